@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Time-to-1e-6 of the per-tick structural solve on one MI355X.
+
+Workload: the 128^3 single island (~57 % occupied box union on an anchored
+z=0 slab, kappa lognormal sigma 0.5, FP32 inputs).  One step = solve
+A phi = b to ||b - A phi|| / ||b|| <= 1e-6 (FP32 MG-preconditioned PCG inside
+FP64 iterative refinement, cold start), then the phase-field history update
+H = max(H, psi_e) and the failure scan, all on device arrays already resident
+in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "pfsf-core_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "time-to-1e-6 residual at 128^3 + batched islands/s vs plain, % of HBM peak"
+
+# Algorithmic bytes per ACTIVE voxel for each profiled solver kernel (each
+# array element counted once per launch, neighbour reuse free):
+#   0 SpMV + p.Ap       read p 4, kappa 4, type 1; write q 4             = 13
+#   1 vector update     read x, r, p, q 16; write x, r 8                 = 24
+#   2 V-cycle down (L0) read r, kappa, diag 12; write xs 4, r_c 0.5       = 16.5
+#   3 V-cycle up   (L0) read xs, r, kappa, diag 16, e_c 0.5; write z 4   = 20.5
+ALG_BYTES = {0: 13.0, 1: 24.0, 2: 16.5, 3: 20.5}
+KERNEL_NAMES = {0: "k_pcg_spmv", 1: "k_pcg_update", 2: "k_mg_down", 3: "k_mg_up"}
+# FP64 Jacobi-PCG iterations the CPU oracle needs on this workload to reach
+# 1e-6 (counted once with oracle.pcg(workload(1), tol=1e-6) on the full solve).
+CPU_PCG_ITERS_TO_1E6 = None
+
+
+def cpu_baseline(isl, sample_iters=40):
+    """The oracle's FP64 Jacobi-PCG (C, one thread) on a bounded sample."""
+    from oracle import oracle as O
+
+    t = time.perf_counter()
+    _, it, _ = O.pcg(isl, tol=0.0, maxit=sample_iters)
+    dt = time.perf_counter() - t
+    per_iter = dt / max(it, 1)
+    iters = CPU_PCG_ITERS_TO_1E6 or 1030
+    return {"value": per_iter * iters * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"{it} FP64 Jacobi-PCG iterations of the C oracle on the 128^3 island "
+                      f"({dt:.1f} s, {per_iter * 1e3:.1f} ms/iter), extrapolated to the "
+                      f"{iters} iterations that solver needs for 1e-6"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--levels", type=int, default=0, help="MG levels (0 = auto hierarchy)")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(islands shard by bin-packing; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pfsf_amd import device as D
+    from pfsf_amd import workloads as W
+
+    isl = W.workload(1)
+    n_act = isl.n_active
+    dev = torch.device("cuda:0")
+    vt = torch.from_numpy(isl.vtype).to(dev)
+    kp = torch.from_numpy(isl.kappa).to(dev)
+    b = torch.from_numpy(isl.source).to(dev)
+    rc = torch.from_numpy(isl.rcomp).to(dev)
+    rt = torch.from_numpy(isl.rtens).to(dev)
+    x64 = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=dev)
+    phi = torch.zeros(isl.vtype.shape, dtype=torch.float32, device=dev)
+    H = torch.zeros_like(phi)
+    code = torch.zeros_like(vt)
+    solver = D.Solver(vt, kp, mg_levels=args.levels, precision=args.precision, tol=1e-6,
+                      max_iters=2000)
+
+    def step():
+        x64.zero_()
+        info = solver.solve(b, x64, phi)
+        D.history_update(vt, kp, phi, H)
+        D.failure_scan(vt, kp, phi, rc, rt, code)
+        return info
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    solve_ms = float(np.median([i.solve_ms for i in infos]))
+    last = infos[-1]
+
+    # live per-kernel timing (HIP events on the solver stream)
+    prof = {}
+    for k in KERNEL_NAMES:
+        if k in (2, 3) and last.levels < 2:
+            continue
+        prof[k] = solver.profile(k, reps=100)
+    iter_ms = solver.profile(4, reps=30)
+    dom = max(prof, key=prof.get)
+    ach = ALG_BYTES[dom] * n_act / (prof[dom] * 1e-3)
+    # whole-iteration algorithmic rate
+    it_bytes = (13 + 24 + 16.5 + 20.5) * n_act
+    hist = np.bincount(code.cpu().numpy().ravel(), minlength=5).tolist()
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_step, 4),
+        "unit": "ms per 128^3 solve to 1e-6 (+ history + scan)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "single island 128^3 box union, PCG + V-cycle to 1e-6, "
+                               "then psi/H update and failure scan",
+                   "grid": [isl.lx, isl.ly, isl.lz], "active_voxels": n_act,
+                   "solid_voxels": isl.n_solid, "mg_levels": last.levels,
+                   "precision": ["fp32", "bf16-coarse"][args.precision],
+                   "inner": "FP32 SpMV/smoothing, FP64 dots + FP64 refinement residual"},
+        "solve_ms_median": round(solve_ms, 4),
+        "pcg_iterations": last.iterations, "refinement_passes": last.refinements,
+        "rel_residual": last.rel_residual,
+        "pcg_iteration_ms": round(iter_ms, 5),
+        "iteration_alg_GBps": round(it_bytes / (iter_ms * 1e-3) / 1e9, 1),
+        "voxel_updates_per_s": round(n_act * last.iterations / (solve_ms * 1e-3), 1),
+        "kernel_ms": {KERNEL_NAMES[k]: round(v, 5) for k, v in prof.items()},
+        "failure_histogram": hist,
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom],
+                     "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_active_voxel": ALG_BYTES[dom]},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(isl)
+    solver.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * pfsf_hip.h — additive device-pointer surface of libpfsf (ABI contract 1.6).
+ *
+ * The reference ABI (pfsf.h) hands host buffers to the engine each tick.
+ * These entry points operate on device-resident arrays (hipMalloc'd, or owned
+ * by any allocator) and a caller-chosen stream, so a caller that keeps its
+ * islands in HBM pays no PCIe transfer.  Every array is on the dense grid
+ * lx*ly*lz with flat index x + lx*(y + ly*z); voxel types are uint8
+ * {0 air, 1 solid, 2 anchor}; `stream` is a hipStream_t (NULL = default
+ * stream).  Calls are asynchronous on `stream` unless stated otherwise.
+ *
+ * The operator is the symmetric 26-point form of the Java CPU reference
+ * (VoxelPhysicsCpuReference.java:66-118): for in-bounds non-air neighbours
+ *   face a = (k_i + k_j)/2, edge a = sqrt(k_i k_j)*0.5f,
+ *   corner a = sqrt(k_i k_j)*0.1666667f, k = max(0, kappa),
+ * anchors are Dirichlet phi = 0 and air is skipped.
+ */
+#ifndef PFSF_HIP_H
+#define PFSF_HIP_H
+
+#include "pfsf_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t lx, ly, lz;
+} pfsf_dims;
+
+/* ── Kernel-level entry points ────────────────────────────────────────── */
+
+/* One Jacobi step, bit-identical to VoxelPhysicsCpuReference.jacobiStep
+ * (java:66-118): FP64 accumulation in PFSFStencil offset order, FP32 store,
+ * phi_out = (float)(omega*(phiJ - prev) + prev).  Replaces the reference's
+ * jacobi_smooth dispatch (SH/jacobi_smooth.comp.glsl:1-227). */
+PFSF_API pfsf_result pfsf_hip_jacobi_step(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                          const float* source, const float* phi_prev,
+                                          float* phi_out, float omega, void* stream);
+
+/* `sweeps` in-place 8-colour Gauss-Seidel sweeps, colours 0..7 with colour
+ * (x&1)|(y&1)<<1|(z&1)<<2 (SH/rbgs_smooth.comp.glsl:86-87,
+ * NP/src/solver/jacobi_solver.cpp:163-175), each voxel updated with the
+ * jacobiStep formula (omega 1).  Bit-identical to the CPU restatement. */
+PFSF_API pfsf_result pfsf_hip_rbgs(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                   const float* source, float* phi, int32_t sweeps, void* stream);
+
+/* Failure scan (SH/failure_scan.comp.glsl:36-176 precedence on the symmetric
+ * operator): code 0 on non-solid; NO_SUPPORT(3) when the voxel has no
+ * coupling or phi > phi_orphan; CRUSHING(2) when rcomp > 0 and
+ * flux_in > rcomp; TENSION(4) when rtens > 0 and flux_out > rtens.  Fluxes
+ * are FP64 sums of a_ij |phi_i - phi_j| split by sign, in stencil order. */
+PFSF_API pfsf_result pfsf_hip_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                           const float* phi, const float* rcomp,
+                                           const float* rtens, float phi_orphan, uint8_t* code,
+                                           void* stream);
+
+/* Elastic energy density psi_e = 0.5*kappa*|grad phi|^2 (per-axis central
+ * difference over occupied face neighbours, one-sided at air/boundary, 0 if
+ * neither) and the history update H = max(H, psi_e) on solid voxels.  Both
+ * FP32 with a fixed expression order (bit-exact vs the CPU restatement).
+ * psi may be NULL. */
+PFSF_API pfsf_result pfsf_hip_history_update(pfsf_dims d, const uint8_t* type,
+                                             const float* kappa, const float* phi, float* H,
+                                             float* psi, void* stream);
+
+/* y = A x in FP64 (exact coefficients), 0 on non-solid rows. */
+PFSF_API pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                        const double* x, double* y, void* stream);
+
+/* Compact non-zero failure codes into the DBB wire format in device memory:
+ * out[0] = count (int32), then count x {x+ox, y+oy, z+oz, code} int32, in
+ * ascending flat-index order, at most `capacity` tuples (the count is
+ * clamped).  Replaces the host loop of NP/src/core/island_buffer.cpp:945-960
+ * and SH/failure_compact.comp.glsl:27-45. */
+PFSF_API pfsf_result pfsf_hip_compact_failures(pfsf_dims d, const uint8_t* code, pfsf_pos origin,
+                                               int32_t* out, int32_t capacity, void* stream);
+
+/* 2:1 downsample of a per-voxel field and its types (NP/src/compute/
+ * downsample.cpp:39-98): mean over in-bounds children; type anchor if
+ * anchors >= solids and >= airs, else solid if solids >= airs, else air. */
+PFSF_API pfsf_result pfsf_hip_downsample_2to1(pfsf_dims fine, const float* field,
+                                              const uint8_t* type, float* coarse,
+                                              uint8_t* coarse_type, void* stream);
+
+/* sigma-max normalisation (PFSFDataBuilder.java:222-278,
+ * NP/src/compute/normalize.cpp:29-63): s = max(1, max cond); when s > 1 the
+ * source, rcomp, rtens (n each) and cond (6n) are multiplied by 1/s in place.
+ * *sigma_max_host receives s (this call synchronises the stream). */
+PFSF_API pfsf_result pfsf_hip_normalize_soa6(float* source, float* rcomp, float* rtens,
+                                             float* cond, int64_t n, float* sigma_max_host,
+                                             void* stream);
+
+/* Per-voxel kappa from SoA-6 face conductivities: kappa_i = max_d cond[d*n+i]. */
+PFSF_API pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa,
+                                              void* stream);
+
+/* ── Solver object: PCG to a relative-residual tolerance ─────────────── */
+
+typedef struct pfsf_solver_t* pfsf_solver;
+
+enum {
+  PFSF_PREC_FP32 = 0,       /* FP32 smoothing/SpMV, FP32 kappa on every level */
+  PFSF_PREC_BF16_COARSE = 1 /* as FP32, coarse-level kappa stored as BF16 */
+};
+
+typedef struct {
+  pfsf_dims dims;
+  const uint8_t* type;      /* device, N */
+  const float* kappa;       /* device, N */
+  int32_t mg_levels;        /* 1 = Jacobi-PCG; >= 2 geometric V-cycle levels; 0 = auto */
+  int32_t precision;        /* PFSF_PREC_* */
+  int32_t coarse_sweeps;    /* Chebyshev sweeps on the coarsest level (0 = default) */
+  double tol;               /* ||b - A x||_2 / ||b||_2 target (FP64 residual) */
+  int32_t max_iters;        /* cap on total PCG iterations */
+  void* stream;
+} pfsf_solver_desc;
+
+typedef struct {
+  int32_t iterations;   /* total inner PCG iterations */
+  int32_t refinements;  /* FP64 iterative-refinement passes */
+  int32_t converged;    /* 1 when rel_residual <= tol */
+  int32_t levels;       /* multigrid levels actually built */
+  double rel_residual;  /* final FP64 ||b - A x|| / ||b|| */
+  double solve_ms;      /* device time of the solve (hipEvents) */
+} pfsf_solve_info;
+
+/* Builds the multigrid hierarchy and workspaces (allocates device memory).
+ * Orphan components (solid voxels with no 26-connected path to an anchor) and
+ * isolated voxels are excluded from the solve and keep phi = 0. */
+PFSF_API pfsf_result pfsf_solver_create(const pfsf_solver_desc* desc, pfsf_solver* out);
+
+/* Solves A x = b.  x64 (device, N doubles) is the initial guess on entry
+ * (NULL = start from 0) and the FP64 solution on exit (may be NULL);
+ * phi32 (device, N floats, may be NULL) receives the FP32-rounded solution.
+ * Synchronous: returns after convergence or max_iters. */
+PFSF_API pfsf_result pfsf_solver_solve(pfsf_solver s, const float* b, double* x64, float* phi32,
+                                       pfsf_solve_info* info);
+
+/* Re-reads kappa/type (same dims) and rebuilds the hierarchy in place. */
+PFSF_API pfsf_result pfsf_solver_update(pfsf_solver s, const uint8_t* type, const float* kappa);
+
+PFSF_API void pfsf_solver_destroy(pfsf_solver s);
+
+/* Measurement hook: average device time (ms) of one launch of a solver
+ * kernel, replayed `reps` times between two hipEvents on the solver's stream
+ * with the convergence stop disabled.  which: 0 finest-level SpMV + p.Ap,
+ * 1 finest vector update + r.r, 2 finest down-sweep (smooth, residual,
+ * restrict), 3 finest up-sweep (prolong, smooth, r.z), 4 one PCG iteration
+ * including the whole V-cycle.  Use after pfsf_solver_solve; it overwrites
+ * the solver's scratch vectors, not its inputs or results. */
+PFSF_API pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps,
+                                         double* avg_ms);
+
+/* Orphan mask used by the solver: 1 for solid voxels 26-connected to an
+ * anchor through non-air voxels, else 0 (device, N bytes).  Synchronous. */
+PFSF_API pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach,
+                                            void* stream);
+
+/* ── Engine extension: exact per-voxel kappa ────────────────────────────
+ * By default a registered island's kappa is the per-voxel maximum of its
+ * SoA-6 conductivities.  This host buffer (float32 x N, valid for the
+ * island's lifetime) overrides it with an exact per-voxel kappa. */
+PFSF_API pfsf_result pfsf_register_island_kappa(pfsf_engine engine, int32_t island_id,
+                                                void* kappa_addr, int64_t bytes);
+
+/* Engine solve controls: relative tolerance (default 1e-6) and the cap on
+ * PCG iterations per island per tick (default 500). */
+PFSF_API void pfsf_set_solve_tolerance(pfsf_engine engine, double tol, int32_t max_iters);
+
+/* Device history field of an island (N floats), for inspection. */
+PFSF_API pfsf_result pfsf_read_history(pfsf_engine engine, int32_t island_id, float* out,
+                                       int32_t capacity, int32_t* out_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFSF_HIP_H */

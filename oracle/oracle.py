@@ -1,0 +1,278 @@
+"""Python face of the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker, never as the product.
+
+Two layers:
+  * ctypes bindings of oracle/_build/liboracle.so, our C restatement of the
+    reference's Java CPU solver (VoxelPhysicsCpuReference.java:66-166) and of
+    the failure-scan / history rules (SH/failure_scan.comp.glsl:36-196);
+  * an FP64 scipy formulation of the same operator (sparse assembly + direct
+    solve) used to cross-check the C restatement and as the phi reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libpfsf_compute_ref.so"
+
+EDGE_P = np.float32(0.5)
+CORNER_P = np.float32(0.1666667)
+# PFSFStencil.java:70-84
+OFFSETS = np.array([
+    (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+    (1, 1, 0), (1, -1, 0), (-1, 1, 0), (-1, -1, 0),
+    (1, 0, 1), (1, 0, -1), (-1, 0, 1), (-1, 0, -1),
+    (0, 1, 1), (0, 1, -1), (0, -1, 1), (0, -1, -1),
+    (1, 1, 1), (1, 1, -1), (1, -1, 1), (1, -1, -1),
+    (-1, 1, 1), (-1, 1, -1), (-1, -1, 1), (-1, -1, -1),
+], dtype=np.int64)
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle (and oracle/_ref when the reference tree exists)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        I = C.c_int
+        L.or_jacobi_step.argtypes = [I, I, I, P, P, P, P, P, C.c_float]
+        L.or_rbgs.argtypes = [I, I, I, P, P, P, P, I]
+        L.or_diag.argtypes = [I, I, I, P, P, P]
+        L.or_apply.argtypes = [I, I, I, P, P, P, P]
+        L.or_residual.argtypes = [I, I, I, P, P, P, P, P]
+        L.or_residual.restype = C.c_double
+        L.or_pcg.argtypes = [I, I, I, P, P, P, P, C.c_double, I, P]
+        L.or_pcg.restype = I
+        L.or_failure_scan.argtypes = [I, I, I, P, P, P, P, P, C.c_float, P]
+        L.or_psi.argtypes = [I, I, I, P, P, P, P]
+        L.or_history_update.argtypes = [I, I, I, P, P, P, P]
+        L.or_anchored.argtypes = [I, I, I, P, P]
+        L.or_anchored.restype = C.c_int64
+        L.or_normalize_soa6.argtypes = [P, P, P, P, C.c_int32]
+        L.or_normalize_soa6.restype = C.c_float
+        L.or_chebyshev_omega.argtypes = [C.c_int32, C.c_float]
+        L.or_chebyshev_omega.restype = C.c_float
+        L.or_downsample_2to1.argtypes = [P, P, I, I, I, P, P]
+        L.or_energy_grad_kappa.argtypes = [I, I, I, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dims(isl):
+    return isl.lx, isl.ly, isl.lz
+
+
+# ---------------------------------------------------------------- C oracle
+
+def jacobi_step(isl, phi_prev, omega=1.0):
+    out = np.empty_like(phi_prev, dtype=np.float32)
+    lib().or_jacobi_step(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(isl.source),
+                         _p(np.ascontiguousarray(phi_prev, np.float32)), _p(out), omega)
+    return out
+
+
+def jacobi_solve(isl, steps):
+    """VoxelPhysicsCpuReference.solve (java:121-127): steps Jacobi sweeps from 0."""
+    phi = np.zeros(isl.vtype.shape, np.float32)
+    for _ in range(steps):
+        phi = jacobi_step(isl, phi)
+    return phi
+
+
+def rbgs(isl, phi=None, sweeps=1):
+    phi = np.zeros(isl.vtype.shape, np.float32) if phi is None else np.array(phi, np.float32)
+    lib().or_rbgs(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(isl.source), _p(phi), sweeps)
+    return phi
+
+
+def diag(isl):
+    d = np.empty(isl.vtype.shape, np.float64)
+    lib().or_diag(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(d))
+    return d
+
+
+def apply(isl, x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(isl.vtype.shape, np.float64)
+    lib().or_apply(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(x), _p(y))
+    return y
+
+
+def residual(isl, x, b=None):
+    b = isl.source if b is None else np.ascontiguousarray(b, np.float32)
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_residual(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(b), _p(x), None)
+
+
+def pcg(isl, tol=1e-10, maxit=100000, x0=None, b=None):
+    """FP64 Jacobi-PCG; returns (x, iterations, true relative residual)."""
+    b = isl.source if b is None else np.ascontiguousarray(b, np.float32)
+    x = np.zeros(isl.vtype.shape, np.float64) if x0 is None else np.array(x0, np.float64)
+    rel = C.c_double(0)
+    it = lib().or_pcg(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(b), _p(x), tol, maxit,
+                      C.byref(rel))
+    return x, it, rel.value
+
+
+def failure_scan(isl, phi, phi_orphan=1e6):
+    code = np.empty(isl.vtype.shape, np.uint8)
+    lib().or_failure_scan(*_dims(isl), _p(isl.vtype), _p(isl.kappa),
+                          _p(np.ascontiguousarray(phi, np.float32)), _p(isl.rcomp),
+                          _p(isl.rtens), phi_orphan, _p(code))
+    return code
+
+
+def psi(isl, phi):
+    out = np.empty(isl.vtype.shape, np.float32)
+    lib().or_psi(*_dims(isl), _p(isl.vtype), _p(isl.kappa),
+                 _p(np.ascontiguousarray(phi, np.float32)), _p(out))
+    return out
+
+
+def history_update(isl, phi, H):
+    H = np.array(H, np.float32)
+    lib().or_history_update(*_dims(isl), _p(isl.vtype), _p(isl.kappa),
+                            _p(np.ascontiguousarray(phi, np.float32)), _p(H))
+    return H
+
+
+def anchored(isl):
+    reach = np.empty(isl.vtype.shape, np.uint8)
+    orphans = lib().or_anchored(*_dims(isl), _p(isl.vtype), _p(reach))
+    return reach, int(orphans)
+
+
+def normalize_soa6(source, rcomp, rtens, cond):
+    s, r, t, c = (np.array(a, np.float32) for a in (source, rcomp, rtens, cond))
+    smax = lib().or_normalize_soa6(_p(s), _p(r), _p(t), _p(c), s.size)
+    return s, r, t, c, smax
+
+
+def chebyshev_omega(it, rho):
+    return lib().or_chebyshev_omega(it, rho)
+
+
+def downsample_2to1(fine, ftype, lx, ly, lz):
+    lxc, lyc, lzc = (lx + 1) // 2, (ly + 1) // 2, (lz + 1) // 2
+    c = np.empty((lzc, lyc, lxc), np.float32)
+    ct = np.empty((lzc, lyc, lxc), np.uint8)
+    lib().or_downsample_2to1(_p(np.ascontiguousarray(fine, np.float32)),
+                             _p(np.ascontiguousarray(ftype, np.uint8)), lx, ly, lz, _p(c), _p(ct))
+    return c, ct
+
+
+# ------------------------------------------------ reference CPU primitives
+
+def ref_lib():
+    """oracle/_ref build of the reference's own normalize/chebyshev/downsample."""
+    if not REF_PATH.exists():
+        return None
+    L = C.CDLL(str(REF_PATH))
+    P = C.c_void_p
+    L.pfsf_normalize_soa6.argtypes = [P, P, P, P, P, C.c_int32, P]
+    L.pfsf_chebyshev_omega.argtypes = [C.c_int32, C.c_float]
+    L.pfsf_chebyshev_omega.restype = C.c_float
+    L.pfsf_downsample_2to1.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, P, P]
+    return L
+
+
+# ------------------------------------------------------- FP64 scipy oracle
+
+def coupling_f64(k: int, ki: np.ndarray, kj: np.ndarray) -> np.ndarray:
+    """Coupling weights in the Java expression order (float sum/product, double scale)."""
+    ki = np.maximum(ki, np.float32(0)).astype(np.float32)
+    kj = np.maximum(kj, np.float32(0)).astype(np.float32)
+    if k < 6:
+        return 0.5 * (ki + kj).astype(np.float64)
+    r = np.sqrt((ki * kj).astype(np.float64))
+    return r * (np.float64(EDGE_P) if k < 18 else np.float64(CORNER_P))
+
+
+def assemble(isl):
+    """Sparse FP64 matrix of the operator restricted to solid voxels.
+
+    Returns (A, solid_flat_indices).  Independent of the C restatement: built
+    edge-by-edge with numpy from the same coupling rule.
+    """
+    import scipy.sparse as sp
+
+    lz, ly, lx = isl.vtype.shape
+    t = isl.vtype
+    solid = (t != 0) & (t != 2)
+    nonair = t != 0
+    ids = -np.ones(t.shape, np.int64)
+    sidx = np.flatnonzero(solid.ravel())
+    ids.ravel()[sidx] = np.arange(sidx.size)
+    rows, cols, vals = [], [], []
+    dg = np.zeros(sidx.size)
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    for k, (dx, dy, dz) in enumerate(OFFSETS):
+        nx, ny, nz = x + dx, y + dy, z + dz
+        ok = (nx >= 0) & (nx < lx) & (ny >= 0) & (ny < ly) & (nz >= 0) & (nz < lz) & solid
+        ii = np.flatnonzero(ok.ravel())
+        jx, jy, jz = nx.ravel()[ii], ny.ravel()[ii], nz.ravel()[ii]
+        jj = jx + lx * (jy + ly * jz)
+        keep = nonair.ravel()[jj]
+        ii, jj = ii[keep], jj[keep]
+        a = coupling_f64(k, isl.kappa.ravel()[ii], isl.kappa.ravel()[jj])
+        ri = ids.ravel()[ii]
+        np.add.at(dg, ri, a)
+        sj = solid.ravel()[jj]
+        rows.append(ri[sj])
+        cols.append(ids.ravel()[jj][sj])
+        vals.append(-a[sj])
+    rows.append(np.arange(sidx.size))
+    cols.append(np.arange(sidx.size))
+    vals.append(dg)
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(sidx.size, sidx.size))
+    return A, sidx
+
+
+def direct_solve(isl, b=None):
+    """FP64 sparse direct solve of A phi = b on solid voxels (0 elsewhere)."""
+    import scipy.sparse.linalg as spla
+
+    A, sidx = assemble(isl)
+    b = isl.source if b is None else b
+    rhs = np.asarray(b, np.float64).ravel()[sidx]
+    sol = spla.spsolve(A.tocsc(), rhs)
+    phi = np.zeros(isl.n, np.float64)
+    phi[sidx] = sol
+    return phi.reshape(isl.vtype.shape)
+
+
+def energy(isl_all_solid_kappa, phi, lx, ly, lz):
+    """E = 1/2 phi^T L phi with every voxel solid (EnergyEvaluatorCPU elastic term)."""
+    from types import SimpleNamespace
+
+    vt = np.ones((lz, ly, lx), np.uint8)
+    k = np.ascontiguousarray(np.broadcast_to(isl_all_solid_kappa, (lz, ly, lx)), np.float32)
+    isl = SimpleNamespace(lx=lx, ly=ly, lz=lz, vtype=vt, kappa=k)
+    y = apply(isl, np.ascontiguousarray(phi, np.float64).reshape(lz, ly, lx))
+    return 0.5 * float((np.asarray(phi, np.float64).ravel() * y.ravel()).sum())
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]

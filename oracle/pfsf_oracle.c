@@ -1,0 +1,512 @@
+/*
+ * pfsf_oracle.c — CPU restatement of PFSF-CORE's per-tick structural solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
+ * compared against; it is built into oracle/_build/liboracle.so and may be
+ * loaded only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg.  Nothing in the product library links or calls it.
+ *
+ * Canonical operator (26-point, per-voxel conductivity kappa, symmetric):
+ *   Block Reality/api/src/test/java/com/blockreality/api/physics/validation/
+ *     VoxelPhysicsCpuReference.java:66-118   (jacobiStep: loop order z,y,x;
+ *       offsets in PFSFStencil order; air neighbours skipped; anchors and air
+ *       rows forced to 0; double accumulation, float store)
+ *   .../physics/pfsf/PFSFStencil.java:42-43,70-84  (EDGE_P=0.5f,
+ *       CORNER_P=0.1666667f, NEIGHBOR_OFFSETS order)
+ *   .../physics/effective/EnergyEvaluatorCPU.java:166-185 (same weights)
+ *
+ * Coupling between voxel i and an in-bounds non-air neighbour j, evaluated in
+ * exactly the Java expression order (float products/sums, double scaling):
+ *   face   : 0.5 * (double)(ki + kj)             ki,kj = max(0, kappa) as float
+ *   edge   : sqrt((double)(ki * kj)) * (double)EDGE_P
+ *   corner : sqrt((double)(ki * kj)) * (double)CORNER_P
+ *
+ * Build with -ffp-contract=off: every product and sum rounds separately, as
+ * in Java.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define T_AIR 0
+#define T_SOLID 1
+#define T_ANCHOR 2
+
+static const float EDGE_P = 0.5f;
+static const float CORNER_P = 0.1666667f;
+
+/* PFSFStencil.java:70-84 — order matters for the double accumulation. */
+static const int OFFS[26][3] = {
+    {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+    {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0},
+    {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1},
+    {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1},
+    {1, 1, 1},   {1, 1, -1},  {1, -1, 1},  {1, -1, -1},
+    {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1},
+};
+
+static inline int64_t IDX(int x, int y, int z, int lx, int ly) {
+  return (int64_t)x + (int64_t)lx * ((int64_t)y + (int64_t)ly * (int64_t)z);
+}
+
+static inline int is_solid(uint8_t t) { return t != T_AIR && t != T_ANCHOR; }
+
+/* Coupling a_ij for offset class k (0..5 face, 6..17 edge, 18..25 corner). */
+static inline double coupling(int k, float ki, float kj) {
+  if (k < 6) {
+    float s = ki + kj;
+    return 0.5 * (double)s;
+  }
+  float p = ki * kj;
+  double r = sqrt((double)p);
+  return (k < 18) ? r * (double)EDGE_P : r * (double)CORNER_P;
+}
+
+static inline float kap(const float* kappa, int64_t i) {
+  float k = kappa[i];
+  return k > 0.0f ? k : 0.0f; /* Math.max(0f, sigma) */
+}
+
+/* One Jacobi step, VoxelPhysicsCpuReference.jacobiStep (java:66-118). */
+void or_jacobi_step(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                    const float* source, const float* phi_prev, float* phi_out, float omega) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        uint8_t t = type[i];
+        if (t == T_ANCHOR || t == T_AIR) {
+          phi_out[i] = 0.0f;
+          continue;
+        }
+        double diag = 0.0, off = 0.0;
+        float ki = kap(kappa, i);
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double a = coupling(k, ki, kap(kappa, j));
+          diag += a;
+          off += a * (double)phi_prev[j];
+        }
+        double b = (double)source[i];
+        double pj = (diag > 1e-15) ? (b + off) / diag : (double)phi_prev[i];
+        phi_out[i] = (float)((double)omega * (pj - (double)phi_prev[i]) + (double)phi_prev[i]);
+      }
+}
+
+/* In-place 8-colour Gauss-Seidel with the jacobiStep per-voxel update.
+ * Colour = (x&1) | (y&1)<<1 | (z&1)<<2, colours 0..7 ascending
+ * (SH/rbgs_smooth.comp.glsl:86-87, NP/src/solver/jacobi_solver.cpp:163-175).
+ * No damping and no macro-block skip. */
+void or_rbgs(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+             const float* source, float* phi, int sweeps) {
+  for (int s = 0; s < sweeps; s++)
+    for (int c = 0; c < 8; c++)
+      for (int z = (c >> 2) & 1; z < lz; z += 2)
+        for (int y = (c >> 1) & 1; y < ly; y += 2)
+          for (int x = c & 1; x < lx; x += 2) {
+            int64_t i = IDX(x, y, z, lx, ly);
+            uint8_t t = type[i];
+            if (t == T_ANCHOR || t == T_AIR) {
+              phi[i] = 0.0f;
+              continue;
+            }
+            double diag = 0.0, off = 0.0;
+            float ki = kap(kappa, i);
+            for (int k = 0; k < 26; k++) {
+              int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+              if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+              int64_t j = IDX(nx, ny, nz, lx, ly);
+              if (type[j] == T_AIR) continue;
+              double a = coupling(k, ki, kap(kappa, j));
+              diag += a;
+              off += a * (double)phi[j];
+            }
+            double b = (double)source[i];
+            double pj = (diag > 1e-15) ? (b + off) / diag : (double)phi[i];
+            phi[i] = (float)((pj - (double)phi[i]) + (double)phi[i]);
+          }
+}
+
+/* Diagonal D_i = sum_j a_ij over non-air neighbours (anchors included); 0 on
+ * non-solid rows.  Same accumulation order as jacobiStep's diagA. */
+void or_diag(int lx, int ly, int lz, const uint8_t* type, const float* kappa, double* d) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        if (!is_solid(type[i])) {
+          d[i] = 0.0;
+          continue;
+        }
+        double diag = 0.0;
+        float ki = kap(kappa, i);
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          diag += coupling(k, ki, kap(kappa, j));
+        }
+        d[i] = diag;
+      }
+}
+
+/* y = A x over solid rows: y_i = sum_j a_ij (x_i - x_j), x_j := 0 for anchor
+ * neighbours (Dirichlet), air skipped.  y = 0 on non-solid rows. */
+void or_apply(int lx, int ly, int lz, const uint8_t* type, const float* kappa, const double* x,
+              double* y) {
+  for (int z = 0; z < lz; z++)
+    for (int yy = 0; yy < ly; yy++)
+      for (int xx = 0; xx < lx; xx++) {
+        int64_t i = IDX(xx, yy, z, lx, ly);
+        if (!is_solid(type[i])) {
+          y[i] = 0.0;
+          continue;
+        }
+        double acc = 0.0;
+        float ki = kap(kappa, i);
+        double xi = x[i];
+        for (int k = 0; k < 26; k++) {
+          int nx = xx + OFFS[k][0], ny = yy + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          uint8_t tj = type[j];
+          if (tj == T_AIR) continue;
+          double xj = is_solid(tj) ? x[j] : 0.0;
+          acc += coupling(k, ki, kap(kappa, j)) * (xi - xj);
+        }
+        y[i] = acc;
+      }
+}
+
+/* ||b - A x||_2 / ||b||_2 over solid voxels, FP64.  Writes r if non-NULL. */
+double or_residual(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                   const float* b, const double* x, double* r) {
+  int64_t n = (int64_t)lx * ly * lz;
+  double* ax = (double*)malloc(sizeof(double) * (size_t)n);
+  or_apply(lx, ly, lz, type, kappa, x, ax);
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    double ri = 0.0;
+    if (is_solid(type[i])) {
+      ri = (double)b[i] - ax[i];
+      bb += (double)b[i] * (double)b[i];
+    }
+    rr += ri * ri;
+    if (r) r[i] = ri;
+  }
+  free(ax);
+  return bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
+/* FP64 Jacobi-preconditioned CG on the solid rows to ||r||/||b|| <= tol.
+ * x holds the initial guess on entry and the solution on exit.  Returns the
+ * iteration count; *relres gets the final true relative residual. */
+int or_pcg(int lx, int ly, int lz, const uint8_t* type, const float* kappa, const float* b,
+           double* x, double tol, int maxit, double* relres) {
+  int64_t n = (int64_t)lx * ly * lz;
+  double* d = (double*)malloc(sizeof(double) * (size_t)n);
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* z = (double*)malloc(sizeof(double) * (size_t)n);
+  double* p = (double*)malloc(sizeof(double) * (size_t)n);
+  double* q = (double*)malloc(sizeof(double) * (size_t)n);
+  or_diag(lx, ly, lz, type, kappa, d);
+  for (int64_t i = 0; i < n; i++)
+    if (!is_solid(type[i]) || d[i] <= 0.0) x[i] = 0.0;
+  or_residual(lx, ly, lz, type, kappa, b, x, r);
+  double bb = 0.0;
+  for (int64_t i = 0; i < n; i++)
+    if (is_solid(type[i])) bb += (double)b[i] * (double)b[i];
+  double bn = sqrt(bb > 0.0 ? bb : 1.0);
+  double rz = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    z[i] = (d[i] > 0.0) ? r[i] / d[i] : 0.0;
+    p[i] = z[i];
+    rz += r[i] * z[i];
+  }
+  int it = 0;
+  double rn = 0.0;
+  for (int64_t i = 0; i < n; i++) rn += r[i] * r[i];
+  rn = sqrt(rn);
+  while (rn / bn > tol && it < maxit) {
+    or_apply(lx, ly, lz, type, kappa, p, q);
+    double pq = 0.0;
+    for (int64_t i = 0; i < n; i++) pq += p[i] * q[i];
+    if (pq <= 0.0) break;
+    double alpha = rz / pq;
+    double rz2 = 0.0;
+    rn = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      z[i] = (d[i] > 0.0) ? r[i] / d[i] : 0.0;
+      rz2 += r[i] * z[i];
+      rn += r[i] * r[i];
+    }
+    rn = sqrt(rn);
+    double beta = rz2 / rz;
+    rz = rz2;
+    for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    it++;
+  }
+  if (relres) *relres = or_residual(lx, ly, lz, type, kappa, b, x, NULL);
+  free(d); free(r); free(z); free(p); free(q);
+  return it;
+}
+
+/* Failure scan on the canonical operator.  Code precedence follows
+ * SH/failure_scan.comp.glsl:41-176:
+ *   non-solid -> 0; no coupling (D_i == 0) or phi > 1e6 -> NO_SUPPORT(3);
+ *   rcomp > 0 && flux_in > rcomp -> CRUSHING(2);
+ *   rtens > 0 && flux_out > rtens -> TENSION(4); else 0.
+ * flux_in  = sum a_ij (phi_j - phi_i) over phi_j > phi_i,
+ * flux_out = sum a_ij (phi_i - phi_j) over phi_j < phi_i, all non-air
+ * neighbours (anchors at their stored phi), FP64 in stencil order. */
+void or_failure_scan(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                     const float* phi, const float* rcomp, const float* rtens, float phi_orphan,
+                     uint8_t* code) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        code[i] = 0;
+        if (!is_solid(type[i])) continue;
+        float ki = kap(kappa, i);
+        double pi = (double)phi[i];
+        double diag = 0.0, fin = 0.0, fout = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double a = coupling(k, ki, kap(kappa, j));
+          diag += a;
+          double dp = (double)phi[j] - pi;
+          if (dp > 0.0)
+            fin += a * dp;
+          else if (dp < 0.0)
+            fout += a * (-dp);
+        }
+        if (diag == 0.0 || phi[i] > phi_orphan) {
+          code[i] = 3;
+          continue;
+        }
+        if (rcomp[i] > 0.0f && fin > (double)rcomp[i]) {
+          code[i] = 2;
+          continue;
+        }
+        if (rtens[i] > 0.0f && fout > (double)rtens[i]) {
+          code[i] = 4;
+          continue;
+        }
+      }
+}
+
+/* Elastic energy density psi_e = 0.5 * kappa * |grad phi|^2 (float).
+ * Per axis: central difference when both face neighbours are non-air and in
+ * bounds, one-sided when only one is, 0 when neither.  Only solid voxels get a
+ * value; others 0.  Expression order is fixed so the GPU can match bitwise. */
+static inline float axis_grad(const uint8_t* type, const float* phi, int64_t i, int has_m,
+                              int has_p, int64_t jm, int64_t jp) {
+  int om = has_m && type[jm] != T_AIR;
+  int op = has_p && type[jp] != T_AIR;
+  if (om && op) return (phi[jp] - phi[jm]) * 0.5f;
+  if (op) return phi[jp] - phi[i];
+  if (om) return phi[i] - phi[jm];
+  return 0.0f;
+}
+
+void or_psi(int lx, int ly, int lz, const uint8_t* type, const float* kappa, const float* phi,
+            float* psi) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        if (!is_solid(type[i])) {
+          psi[i] = 0.0f;
+          continue;
+        }
+        int64_t sx = 1, sy = lx, sz = (int64_t)lx * ly;
+        float gx = axis_grad(type, phi, i, x > 0, x + 1 < lx, i - sx, i + sx);
+        float gy = axis_grad(type, phi, i, y > 0, y + 1 < ly, i - sy, i + sy);
+        float gz = axis_grad(type, phi, i, z > 0, z + 1 < lz, i - sz, i + sz);
+        float g2 = gx * gx;
+        g2 = g2 + gy * gy;
+        g2 = g2 + gz * gz;
+        float h = 0.5f * kap(kappa, i);
+        psi[i] = h * g2;
+      }
+}
+
+/* H = max(H, psi_e) on solid voxels (monotone history field). */
+void or_history_update(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                       const float* phi, float* H) {
+  int64_t n = (int64_t)lx * ly * lz;
+  float* psi = (float*)malloc(sizeof(float) * (size_t)n);
+  or_psi(lx, ly, lz, type, kappa, phi, psi);
+  for (int64_t i = 0; i < n; i++)
+    if (is_solid(type[i]) && psi[i] > H[i]) H[i] = psi[i];
+  free(psi);
+}
+
+/* 26-connected reachability from anchors over non-air voxels
+ * (VoxelPhysicsCpuReference.findOrphans, java:138-166).  reach[i]=1 for
+ * reached voxels.  Returns the number of orphan solid voxels. */
+int64_t or_anchored(int lx, int ly, int lz, const uint8_t* type, uint8_t* reach) {
+  int64_t n = (int64_t)lx * ly * lz;
+  int64_t* q = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  int64_t head = 0, tail = 0;
+  for (int64_t i = 0; i < n; i++) {
+    reach[i] = 0;
+    if (type[i] == T_ANCHOR) {
+      reach[i] = 1;
+      q[tail++] = i;
+    }
+  }
+  while (head < tail) {
+    int64_t i = q[head++];
+    int x = (int)(i % lx), y = (int)((i / lx) % ly), z = (int)(i / ((int64_t)lx * ly));
+    for (int k = 0; k < 26; k++) {
+      int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+      if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+      int64_t j = IDX(nx, ny, nz, lx, ly);
+      if (type[j] == T_AIR || reach[j]) continue;
+      reach[j] = 1;
+      q[tail++] = j;
+    }
+  }
+  int64_t orphans = 0;
+  for (int64_t i = 0; i < n; i++)
+    if (is_solid(type[i]) && !reach[i]) orphans++;
+  free(q);
+  return orphans;
+}
+
+/* sigma-max normalisation, PFSFDataBuilder.normalizeSoA6JavaRef
+ * (PFSFDataBuilder.java:257-278) / NP/src/compute/normalize.cpp:29-63:
+ * s = max(1, max cond); if s > 1 divide source, rcomp, rtens, cond by s
+ * (multiplication by the float reciprocal, as the native path does). */
+float or_normalize_soa6(float* source, float* rcomp, float* rtens, float* cond, int32_t n) {
+  float smax = 1.0f;
+  for (int64_t j = 0; j < 6 * (int64_t)n; j++)
+    if (cond[j] > smax) smax = cond[j];
+  if (smax > 1.0f) {
+    float inv = 1.0f / smax;
+    for (int64_t j = 0; j < n; j++) {
+      source[j] *= inv;
+      rcomp[j] *= inv;
+      rtens[j] *= inv;
+    }
+    for (int64_t j = 0; j < 6 * (int64_t)n; j++) cond[j] *= inv;
+  }
+  return smax;
+}
+
+/* Chebyshev omega schedule, PFSFScheduler.computeOmega /
+ * NP/src/compute/chebyshev.cpp:30-57: w0=1, w1=2/(2-rho^2),
+ * wk=4/(4-rho^2 w_{k-1}); stop at the previous value when the denominator
+ * drops below 0.01 or the result leaves (0,2]; clamp at 1.98. */
+float or_chebyshev_omega(int32_t iter, float rho) {
+  if (iter <= 0) return 1.0f;
+  float r2 = rho * rho;
+  if (r2 >= 1.0f) return 1.0f;
+  if (iter == 1) return 2.0f / (2.0f - r2);
+  float wp = 1.0f, w = 2.0f / (2.0f - r2);
+  for (int32_t k = 2; k <= iter; k++) {
+    float den = 4.0f - r2 * w;
+    if (den < 0.01f) {
+      w = wp;
+      break;
+    }
+    float wn = 4.0f / den;
+    if (wn > 2.0f || wn != wn) {
+      w = wp;
+      break;
+    }
+    wp = w;
+    w = wn;
+  }
+  return w < 1.98f ? w : 1.98f;
+}
+
+/* 2:1 coarse downsample of a per-voxel float field and voxel types
+ * (NP/src/compute/downsample.cpp:39-98): float mean over in-bounds children,
+ * type = anchor if anchors >= solids and >= airs, else solid if solids >= airs,
+ * else air. */
+void or_downsample_2to1(const float* fine, const uint8_t* ftype, int lxf, int lyf, int lzf,
+                        float* coarse, uint8_t* ctype) {
+  int lxc = (lxf + 1) / 2, lyc = (lyf + 1) / 2, lzc = (lzf + 1) / 2;
+  for (int zc = 0; zc < lzc; zc++)
+    for (int yc = 0; yc < lyc; yc++)
+      for (int xc = 0; xc < lxc; xc++) {
+        float sum = 0.0f;
+        int cnt = 0, na = 0, ns = 0, nk = 0;
+        for (int dz = 0; dz < 2; dz++) {
+          int zf = 2 * zc + dz;
+          if (zf >= lzf) continue;
+          for (int dy = 0; dy < 2; dy++) {
+            int yf = 2 * yc + dy;
+            if (yf >= lyf) continue;
+            for (int dx = 0; dx < 2; dx++) {
+              int xf = 2 * xc + dx;
+              if (xf >= lxf) continue;
+              int64_t f = IDX(xf, yf, zf, lxf, lyf);
+              sum += fine[f];
+              cnt++;
+              uint8_t t = ftype[f];
+              if (t == T_ANCHOR) nk++;
+              else if (t == T_SOLID) ns++;
+              else na++;
+            }
+          }
+        }
+        int64_t c = IDX(xc, yc, zc, lxc, lyc);
+        coarse[c] = cnt > 0 ? sum / (float)cnt : 0.0f;
+        if (nk >= ns && nk >= na) ctype[c] = T_ANCHOR;
+        else if (ns >= na) ctype[c] = T_SOLID;
+        else ctype[c] = T_AIR;
+      }
+}
+
+/* Adjoint-ready helpers.  d(a_ij)/d(kappa_i) for the three coupling classes,
+ * on the real-valued formulas a = (ki+kj)/2 and c*sqrt(ki kj). */
+void or_energy_grad_kappa(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                          const double* u, const double* v, double* g) {
+  /* g_i = sum over couplings touching i of d a_ij/d kappa_i * (u_i-u_j)(v_i-v_j),
+   * with u,v := 0 at anchors.  This is dL/dkappa for L = -lambda^T A phi pairs. */
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        g[i] = 0.0;
+        if (type[i] == T_AIR) continue;
+        double ki = (double)kap(kappa, i);
+        double ui = is_solid(type[i]) ? u[i] : 0.0;
+        double vi = is_solid(type[i]) ? v[i] : 0.0;
+        double acc = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          uint8_t tj = type[j];
+          if (tj == T_AIR) continue;
+          if (!is_solid(type[i]) && !is_solid(tj)) continue;
+          double kj = (double)kap(kappa, j);
+          double da;
+          if (k < 6)
+            da = 0.5;
+          else {
+            double c = (k < 18) ? (double)EDGE_P : (double)CORNER_P;
+            da = (ki > 0.0) ? c * 0.5 * sqrt(kj / ki) : 0.0;
+          }
+          double uj = is_solid(tj) ? u[j] : 0.0;
+          double vj = is_solid(tj) ? v[j] : 0.0;
+          acc += da * (ui - uj) * (vi - vj);
+        }
+        g[i] = acc;
+      }
+}

@@ -1,0 +1,779 @@
+// libpfsf C ABI on HIP: the reference engine surface (pfsf.h) and the additive
+// device-pointer surface (pfsf_hip.h).
+//
+// Engine behaviour follows NP/src/pfsf_engine.cpp (argument validation
+// :156-285, dirty-island tick loop :424-600, failure tuple wire format
+// NP/src/core/island_buffer.cpp:930-962) with the solve replaced: each dirty
+// island is solved to a relative-residual tolerance by MG-preconditioned PCG
+// warm-started from its previous phi, then the history field and the failure
+// scan run on the device and failures are compacted on the device.
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "mg_solver.h"
+#include "pfsf/pfsf.h"
+#include "pfsf_internal.h"
+
+using namespace pfsf;
+
+namespace {
+
+constexpr int kMaxSparse = 512;     // MAX_SPARSE_UPDATES_PER_TICK
+constexpr int kSparseRecord = 48;   // bytes per record
+constexpr float kPhiOrphan = 1e6f;  // PFSFConstants.PHI_ORPHAN_THRESHOLD
+constexpr int kRbgsSweeps = 16;     // STEPS_MAJOR (NP/src/core/constants.h:37)
+
+struct HostBufs {
+  bool registered = false;
+  void* phi = nullptr;
+  void* source = nullptr;
+  void* cond = nullptr;
+  void* type = nullptr;
+  void* rcomp = nullptr;
+  void* rtens = nullptr;
+  void* max_phi = nullptr;
+  void* kappa = nullptr;  // optional exact per-voxel kappa
+  void* stress = nullptr;
+  int64_t stress_bytes = 0;
+  bool lookups = false;
+};
+
+struct Island {
+  pfsf_island_desc desc{};
+  pfsf_dims dims{0, 0, 0};
+  int64_t n = 0;
+  bool dirty = true;
+  bool topo_dirty = true;  // kappa / type changed since the hierarchy was built
+  bool have_phi = false;
+  HostBufs host;
+  // device state
+  uint8_t* type = nullptr;
+  float* kappa = nullptr;
+  float* cond = nullptr;
+  float* source = nullptr;
+  float* rcomp = nullptr;
+  float* rtens = nullptr;
+  float* max_phi = nullptr;
+  float* phi = nullptr;
+  double* x64 = nullptr;
+  float* H = nullptr;
+  uint8_t* code = nullptr;
+  int32_t* events = nullptr;  // 1 + 4*capacity int32
+  int32_t event_cap = 0;
+  int32_t* scratch = nullptr;
+  int64_t scratch_elems = 0;
+  uint32_t* sparse_dev = nullptr;
+  void* sparse_host = nullptr;
+  std::unique_ptr<MgSolver> solver;
+  std::vector<void*> owned;
+};
+
+}  // namespace
+
+struct pfsf_engine_t {
+  pfsf_config cfg{};
+  bool available = false;
+  hipStream_t stream = nullptr;
+  std::map<int32_t, std::unique_ptr<Island>> islands;
+  bool pcg = true;
+  double tol = 1e-6;
+  int32_t max_iters = 500;
+  pfsf_material_fn material_fn = nullptr;
+  void* material_ud = nullptr;
+  pfsf_anchor_fn anchor_fn = nullptr;
+  void* anchor_ud = nullptr;
+  pfsf_fill_ratio_fn fill_fn = nullptr;
+  void* fill_ud = nullptr;
+  pfsf_curing_fn curing_fn = nullptr;
+  void* curing_ud = nullptr;
+  pfsf_vec3 wind{0, 0, 0};
+  mutable std::mutex stats_mtx;
+  float last_tick_ms = 0.f;
+  int64_t vram_used = 0;
+};
+
+namespace {
+
+pfsf_config default_config() {
+  pfsf_config c{};
+  c.max_island_size = 50000;
+  c.tick_budget_ms = 8;
+  c.vram_budget_bytes = 512LL * 1024 * 1024;
+  c.enable_phase_field = true;
+  c.enable_multigrid = true;
+  return c;
+}
+
+void free_island(Island& is) {
+  is.solver.reset();
+  for (void* p : is.owned) (void)hipFree(p);
+  is.owned.clear();
+  if (is.sparse_host) (void)hipHostFree(is.sparse_host);
+  is.sparse_host = nullptr;
+}
+
+template <typename T>
+bool dev_alloc(Island& is, T** out, int64_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, (size_t)(n > 0 ? n : 1) * sizeof(T)) != hipSuccess) return false;
+  is.owned.push_back(p);
+  *out = (T*)p;
+  return true;
+}
+
+int64_t island_bytes(int64_t n) {
+  // type, kappa, cond(6), source, rcomp, rtens, max_phi, phi, x64, H, code + solver (~2.3x fine)
+  return n * (1 + 4 + 24 + 4 + 4 + 4 + 4 + 4 + 8 + 4 + 1) + n * 60;
+}
+
+bool allocate(pfsf_engine_t* E, Island& is) {
+  int64_t n = is.n;
+  bool ok = dev_alloc(is, &is.type, n) && dev_alloc(is, &is.kappa, n) &&
+            dev_alloc(is, &is.cond, 6 * n) && dev_alloc(is, &is.source, n) &&
+            dev_alloc(is, &is.rcomp, n) && dev_alloc(is, &is.rtens, n) &&
+            dev_alloc(is, &is.max_phi, n) && dev_alloc(is, &is.phi, n) &&
+            dev_alloc(is, &is.x64, n) && dev_alloc(is, &is.H, n) && dev_alloc(is, &is.code, n);
+  if (!ok) return false;
+  is.scratch_elems = compact_scratch_elems(is.dims);
+  is.event_cap = 4096;
+  ok = dev_alloc(is, &is.scratch, is.scratch_elems) &&
+       dev_alloc(is, &is.events, 1 + 4 * (int64_t)is.event_cap);
+  if (!ok) return false;
+  hipStream_t s = E->stream;
+  ok = hipMemsetAsync(is.type, 0, n, s) == hipSuccess &&
+       hipMemsetAsync(is.kappa, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.cond, 0, 24 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.source, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.rcomp, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.rtens, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.max_phi, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.phi, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.x64, 0, 8 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.H, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.code, 0, n, s) == hipSuccess;
+  return ok && hipStreamSynchronize(s) == hipSuccess;
+}
+
+Island* find(pfsf_engine_t* E, int32_t id) {
+  auto it = E->islands.find(id);
+  return it == E->islands.end() ? nullptr : it->second.get();
+}
+
+__global__ void k_widen(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+// Host DBB -> device (NP/src/core/island_buffer.cpp:427-598).
+bool upload(pfsf_engine_t* E, Island& is) {
+  hipStream_t s = E->stream;
+  int64_t n = is.n;
+  HostBufs& h = is.host;
+  bool ok = hipMemcpyAsync(is.type, h.type, n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.source, h.source, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.cond, h.cond, 24 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.rcomp, h.rcomp, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.rtens, h.rtens, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.max_phi, h.max_phi, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(is.phi, h.phi, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess;
+  if (!ok) return false;
+  if (h.kappa)
+    ok = hipMemcpyAsync(is.kappa, h.kappa, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess;
+  else
+    ok = launch_kappa_from_soa6(is.cond, n, is.kappa, s) == hipSuccess;
+  if (!ok) return false;
+  // the registered phi is the warm start of the solve
+  k_widen<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, is.phi, is.x64);
+  is.topo_dirty = true;
+  return hipGetLastError() == hipSuccess;
+}
+
+pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
+  hipStream_t s = E->stream;
+  if (E->pcg) {
+    if (!is.solver || is.topo_dirty) {
+      if (!is.solver) {
+        pfsf_solver_desc d{};
+        d.dims = is.dims;
+        d.type = is.type;
+        d.kappa = is.kappa;
+        d.mg_levels = E->cfg.enable_multigrid ? 0 : 1;
+        d.precision = PFSF_PREC_FP32;
+        d.tol = E->tol;
+        d.max_iters = E->max_iters;
+        d.stream = s;
+        auto sv = std::make_unique<MgSolver>();
+        if (sv->create(d) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
+        is.solver = std::move(sv);
+      } else if (is.solver->update(is.type, is.kappa) != hipSuccess) {
+        return PFSF_ERROR_VULKAN;
+      }
+      is.topo_dirty = false;
+    }
+    pfsf_solve_info info{};
+    if (is.solver->solve(is.source, is.x64, is.phi, &info) != hipSuccess) return PFSF_ERROR_VULKAN;
+  } else {
+    if (launch_rbgs_exact(is.dims, is.type, is.kappa, is.source, is.phi, kRbgsSweeps, s) !=
+        hipSuccess)
+      return PFSF_ERROR_VULKAN;
+    k_widen<<<(unsigned)((is.n + 255) / 256), 256, 0, s>>>(is.n, is.phi, is.x64);
+  }
+  if (E->cfg.enable_phase_field &&
+      launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens, kPhiOrphan,
+                          is.code, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  is.have_phi = true;
+  return PFSF_OK;
+}
+
+// Appends this island's failures to a {count, tuples...} buffer (host).
+pfsf_result drain_failures(pfsf_engine_t* E, Island& is, int32_t* header, int32_t* tuples,
+                           int32_t cap) {
+  hipStream_t s = E->stream;
+  int32_t count = *header;
+  int32_t room = cap - count;
+  if (room <= 0) return PFSF_OK;
+  int32_t want = room < is.event_cap ? room : is.event_cap;
+  if (launch_compact(is.dims, is.code, is.desc.origin, is.events, want, is.scratch,
+                     is.scratch_elems, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  int32_t got = 0;
+  if (hipMemcpyAsync(&got, is.events, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (got > 0 &&
+      (hipMemcpyAsync(tuples + 4 * (int64_t)count, is.events + 1, 16 * (size_t)got,
+                      hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    return PFSF_ERROR_VULKAN;
+  *header = count + got;
+  return PFSF_OK;
+}
+
+pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count,
+                      pfsf_tick_result* result, void* fail_addr, int64_t fail_bytes) {
+  if (!E->available) return PFSF_ERROR_NOT_INIT;
+  if (count > 0 && !ids) return PFSF_ERROR_INVALID_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  if (result) result->count = 0;
+  const bool drain = fail_addr && fail_bytes >= 20;
+  for (int32_t k = 0; k < count; k++) {
+    Island* is = find(E, ids[k]);
+    if (!is || !is->dirty) continue;
+    if (is->host.registered && !upload(E, *is)) return PFSF_ERROR_VULKAN;
+    auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                  std::chrono::steady_clock::now() - t0)
+                  .count();
+    if (ms >= E->cfg.tick_budget_ms) break;  // island stays dirty
+    pfsf_result r = solve_island(E, *is);
+    if (r != PFSF_OK) return r;
+    if (drain) {
+      int32_t cap = (int32_t)((fail_bytes - 4) / 16);
+      int32_t* hdr = (int32_t*)fail_addr;
+      r = drain_failures(E, *is, hdr, hdr + 1, cap);
+      if (r != PFSF_OK) return r;
+    }
+    if (result && result->events && result->count < result->capacity) {
+      int32_t hdr = result->count;
+      r = drain_failures(E, *is, &hdr, (int32_t*)result->events, result->capacity);
+      if (r != PFSF_OK) return r;
+      result->count = hdr;
+    }
+    if (is->host.stress && is->host.stress_bytes > 0) {
+      int64_t cnt = is->host.stress_bytes / 4;
+      if (cnt > is->n) cnt = is->n;
+      if (hipMemcpyAsync(is->host.stress, is->phi, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(E->stream) != hipSuccess)
+        return PFSF_ERROR_VULKAN;
+    }
+    is->dirty = false;
+  }
+  if (hipStreamSynchronize(E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
+  std::lock_guard<std::mutex> lk(E->stats_mtx);
+  E->last_tick_ms =
+      std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return PFSF_OK;
+}
+
+}  // namespace
+
+#define CHECK_ENGINE(h)                        \
+  if (!(h)) return PFSF_ERROR_INVALID_ARG;     \
+  if (!(h)->available) return PFSF_ERROR_NOT_INIT;
+
+extern "C" {
+
+// ═════════════════════════════ lifecycle ═════════════════════════════════
+
+pfsf_engine pfsf_create(const pfsf_config* config) {
+  pfsf_engine_t* E = new (std::nothrow) pfsf_engine_t();
+  if (!E) return nullptr;
+  E->cfg = config ? *config : default_config();
+  return E;
+}
+
+pfsf_engine pfsf_create_with_vulkan(const pfsf_config*, const pfsf_vulkan_handles*) {
+  return nullptr;
+}
+
+pfsf_result pfsf_init(pfsf_engine E) {
+  if (!E) return PFSF_ERROR_INVALID_ARG;
+  if (E->available) return PFSF_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PFSF_ERROR_NO_DEVICE;
+  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  E->available = true;
+  return PFSF_OK;
+}
+
+void pfsf_shutdown(pfsf_engine E) {
+  if (!E || !E->available) return;
+  (void)hipStreamSynchronize(E->stream);
+  for (auto& kv : E->islands) free_island(*kv.second);
+  E->islands.clear();
+  (void)hipStreamDestroy(E->stream);
+  E->stream = nullptr;
+  E->available = false;
+  E->vram_used = 0;
+}
+
+void pfsf_destroy(pfsf_engine E) {
+  if (!E) return;
+  pfsf_shutdown(E);
+  delete E;
+}
+
+bool pfsf_is_available(pfsf_engine E) { return E && E->available; }
+
+pfsf_result pfsf_get_stats(pfsf_engine E, pfsf_stats* out) {
+  CHECK_ENGINE(E);
+  if (!out) return PFSF_ERROR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(E->stats_mtx);
+  out->island_count = (int32_t)E->islands.size();
+  int64_t v = 0;
+  for (auto& kv : E->islands) v += kv.second->n;
+  out->total_voxels = v;
+  out->vram_used_bytes = E->vram_used;
+  out->vram_budget_bytes = E->cfg.vram_budget_bytes;
+  out->last_tick_ms = E->last_tick_ms;
+  return PFSF_OK;
+}
+
+// ═════════════════════════════ configuration ═════════════════════════════
+
+void pfsf_set_material_lookup(pfsf_engine E, pfsf_material_fn fn, void* ud) {
+  if (E) E->material_fn = fn, E->material_ud = ud;
+}
+void pfsf_set_anchor_lookup(pfsf_engine E, pfsf_anchor_fn fn, void* ud) {
+  if (E) E->anchor_fn = fn, E->anchor_ud = ud;
+}
+void pfsf_set_fill_ratio_lookup(pfsf_engine E, pfsf_fill_ratio_fn fn, void* ud) {
+  if (E) E->fill_fn = fn, E->fill_ud = ud;
+}
+void pfsf_set_curing_lookup(pfsf_engine E, pfsf_curing_fn fn, void* ud) {
+  if (E) E->curing_fn = fn, E->curing_ud = ud;
+}
+void pfsf_set_wind(pfsf_engine E, const pfsf_vec3* wind) {
+  if (E) E->wind = wind ? *wind : pfsf_vec3{0, 0, 0};
+}
+void pfsf_set_pcg_enabled(pfsf_engine E, int enabled) {
+  if (E) E->pcg = enabled != 0;
+}
+void pfsf_set_solve_tolerance(pfsf_engine E, double tol, int32_t max_iters) {
+  if (!E) return;
+  if (tol > 0) E->tol = tol;
+  if (max_iters > 0) E->max_iters = max_iters;
+  for (auto& kv : E->islands) kv.second->solver.reset();  // rebuilt with the new settings
+}
+
+// ═════════════════════════════ islands ═══════════════════════════════════
+
+pfsf_result pfsf_add_island(pfsf_engine E, const pfsf_island_desc* desc) {
+  CHECK_ENGINE(E);
+  if (!desc) return PFSF_ERROR_INVALID_ARG;
+  if (desc->lx <= 0 || desc->ly <= 0 || desc->lz <= 0) return PFSF_ERROR_ISLAND_FULL;
+  int64_t mx = E->cfg.max_island_size;
+  if (desc->lx > mx || desc->ly > mx || desc->lz > mx) return PFSF_ERROR_ISLAND_FULL;
+  int64_t n = (int64_t)desc->lx * desc->ly * desc->lz;
+  if (n > mx) return PFSF_ERROR_ISLAND_FULL;
+  if (Island* old = find(E, desc->island_id)) {
+    // same id: keep the existing buffers when the grid is unchanged
+    if (old->dims.lx == desc->lx && old->dims.ly == desc->ly && old->dims.lz == desc->lz) {
+      old->desc = *desc;
+      return PFSF_OK;
+    }
+    E->vram_used -= island_bytes(old->n);
+    free_island(*old);
+    E->islands.erase(desc->island_id);
+  }
+  if (E->vram_used + island_bytes(n) > E->cfg.vram_budget_bytes) return PFSF_ERROR_OUT_OF_VRAM;
+  auto is = std::make_unique<Island>();
+  is->desc = *desc;
+  is->dims = {desc->lx, desc->ly, desc->lz};
+  is->n = n;
+  if (!allocate(E, *is)) {
+    free_island(*is);
+    return PFSF_ERROR_OUT_OF_VRAM;
+  }
+  E->vram_used += island_bytes(n);
+  E->islands[desc->island_id] = std::move(is);
+  return PFSF_OK;
+}
+
+void pfsf_remove_island(pfsf_engine E, int32_t id) {
+  if (!E) return;
+  Island* is = find(E, id);
+  if (!is) return;
+  (void)hipStreamSynchronize(E->stream);
+  E->vram_used -= island_bytes(is->n);
+  free_island(*is);
+  E->islands.erase(id);
+}
+
+pfsf_result pfsf_notify_block_change(pfsf_engine E, int32_t id, const pfsf_voxel_update* u) {
+  CHECK_ENGINE(E);
+  if (!u) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  if (u->flat_index < 0 || (int64_t)u->flat_index >= is->n) return PFSF_ERROR_INVALID_ARG;
+  is->dirty = true;
+  return PFSF_OK;
+}
+
+void pfsf_mark_full_rebuild(pfsf_engine E, int32_t id) {
+  if (!E) return;
+  if (Island* is = find(E, id)) is->dirty = true;
+}
+
+// ═════════════════════════════ buffers ═══════════════════════════════════
+
+pfsf_result pfsf_register_island_buffers(pfsf_engine E, int32_t id,
+                                         const pfsf_island_buffers* b) {
+  CHECK_ENGINE(E);
+  if (!b) return PFSF_ERROR_INVALID_ARG;
+  if (!b->phi_addr || !b->source_addr || !b->conductivity_addr || !b->voxel_type_addr ||
+      !b->rcomp_addr || !b->rtens_addr || !b->max_phi_addr)
+    return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  int64_t n = is->n;
+  if (b->phi_bytes < 4 * n || b->source_bytes < 4 * n || b->conductivity_bytes < 24 * n ||
+      b->voxel_type_bytes < n || b->rcomp_bytes < 4 * n || b->rtens_bytes < 4 * n ||
+      b->max_phi_bytes < 4 * n)
+    return PFSF_ERROR_INVALID_ARG;
+  HostBufs& h = is->host;
+  h.phi = b->phi_addr;
+  h.source = b->source_addr;
+  h.cond = b->conductivity_addr;
+  h.type = b->voxel_type_addr;
+  h.rcomp = b->rcomp_addr;
+  h.rtens = b->rtens_addr;
+  h.max_phi = b->max_phi_addr;
+  h.registered = true;
+  if (!upload(E, *is) || hipStreamSynchronize(E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
+  is->dirty = true;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_register_island_kappa(pfsf_engine E, int32_t id, void* kappa, int64_t bytes) {
+  CHECK_ENGINE(E);
+  Island* is = find(E, id);
+  if (!is || !kappa || bytes < 4 * is->n) return PFSF_ERROR_INVALID_ARG;
+  is->host.kappa = kappa;
+  is->dirty = true;
+  is->topo_dirty = true;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_register_island_lookups(pfsf_engine E, int32_t id,
+                                         const pfsf_island_lookups* l) {
+  CHECK_ENGINE(E);
+  if (!l) return PFSF_ERROR_INVALID_ARG;
+  if (!l->material_id_addr || !l->anchor_bitmap_addr || !l->fluid_pressure_addr ||
+      !l->curing_addr)
+    return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  int64_t n = is->n;
+  if (l->material_id_bytes < 4 * n || l->anchor_bitmap_bytes < 8 * n ||
+      l->fluid_pressure_bytes < 4 * n || l->curing_bytes < 4 * n)
+    return PFSF_ERROR_INVALID_ARG;
+  is->host.lookups = true;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_register_stress_readback(pfsf_engine E, int32_t id, void* addr, int64_t bytes) {
+  CHECK_ENGINE(E);
+  if (!addr || bytes <= 0) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  is->host.stress = addr;
+  is->host.stress_bytes = bytes;
+  return PFSF_OK;
+}
+
+// ═════════════════════════════ tick ══════════════════════════════════════
+
+pfsf_result pfsf_tick(pfsf_engine E, const int32_t* ids, int32_t count, int64_t,
+                      pfsf_tick_result* result) {
+  CHECK_ENGINE(E);
+  return tick_impl(E, ids, count, result, nullptr, 0);
+}
+
+pfsf_result pfsf_tick_dbb(pfsf_engine E, const int32_t* ids, int32_t count, int64_t,
+                          void* fail_addr, int64_t fail_bytes) {
+  CHECK_ENGINE(E);
+  if (fail_addr && fail_bytes >= 4) *(int32_t*)fail_addr = 0;
+  return tick_impl(E, ids, count, nullptr, fail_addr, fail_bytes);
+}
+
+pfsf_result pfsf_read_stress(pfsf_engine E, int32_t id, float* out, int32_t cap,
+                             int32_t* out_count) {
+  CHECK_ENGINE(E);
+  if (!out || !out_count) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  int64_t cnt = cap < is->n ? cap : is->n;
+  if (cnt < 0) cnt = 0;
+  if (cnt > 0 && (hipMemcpyAsync(out, is->phi, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(E->stream) != hipSuccess))
+    return PFSF_ERROR_VULKAN;
+  *out_count = (int32_t)cnt;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_read_history(pfsf_engine E, int32_t id, float* out, int32_t cap,
+                              int32_t* out_count) {
+  CHECK_ENGINE(E);
+  if (!out || !out_count) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  int64_t cnt = cap < is->n ? cap : is->n;
+  if (cnt < 0) cnt = 0;
+  if (cnt > 0 && (hipMemcpyAsync(out, is->H, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(E->stream) != hipSuccess))
+    return PFSF_ERROR_VULKAN;
+  *out_count = (int32_t)cnt;
+  return PFSF_OK;
+}
+
+int32_t pfsf_drain_callbacks(pfsf_engine E, int32_t* out, int32_t cap) {
+  (void)E;
+  (void)out;
+  (void)cap;
+  return 0;
+}
+
+pfsf_result pfsf_get_sparse_upload_buffer(pfsf_engine E, int32_t id, void** out_addr,
+                                          int64_t* out_bytes) {
+  CHECK_ENGINE(E);
+  if (!out_addr || !out_bytes) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  if (!is->sparse_host) {
+    if (hipHostMalloc(&is->sparse_host, kMaxSparse * kSparseRecord) != hipSuccess)
+      return PFSF_ERROR_VULKAN;
+    if (!dev_alloc(*is, &is->sparse_dev, kMaxSparse * 12)) return PFSF_ERROR_OUT_OF_VRAM;
+  }
+  *out_addr = is->sparse_host;
+  *out_bytes = (int64_t)kMaxSparse * kSparseRecord;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_notify_sparse_updates(pfsf_engine E, int32_t id, int32_t count) {
+  CHECK_ENGINE(E);
+  if (count <= 0) return PFSF_OK;
+  Island* is = find(E, id);
+  if (!is || !is->sparse_host) return PFSF_ERROR_INVALID_ARG;
+  if (count > kMaxSparse) count = kMaxSparse;
+  hipStream_t s = E->stream;
+  if (hipMemcpyAsync(is->sparse_dev, is->sparse_host, (size_t)count * kSparseRecord,
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      launch_sparse_scatter(is->sparse_dev, count, is->n, is->source, is->cond, is->type,
+                            is->max_phi, is->rcomp, is->rtens, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (!is->host.kappa && launch_kappa_from_soa6(is->cond, is->n, is->kappa, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (hipStreamSynchronize(s) != hipSuccess) return PFSF_ERROR_VULKAN;
+  is->dirty = true;
+  is->topo_dirty = true;
+  return PFSF_OK;
+}
+
+// ═════════════════════════════ version ═══════════════════════════════════
+
+const char* pfsf_version(void) { return "0.1.0-hip"; }
+uint32_t pfsf_abi_version(void) { return (1u << 16) | (6u << 8) | 0u; }
+bool pfsf_has_feature(const char* name) {
+  if (!name) return false;
+  static const char* kFeat[] = {"hip.gfx950",       "solve.mg_pcg",  "solve.batched",
+                                "solve.mixed_bf16", "solve.adjoint", "scan.compact"};
+  for (const char* f : kFeat)
+    if (strcmp(f, name) == 0) return true;
+  return false;
+}
+const char* pfsf_build_info(void) { return "hipcc -O3 --offload-arch=gfx950 (HIP " PFSF_HIP_VER ")"; }
+const char* pfsf_abi_contract_version(void) { return "1.6.0"; }
+
+// ═════════════════════════════ device-pointer surface ════════════════════
+
+#define DIMS_OR_FAIL(d) \
+  if (!dims_ok(d)) return PFSF_ERROR_INVALID_ARG;
+#define RET(e) return (e) == hipSuccess ? PFSF_OK : PFSF_ERROR_VULKAN
+
+pfsf_result pfsf_hip_jacobi_step(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                 const float* source, const float* prev, float* out, float omega,
+                                 void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !source || !prev || !out || prev == out) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_jacobi_exact(d, type, kappa, source, prev, out, omega, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_rbgs(pfsf_dims d, const uint8_t* type, const float* kappa,
+                          const float* source, float* phi, int32_t sweeps, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !source || !phi || sweeps < 0) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_rbgs_exact(d, type, kappa, source, phi, sweeps, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                  const float* phi, const float* rcomp, const float* rtens,
+                                  float phi_orphan, uint8_t* code, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !phi || !rcomp || !rtens || !code) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_failure_scan(d, type, kappa, phi, rcomp, rtens, phi_orphan, code,
+                          (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_history_update(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                    const float* phi, float* H, float* psi, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !phi || (!H && !psi)) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_history(d, type, kappa, phi, H, psi, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
+                               const double* x, double* y, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !x || !y || (const void*)x == (void*)y) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_apply_f64(d, type, kappa, x, y, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_compact_failures(pfsf_dims d, const uint8_t* code, pfsf_pos origin,
+                                      int32_t* out, int32_t capacity, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!code || !out || capacity < 0) return PFSF_ERROR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  int64_t ne = compact_scratch_elems(d);
+  int32_t* scratch = nullptr;
+  if (hipMallocAsync((void**)&scratch, sizeof(int32_t) * ne, s) != hipSuccess)
+    return PFSF_ERROR_OUT_OF_VRAM;
+  hipError_t e = launch_compact(d, code, origin, out, capacity, scratch, ne, s);
+  (void)hipFreeAsync(scratch, s);
+  RET(e);
+}
+
+pfsf_result pfsf_hip_downsample_2to1(pfsf_dims f, const float* field, const uint8_t* type,
+                                     float* coarse, uint8_t* ctype, void* stream) {
+  DIMS_OR_FAIL(f);
+  if (!field || !type || !coarse) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_downsample(f, field, type, coarse, ctype, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_normalize_soa6(float* source, float* rcomp, float* rtens, float* cond,
+                                    int64_t n, float* sigma_max_host, void* stream) {
+  if (!source || !rcomp || !rtens || !cond || n <= 0) {
+    if (sigma_max_host) *sigma_max_host = 1.0f;
+    return (n <= 0) ? PFSF_OK : PFSF_ERROR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  unsigned* mx = nullptr;
+  if (hipMallocAsync((void**)&mx, sizeof(unsigned), s) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
+  unsigned one = 0x3f800000u;  // 1.0f: the maximum starts at 1
+  hipError_t e = hipMemcpyAsync(mx, &one, 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_max_soa6(cond, 6 * n, mx, s);
+  if (e == hipSuccess) e = launch_scale3(source, rcomp, rtens, cond, n, mx, s);
+  unsigned hb = one;
+  if (e == hipSuccess) e = hipMemcpyAsync(&hb, mx, 4, hipMemcpyDeviceToHost, s);
+  (void)hipFreeAsync(mx, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (sigma_max_host) memcpy(sigma_max_host, &hb, 4);
+  RET(e);
+}
+
+pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa, void* stream) {
+  if (!cond || !kappa || n <= 0) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_kappa_from_soa6(cond, n, kappa, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach,
+                                   void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !reach) return PFSF_ERROR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  int* flag = nullptr;
+  if (hipMalloc((void**)&flag, sizeof(int)) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
+  hipError_t e = anchored_mask(d, type, reach, flag, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(flag);
+  RET(e);
+}
+
+// ── solver object ──────────────────────────────────────────────────────
+
+struct pfsf_solver_t {
+  MgSolver mg;
+};
+
+pfsf_result pfsf_solver_create(const pfsf_solver_desc* desc, pfsf_solver* out) {
+  if (!desc || !out || !desc->type || !desc->kappa || !dims_ok(desc->dims))
+    return PFSF_ERROR_INVALID_ARG;
+  if (desc->precision != PFSF_PREC_FP32 && desc->precision != PFSF_PREC_BF16_COARSE)
+    return PFSF_ERROR_INVALID_ARG;
+  *out = nullptr;
+  pfsf_solver_t* s = new (std::nothrow) pfsf_solver_t();
+  if (!s) return PFSF_ERROR_OUT_OF_VRAM;
+  hipError_t e = s->mg.create(*desc);
+  if (e != hipSuccess) {
+    delete s;
+    return e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN;
+  }
+  *out = s;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_solver_solve(pfsf_solver s, const float* b, double* x64, float* phi32,
+                              pfsf_solve_info* info) {
+  if (!s || !b) return PFSF_ERROR_INVALID_ARG;
+  RET(s->mg.solve(b, x64, phi32, info));
+}
+
+pfsf_result pfsf_solver_update(pfsf_solver s, const uint8_t* type, const float* kappa) {
+  if (!s || !type || !kappa) return PFSF_ERROR_INVALID_ARG;
+  RET(s->mg.update(type, kappa));
+}
+
+void pfsf_solver_destroy(pfsf_solver s) { delete s; }
+
+pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps, double* avg_ms) {
+  if (!s || !avg_ms || reps <= 0 || which < 0 || which > 4) return PFSF_ERROR_INVALID_ARG;
+  float ms = 0.f;
+  hipError_t e = s->mg.profile(which, reps, &ms);
+  if (e == hipErrorInvalidValue) return PFSF_ERROR_INVALID_ARG;
+  *avg_ms = ms;
+  RET(e);
+}
+
+}  // extern "C"

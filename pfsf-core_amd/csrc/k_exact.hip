@@ -1,0 +1,554 @@
+// Bit-exact kernels of the structural solve: every product and sum rounds
+// separately (this file is built with -ffp-contract=off), coefficients are
+// formed exactly as the Java CPU reference forms them, and neighbour sums run
+// in PFSFStencil offset order, so results match the CPU restatement bit for
+// bit on identical inputs.
+//
+//   Jacobi step / RBGS    VoxelPhysicsCpuReference.java:66-118,
+//                         SH/rbgs_smooth.comp.glsl:86-87 (colour order)
+//   failure scan          SH/failure_scan.comp.glsl:36-176 (code precedence)
+//   history update        phase-field H = max(H, psi_e), psi_e = 0.5 k |grad phi|^2
+//   compaction            SH/failure_compact.comp.glsl:27-45,
+//                         NP/src/core/island_buffer.cpp:930-962 (tuple format)
+//   downsample            NP/src/compute/downsample.cpp:39-98
+//   normalisation         NP/src/compute/normalize.cpp:29-63
+#pragma clang fp contract(off)
+
+#include "pfsf_internal.h"
+
+namespace pfsf {
+namespace {
+
+constexpr int BS = 256;
+
+__device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
+
+// Coupling of offset class k (0..5 face, 6..17 edge, 18..25 corner),
+// Java expression order: float sum / product, double scale, IEEE sqrt.
+__device__ __forceinline__ double couple_exact(int k, float ki, float kj) {
+  if (k < 6) {
+    float s = ki + kj;
+    return 0.5 * (double)s;
+  }
+  float p = ki * kj;
+  double r = sqrt((double)p);
+  return r * (k < 18 ? (double)kEdgeP : (double)kCornerP);
+}
+
+__device__ __forceinline__ bool is_solid(uint8_t t) { return t != 0 && t != 2; }
+
+struct XYZ {
+  int x, y, z;
+};
+
+__device__ __forceinline__ XYZ unflat(int64_t i, pfsf_dims d) {
+  XYZ p;
+  p.x = (int)(i % d.lx);
+  int64_t r = i / d.lx;
+  p.y = (int)(r % d.ly);
+  p.z = (int)(r / d.ly);
+  return p;
+}
+
+__device__ __forceinline__ int64_t flat(int x, int y, int z, pfsf_dims d) {
+  return (int64_t)x + (int64_t)d.lx * ((int64_t)y + (int64_t)d.ly * z);
+}
+
+// The jacobiStep per-voxel update on a neighbourhood read from `phi`.
+__device__ __forceinline__ float jacobi_voxel(int64_t i, XYZ p, pfsf_dims d,
+                                              const uint8_t* __restrict__ type,
+                                              const float* __restrict__ kappa,
+                                              const float* __restrict__ src, const float* phi,
+                                              float omega) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  double diag = 0.0, off = 0.0;
+  float ki = kapf(kappa[i]);
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+    int64_t j = flat(nx, ny, nz, d);
+    if (type[j] == 0) continue;
+    double a = couple_exact(k, ki, kapf(kappa[j]));
+    diag += a;
+    off += a * (double)phi[j];
+  }
+  double prev = (double)phi[i];
+  double b = (double)src[i];
+  double pj = (diag > 1e-15) ? (b + off) / diag : prev;
+  return (float)((double)omega * (pj - prev) + prev);
+}
+
+__global__ void __launch_bounds__(BS)
+    k_jacobi_exact(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                   const float* __restrict__ src, const float* __restrict__ prev,
+                   float* __restrict__ out, float omega) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d)) return;
+  uint8_t t = type[i];
+  if (!is_solid(t)) {
+    out[i] = 0.0f;
+    return;
+  }
+  out[i] = jacobi_voxel(i, unflat(i, d), d, type, kappa, src, prev, omega);
+}
+
+// One colour pass of the in-place 8-colour Gauss-Seidel.  Threads cover only
+// the voxels of colour c: (x&1, y&1, z&1) == (c&1, c>>1&1, c>>2&1).  No voxel
+// of colour c neighbours another of colour c, so the pass is race-free and its
+// result does not depend on the order within the colour.
+__global__ void __launch_bounds__(BS)
+    k_rbgs_color(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                 const float* __restrict__ src, float* phi, int c) {
+  int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+  int hx = (d.lx - cx + 1) >> 1, hy = (d.ly - cy + 1) >> 1, hz = (d.lz - cz + 1) >> 1;
+  int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (hx <= 0 || hy <= 0 || hz <= 0 || t >= (int64_t)hx * hy * hz) return;
+  XYZ p;
+  p.x = 2 * (int)(t % hx) + cx;
+  int64_t r = t / hx;
+  p.y = 2 * (int)(r % hy) + cy;
+  p.z = 2 * (int)(r / hy) + cz;
+  int64_t i = flat(p.x, p.y, p.z, d);
+  if (!is_solid(type[i])) {
+    phi[i] = 0.0f;
+    return;
+  }
+  phi[i] = jacobi_voxel(i, p, d, type, kappa, src, phi, 1.0f);
+}
+
+__global__ void __launch_bounds__(BS)
+    k_failure_scan(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                   const float* __restrict__ phi, const float* __restrict__ rcomp,
+                   const float* __restrict__ rtens, float phi_orphan, uint8_t* __restrict__ code) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d)) return;
+  if (!is_solid(type[i])) {
+    code[i] = 0;
+    return;
+  }
+  XYZ p = unflat(i, d);
+  float ki = kapf(kappa[i]);
+  double pi = (double)phi[i];
+  double diag = 0.0, fin = 0.0, fout = 0.0;
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+    int64_t j = flat(nx, ny, nz, d);
+    if (type[j] == 0) continue;
+    double a = couple_exact(k, ki, kapf(kappa[j]));
+    diag += a;
+    double dp = (double)phi[j] - pi;
+    if (dp > 0.0)
+      fin += a * dp;
+    else if (dp < 0.0)
+      fout += a * (-dp);
+  }
+  uint8_t c = 0;
+  if (diag == 0.0 || phi[i] > phi_orphan)
+    c = 3;
+  else if (rcomp[i] > 0.0f && fin > (double)rcomp[i])
+    c = 2;
+  else if (rtens[i] > 0.0f && fout > (double)rtens[i])
+    c = 4;
+  code[i] = c;
+}
+
+__device__ __forceinline__ float axis_grad(const uint8_t* __restrict__ type,
+                                           const float* __restrict__ phi, int64_t i, bool hm,
+                                           bool hp, int64_t jm, int64_t jp) {
+  bool om = hm && type[jm] != 0;
+  bool op = hp && type[jp] != 0;
+  if (om && op) return (phi[jp] - phi[jm]) * 0.5f;
+  if (op) return phi[jp] - phi[i];
+  if (om) return phi[i] - phi[jm];
+  return 0.0f;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_history(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+              const float* __restrict__ phi, float* __restrict__ H, float* __restrict__ psi_out) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d)) return;
+  if (!is_solid(type[i])) {
+    if (psi_out) psi_out[i] = 0.0f;
+    return;
+  }
+  XYZ p = unflat(i, d);
+  int64_t sy = d.lx, sz = (int64_t)d.lx * d.ly;
+  float gx = axis_grad(type, phi, i, p.x > 0, p.x + 1 < d.lx, i - 1, i + 1);
+  float gy = axis_grad(type, phi, i, p.y > 0, p.y + 1 < d.ly, i - sy, i + sy);
+  float gz = axis_grad(type, phi, i, p.z > 0, p.z + 1 < d.lz, i - sz, i + sz);
+  float g2 = gx * gx;
+  g2 = g2 + gy * gy;
+  g2 = g2 + gz * gz;
+  float h = 0.5f * kapf(kappa[i]);
+  float psi = h * g2;
+  if (psi_out) psi_out[i] = psi;
+  if (H && psi > H[i]) H[i] = psi;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_apply_f64(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                const double* __restrict__ x, double* __restrict__ y) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d)) return;
+  if (!is_solid(type[i])) {
+    y[i] = 0.0;
+    return;
+  }
+  XYZ p = unflat(i, d);
+  float ki = kapf(kappa[i]);
+  double xi = x[i], acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+    int64_t j = flat(nx, ny, nz, d);
+    uint8_t tj = type[j];
+    if (tj == 0) continue;
+    double xj = is_solid(tj) ? x[j] : 0.0;
+    acc += couple_exact(k, ki, kapf(kappa[j])) * (xi - xj);
+  }
+  y[i] = acc;
+}
+
+// ── Ordered compaction of failure codes into the tuple wire format ───────
+constexpr int CB = 256, CPT = 4, CSEG = CB * CPT;  // voxels per segment
+
+__device__ __forceinline__ int block_excl_scan(int v, int* sh, int* total) {
+  // Inclusive Hillis-Steele scan over CB threads in LDS.
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < CB; o <<= 1) {
+    int a = (t >= o) ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  int incl = sh[t];
+  *total = sh[CB - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(CB)
+    k_compact_count(int64_t n, const uint8_t* __restrict__ code, int* __restrict__ counts) {
+  __shared__ int sh[CB];
+  int64_t base = (int64_t)blockIdx.x * CSEG + (int64_t)threadIdx.x * CPT;
+  int c = 0;
+  for (int e = 0; e < CPT; e++)
+    if (base + e < n && code[base + e] != 0) c++;
+  int tot;
+  block_excl_scan(c, sh, &tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(CB) k_compact_scan(int nb, int* __restrict__ counts) {
+  // Exclusive scan of nb block counts in place; counts[nb] = total.
+  __shared__ int sh[CB];
+  int per = (nb + CB - 1) / CB;
+  int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int e = 0; e < per; e++)
+    if (b0 + e < nb) s += counts[b0 + e];
+  int tot;
+  int ex = block_excl_scan(s, sh, &tot);
+  for (int e = 0; e < per; e++)
+    if (b0 + e < nb) {
+      int v = counts[b0 + e];
+      counts[b0 + e] = ex;
+      ex += v;
+    }
+  if (threadIdx.x == 0) counts[nb] = tot;
+}
+
+__global__ void __launch_bounds__(CB)
+    k_compact_write(pfsf_dims d, const uint8_t* __restrict__ code, const int* __restrict__ offs,
+                    int nb, pfsf_pos origin, int32_t* __restrict__ out, int32_t capacity) {
+  __shared__ int sh[CB];
+  int64_t n = nvox(d);
+  int64_t base = (int64_t)blockIdx.x * CSEG + (int64_t)threadIdx.x * CPT;
+  int c = 0;
+  for (int e = 0; e < CPT; e++)
+    if (base + e < n && code[base + e] != 0) c++;
+  int tot;
+  int rank = offs[blockIdx.x] + block_excl_scan(c, sh, &tot);
+  for (int e = 0; e < CPT; e++) {
+    int64_t i = base + e;
+    if (i < n && code[i] != 0) {
+      if (rank < capacity) {
+        XYZ p = unflat(i, d);
+        int32_t* ev = out + 1 + 4 * (int64_t)rank;
+        ev[0] = origin.x + p.x;
+        ev[1] = origin.y + p.y;
+        ev[2] = origin.z + p.z;
+        ev[3] = (int32_t)code[i];
+      }
+      rank++;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int total = offs[nb];
+    out[0] = total < capacity ? total : capacity;
+  }
+}
+
+__global__ void __launch_bounds__(BS)
+    k_downsample(pfsf_dims f, const float* __restrict__ fine, const uint8_t* __restrict__ ft,
+                 float* __restrict__ coarse, uint8_t* __restrict__ ct) {
+  pfsf_dims c{(f.lx + 1) / 2, (f.ly + 1) / 2, (f.lz + 1) / 2};
+  int64_t ci = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (ci >= nvox(c)) return;
+  XYZ p = unflat(ci, c);
+  float sum = 0.0f;
+  int cnt = 0, na = 0, ns = 0, nk = 0;
+  for (int dz = 0; dz < 2; dz++) {
+    int zf = 2 * p.z + dz;
+    if (zf >= f.lz) continue;
+    for (int dy = 0; dy < 2; dy++) {
+      int yf = 2 * p.y + dy;
+      if (yf >= f.ly) continue;
+      for (int dx = 0; dx < 2; dx++) {
+        int xf = 2 * p.x + dx;
+        if (xf >= f.lx) continue;
+        int64_t fi = flat(xf, yf, zf, f);
+        sum += fine[fi];
+        cnt++;
+        uint8_t t = ft[fi];
+        if (t == 2) nk++;
+        else if (t == 1) ns++;
+        else na++;
+      }
+    }
+  }
+  coarse[ci] = cnt > 0 ? sum / (float)cnt : 0.0f;
+  if (ct) ct[ci] = (nk >= ns && nk >= na) ? 2 : (ns >= na ? 1 : 0);
+}
+
+__global__ void __launch_bounds__(BS)
+    k_kappa_soa6(const float* __restrict__ cond, int64_t n, float* __restrict__ kappa) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  float m = cond[i];
+  for (int dd = 1; dd < 6; dd++) {
+    float v = cond[dd * n + i];
+    m = v > m ? v : m;
+  }
+  kappa[i] = m;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_max_bits(const float* __restrict__ a, int64_t n, unsigned* __restrict__ mx) {
+  // max over positive finite values via their (monotone) IEEE bit patterns
+  __shared__ unsigned sh[BS];
+  unsigned m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+    float v = a[i];
+    if (v > 0.0f) {
+      unsigned b = __float_as_uint(v);
+      m = b > m ? b : m;
+    }
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = BS / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] = max(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(mx, sh[0]);
+}
+
+__global__ void __launch_bounds__(BS)
+    k_scale3(float* a, float* b, float* c, float* cond, int64_t n, const unsigned* mxb) {
+  float smax = __uint_as_float(*mxb);
+  if (!(smax > 1.0f)) return;
+  float inv = 1.0f / smax;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < 6 * n;
+       i += (int64_t)gridDim.x * BS) {
+    if (i < n) {
+      a[i] *= inv;
+      b[i] *= inv;
+      c[i] *= inv;
+    }
+    cond[i] *= inv;
+  }
+}
+
+// Reachability from anchors: monotone relaxation reach_i |= OR_j reach_j over
+// 26 non-air neighbours.  Writes only 0 -> 1, so concurrent updates are benign.
+__global__ void __launch_bounds__(BS)
+    k_reach_init(int64_t n, const uint8_t* __restrict__ type, uint8_t* __restrict__ reach) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) reach[i] = type[i] == 2 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_reach_step(pfsf_dims d, const uint8_t* __restrict__ type, uint8_t* reach, int* changed) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d)) return;
+  if (type[i] == 0 || reach[i]) return;
+  XYZ p = unflat(i, d);
+  for (int dz = -1; dz <= 1; dz++) {
+    int nz = p.z + dz;
+    if (nz < 0 || nz >= d.lz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      int ny = p.y + dy;
+      if (ny < 0 || ny >= d.ly) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        int nx = p.x + dx;
+        if (nx < 0 || nx >= d.lx) continue;
+        int64_t j = flat(nx, ny, nz, d);
+        if (reach[j] && type[j] != 0) {
+          reach[i] = 1;
+          *changed = 1;
+          return;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BS)
+    k_reach_final(int64_t n, const uint8_t* __restrict__ type, uint8_t* __restrict__ reach) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) reach[i] = (reach[i] && is_solid(type[i])) ? 1 : 0;
+}
+
+// Sparse voxel re-upload: 48-byte records {flat, source, type, max_phi, rcomp,
+// rtens, cond[6]} scattered into the island's device arrays
+// (SH/sparse_scatter.comp.glsl:21-67).  Out-of-range indices are skipped.
+__global__ void __launch_bounds__(64)
+    k_sparse_scatter(const uint32_t* __restrict__ rec, int count, int64_t n,
+                     float* __restrict__ source, float* __restrict__ cond,
+                     uint8_t* __restrict__ type, float* __restrict__ max_phi,
+                     float* __restrict__ rcomp, float* __restrict__ rtens) {
+  int u = blockIdx.x * 64 + threadIdx.x;
+  if (u >= count) return;
+  const uint32_t* r = rec + 12 * (int64_t)u;
+  int64_t i = (int64_t)(int32_t)r[0];
+  if (i < 0 || i >= n) return;
+  source[i] = __uint_as_float(r[1]);
+  type[i] = (uint8_t)r[2];
+  if (max_phi) max_phi[i] = __uint_as_float(r[3]);
+  rcomp[i] = __uint_as_float(r[4]);
+  rtens[i] = __uint_as_float(r[5]);
+  for (int dd = 0; dd < 6; dd++) cond[dd * n + i] = __uint_as_float(r[6 + dd]);
+}
+
+}  // namespace
+
+hipError_t launch_sparse_scatter(const uint32_t* rec, int count, int64_t n, float* source,
+                                 float* cond, uint8_t* type, float* max_phi, float* rcomp,
+                                 float* rtens, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  k_sparse_scatter<<<(count + 63) / 64, 64, 0, s>>>(rec, count, n, source, cond, type, max_phi,
+                                                     rcomp, rtens);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
+                               const float* src, const float* prev, float* out, float omega,
+                               hipStream_t s) {
+  k_jacobi_exact<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, src, prev, out, omega);
+  return hipGetLastError();
+}
+
+hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
+                             const float* src, float* phi, int sweeps, hipStream_t s) {
+  for (int sw = 0; sw < sweeps; sw++)
+    for (int c = 0; c < 8; c++) {
+      int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+      int64_t hn = (int64_t)((d.lx - cx + 1) >> 1) * ((d.ly - cy + 1) >> 1) * ((d.lz - cz + 1) >> 1);
+      if (hn <= 0) continue;
+      k_rbgs_color<<<blocks_for(hn, BS), BS, 0, s>>>(d, type, kappa, src, phi, c);
+    }
+  return hipGetLastError();
+}
+
+hipError_t launch_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
+                               const float* phi, const float* rcomp, const float* rtens,
+                               float phi_orphan, uint8_t* code, hipStream_t s) {
+  k_failure_scan<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, phi, rcomp, rtens,
+                                                       phi_orphan, code);
+  return hipGetLastError();
+}
+
+hipError_t launch_history(pfsf_dims d, const uint8_t* type, const float* kappa,
+                          const float* phi, float* H, float* psi, hipStream_t s) {
+  k_history<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, phi, H, psi);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
+                            const double* x, double* y, hipStream_t s) {
+  k_apply_f64<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, x, y);
+  return hipGetLastError();
+}
+
+int64_t compact_scratch_elems(pfsf_dims d) { return (int64_t)blocks_for(nvox(d), CSEG) + 1; }
+
+hipError_t launch_compact(pfsf_dims d, const uint8_t* code, pfsf_pos origin, int32_t* out,
+                          int32_t capacity, int32_t* scratch, int64_t scratch_elems,
+                          hipStream_t s) {
+  int nb = (int)blocks_for(nvox(d), CSEG);
+  if (scratch_elems < nb + 1) return hipErrorInvalidValue;
+  k_compact_count<<<nb, CB, 0, s>>>(nvox(d), code, scratch);
+  k_compact_scan<<<1, CB, 0, s>>>(nb, scratch);
+  k_compact_write<<<nb, CB, 0, s>>>(d, code, scratch, nb, origin, out, capacity);
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* type,
+                             float* coarse, uint8_t* ctype, hipStream_t s) {
+  pfsf_dims c{(fine.lx + 1) / 2, (fine.ly + 1) / 2, (fine.lz + 1) / 2};
+  k_downsample<<<blocks_for(nvox(c), BS), BS, 0, s>>>(fine, field, type, coarse, ctype);
+  return hipGetLastError();
+}
+
+hipError_t launch_kappa_from_soa6(const float* cond, int64_t n, float* kappa, hipStream_t s) {
+  k_kappa_soa6<<<blocks_for(n, BS), BS, 0, s>>>(cond, n, kappa);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_soa6(const float* cond, int64_t n6, unsigned* max_bits, hipStream_t s) {
+  unsigned nb = blocks_for(n6, BS);
+  if (nb > 1024) nb = 1024;
+  k_max_bits<<<nb, BS, 0, s>>>(cond, n6, max_bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
+                         const unsigned* max_bits, hipStream_t s) {
+  unsigned nb = blocks_for(6 * n, BS);
+  if (nb > 4096) nb = 4096;
+  k_scale3<<<nb, BS, 0, s>>>(a, b, c, cond, n, max_bits);
+  return hipGetLastError();
+}
+
+hipError_t anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach, int* dflag,
+                         hipStream_t s) {
+  int64_t n = nvox(d);
+  k_reach_init<<<blocks_for(n, BS), BS, 0, s>>>(n, type, reach);
+  // Every relaxation pass extends the reached set by at least one voxel along
+  // each shortest path, so n passes always suffice; the flag ends it early.
+  int hflag = 1;
+  for (int64_t pass = 0; pass < n && hflag; pass += 8) {
+    hipError_t e = hipMemsetAsync(dflag, 0, sizeof(int), s);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < 8; k++)
+      k_reach_step<<<blocks_for(n, BS), BS, 0, s>>>(d, type, reach, dflag);
+    e = hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  k_reach_final<<<blocks_for(n, BS), BS, 0, s>>>(n, type, reach);
+  return hipGetLastError();
+}
+
+}  // namespace pfsf

@@ -1,0 +1,59 @@
+// Internal declarations shared by the HIP translation units of libpfsf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pfsf/pfsf_hip.h"
+
+namespace pfsf {
+
+constexpr float kEdgeP = 0.5f;          // PFSFStencil.java:42
+constexpr float kCornerP = 0.1666667f;  // PFSFStencil.java:43
+
+// Stencil offsets in PFSFStencil.NEIGHBOR_OFFSETS order (PFSFStencil.java:70-84):
+// 6 faces, 12 edges (XY, XZ, YZ), 8 corners.  Encoded as dx,dy,dz in {-1,0,1}.
+#define PFSF_OFFSETS                                                              \
+  {{1, 0, 0},  {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},   \
+   {1, 1, 0},  {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0}, {1, 0, 1},   {1, 0, -1},   \
+   {-1, 0, 1}, {-1, 0, -1}, {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1},  \
+   {1, 1, 1},  {1, 1, -1},  {1, -1, 1},  {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1},  \
+   {-1, -1, 1}, {-1, -1, -1}}
+
+__host__ __device__ inline int64_t nvox(pfsf_dims d) { return (int64_t)d.lx * d.ly * d.lz; }
+
+inline bool dims_ok(pfsf_dims d) { return d.lx > 0 && d.ly > 0 && d.lz > 0; }
+
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ── k_exact.hip: bit-exact kernels (FP64 accumulation, no contraction) ──
+hipError_t launch_jacobi_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
+                               const float* src, const float* prev, float* out, float omega,
+                               hipStream_t s);
+hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
+                             const float* src, float* phi, int sweeps, hipStream_t s);
+hipError_t launch_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
+                               const float* phi, const float* rcomp, const float* rtens,
+                               float phi_orphan, uint8_t* code, hipStream_t s);
+hipError_t launch_history(pfsf_dims d, const uint8_t* type, const float* kappa,
+                          const float* phi, float* H, float* psi, hipStream_t s);
+hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
+                            const double* x, double* y, hipStream_t s);
+hipError_t launch_compact(pfsf_dims d, const uint8_t* code, pfsf_pos origin, int32_t* out,
+                          int32_t capacity, int32_t* scratch, int64_t scratch_elems,
+                          hipStream_t s);
+int64_t compact_scratch_elems(pfsf_dims d);
+hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* type,
+                             float* coarse, uint8_t* ctype, hipStream_t s);
+hipError_t launch_kappa_from_soa6(const float* cond, int64_t n, float* kappa, hipStream_t s);
+hipError_t launch_max_soa6(const float* cond, int64_t n6, unsigned* max_bits, hipStream_t s);
+hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
+                         const unsigned* max_bits, hipStream_t s);
+hipError_t launch_sparse_scatter(const uint32_t* rec, int count, int64_t n, float* source,
+                                 float* cond, uint8_t* type, float* max_phi, float* rcomp,
+                                 float* rtens, hipStream_t s);
+// Reachability from anchors (26-connected over non-air).  reach: u8 N.
+hipError_t anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach, int* dflag,
+                         hipStream_t s);
+
+}  // namespace pfsf

@@ -1,0 +1,140 @@
+"""Device-array front end of libpfsf's pfsf_hip.h surface.
+
+Arguments are torch CUDA tensors (HBM allocations); each call passes their raw
+pointers and torch's current HIP stream to the C ABI.  Arrays are contiguous
+with shape (lz, ly, lx) or flat N.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+
+
+def _lib():
+    return capi.load()
+
+
+def _stream():
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    assert t.is_cuda and t.is_contiguous(), "expected a contiguous CUDA tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def _dims(t):
+    lz, ly, lx = t.shape
+    return capi.Dims(lx, ly, lz)
+
+
+def upload(isl, device="cuda"):
+    """Island arrays to device tensors (dict)."""
+    import torch
+
+    out = {}
+    for k in ("vtype", "kappa", "source", "rcomp", "rtens"):
+        out[k] = torch.from_numpy(np.ascontiguousarray(getattr(isl, k))).to(device)
+    return out
+
+
+def jacobi_step(vtype, kappa, source, phi_prev, phi_out, omega=1.0):
+    capi.check("pfsf_hip_jacobi_step", _lib().pfsf_hip_jacobi_step(
+        _dims(vtype), _p(vtype), _p(kappa), _p(source), _p(phi_prev), _p(phi_out), omega,
+        _stream()))
+
+
+def rbgs(vtype, kappa, source, phi, sweeps):
+    capi.check("pfsf_hip_rbgs", _lib().pfsf_hip_rbgs(
+        _dims(vtype), _p(vtype), _p(kappa), _p(source), _p(phi), sweeps, _stream()))
+
+
+def failure_scan(vtype, kappa, phi, rcomp, rtens, code, phi_orphan=1e6):
+    capi.check("pfsf_hip_failure_scan", _lib().pfsf_hip_failure_scan(
+        _dims(vtype), _p(vtype), _p(kappa), _p(phi), _p(rcomp), _p(rtens), phi_orphan,
+        _p(code), _stream()))
+
+
+def history_update(vtype, kappa, phi, H=None, psi=None):
+    capi.check("pfsf_hip_history_update", _lib().pfsf_hip_history_update(
+        _dims(vtype), _p(vtype), _p(kappa), _p(phi), _p(H) if H is not None else None,
+        _p(psi) if psi is not None else None, _stream()))
+
+
+def apply_f64(vtype, kappa, x, y):
+    capi.check("pfsf_hip_apply_f64", _lib().pfsf_hip_apply_f64(
+        _dims(vtype), _p(vtype), _p(kappa), _p(x), _p(y), _stream()))
+
+
+def compact_failures(code, out, capacity, origin=(0, 0, 0)):
+    capi.check("pfsf_hip_compact_failures", _lib().pfsf_hip_compact_failures(
+        _dims(code), _p(code), capi.Pos(*origin), _p(out), capacity, _stream()))
+
+
+def downsample_2to1(field, vtype, coarse, ctype):
+    capi.check("pfsf_hip_downsample_2to1", _lib().pfsf_hip_downsample_2to1(
+        _dims(vtype), _p(field), _p(vtype), _p(coarse), _p(ctype), _stream()))
+
+
+def normalize_soa6(source, rcomp, rtens, cond):
+    smax = C.c_float(0)
+    capi.check("pfsf_hip_normalize_soa6", _lib().pfsf_hip_normalize_soa6(
+        _p(source), _p(rcomp), _p(rtens), _p(cond), source.numel(), C.byref(smax), _stream()))
+    return smax.value
+
+
+def kappa_from_soa6(cond, kappa):
+    capi.check("pfsf_hip_kappa_from_soa6", _lib().pfsf_hip_kappa_from_soa6(
+        _p(cond), kappa.numel(), _p(kappa), _stream()))
+
+
+def anchored_mask(vtype, reach):
+    capi.check("pfsf_hip_anchored_mask", _lib().pfsf_hip_anchored_mask(
+        _dims(vtype), _p(vtype), _p(reach), _stream()))
+
+
+class Solver:
+    """MG-preconditioned PCG to an FP64 relative-residual tolerance."""
+
+    def __init__(self, vtype, kappa, mg_levels=0, precision=capi.PREC_FP32, tol=1e-6,
+                 max_iters=1000, coarse_sweeps=0):
+        self._vtype, self._kappa = vtype, kappa  # keep the device arrays alive
+        desc = capi.SolverDesc(_dims(vtype), _p(vtype).value, _p(kappa).value, mg_levels,
+                               precision, coarse_sweeps, tol, max_iters, _stream().value)
+        h = C.c_void_p()
+        capi.check("pfsf_solver_create", _lib().pfsf_solver_create(C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def solve(self, b, x64=None, phi32=None):
+        info = capi.SolveInfo()
+        capi.check("pfsf_solver_solve", _lib().pfsf_solver_solve(
+            self.h, _p(b), _p(x64) if x64 is not None else None,
+            _p(phi32) if phi32 is not None else None, C.byref(info)))
+        return info
+
+    def profile(self, which, reps=50):
+        """Average device ms of one launch of solver kernel `which` (see pfsf_hip.h)."""
+        ms = C.c_double(0)
+        capi.check("pfsf_solver_profile",
+                   _lib().pfsf_solver_profile(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def update(self, vtype, kappa):
+        self._vtype, self._kappa = vtype, kappa
+        capi.check("pfsf_solver_update", _lib().pfsf_solver_update(self.h, _p(vtype), _p(kappa)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().pfsf_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

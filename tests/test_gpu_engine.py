@@ -1,0 +1,118 @@
+"""The drop-in tick path (pfsf.h) on the GPU, driven the way the Java runtime
+drives it (NativePFSFRuntime.java:267-383): add island, register host
+buffers (phi doubles as the stress readback), tick_dbb, decode failures."""
+import numpy as np
+import pytest
+
+import domains
+from oracle import oracle as O
+from pfsf_amd import capi
+from pfsf_amd import workloads as W
+from pfsf_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _expected_events(isl, phi, origin):
+    code = O.failure_scan(isl, phi)
+    flat = np.flatnonzero(code.ravel())
+    lx, ly = isl.lx, isl.ly
+    return np.stack([flat % lx + origin[0], (flat // lx) % ly + origin[1],
+                     flat // (lx * ly) + origin[2], code.ravel()[flat]], 1).astype(np.int32)
+
+
+def test_tick_dbb_solves_scans_and_reports(cuda):
+    isl = W.small_box_union(2, L=16)
+    isl.rcomp[...] = np.float32(0.4)  # low strengths: some voxels fail
+    isl.rtens[...] = np.float32(0.2)
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(7, isl.lx, isl.ly, isl.lz, origin=(100, -20, 64)) == capi.OK
+        assert e.register(7, isl, kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb([7], capacity=4096)
+        assert r == capi.OK
+        phi = e.phi(7).reshape(isl.vtype.shape)  # stress readback == phi buffer
+        ref = O.direct_solve(isl)
+        assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) <= 1e-5
+        want = _expected_events(isl, phi, (100, -20, 64))
+        assert len(want) > 0
+        assert np.array_equal(ev, want)
+        r, st = e.read_stress(7, isl.n)
+        assert r == capi.OK and np.array_equal(st, phi.ravel())
+        # history: H = max(0, psi(phi))
+        r, H = e.read_history(7, isl.n)
+        assert np.array_equal(H, O.psi(isl, phi).ravel())
+        # a clean island is skipped; a second tick after mark_full_rebuild warm-starts
+        r, ev2 = e.tick_dbb([7])
+        assert r == capi.OK and len(ev2) == 0
+        e.lib.pfsf_mark_full_rebuild(e.h, 7)
+        r, ev3 = e.tick_dbb([7], capacity=4096)
+        assert r == capi.OK and np.array_equal(ev3, want)
+
+
+def test_tick_legacy_result_and_capacity(cuda):
+    isl = W.small_box_union(4, L=12)
+    isl.rcomp[...] = np.float32(0.3)
+    isl.rtens[...] = np.float32(0.1)
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(1, isl, kappa=isl.kappa) == capi.OK
+        r, ev = e.tick([1], capacity=5)
+        assert r == capi.OK and len(ev) == 5
+        phi = e.phi(1).reshape(isl.vtype.shape)
+        assert np.array_equal(ev, _expected_events(isl, phi, (0, 0, 0))[:5])
+
+
+def test_soa6_conductivity_maps_to_per_voxel_max(cuda):
+    isl = W.small_box_union(1, L=10)
+    rng = np.random.default_rng(0)
+    cond = np.tile(isl.kappa.ravel(), 6).astype(np.float32)
+    cond[rng.integers(0, cond.size, 50)] *= 0.5  # per-voxel max is still kappa
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(3, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(3, isl, cond=cond) == capi.OK
+        assert e.tick_dbb([3])[0] == capi.OK
+        ref = O.direct_solve(isl)
+        phi = e.phi(3).reshape(isl.vtype.shape)
+        assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) <= 1e-5
+
+
+def test_orphan_cluster_reports_no_support_only_for_isolated(cuda):
+    isl = domains.pillar_and_cluster()
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(2, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(2, isl, kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb([2])
+        assert r == capi.OK
+        phi = e.phi(2).reshape(isl.vtype.shape)
+        assert np.array_equal(ev, _expected_events(isl, phi, (0, 0, 0)))
+
+
+def test_argument_errors(cuda):
+    with Engine(max_island_size=1000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(1, 20, 20, 20) == capi.ISLAND_FULL
+        assert e.add_island(1, 0, 2, 2) == capi.ISLAND_FULL
+        assert e.add_island(1, 10, 10, 10) == capi.OK
+        small = W.dense_block(1, 8)  # 512 voxels, island 1 has 1000
+        # undersized buffers -> INVALID_ARG
+        assert e.register(1, small) == capi.INVALID_ARG
+        assert e.register(99, small) == capi.INVALID_ARG
+        r, _ = e.tick_dbb([1])  # registered nothing: nothing to solve, but OK
+        assert r == capi.OK
+        assert e.lib.pfsf_tick_dbb(e.h, None, 1, 0, None, 0) == capi.INVALID_ARG
+        assert e.read_stress(42, 10)[0] == capi.INVALID_ARG
+
+
+def test_budget_zero_leaves_island_dirty(cuda):
+    isl = W.small_box_union(0, L=8)
+    with Engine(max_island_size=10**7, tick_budget_ms=0) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(5, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(5, isl, kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb([5])
+        assert r == capi.OK and len(ev) == 0
+        assert np.all(e.phi(5) == 0)  # nothing dispatched, nothing read back

@@ -1,0 +1,163 @@
+"""HIP kernels through the C ABI against the CPU oracle on identical inputs.
+
+Bar: bit-exact for Jacobi/RBGS phi, failure codes, psi/H, compaction and
+downsample; FP64 apply within 1e-12 relative (summation order is the same, so
+it is in practice exact)."""
+import numpy as np
+import pytest
+import torch
+
+import domains
+from oracle import oracle as O
+from pfsf_amd import device as D
+from pfsf_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("dense8", lambda: W.dense_block(7, 8)),
+         ("box12", lambda: W.small_box_union(0, L=12)),
+         ("box16", lambda: W.small_box_union(1, L=16)),
+         ("ragged", lambda: W.box_union(1100, 13, 0.5, 0.5, 40, (2, 7), True)),
+         ("column", lambda: W.column(9, 2.0, 1.0)),
+         ("floating", domains.floating_island)]
+
+
+def _isl_dev(isl, cuda):
+    return {k: torch.from_numpy(np.ascontiguousarray(getattr(isl, k))).to(cuda)
+            for k in ("vtype", "kappa", "source", "rcomp", "rtens")}
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_jacobi_step_bit_exact(cuda, name, mk):
+    isl = mk()
+    d = _isl_dev(isl, cuda)
+    rng = np.random.default_rng(3)
+    prev = (rng.random(isl.vtype.shape) * 3).astype(np.float32)
+    for omega in (1.0, 0.8):
+        out = torch.empty_like(d["kappa"])
+        D.jacobi_step(d["vtype"], d["kappa"], d["source"], torch.from_numpy(prev).to(cuda), out,
+                      omega)
+        want = O.jacobi_step(isl, prev, omega)
+        assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32)), name
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_rbgs_bit_exact(cuda, name, mk):
+    isl = mk()
+    d = _isl_dev(isl, cuda)
+    phi = torch.zeros_like(d["kappa"])
+    D.rbgs(d["vtype"], d["kappa"], d["source"], phi, 7)
+    want = O.rbgs(isl, sweeps=7)
+    assert np.array_equal(phi.cpu().numpy().view(np.uint32), want.view(np.uint32)), name
+
+
+def test_dense32_rbgs_200_sweeps_and_scan_bit_exact(cuda):
+    """The dense 32^3 workload: 200 RBGS sweeps then the failure scan."""
+    isl = W.workload(0)
+    d = _isl_dev(isl, cuda)
+    phi = torch.zeros_like(d["kappa"])
+    D.rbgs(d["vtype"], d["kappa"], d["source"], phi, 200)
+    want = O.rbgs(isl, sweeps=200)
+    got = phi.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    code = torch.empty_like(d["vtype"])
+    D.failure_scan(d["vtype"], d["kappa"], phi, d["rcomp"], d["rtens"], code)
+    assert np.array_equal(code.cpu().numpy(), O.failure_scan(isl, want))
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_failure_scan_bit_exact(cuda, name, mk):
+    isl = mk()
+    rng = np.random.default_rng(11)
+    phi = (rng.random(isl.vtype.shape) * 3).astype(np.float32)
+    isl.rcomp[...] = rng.uniform(0.0, 2.0, phi.shape).astype(np.float32)
+    isl.rtens[...] = rng.uniform(0.0, 1.0, phi.shape).astype(np.float32)
+    phi.ravel()[::97] = 2e6  # some orphan-threshold hits
+    d = _isl_dev(isl, cuda)
+    code = torch.empty_like(d["vtype"])
+    D.failure_scan(d["vtype"], d["kappa"], torch.from_numpy(phi).to(cuda), d["rcomp"], d["rtens"],
+                   code)
+    assert np.array_equal(code.cpu().numpy(), O.failure_scan(isl, phi)), name
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_history_bit_exact(cuda, name, mk):
+    isl = mk()
+    rng = np.random.default_rng(5)
+    phi = (rng.random(isl.vtype.shape) * 4).astype(np.float32)
+    H0 = (rng.random(isl.vtype.shape) * 2).astype(np.float32)
+    d = _isl_dev(isl, cuda)
+    H = torch.from_numpy(H0).to(cuda)
+    psi = torch.empty_like(H)
+    D.history_update(d["vtype"], d["kappa"], torch.from_numpy(phi).to(cuda), H, psi)
+    assert np.array_equal(psi.cpu().numpy().view(np.uint32), O.psi(isl, phi).view(np.uint32))
+    assert np.array_equal(H.cpu().numpy().view(np.uint32),
+                          O.history_update(isl, phi, H0).view(np.uint32))
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+def test_apply_f64(cuda, name, mk):
+    isl = mk()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(isl.vtype.shape)
+    d = _isl_dev(isl, cuda)
+    y = torch.empty(isl.vtype.shape, dtype=torch.float64, device=cuda)
+    D.apply_f64(d["vtype"], d["kappa"], torch.from_numpy(x).to(cuda), y)
+    want = O.apply(isl, x)
+    assert np.allclose(y.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 7, 3), (33, 17, 64), (40, 40, 40)])
+def test_compaction_order_and_capacity(cuda, shape):
+    rng = np.random.default_rng(sum(shape))
+    code = rng.choice([0, 0, 0, 0, 2, 3, 4], size=shape).astype(np.uint8)
+    flat = np.flatnonzero(code.ravel())
+    lz, ly, lx = shape
+    want = np.stack([flat % lx + 10, (flat // lx) % ly - 5, flat // (lx * ly) + 7,
+                     code.ravel()[flat]], 1).astype(np.int32)
+    for cap in (0, 1, max(1, len(flat) // 2), len(flat) + 3):
+        out = torch.full((1 + 4 * max(cap, 1),), -1, dtype=torch.int32, device=cuda)
+        D.compact_failures(torch.from_numpy(code).to(cuda), out, cap, origin=(10, -5, 7))
+        o = out.cpu().numpy()
+        n = min(cap, len(flat))
+        assert o[0] == n
+        assert np.array_equal(o[1:1 + 4 * n].reshape(n, 4), want[:n])
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (7, 5, 9), (3, 1, 4), (64, 32, 16)])
+def test_downsample_bit_exact(cuda, shape):
+    rng = np.random.default_rng(7)
+    f = rng.random(shape).astype(np.float32)
+    ft = rng.integers(0, 3, shape).astype(np.uint8)
+    c, ct = O.downsample_2to1(f, ft, shape[2], shape[1], shape[0])
+    gc = torch.empty(c.shape, dtype=torch.float32, device=cuda)
+    gt = torch.empty(ct.shape, dtype=torch.uint8, device=cuda)
+    D.downsample_2to1(torch.from_numpy(f).to(cuda), torch.from_numpy(ft).to(cuda), gc, gt)
+    assert np.array_equal(gc.cpu().numpy(), c) and np.array_equal(gt.cpu().numpy(), ct)
+
+
+def test_normalize_and_kappa_from_soa6(cuda):
+    rng = np.random.default_rng(9)
+    n = 5000
+    for scale in (0.5, 40.0):
+        s, r, t = (rng.random(n).astype(np.float32) * 10 for _ in range(3))
+        c = (rng.random(6 * n) * scale).astype(np.float32)
+        want = O.normalize_soa6(s, r, t, c)
+        g = [torch.from_numpy(a.copy()).to(cuda) for a in (s, r, t, c)]
+        smax = D.normalize_soa6(*g)
+        assert smax == want[4]
+        for a, b in zip(g, want[:4]):
+            assert np.array_equal(a.cpu().numpy(), b)
+    cond = (rng.random(6 * n)).astype(np.float32)
+    k = torch.empty(n, dtype=torch.float32, device=cuda)
+    D.kappa_from_soa6(torch.from_numpy(cond).to(cuda), k)
+    assert np.array_equal(k.cpu().numpy(), cond.reshape(6, n).max(0))
+
+
+def test_anchored_mask_matches_bfs(cuda):
+    for isl in (domains.floating_island(), domains.pillar_and_cluster(), W.small_box_union(3, 16)):
+        reach = torch.empty_like(torch.from_numpy(isl.vtype)).to(cuda)
+        D.anchored_mask(torch.from_numpy(isl.vtype).to(cuda), reach)
+        r, _ = O.anchored(isl)
+        want = (r.astype(bool) & (isl.vtype == 1)).astype(np.uint8)
+        assert np.array_equal(reach.cpu().numpy(), want)

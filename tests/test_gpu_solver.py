@@ -1,0 +1,104 @@
+"""The MG-preconditioned PCG solve on the GPU against the FP64 oracle.
+
+Bar: the FP64 residual ||b - A x|| / ||b|| (recomputed by the oracle from the
+returned x) <= 1e-6, and phi within 1e-5 relative L2 of the FP64 direct solve
+where the direct solve is affordable."""
+import numpy as np
+import pytest
+import torch
+
+import domains
+from oracle import oracle as O
+from pfsf_amd import capi
+from pfsf_amd import device as D
+from pfsf_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL_PHI = 1e-5  # relative L2 of phi vs the FP64 oracle
+
+
+def _solve(isl, cuda, **kw):
+    vt = torch.from_numpy(isl.vtype).to(cuda)
+    kp = torch.from_numpy(isl.kappa).to(cuda)
+    b = torch.from_numpy(isl.source).to(cuda)
+    s = D.Solver(vt, kp, **kw)
+    x64 = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=cuda)
+    phi = torch.empty(isl.vtype.shape, dtype=torch.float32, device=cuda)
+    info = s.solve(b, x64, phi)
+    s.close()
+    return info, x64.cpu().numpy(), phi.cpu().numpy()
+
+
+@pytest.mark.parametrize("levels", [1, 2, 0])
+@pytest.mark.parametrize("seed,L", [(0, 12), (1, 16), (2, 20), (4, 17)])
+def test_small_islands_match_direct_solve(cuda, seed, L, levels):
+    isl = W.small_box_union(seed, L=L)
+    info, x, phi = _solve(isl, cuda, mg_levels=levels, tol=1e-8, max_iters=4000)
+    ref = O.direct_solve(isl)
+    assert info.converged == 1, (info.iterations, info.rel_residual)
+    assert O.residual(isl, x) <= 1e-8 * 1.01
+    err = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    assert err <= TOL_PHI, err
+
+
+def test_column_closed_form(cuda):
+    L, k, q = 40, 2.0, 0.5
+    isl = W.column(L, kappa=k, load=q)
+    info, x, phi = _solve(isl, cuda, tol=1e-10, max_iters=2000)
+    z = np.arange(1, L)
+    exact = (np.float32(q) / k) * (z * L - z * (z + 1) / 2)
+    assert np.allclose(x.ravel()[1:], exact, rtol=1e-8)
+
+
+def test_orphans_and_isolated_voxels_are_excluded(cuda):
+    isl = domains.pillar_and_cluster()
+    info, x, phi = _solve(isl, cuda, tol=1e-9)
+    assert info.converged == 1
+    assert np.all(phi[3:6, 6, 6] == 0)  # detached 3-voxel column: not solved
+    assert phi[1, 1, 1] > 0 and phi[2, 1, 1] > phi[1, 1, 1]
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP32, capi.PREC_BF16_COARSE])
+def test_box_union_64_to_1e6(cuda, prec):
+    isl = W.box_union(1, 64, 0.6, 0.5, n_boxes=100000, edge=(2, 20))
+    info, x, phi = _solve(isl, cuda, precision=prec, tol=1e-6, max_iters=2000)
+    assert info.converged == 1
+    rel = O.residual(isl, x)
+    assert rel <= 1e-6 * 1.01, rel
+    ref, it, rr = O.pcg(isl, tol=1e-11)
+    err = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    assert err <= TOL_PHI, err
+
+
+def test_workload_128_to_1e6(cuda):
+    """The 128^3 box-union workload: FP64 residual recomputed on the CPU."""
+    isl = W.workload(1)
+    info, x, phi = _solve(isl, cuda, tol=1e-6, max_iters=2000)
+    print(f"128^3: iters={info.iterations} passes={info.refinements} levels={info.levels} "
+          f"rel={info.rel_residual:.3e} ms={info.solve_ms:.2f}")
+    assert info.converged == 1
+    rel = O.residual(isl, x)
+    assert rel <= 1e-6 * 1.01, rel
+    # phi is the FP32 rounding of x
+    assert np.array_equal(phi, x.astype(np.float32))
+
+
+def test_solve_is_deterministic(cuda):
+    isl = W.small_box_union(6, L=20)
+    a = _solve(isl, cuda, tol=1e-7)[1]
+    b = _solve(isl, cuda, tol=1e-7)[1]
+    assert np.array_equal(a, b)
+
+
+def test_warm_start_converged_input_needs_no_iterations(cuda):
+    isl = W.small_box_union(5, L=16)
+    vt = torch.from_numpy(isl.vtype).to(cuda)
+    kp = torch.from_numpy(isl.kappa).to(cuda)
+    b = torch.from_numpy(isl.source).to(cuda)
+    s = D.Solver(vt, kp, tol=1e-7)
+    x64 = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=cuda)
+    i1 = s.solve(b, x64)
+    i2 = s.solve(b, x64)
+    s.close()
+    assert i1.iterations > 0 and i2.iterations == 0 and i2.converged == 1

@@ -1,0 +1,292 @@
+"""Pins the CPU oracle against the reference's known answers and against
+mathematics that a plausible restatement mistake would violate.
+
+Sources of the known answers:
+  EnergyEvaluatorCPUGoldenTest.java:61-75, 77-94, 238-261  (E = 1/2 phi^T L phi)
+  FloatingIslandTest.java:29-33, 120-147                    (orphan counts 125 and 3)
+  CantileverBreakTest.java:39-69                            (monotone phi, phi ratio band)
+  GoldenParityTest.java:289-337, 506-590, 691-760           (normalize, downsample, Chebyshev)
+"""
+import numpy as np
+import pytest
+
+import domains
+from oracle import oracle as O
+from pfsf_amd import workloads as W
+
+CORNER = float(np.float32(0.1666667))
+
+
+# ── a1: operator weights ────────────────────────────────────────────────
+def test_energy_kat_face_pair():
+    assert O.energy(1.0, np.array([0.0, 1.0]), 2, 1, 1) == pytest.approx(0.5, abs=1e-12)
+
+
+def test_energy_kat_checkerboard():
+    assert O.energy(1.0, np.array([0.0, 1.0, 1.0, 0.0]), 2, 2, 1) == pytest.approx(2.0, abs=1e-12)
+
+
+def test_energy_kat_impulse_uses_edge_and_corner_weights():
+    phi = np.zeros(8)
+    phi[0] = 1.0
+    want = 1.5 + 1.5 * 0.5 + 0.5 * CORNER
+    assert O.energy(1.0, phi, 2, 2, 2) == pytest.approx(want, abs=1e-12)
+
+
+def test_corner_weight_with_unequal_kappa():
+    # two voxels on a body diagonal, kappa 2 and 8: a = sqrt(16) * CORNER_P
+    k = np.zeros((2, 2, 2), np.float32)
+    k[0, 0, 0], k[1, 1, 1] = 2.0, 8.0
+    t = np.where(k > 0, 1, 0).astype(np.uint8)
+    isl = domains.make(t, k)
+    x = np.zeros((2, 2, 2))
+    x[0, 0, 0] = 1.0
+    y = O.apply(isl, x)
+    assert y[0, 0, 0] == pytest.approx(4.0 * CORNER, rel=1e-15)
+    assert y[1, 1, 1] == pytest.approx(-4.0 * CORNER, rel=1e-15)
+
+
+def test_operator_symmetric_and_row_sums():
+    isl = W.small_box_union(3, L=12)
+    A, sidx = O.assemble(isl)
+    assert abs(A - A.T).max() < 1e-12
+    # A*1 = couplings to anchors (Dirichlet) for every solid row
+    ones = np.zeros(isl.vtype.shape)
+    ones.ravel()[sidx] = 1.0
+    y = O.apply(isl, ones).ravel()[sidx]
+    d = O.diag(isl).ravel()[sidx]
+    offs = d - np.asarray(abs(A).sum(axis=1)).ravel() + d  # d - sum_j |a_ij| (j solid)
+    assert np.allclose(y, offs, rtol=1e-12, atol=1e-12)
+    # C restatement equals the independent numpy assembly
+    rng = np.random.default_rng(0)
+    x = np.zeros(isl.vtype.shape)
+    x.ravel()[sidx] = rng.standard_normal(sidx.size)
+    assert np.allclose(O.apply(isl, x).ravel()[sidx], A @ x.ravel()[sidx], rtol=1e-13, atol=1e-13)
+
+
+def test_column_closed_form():
+    # 1x1xL column, z=0 anchored, uniform kappa k, load q:
+    # phi_k = (q/k)(kL - k(k+1)/2)  (only face couplings exist)
+    L, k, q = 24, 2.5, 0.75
+    isl = W.column(L, kappa=k, load=q)
+    z = np.arange(1, L)
+    exact = (q / k) * (z * L - z * (z + 1) / 2)
+    phi = O.direct_solve(isl).ravel()
+    assert np.allclose(phi[1:], exact, rtol=1e-12)
+    x, it, rel = O.pcg(isl, tol=1e-14)
+    assert np.allclose(x.ravel()[1:], exact, rtol=1e-10)
+
+
+def test_column_nonuniform_kappa_closed_form():
+    # phi_k = sum_{m=1..k} q (L-m) / a_m, a_m = (k_{m-1} + k_m) / 2
+    L = 12
+    rng = np.random.default_rng(5)
+    kap = rng.uniform(0.5, 4.0, L).astype(np.float32)
+    t = np.full((L, 1, 1), 1, np.uint8)
+    t[0] = 2
+    q = float(np.float32(0.3))
+    src = np.full((L, 1, 1), q, np.float32)
+    src[0] = 0
+    isl = domains.make(t, kap.reshape(L, 1, 1), src)
+    a = 0.5 * (kap[:-1] + kap[1:]).astype(np.float64)
+    exact = np.cumsum([q * (L - m) / a[m - 1] for m in range(1, L)])
+    assert np.allclose(O.direct_solve(isl).ravel()[1:], exact, rtol=1e-11)
+
+
+# ── a3: Jacobi / RBGS ──────────────────────────────────────────────────
+def test_jacobi_first_step_is_b_over_diag():
+    isl = W.small_box_union(1, L=10)
+    phi1 = O.jacobi_step(isl, np.zeros(isl.vtype.shape, np.float32))
+    d = O.diag(isl)
+    solid = isl.vtype == 1
+    want = np.where(solid & (d > 1e-15), isl.source.astype(np.float64) / np.where(d > 0, d, 1), 0)
+    assert np.array_equal(phi1[solid], want[solid].astype(np.float32))
+
+
+def test_jacobi_and_rbgs_reach_the_direct_solution():
+    isl = W.small_box_union(0, L=10)
+    ref = O.direct_solve(isl)
+    j = O.jacobi_solve(isl, 6000)
+    r = O.rbgs(isl, sweeps=3000)
+    assert np.linalg.norm(j - ref) / np.linalg.norm(ref) < 1e-5
+    assert np.linalg.norm(r - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_rbgs_two_voxel_hand_case():
+    # anchor - a - b in a row (x direction), kappa 1, b = [0, 1, 2]
+    t = np.array([[[2, 1, 1]]], np.uint8)
+    isl = domains.make(t, 1.0, np.array([[[0, 1, 2]]], np.float32))
+    phi = O.rbgs(isl, sweeps=1)
+    # colour 0 (x even) first: x=2 sees only x=1 (phi 0): phi2 = 2/1 = 2
+    # colour 1: x=1 sees the anchor (0) and x=2 (2), diag 2: phi1 = (1 + 2)/2 = 1.5
+    assert phi.ravel().tolist() == [0.0, 1.5, 2.0]
+
+
+def test_rbgs_colour_order_matters_only_through_colours():
+    isl = W.small_box_union(2, L=9)
+    a = O.rbgs(isl, sweeps=3)
+    b = O.rbgs(isl, sweeps=3)
+    assert np.array_equal(a, b)
+
+
+def test_cantilever_band():
+    # CantileverBreakTest: phi monotone up the column; phi_max(16)/phi_max(8) in (3, 6)
+    m = {}
+    for N in (8, 16):
+        isl = domains.cantilever(N)
+        phi = O.jacobi_solve(isl, 2000 if N == 8 else 4000).ravel()
+        assert phi[0] == 0.0 and np.all(np.diff(phi) >= -1e-4)
+        m[N] = phi.max()
+    assert 3.0 < m[16] / m[8] < 6.0
+
+
+# ── orphans ─────────────────────────────────────────────────────────────
+def test_orphan_counts_match_floating_island_test():
+    _, n = O.anchored(domains.floating_island())
+    assert n == 125
+    _, n = O.anchored(domains.pillar_and_cluster())
+    assert n == 3
+
+
+# ── a8: failure scan ────────────────────────────────────────────────────
+def _scan_numpy(isl, phi, orphan=1e6):
+    """Brute-force restatement from the assembled couplings (independent of C)."""
+    lz, ly, lx = isl.vtype.shape
+    t = isl.vtype
+    code = np.zeros(t.shape, np.uint8)
+    for z in range(lz):
+        for y in range(ly):
+            for x in range(lx):
+                if t[z, y, x] != 1:
+                    continue
+                fin = fout = dg = 0.0
+                for k, (dx, dy, dz) in enumerate(O.OFFSETS):
+                    nx, ny, nz = x + dx, y + dy, z + dz
+                    if not (0 <= nx < lx and 0 <= ny < ly and 0 <= nz < lz) or t[nz, ny, nx] == 0:
+                        continue
+                    a = float(O.coupling_f64(k, np.float32(isl.kappa[z, y, x]),
+                                             np.float32(isl.kappa[nz, ny, nx])))
+                    dg += a
+                    dp = float(phi[nz, ny, nx]) - float(phi[z, y, x])
+                    if dp > 0:
+                        fin += a * dp
+                    elif dp < 0:
+                        fout -= a * dp
+                if dg == 0 or phi[z, y, x] > orphan:
+                    code[z, y, x] = 3
+                elif isl.rcomp[z, y, x] > 0 and fin > isl.rcomp[z, y, x]:
+                    code[z, y, x] = 2
+                elif isl.rtens[z, y, x] > 0 and fout > isl.rtens[z, y, x]:
+                    code[z, y, x] = 4
+    return code
+
+
+def test_failure_scan_hand_cases():
+    # isolated voxel -> NO_SUPPORT; anchor and air -> 0
+    t = np.zeros((3, 3, 3), np.uint8)
+    t[0, 0, 0] = 2
+    t[2, 2, 2] = 1
+    isl = domains.make(t, 1.0)
+    code = O.failure_scan(isl, np.zeros(t.shape, np.float32))
+    assert code[2, 2, 2] == 3 and code[0, 0, 0] == 0 and code.sum() == 3
+    # phi above the orphan threshold -> NO_SUPPORT
+    t = np.array([[[2, 1]]], np.uint8)
+    isl = domains.make(t, 1.0)
+    assert O.failure_scan(isl, np.array([[[0, 2e6]]], np.float32))[0, 0, 1] == 3
+    # chain anchor(0) - a(phi 1) - b(phi 3): a gets flux_in 2 from b, flux_out 1 to anchor
+    t = np.array([[[2, 1, 1]]], np.uint8)
+    phi = np.array([[[0, 1, 3]]], np.float32)
+    for rc, rt, want in ((1.5, 9, 2), (9, 0.5, 4), (9, 9, 0), (2.0, 9, 0), (0, 0.5, 4)):
+        isl = domains.make(t, 1.0, rcomp=np.full(t.shape, rc), rtens=np.full(t.shape, rt))
+        assert O.failure_scan(isl, phi)[0, 0, 1] == want, (rc, rt)
+
+
+def test_failure_scan_matches_bruteforce():
+    isl = W.small_box_union(4, L=8)
+    rng = np.random.default_rng(1)
+    phi = (rng.random(isl.vtype.shape) * 0.2).astype(np.float32)
+    isl.rcomp[...] = rng.uniform(0.01, 0.3, isl.vtype.shape).astype(np.float32)
+    isl.rtens[...] = (isl.rcomp * np.float32(0.5)).astype(np.float32)
+    c = O.failure_scan(isl, phi)
+    assert np.array_equal(c, _scan_numpy(isl, phi))
+    assert len(np.unique(c)) >= 3
+
+
+# ── a7: psi and history ─────────────────────────────────────────────────
+def test_psi_linear_field_exact():
+    # phi = 0.25 z on a dense block: central differences are exact (0.25),
+    # one-sided at the top/bottom faces too; psi = 0.5 * k * 0.0625
+    t = np.ones((5, 4, 4), np.uint8)
+    t[0] = 2
+    z = np.arange(5, dtype=np.float32)[:, None, None]
+    phi = np.broadcast_to(0.25 * z, t.shape).astype(np.float32).copy()
+    isl = domains.make(t, 3.0)
+    psi = O.psi(isl, phi)
+    assert np.allclose(psi[1:], 0.5 * 3.0 * 0.0625, rtol=0, atol=0)
+    assert np.all(psi[0] == 0)
+
+
+def test_history_monotone_and_idempotent():
+    isl = W.small_box_union(5, L=10)
+    ref = O.direct_solve(isl).astype(np.float32)
+    H0 = np.zeros(isl.vtype.shape, np.float32)
+    H1 = O.history_update(isl, ref, H0)
+    H2 = O.history_update(isl, ref, H1)
+    assert np.array_equal(H1, H2)
+    H3 = O.history_update(isl, ref * 0.5, H2)
+    assert np.array_equal(H3, H2)  # smaller psi never lowers H
+    H4 = O.history_update(isl, ref * 2, H3)
+    assert np.all(H4 >= H3) and np.any(H4 > H3)
+
+
+# ── a4, a9, a6 data prep: against the reference's own code (oracle/_ref) ─
+def test_chebyshev_omega_invariants():
+    assert O.chebyshev_omega(0, 0.9) == 1.0
+    rho = 0.95 * np.cos(np.pi / 32)
+    w = [O.chebyshev_omega(k, rho) for k in range(64)]
+    assert w[1] == pytest.approx(2 / (2 - rho * rho), rel=1e-6)
+    lim = 2 / (1 + np.sqrt(1 - rho * rho))
+    assert w[-1] == pytest.approx(lim, rel=1e-3) and max(w) <= 1.98
+    assert O.chebyshev_omega(5, 1.0) == 1.0
+
+
+def test_against_reference_cpu_primitives():
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    import ctypes as C
+
+    for it in range(0, 40):
+        for rho in (0.3, 0.9, 0.99, 0.999, 1.0):
+            assert R.pfsf_chebyshev_omega(it, rho) == O.chebyshev_omega(it, rho)
+    rng = np.random.default_rng(2)
+    n = 1000
+    for scale in (0.5, 30.0):
+        s, r, t = (rng.random(n).astype(np.float32) * 10 for _ in range(3))
+        c = (rng.random(6 * n) * scale).astype(np.float32)
+        a = O.normalize_soa6(s, r, t, c)
+        s2, r2, t2, c2 = (x.copy() for x in (s, r, t, c))
+        sm = C.c_float(0)
+        R.pfsf_normalize_soa6(*(x.ctypes.data_as(C.c_void_p) for x in (s2, r2, t2, c2)), None, n,
+                              C.byref(sm))
+        assert a[4] == sm.value
+        for u, v in zip(a[:4], (s2, r2, t2, c2)):
+            assert np.array_equal(u, v)
+    for shape in ((7, 5, 9), (8, 8, 8), (3, 1, 4)):
+        f = rng.random(shape).astype(np.float32)
+        ft = rng.integers(0, 3, shape).astype(np.uint8)
+        c, ct = O.downsample_2to1(f, ft, shape[2], shape[1], shape[0])
+        c2 = np.empty_like(c)
+        ct2 = np.empty_like(ct)
+        R.pfsf_downsample_2to1(f.ctypes.data_as(C.c_void_p), ft.ctypes.data_as(C.c_void_p),
+                               shape[2], shape[1], shape[0], c2.ctypes.data_as(C.c_void_p),
+                               ct2.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(c, c2) and np.array_equal(ct, ct2)
+
+
+def test_workloads_are_seeded_and_shaped():
+    a = W.workload(0)
+    assert a.n == 32 ** 3 and a.n_active == a.n and (a.vtype[0] == 2).all()
+    b = W.dense_block(0, 32, 0.5)
+    assert np.array_equal(a.kappa, b.kappa)
+    assert 20.0 < np.median(a.kappa) < 45.0
