@@ -24,14 +24,15 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "time-to-1e-6 residual at 128^3 + batched islands/s vs plain, % of HBM peak"
 
-# Algorithmic bytes per ACTIVE voxel for each profiled solver kernel (each
-# array element counted once per launch, neighbour reuse free):
-#   0 SpMV + p.Ap       read p 4, kappa 4, type 1; write q 4             = 13
-#   1 vector update     read x, r, p, q 16; write x, r 8                 = 24
-#   2 V-cycle down (L0) read r, kappa, diag 12; write xs 4, r_c 0.5       = 16.5
-#   3 V-cycle up   (L0) read xs, r, kappa, diag 16, e_c 0.5; write z 4   = 20.5
-ALG_BYTES = {0: 13.0, 1: 24.0, 2: 16.5, 3: 20.5}
-KERNEL_NAMES = {0: "k_pcg_spmv", 1: "k_pcg_update", 2: "k_mg_down", 3: "k_mg_up"}
+# Algorithmic bytes per ACTIVE voxel of each profiled solver kernel (each array
+# element counted once per launch, neighbour reuse free):
+#   0 k_spmv_dir  read z, p_in, kappa, anchor mass 16; write p_out, q 8      = 24
+#   1 k_down      read r_in, q, kappa, diag, anchor mass, p 24, x (f64) 8;
+#                 write x (f64) 8, r_out 4, coarse rhs 0.5                   = 44.5
+#   3 k_up        read r, kappa, diag, anchor mass 16, coarse correction 0.5;
+#                 write z 4                                                  = 20.5
+ALG_BYTES = {0: 24.0, 1: 44.5, 3: 20.5}
+KERNEL_NAMES = {0: "k_spmv_dir", 1: "k_down(update)", 3: "k_up(finest)"}
 # FP64 Jacobi-PCG iterations the CPU oracle needs on this workload to reach
 # 1e-6 (counted once with oracle.pcg(workload(1), tol=1e-6) on the full solve).
 CPU_PCG_ITERS_TO_1E6 = None
@@ -110,14 +111,14 @@ def main():
     # live per-kernel timing (HIP events on the solver stream)
     prof = {}
     for k in KERNEL_NAMES:
-        if k in (2, 3) and last.levels < 2:
+        if k == 3 and last.levels < 2:
             continue
         prof[k] = solver.profile(k, reps=100)
     iter_ms = solver.profile(4, reps=30)
     dom = max(prof, key=prof.get)
     ach = ALG_BYTES[dom] * n_act / (prof[dom] * 1e-3)
     # whole-iteration algorithmic rate
-    it_bytes = (13 + 24 + 16.5 + 20.5) * n_act
+    it_bytes = sum(ALG_BYTES.values()) * n_act
     hist = np.bincount(code.cpu().numpy().ravel(), minlength=5).tolist()
 
     out = {

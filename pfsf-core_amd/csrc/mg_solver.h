@@ -13,34 +13,40 @@ struct PcgScalars {
   int done;  // 1 converged / capped, 2 breakdown
   int iters;
   int max_iters;
-  int have_dir;
-  unsigned counter[4];
+  int have_dir;  // a search direction exists (set after the first SpMV)
 };
 
 struct IrScalars {
   double bb, rr, rel;
-  unsigned counter;
-  unsigned pad;
+};
+
+// Stencil tiles: 32 x 8 x tz voxels per 256-thread work-group.
+constexpr int kTX = 32, kTY = 8;
+
+struct TileList {
+  int tz = 8;                     // tile depth (8, 4 or 2) chosen per level
+  int ntx = 0, nty = 0, ntz = 0;  // tiles per axis
+  int nlaunch = 0;                // launch size (multiple of 8, padded with -1)
+  int nactive = 0;                // tiles holding at least one free voxel
+  int* ids = nullptr;             // device: tile id per work-group, XCD-grouped
 };
 
 struct MgLevel {
   pfsf_dims d{0, 0, 0};
   int64_t n = 0;
   bool bf16 = false;
-  void* kap = nullptr;  // float or bf16, 0 outside the free set
-  void* sq = nullptr;   // sqrt(kappa), same storage type
-  float* dg = nullptr;  // diagonal (0 outside the free set)
-  float* dinv = nullptr;
-  float* am = nullptr;   // anchor mass
-  uint8_t* fr = nullptr; // free mask
-  float* r = nullptr;    // restricted residual (levels > 0)
-  float* e = nullptr;    // correction returned to the finer level (levels > 0)
-  float* xs = nullptr;   // pre-smoothed iterate
-  float* xb = nullptr;   // Chebyshev ping-pong (coarsest)
-  float* dv = nullptr;   // Chebyshev direction (coarsest)
-  float scale = 1.f;
-  float omega = 1.f;
-  float lmax = 2.f;
+  void* kap = nullptr;    // float or bf16 kappa on the free set, -1 elsewhere
+  float* dg = nullptr;    // diagonal (0 outside the free set)
+  float* am = nullptr;    // anchor mass (couplings to Dirichlet voxels)
+  uint8_t* fr = nullptr;  // free mask
+  float* r = nullptr;     // restricted residual (levels > 0)
+  float* e = nullptr;     // correction returned to the finer level (levels > 0)
+  float* xb = nullptr;    // Chebyshev ping-pong (coarsest)
+  float* dv = nullptr;    // Chebyshev direction (coarsest)
+  float scale = 1.f;      // coupling scale 2^level
+  float omega = 1.f;      // damped-Jacobi weight
+  float lmax = 2.f;       // estimate of lambda_max(D^-1 A)
+  TileList tiles;
 };
 
 class MgSolver {
@@ -56,8 +62,12 @@ class MgSolver {
 
  private:
   hipError_t build();
-  hipError_t precondition();
-  hipError_t iterate();
+  hipError_t build_tiles(MgLevel& L);
+  hipError_t precondition();            // z = M r[0], r.z
+  hipError_t true_residual(const float* b, double* rel);
+  hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
+  hipError_t vcycle_below(int from);   // levels from..L-1 given level `from` residual
+  hipError_t chunk(int pairs);         // 2*pairs iterations (graph replay when captured)
   void release();
 
   pfsf_solver_desc desc_{};
@@ -76,12 +86,18 @@ class MgSolver {
   IrScalars* ir_ = nullptr;
   PcgScalars* sc_host_ = nullptr;  // pinned: PcgScalars followed by IrScalars
   double* partials_ = nullptr;
+  int64_t npartials_ = 0;
   uint8_t* reach_ = nullptr;
   int* flag_ = nullptr;
-  float *x32_ = nullptr, *r32_ = nullptr, *p_ = nullptr, *q_ = nullptr, *z_ = nullptr;
-  double *xa_ = nullptr, *xb_ = nullptr, *last_xc_ = nullptr;
+  double* x_ = nullptr;  // FP64 iterate
+  float* r_[2] = {nullptr, nullptr};
+  float* p_[2] = {nullptr, nullptr};
+  float *q_ = nullptr, *z_ = nullptr;
   float* coef_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  bool use_graph_ = true;
 };
 
 }  // namespace pfsf

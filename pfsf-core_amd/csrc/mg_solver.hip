@@ -1,20 +1,33 @@
-// FP32 multigrid-preconditioned PCG inside an FP64 iterative-refinement loop.
+// FP32 multigrid-preconditioned PCG with an FP64 iterate and FP64 restarts.
 //
-// Operator (every level): y_i = D_i x_i - s_l * sum_j w(k_i, k_j) x_j over the
-// 26 neighbours, w = (k_i+k_j)/2 on faces, 0.5*sqrt(k_i)sqrt(k_j) on edges,
-// 0.1666667*sqrt(k_i)sqrt(k_j) on corners (the coupling rule of
-// VoxelPhysicsCpuReference.java:97-105).  Vectors are zero outside the free
-// set (solid voxels connected to an anchor, with D > 0), so the neighbour sum
-// needs no masks; the diagonal D is precomputed and already contains every
-// coupling to anchors.
+// Operator (every level), in difference form:
+//   y_i = s_l * sum_{j free} w(k_i, k_j) (x_i - x_j) + m_i x_i
+// over the 26 neighbours, w = (k_i+k_j)/2 on faces, 0.5*sqrt(k_i)sqrt(k_j) on
+// edges, 0.1666667*sqrt(k_i)sqrt(k_j) on corners (the coupling rule of
+// VoxelPhysicsCpuReference.java:97-105); m_i ("anchor mass") is the sum of the
+// couplings to Dirichlet neighbours.  Forming x_i - x_j before the product
+// keeps the FP32 operator accurate on the smooth vectors PCG produces (the
+// expanded form D x_i - sum w x_j cancels catastrophically there).  The free
+// set (solid voxels connected to an anchor, with a non-zero diagonal) is
+// encoded in the kappa array itself: kappa < 0 marks a cell outside it.
 //
 // Coarse levels are rediscretised: a coarse cell is free when any child is,
 // its kappa is the sum of its free children's kappa / 8, couplings are scaled
 // by 2 per level (the h-scaling of a cell-centred 3D Laplacian), and the
-// Dirichlet couplings of its children ("anchor mass") are summed into its
-// diagonal.  Prolongation is piecewise constant and restriction its
-// transpose, so with symmetric damped-Jacobi smoothing and a fixed-degree
-// Chebyshev coarse solve the V-cycle is a symmetric preconditioner.
+// children's anchor masses add up.  Prolongation is piecewise constant and
+// restriction its transpose; damped-Jacobi pre/post smoothing with the same
+// weight and a fixed-degree Chebyshev coarsest solve keep the V-cycle a
+// symmetric positive-definite preconditioner.
+//
+// Hot kernels work on 32 x 8 x TZ voxel tiles staged in LDS with a one-voxel
+// halo (value, kappa, sqrt kappa), so every output reads its 26 neighbours
+// from LDS.  TZ is chosen per level so that small levels still launch enough
+// work-groups.  Work-groups cover only tiles holding free voxels; the tile
+// list is ordered so that consecutive work-groups on one XCD (work-groups are
+// dealt to the 8 XCDs round-robin) take neighbouring tiles and share halos in
+// that XCD's L2.  Dot products are FP64: each work-group writes a partial and
+// a one-block finalize kernel sums them in a fixed order (deterministic, and
+// no device-scope fence per work-group).
 //
 // Reference cycle for contrast: SH/mg_restrict.comp.glsl:118-195,
 // SH/mg_prolong.comp.glsl:39-80, NP/src/solver/dispatcher_vcycle.cpp:342-392
@@ -32,8 +45,11 @@ namespace pfsf {
 namespace {
 
 constexpr int BS = 256;
+constexpr int TX = kTX, TY = kTY;
+constexpr int HX = TX + 2, HY = TY + 2, HXY = HX * HY;
+static_assert(TX * TY == BS, "one thread per (x, y) column of a tile");
 
-// ── storage helpers (FP32 or BF16 kappa / sqrt-kappa) ───────────────────
+// ── storage helpers (FP32 or BF16 kappa) ───────────────────────────────
 __device__ __forceinline__ float ldk(const float* p, int64_t i) { return p[i]; }
 __device__ __forceinline__ float ldk(const uint16_t* p, int64_t i) {
   return __uint_as_float(((unsigned)p[i]) << 16);
@@ -49,72 +65,6 @@ __device__ __forceinline__ int64_t flat(int x, int y, int z, pfsf_dims d) {
   return (int64_t)x + (int64_t)d.lx * ((int64_t)y + (int64_t)d.ly * z);
 }
 
-template <typename KT>
-struct Lv {
-  pfsf_dims d;
-  const KT* kap;
-  const KT* sq;
-  const float* dg;
-  const float* dinv;
-  float scale;
-  float omega;
-};
-
-// sum_j w_ij v_j for voxel (x,y,z); v(j) supplies the neighbour value.
-template <typename KT, typename F>
-__device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, float ki, float si,
-                                        F v) {
-  const pfsf_dims d = L.d;
-  bool xm = x > 0, xp = x + 1 < d.lx, ym = y > 0, yp = y + 1 < d.ly, zm = z > 0,
-       zp = z + 1 < d.lz;
-  float af = 0.f, ae = 0.f, ac = 0.f;
-  auto face = [&](bool ok, int nx, int ny, int nz) {
-    if (ok) {
-      int64_t j = flat(nx, ny, nz, d);
-      af += (ki + ldk(L.kap, j)) * v(j);
-    }
-  };
-  auto edge = [&](bool ok, int nx, int ny, int nz) {
-    if (ok) {
-      int64_t j = flat(nx, ny, nz, d);
-      ae += ldk(L.sq, j) * v(j);
-    }
-  };
-  auto corner = [&](bool ok, int nx, int ny, int nz) {
-    if (ok) {
-      int64_t j = flat(nx, ny, nz, d);
-      ac += ldk(L.sq, j) * v(j);
-    }
-  };
-  face(xp, x + 1, y, z);
-  face(xm, x - 1, y, z);
-  face(yp, x, y + 1, z);
-  face(ym, x, y - 1, z);
-  face(zp, x, y, z + 1);
-  face(zm, x, y, z - 1);
-  edge(xp && yp, x + 1, y + 1, z);
-  edge(xp && ym, x + 1, y - 1, z);
-  edge(xm && yp, x - 1, y + 1, z);
-  edge(xm && ym, x - 1, y - 1, z);
-  edge(xp && zp, x + 1, y, z + 1);
-  edge(xp && zm, x + 1, y, z - 1);
-  edge(xm && zp, x - 1, y, z + 1);
-  edge(xm && zm, x - 1, y, z - 1);
-  edge(yp && zp, x, y + 1, z + 1);
-  edge(yp && zm, x, y + 1, z - 1);
-  edge(ym && zp, x, y - 1, z + 1);
-  edge(ym && zm, x, y - 1, z - 1);
-  corner(xp && yp && zp, x + 1, y + 1, z + 1);
-  corner(xp && yp && zm, x + 1, y + 1, z - 1);
-  corner(xp && ym && zp, x + 1, y - 1, z + 1);
-  corner(xp && ym && zm, x + 1, y - 1, z - 1);
-  corner(xm && yp && zp, x - 1, y + 1, z + 1);
-  corner(xm && yp && zm, x - 1, y + 1, z - 1);
-  corner(xm && ym && zp, x - 1, y - 1, z + 1);
-  corner(xm && ym && zm, x - 1, y - 1, z - 1);
-  return L.scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac));
-}
-
 __device__ __forceinline__ void unflat(int64_t i, pfsf_dims d, int& x, int& y, int& z) {
   x = (int)(i % d.lx);
   int64_t r = i / d.lx;
@@ -122,7 +72,159 @@ __device__ __forceinline__ void unflat(int64_t i, pfsf_dims d, int& x, int& y, i
   z = (int)(r / d.ly);
 }
 
-// ── deterministic FP64 block reductions + last-block finalisation ───────
+__device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
+
+template <typename KT>
+struct Lv {
+  pfsf_dims d;
+  const KT* kap;     // kappa on the free set, -1 elsewhere
+  const float* dg;   // diagonal (0 outside the free set)
+  const float* am;   // anchor mass
+  float scale;
+  float omega;
+};
+
+struct TL {
+  int ntx, nty;
+  const int* ids;
+};
+
+// ── gather form of the operator (setup) ────────────────────────────────
+// sum over free neighbours of w_ij * v(j).
+template <typename KT, typename F>
+__device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, float ki, F v) {
+  const pfsf_dims d = L.d;
+  const float si = sqrtf(ki);
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  for (int dz = -1; dz <= 1; dz++) {
+    int nz = z + dz;
+    if (nz < 0 || nz >= d.lz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      int ny = y + dy;
+      if (ny < 0 || ny >= d.ly) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        int nx = x + dx;
+        if (nx < 0 || nx >= d.lx) continue;
+        int cls = (dx != 0) + (dy != 0) + (dz != 0);
+        if (cls == 0) continue;
+        int64_t j = flat(nx, ny, nz, d);
+        float kj = ldk(L.kap, j);
+        if (kj < 0.f) continue;
+        float vj = v(j);
+        if (cls == 1)
+          af += (ki + kj) * vj;
+        else if (cls == 2)
+          ae += sqrtf(kj) * vj;
+        else
+          ac += sqrtf(kj) * vj;
+      }
+    }
+  }
+  return L.scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac));
+}
+
+// ── LDS tile machinery ─────────────────────────────────────────────────
+template <int TZ>
+struct Tile {
+  static constexpr int HN = HXY * (TZ + 2);
+  float v[HN];  // value (0 outside the free set)
+  float k[HN];  // kappa (< 0 outside the free set)
+  float s[HN];  // sqrt kappa (0 outside the free set)
+};
+
+template <int TZ>
+__device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& z0) {
+  int t = M.ids[blockIdx.x];
+  if (t < 0) return false;
+  int tx = t % M.ntx;
+  int r = t / M.ntx;
+  x0 = tx * TX;
+  y0 = (r % M.nty) * TY;
+  z0 = (r / M.nty) * TZ;
+  return true;
+}
+
+// Stage the halo box of a tile.  val(j, gx, gy, gz) gives the vector value at
+// an in-bounds free voxel j; other cells hold v = 0, k = -1, s = 0.  The loop
+// is fully unrolled so every thread keeps all of its loads in flight.
+template <int TZ, typename KT, typename F>
+__device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int y0, int z0,
+                                      F val) {
+  constexpr int HN = Tile<TZ>::HN;
+  constexpr int IT = (HN + BS - 1) / BS;
+  const pfsf_dims d = L.d;
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    int e = threadIdx.x + it * BS;
+    if (e < HN) {
+      int ex = e % HX;
+      int r = e / HX;
+      int ey = r % HY;
+      int ez = r / HY;
+      int gx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
+      float v = 0.f, k = -1.f;
+      if (gx >= 0 && gx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz) {
+        int64_t j = flat(gx, gy, gz, d);
+        k = ldk(L.kap, j);
+        if (k >= 0.f) v = val(j, gx, gy, gz);
+      }
+      T.v[e] = v;
+      T.k[e] = k;
+      T.s[e] = k > 0.f ? sqrtf(k) : 0.f;
+    }
+  }
+}
+
+__device__ __forceinline__ int tcell(int tx, int ty, int tz) {
+  return (tz + 1) * HXY + (ty + 1) * HX + (tx + 1);
+}
+
+// (A v)_i at LDS cell c: difference form over free neighbours + anchor mass.
+template <int TZ>
+__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
+                                            float scale) {
+  const float* v = T.v;
+  const float* k = T.k;
+  const float* s = T.s;
+  const float vc = v[c];
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  auto face = [&](int o) {
+    float kj = k[c + o];
+    float w = kj >= 0.f ? ki + kj : 0.f;
+    af += w * (vc - v[c + o]);
+  };
+  auto edge = [&](int o) { ae += s[c + o] * (vc - v[c + o]); };
+  auto corner = [&](int o) { ac += s[c + o] * (vc - v[c + o]); };
+  face(-1);
+  face(1);
+  face(-HX);
+  face(HX);
+  face(-HXY);
+  face(HXY);
+  edge(-1 - HX);
+  edge(1 - HX);
+  edge(-1 + HX);
+  edge(1 + HX);
+  edge(-1 - HXY);
+  edge(1 - HXY);
+  edge(-1 + HXY);
+  edge(1 + HXY);
+  edge(-HX - HXY);
+  edge(HX - HXY);
+  edge(-HX + HXY);
+  edge(HX + HXY);
+  corner(-1 - HX - HXY);
+  corner(1 - HX - HXY);
+  corner(-1 + HX - HXY);
+  corner(1 + HX - HXY);
+  corner(-1 - HX + HXY);
+  corner(1 - HX + HXY);
+  corner(-1 + HX + HXY);
+  corner(1 + HX + HXY);
+  return scale * (0.5f * af + sqrtf(ki) * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+}
+
+// ── FP64 reductions ────────────────────────────────────────────────────
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[BS / 64]) {
 #pragma unroll
@@ -142,56 +244,402 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[BS / 64])
   }
 }
 
-// Writes this block's partials, and returns true in the block that finishes
-// last; that block then sees every other block's partials.
+// Block partials go to partials[k * gridDim.x + blockIdx.x]; a one-block
+// finalize kernel sums them in a fixed order, so results are deterministic
+// and the kernel boundary orders the writes.
 template <int N>
-__device__ __forceinline__ bool publish_partials(const double (&v)[N], double* partials,
-                                                 unsigned* counter) {
-  __shared__ bool last;
+__device__ __forceinline__ void write_partials(double (&v)[N], double* partials,
+                                               double (*sh)[BS / 64]) {
+  block_sum<N>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < N; k++) partials[(int64_t)k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
 }
 
-// Fixed-order sum of the partials by the last block (deterministic).
-template <int N>
-__device__ __forceinline__ void reduce_partials(double (&out)[N], const double* partials,
-                                                unsigned nblk, double (*sh)[BS / 64]) {
-  double v[N];
+enum FinMode {
+  FIN_ALPHA = 0,   // p.q -> alpha = rz / pq
+  FIN_UPDATE,      // r.r -> rr, iteration count, convergence
+  FIN_UPDATE_JAC,  // r.r, r.z -> as FIN_UPDATE plus beta, rz (Jacobi M)
+  FIN_BETA,        // r.z -> beta = rz_new / rz, rz
+  FIN_BNORM,       // b.b -> bb
+  FIN_IR,          // r.r -> rr, rel
+};
+
+__global__ void __launch_bounds__(BS)
+    k_finalize(int mode, const double* __restrict__ partials, unsigned nblk, PcgScalars* sc,
+               IrScalars* ir) {
+  if (mode <= FIN_BETA && sc->done) return;
+  __shared__ double sh[2][BS / 64];
+  double v[2] = {0.0, 0.0};
+  int nv = (mode == FIN_UPDATE_JAC) ? 2 : 1;
+  for (int k = 0; k < nv; k++)
+    for (unsigned b = threadIdx.x; b < nblk; b += BS) v[k] += partials[(int64_t)k * nblk + b];
+  block_sum<2>(v, sh);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case FIN_ALPHA:
+      sc->pq = v[0];
+      if (v[0] > 0.0) {
+        sc->alpha = sc->rz / v[0];
+        sc->have_dir = 1;
+      } else {
+        sc->alpha = 0.0;
+        sc->done = 2;  // breakdown: the direction carries no energy
+      }
+      break;
+    case FIN_UPDATE:
+    case FIN_UPDATE_JAC:
+      sc->rr = v[0];
+      sc->iters += 1;
+      if (mode == FIN_UPDATE_JAC) {
+        sc->beta = (sc->rz > 0.0) ? v[1] / sc->rz : 0.0;
+        sc->rz = v[1];
+      }
+      if (v[0] <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
+      break;
+    case FIN_BETA:
+      sc->beta = (sc->have_dir && sc->rz > 0.0) ? v[0] / sc->rz : 0.0;
+      sc->rz = v[0];
+      break;
+    case FIN_BNORM:
+      ir->bb = v[0];
+      break;
+    case FIN_IR:
+      ir->rr = v[0];
+      ir->rel = ir->bb > 0.0 ? sqrt(v[0] / ir->bb) : sqrt(v[0]);
+      break;
+  }
+}
+
+// ── PCG + V-cycle kernels ──────────────────────────────────────────────
+// p_out = z + beta p_in (z alone for the first direction); q = A p_out; p.q.
+template <int TZ, typename KT>
+__global__ void __launch_bounds__(BS)
+    k_spmv_dir(Lv<KT> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
+               float* __restrict__ pout, float* __restrict__ q, const PcgScalars* sc,
+               double* partials) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  int x0, y0, z0;
+  if (tile_origin<TZ>(M, x0, y0, z0)) {
+    const bool first = !sc->have_dir;
+    const float beta = (float)sc->beta;
+    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) {
+      float zj = z[j];
+      return first ? zj : zj + beta * pin[j];
+    });
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const pfsf_dims d = L.d;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < d.lx && gy < d.ly) {
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    double a = 0.0;
-    for (unsigned b = threadIdx.x; b < nblk; b += BS) a += partials[(int64_t)k * nblk + b];
-    v[k] = a;
+      for (int tz = 0; tz < TZ; tz++) {
+        int gz = z0 + tz;
+        if (gz < d.lz) {
+          int64_t i = flat(gx, gy, gz, d);
+          int c = tcell(tx, ty, tz);
+          float pv = T.v[c], qi = 0.f;
+          float ki = T.k[c];
+          if (ki >= 0.f) {
+            qi = tile_apply<TZ>(T, c, ki, L.am[i], L.scale);
+            acc[0] += (double)pv * qi;
+          }
+          pout[i] = pv;
+          q[i] = qi;
+        }
+      }
+    }
+  }
+  write_partials<1>(acc, partials, sh);
+}
+
+// Sum over each 2x2x2 group of children held by one tile, from per-thread
+// values res[tz].  Lanes tx^1 and ty^1 (= lane^32) hold the x and y
+// siblings.  Writes rc for every in-bounds coarse cell.
+template <int TZ>
+__device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, int y0, int z0,
+                                              int tx, int ty, pfsf_dims c, float* rc) {
+#pragma unroll
+  for (int k = 0; k < TZ / 2; k++) {
+    float a = res[2 * k] + res[2 * k + 1];
+    a = a + __shfl_xor(a, 1, 64);
+    a = a + __shfl_xor(a, 32, 64);
+    int X = (x0 + tx) >> 1, Y = (y0 + ty) >> 1, Z = (z0 >> 1) + k;
+    if (!(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly && Z < c.lz) rc[flat(X, Y, Z, c)] = a;
+  }
+}
+
+// Down-sweep from level l: the pre-smoothed iterate x = omega D^-1 r is formed
+// on the fly, and the coarse right-hand side is rc = P^T (r - A x).
+// With UPDATE (finest level inside an iteration): first x64 += alpha p and
+// r_out = r_in - alpha q, then the same on r_out, plus r.r for convergence.
+template <int TZ, typename KT, bool UPDATE>
+__global__ void __launch_bounds__(BS)
+    k_down(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ rin, float* __restrict__ rout,
+           const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
+           float* __restrict__ rc, const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  int x0, y0, z0;
+  const pfsf_dims d = L.d;
+  const float om = L.omega;
+  if (tile_origin<TZ>(M, x0, y0, z0)) {
+    const float alpha = UPDATE ? (float)sc->alpha : 0.f;
+    const double alpha64 = UPDATE ? sc->alpha : 0.0;
+    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) {
+      float rj = UPDATE ? rin[j] - alpha * q[j] : rin[j];
+      return om * rj / L.dg[j];
+    });
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    float res[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      res[tz] = 0.f;
+      int gz = z0 + tz;
+      int cc = tcell(tx, ty, tz);
+      float ki = T.k[cc];
+      if (gx < d.lx && gy < d.ly && gz < d.lz && ki >= 0.f) {
+        int64_t i = flat(gx, gy, gz, d);
+        float ri = rin[i];
+        if (UPDATE) {
+          ri = ri - alpha * q[i];
+          x[i] = x[i] + alpha64 * (double)p[i];
+          rout[i] = ri;
+          acc[0] += (double)ri * ri;
+        }
+        res[tz] = ri - tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
+      }
+    }
+    restrict_tile<TZ>(res, x0, y0, z0, tx, ty, c, rc);
+  }
+  if (UPDATE) write_partials<1>(acc, partials, sh);
+}
+
+// Up-sweep at level l: x~ = omega D^-1 r + P e_coarse on the free set, then
+// one damped-Jacobi post-smoothing step out = x~ + omega D^-1 (r - A x~).
+// At the finest level it also accumulates r.z.
+template <int TZ, typename KT, bool FINEST>
+__global__ void __launch_bounds__(BS)
+    k_up(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ r, const float* __restrict__ ec,
+         float* __restrict__ out, const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  int x0, y0, z0;
+  const pfsf_dims d = L.d;
+  const float om = L.omega;
+  if (tile_origin<TZ>(M, x0, y0, z0)) {
+    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int gx, int gy, int gz) {
+      return om * r[j] / L.dg[j] + ec[flat(gx >> 1, gy >> 1, gz >> 1, c)];
+    });
+    __syncthreads();
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx < d.lx && gy < d.ly) {
+#pragma unroll
+      for (int tz = 0; tz < TZ; tz++) {
+        int gz = z0 + tz;
+        if (gz < d.lz) {
+          int64_t i = flat(gx, gy, gz, d);
+          int cc = tcell(tx, ty, tz);
+          float ki = T.k[cc], zi = 0.f;
+          if (ki >= 0.f) {
+            float xi = T.v[cc];
+            float ri = r[i];
+            float ax = tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
+            zi = xi + om * (ri - ax) / L.dg[i];
+            if (FINEST) acc[0] += (double)ri * zi;
+          }
+          out[i] = zi;
+        }
+      }
+    }
+  }
+  if (FINEST) write_partials<1>(acc, partials, sh);
+}
+
+// Jacobi-preconditioned variant of the update (single-level solve):
+// x64 += alpha p; r_out = r_in - alpha q; z = D^-1 r_out; r.r and r.z.
+__global__ void __launch_bounds__(BS)
+    k_update_jacobi(int64_t n, const float* __restrict__ dg, double* __restrict__ x,
+                    const float* __restrict__ rin, float* __restrict__ rout,
+                    const float* __restrict__ p, const float* __restrict__ q, float* __restrict__ z,
+                    const PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  __shared__ double sh[2][BS / 64];
+  double acc[2] = {0.0, 0.0};
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  float alpha = (float)sc->alpha;
+  if (i < n) {
+    float D = dg[i];
+    float ri = rin[i] - alpha * q[i];
+    x[i] = x[i] + sc->alpha * (double)p[i];
+    rout[i] = ri;
+    float zi = D > 0.f ? ri / D : 0.f;
+    z[i] = zi;
+    acc[0] = (double)ri * ri;
+    acc[1] = (double)ri * zi;
+  }
+  write_partials<2>(acc, partials, sh);
+}
+
+// z = D^-1 r and r.z (Jacobi preconditioner at the start of a solve pass).
+__global__ void __launch_bounds__(BS)
+    k_jacobi_apply(int64_t n, const float* __restrict__ r, const float* __restrict__ dg,
+                   float* __restrict__ z, double* partials) {
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) {
+    float D = dg[i];
+    float zi = D > 0.f ? r[i] / D : 0.f;
+    z[i] = zi;
+    acc[0] = (double)r[i] * zi;
+  }
+  write_partials<1>(acc, partials, sh);
+}
+
+// One Chebyshev step on the coarsest level (from x = 0), tiled:
+//   first:  d = c2 D^-1 r, x = d
+//   else :  d = c1 d + c2 D^-1 (r - A x_in), x_out = x_in + d
+template <int TZ, typename KT>
+__global__ void __launch_bounds__(BS)
+    k_cheb_tiled(Lv<KT> L, TL M, const float* __restrict__ r, const float* __restrict__ xin,
+                 float* __restrict__ xout, float* __restrict__ dv, float c1, float c2, int first,
+                 const PcgScalars* sc) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  int x0, y0, z0;
+  if (!tile_origin<TZ>(M, x0, y0, z0)) return;
+  const pfsf_dims d = L.d;
+  stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) { return first ? 0.f : xin[j]; });
+  __syncthreads();
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gx = x0 + tx, gy = y0 + ty;
+  if (gx >= d.lx || gy >= d.ly) return;
+  for (int tz = 0; tz < TZ; tz++) {
+    int gz = z0 + tz;
+    if (gz >= d.lz) break;
+    int64_t i = flat(gx, gy, gz, d);
+    int cc = tcell(tx, ty, tz);
+    float ki = T.k[cc];
+    if (ki < 0.f) {
+      xout[i] = 0.f;
+      if (first) dv[i] = 0.f;
+      continue;
+    }
+    float D = L.dg[i];
+    if (first) {
+      float di = c2 * r[i] / D;
+      dv[i] = di;
+      xout[i] = di;
+    } else {
+      float xi = T.v[cc];
+      float ax = tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
+      float di = c1 * dv[i] + c2 * (r[i] - ax) / D;
+      dv[i] = di;
+      xout[i] = xi + di;
+    }
+  }
+}
+
+// Whole Chebyshev solve of the coarsest level held in one work-group's LDS
+// (n <= kMaxLdsCoarse cells): kappa, sqrt kappa, diagonal, rhs, iterate and
+// direction.
+constexpr int kMaxLdsCoarse = 2048;
+
+template <typename KT>
+__global__ void __launch_bounds__(1024)
+    k_cheb_lds(Lv<KT> L, const float* __restrict__ r, float* __restrict__ out,
+               const float* __restrict__ coef, int steps, const PcgScalars* sc) {
+  if (sc->done) return;
+  __shared__ float sk[kMaxLdsCoarse], ss[kMaxLdsCoarse], sd[kMaxLdsCoarse];
+  __shared__ float sr[kMaxLdsCoarse], sx[2][kMaxLdsCoarse], sdv[kMaxLdsCoarse];
+  const pfsf_dims d = L.d;
+  const int n = d.lx * d.ly * d.lz;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    float k = ldk(L.kap, i);
+    float D = L.dg[i];
+    sk[i] = k;
+    ss[i] = k > 0.f ? sqrtf(k) : 0.f;
+    sd[i] = D;
+    sr[i] = r[i];
+    float di = k >= 0.f ? coef[1] * r[i] / D : 0.f;
+    sdv[i] = di;
+    sx[0][i] = di;
   }
   __syncthreads();
-  block_sum<N>(v, sh);
-#pragma unroll
-  for (int k = 0; k < N; k++) out[k] = v[k];
+  int cur = 0;
+  for (int st = 1; st < steps; st++) {
+    const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
+    const float* xin = sx[cur];
+    float* xo = sx[cur ^ 1];
+    for (int i = threadIdx.x; i < n; i += 1024) {
+      const float ki = sk[i];
+      if (ki < 0.f) {
+        xo[i] = 0.f;
+        continue;
+      }
+      int x, y, z;
+      unflat(i, d, x, y, z);
+      const float xi = xin[i];
+      float af = 0.f, ae = 0.f, ac = 0.f;
+      for (int dz = -1; dz <= 1; dz++) {
+        int nz = z + dz;
+        if (nz < 0 || nz >= d.lz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+          int ny = y + dy;
+          if (ny < 0 || ny >= d.ly) continue;
+          for (int dx = -1; dx <= 1; dx++) {
+            int nx = x + dx;
+            if (nx < 0 || nx >= d.lx) continue;
+            int cls = (dx != 0) + (dy != 0) + (dz != 0);
+            if (cls == 0) continue;
+            int j = nx + d.lx * (ny + d.ly * nz);
+            float kj = sk[j];
+            if (kj < 0.f) continue;
+            float dv = xi - xin[j];
+            if (cls == 1)
+              af += (ki + kj) * dv;
+            else if (cls == 2)
+              ae += ss[j] * dv;
+            else
+              ac += ss[j] * dv;
+          }
+        }
+      }
+      float ax = L.scale * (0.5f * af + ss[i] * (kEdgeP * ae + kCornerP * ac)) + L.am[i] * xi;
+      float di = c1 * sdv[i] + c2 * (sr[i] - ax) / sd[i];
+      sdv[i] = di;
+      xo[i] = xi + di;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int i = threadIdx.x; i < n; i += 1024) out[i] = sx[cur][i];
 }
 
 // ── setup kernels ──────────────────────────────────────────────────────
-__device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
-
-// Fine level: free mask, kappa, sqrt kappa, diagonal and anchor mass.  The
-// diagonal is summed in FP64 from the FP32 coupling rule the SpMV uses.
+// Fine level: free mask, kappa (-1 outside the free set) and anchor mass
+// (FP32 coupling rule summed in FP64).
 __global__ void __launch_bounds__(BS)
     k_fine_setup(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
                  const uint8_t* __restrict__ reach, float* __restrict__ kap0,
-                 float* __restrict__ sq0, float* __restrict__ am0, uint8_t* __restrict__ free0) {
+                 float* __restrict__ am0, uint8_t* __restrict__ free0) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= (int64_t)d.lx * d.ly * d.lz) return;
   bool fr = reach[i] != 0;  // solid and connected to an anchor
-  float k = fr ? kapf(kappa[i]) : 0.0f;
+  float k = fr ? kapf(kappa[i]) : -1.0f;
   kap0[i] = k;
-  sq0[i] = sqrtf(k);
   free0[i] = fr ? 1 : 0;
   if (!fr) {
     am0[i] = 0.0f;
@@ -217,13 +665,13 @@ __global__ void __launch_bounds__(BS)
   am0[i] = (float)m;
 }
 
-// Coarse cell: free if any child is free; kappa = sum(free children kappa)/8;
-// anchor mass = sum of children's anchor mass.
+// Coarse cell: free if any child is free; kappa = sum(free children kappa)/8
+// (-1 when none is free); anchor mass = sum of children's anchor mass.
 template <typename KF, typename KC>
 __global__ void __launch_bounds__(BS)
     k_coarsen(pfsf_dims f, const KF* __restrict__ kf, const float* __restrict__ amf,
               const uint8_t* __restrict__ freef, pfsf_dims c, KC* __restrict__ kc,
-              KC* __restrict__ sqc, float* __restrict__ amc, uint8_t* __restrict__ freec) {
+              float* __restrict__ amc, uint8_t* __restrict__ freec) {
   int64_t ci = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (ci >= (int64_t)c.lx * c.ly * c.lz) return;
   int X, Y, Z;
@@ -241,9 +689,7 @@ __global__ void __launch_bounds__(BS)
         ks += ldk(kf, i);
         ms += amf[i];
       }
-  float k = ks * 0.125f;
-  stk(kc, ci, k);
-  stk(sqc, ci, sqrtf(k));
+  stk(kc, ci, nf > 0 ? ks * 0.125f : -1.0f);
   amc[ci] = ms;
   freec[ci] = nf > 0 ? 1 : 0;
 }
@@ -251,349 +697,52 @@ __global__ void __launch_bounds__(BS)
 // Diagonal of a level: scale * sum over free neighbours of w + anchor mass.
 template <typename KT>
 __global__ void __launch_bounds__(BS)
-    k_level_diag(Lv<KT> L, const float* __restrict__ am, const uint8_t* __restrict__ fr,
-                 float* __restrict__ dg, float* __restrict__ dinv) {
+    k_level_diag(Lv<KT> L, const uint8_t* __restrict__ fr, float* __restrict__ dg) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   pfsf_dims d = L.d;
   if (i >= (int64_t)d.lx * d.ly * d.lz) return;
   if (!fr[i]) {
     dg[i] = 0.f;
-    dinv[i] = 0.f;
     return;
   }
   int x, y, z;
   unflat(i, d, x, y, z);
-  float ki = ldk(L.kap, i), si = ldk(L.sq, i);
-  // the neighbour "value" is the free indicator: sum_j w_ij [j free]
-  float s = offsum(L, x, y, z, ki, si, [&](int64_t j) { return fr[j] ? 1.0f : 0.0f; });
-  float D = s + am[i];
-  if (!(D > 0.f)) {
-    dg[i] = 0.f;
-    dinv[i] = 0.f;
-  } else {
-    dg[i] = D;
-    dinv[i] = 1.0f / D;
-  }
+  float s = offsum(L, x, y, z, ldk(L.kap, i), [&](int64_t) { return 1.0f; });
+  float D = s + L.am[i];
+  dg[i] = D > 0.f ? D : 0.f;
 }
 
-// Power iteration step on D^-1 A: w = D^-1 A v, partial sums of v.Av, v.Dv, w.Dw.
+// The free set follows the diagonal: cells whose couplings all vanish leave
+// the solve (mask cleared, kappa flagged negative).
 template <typename KT>
 __global__ void __launch_bounds__(BS)
-    k_power(Lv<KT> L, const float* __restrict__ v, float* __restrict__ w, double* partials) {
-  __shared__ double sh[3][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  pfsf_dims d = L.d;
-  double acc[3] = {0, 0, 0};
-  if (i < (int64_t)d.lx * d.ly * d.lz) {
-    float D = L.dg[i];
-    float wi = 0.f;
-    if (D > 0.f) {
-      int x, y, z;
-      unflat(i, d, x, y, z);
-      float vi = v[i];
-      float av = D * vi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i),
-                                 [&](int64_t j) { return v[j]; });
-      wi = av * L.dinv[i];
-      acc[0] = (double)vi * av;
-      acc[1] = (double)vi * D * vi;
-      acc[2] = (double)wi * D * wi;
-    }
-    w[i] = wi;
-  }
-  block_sum<3>(acc, sh);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 3; k++) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
-}
-
-__global__ void __launch_bounds__(BS)
-    k_power_init(int64_t n, const float* __restrict__ dg, float* __restrict__ v) {
+    k_sync_free(int64_t n, const float* __restrict__ dg, uint8_t* __restrict__ fr, KT* kap) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  unsigned h = (unsigned)(i * 2654435761u) ^ 0x9E3779B9u;
-  h ^= h >> 15;
-  h *= 2246822519u;
-  h ^= h >> 13;
-  v[i] = dg[i] > 0.f ? ((float)(h & 0xFFFF) / 65535.0f + 0.5f) : 0.f;
+  bool f = dg[i] > 0.f;
+  fr[i] = f ? 1 : 0;
+  if (!f) stk(kap, i, -1.0f);
 }
 
-__global__ void __launch_bounds__(BS) k_scale(int64_t n, float* __restrict__ v, float a) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) v[i] *= a;
-}
-
-// ── PCG kernels (level 0, FP32 vectors, FP64 dots) ─────────────────────
-// q = A p, pq = p.q; last block: alpha = rz / pq.
-template <typename KT>
+// Tiles that hold at least one free voxel.
 __global__ void __launch_bounds__(BS)
-    k_pcg_spmv(Lv<KT> L, const float* __restrict__ p, float* __restrict__ q, PcgScalars* sc,
-               double* partials) {
-  if (sc->done) return;
-  __shared__ double sh[1][BS / 64];
+    k_tile_flags(pfsf_dims d, const uint8_t* __restrict__ fr, int ntx, int nty, int tz,
+                 int* __restrict__ flags) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  pfsf_dims d = L.d;
-  double acc[1] = {0.0};
-  if (i < (int64_t)d.lx * d.ly * d.lz) {
-    float D = L.dg[i], qi = 0.f;
-    if (D > 0.f) {
-      int x, y, z;
-      unflat(i, d, x, y, z);
-      float pi = p[i];
-      qi = D * pi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i),
-                           [&](int64_t j) { return p[j]; });
-      acc[0] = (double)pi * qi;
-    }
-    q[i] = qi;
-  }
-  block_sum<1>(acc, sh);
-  if (publish_partials<1>(acc, partials, &sc->counter[0])) {
-    double t[1];
-    reduce_partials<1>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      sc->pq = t[0];
-      sc->alpha = (t[0] > 0.0) ? sc->rz / t[0] : 0.0;
-      if (!(t[0] > 0.0)) sc->done = 2;  // breakdown: direction has no energy
-      sc->counter[0] = 0;
-    }
-  }
-}
-
-// x += alpha p; r -= alpha q; rr = r.r; with Jacobi M: z = D^-1 r, rz = r.z and
-// beta = rz_new / rz.  Convergence when rr <= stop2 (absolute, set by host).
-__global__ void __launch_bounds__(BS)
-    k_pcg_update(int64_t n, float* __restrict__ x, float* __restrict__ r,
-                 const float* __restrict__ p, const float* __restrict__ q,
-                 const float* __restrict__ dinv, float* __restrict__ z, int jacobi,
-                 PcgScalars* sc, double* partials) {
-  if (sc->done) return;
-  __shared__ double sh[2][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  float alpha = (float)sc->alpha;
-  double acc[2] = {0.0, 0.0};
-  if (i < n) {
-    float xi = x[i] + alpha * p[i];
-    float ri = r[i] - alpha * q[i];
-    x[i] = xi;
-    r[i] = ri;
-    acc[0] = (double)ri * ri;
-    if (jacobi) {
-      float zi = ri * dinv[i];
-      z[i] = zi;
-      acc[1] = (double)ri * zi;
-    }
-  }
-  block_sum<2>(acc, sh);
-  if (publish_partials<2>(acc, partials, &sc->counter[1])) {
-    double t[2];
-    reduce_partials<2>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      sc->rr = t[0];
-      sc->iters += 1;
-      if (jacobi) {
-        sc->beta = (sc->rz > 0.0) ? t[1] / sc->rz : 0.0;
-        sc->rz = t[1];
-      }
-      if (t[0] <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
-      sc->counter[1] = 0;
-    }
-  }
-}
-
-// p = z + beta p (beta = 0 on the first direction).
-__global__ void __launch_bounds__(BS)
-    k_pcg_dir(int64_t n, float* __restrict__ p, const float* __restrict__ z, PcgScalars* sc) {
-  if (sc->done) return;
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  float beta = (float)sc->beta;
-  if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-// z = D^-1 r (Jacobi apply at init), rz = r.z
-__global__ void __launch_bounds__(BS)
-    k_jacobi_apply(int64_t n, const float* __restrict__ r, const float* __restrict__ dinv,
-                   float* __restrict__ z, PcgScalars* sc, double* partials) {
-  __shared__ double sh[1][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double acc[1] = {0.0};
-  if (i < n) {
-    float zi = r[i] * dinv[i];
-    z[i] = zi;
-    acc[0] = (double)r[i] * zi;
-  }
-  block_sum<1>(acc, sh);
-  if (publish_partials<1>(acc, partials, &sc->counter[2])) {
-    double t[1];
-    reduce_partials<1>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      sc->rz = t[0];
-      sc->beta = 0.0;
-      sc->counter[2] = 0;
-    }
-  }
-}
-
-// ── V-cycle kernels ────────────────────────────────────────────────────
-// Down-sweep at level l, one thread per coarse cell I: each free child i gets
-// the pre-smoothed value x_i = omega D_i^-1 r_i (stored), and the coarse
-// right-hand side is rc_I = sum_children (r_i - (A x)_i), where x_j at the
-// neighbours is recomputed from r_j the same way.
-template <typename KT, typename KC>
-__global__ void __launch_bounds__(BS)
-    k_mg_down(Lv<KT> L, pfsf_dims c, const float* __restrict__ r, float* __restrict__ xs,
-              float* __restrict__ rc, const PcgScalars* sc) {
-  if (sc->done) return;
-  int64_t ci = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (ci >= (int64_t)c.lx * c.ly * c.lz) return;
-  int X, Y, Z;
-  unflat(ci, c, X, Y, Z);
-  const pfsf_dims d = L.d;
-  const float om = L.omega;
-  float acc = 0.f;
-  for (int dz = 0; dz < 2; dz++)
-    for (int dy = 0; dy < 2; dy++)
-      for (int dx = 0; dx < 2; dx++) {
-        int x = 2 * X + dx, y = 2 * Y + dy, z = 2 * Z + dz;
-        if (x >= d.lx || y >= d.ly || z >= d.lz) continue;
-        int64_t i = flat(x, y, z, d);
-        float D = L.dg[i];
-        if (!(D > 0.f)) {
-          xs[i] = 0.f;
-          continue;
-        }
-        float ri = r[i];
-        float xi = om * ri * L.dinv[i];
-        xs[i] = xi;
-        float ax = D * xi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i),
-                                   [&](int64_t j) { return om * r[j] * L.dinv[j]; });
-        acc += ri - ax;
-      }
-  rc[ci] = acc;
-}
-
-// Up-sweep at level l, one thread per fine voxel: x~ = xs + P e_coarse, then
-// one damped-Jacobi post-smoothing step z = x~ + omega D^-1 (r - A x~).  At the
-// finest level the block also accumulates r.z and the last block sets beta.
-template <typename KT>
-__global__ void __launch_bounds__(BS)
-    k_mg_up(Lv<KT> L, pfsf_dims c, const float* __restrict__ r, const float* __restrict__ xs,
-            const float* __restrict__ ec, float* __restrict__ out, PcgScalars* sc,
-            double* partials, int finest) {
-  if (sc->done) return;
-  __shared__ double sh[1][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  const pfsf_dims d = L.d;
-  double acc[1] = {0.0};
-  if (i < (int64_t)d.lx * d.ly * d.lz) {
-    float D = L.dg[i], zi = 0.f;
-    if (D > 0.f) {
-      int x, y, z;
-      unflat(i, d, x, y, z);
-      // prolonged iterate at a neighbour; 0 outside the free set
-      auto xt = [&](int64_t j) {
-        if (!(L.dg[j] > 0.f)) return 0.f;
-        int jx, jy, jz;
-        unflat(j, d, jx, jy, jz);
-        return xs[j] + ec[flat(jx >> 1, jy >> 1, jz >> 1, c)];
-      };
-      float xi = xs[i] + ec[flat(x >> 1, y >> 1, z >> 1, c)];
-      float ax = D * xi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i), xt);
-      float ri = r[i];
-      zi = xi + L.omega * (ri - ax) * L.dinv[i];
-      acc[0] = (double)ri * zi;
-    }
-    out[i] = zi;
-  }
-  if (!finest) return;
-  block_sum<1>(acc, sh);
-  if (publish_partials<1>(acc, partials, &sc->counter[3])) {
-    double t[1];
-    reduce_partials<1>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      sc->beta = (sc->rz > 0.0 && sc->have_dir) ? t[0] / sc->rz : 0.0;
-      sc->rz = t[0];
-      sc->counter[3] = 0;
-    }
-  }
-}
-
-// One Chebyshev step on the coarsest level (from x = 0):
-//   first:  d = c2 D^-1 r, x = d
-//   else :  d = c1 d + c2 D^-1 (r - A x_in), x_out = x_in + d
-template <typename KT>
-__global__ void __launch_bounds__(BS)
-    k_cheb(Lv<KT> L, const float* __restrict__ r, const float* __restrict__ xin,
-           float* __restrict__ xout, float* __restrict__ dv, float c1, float c2, int first,
-           const PcgScalars* sc) {
-  if (sc->done) return;
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  const pfsf_dims d = L.d;
   if (i >= (int64_t)d.lx * d.ly * d.lz) return;
-  float D = L.dg[i];
-  if (!(D > 0.f)) {
-    xout[i] = 0.f;
-    if (first) dv[i] = 0.f;
-    return;
-  }
-  if (first) {
-    float di = c2 * r[i] * L.dinv[i];
-    dv[i] = di;
-    xout[i] = di;
-    return;
-  }
+  if (!fr[i]) return;
   int x, y, z;
   unflat(i, d, x, y, z);
-  float xi = xin[i];
-  float ax = D * xi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i),
-                             [&](int64_t j) { return xin[j]; });
-  float di = c1 * dv[i] + c2 * (r[i] - ax) * L.dinv[i];
-  dv[i] = di;
-  xout[i] = xi + di;
+  int t = x / TX + ntx * (y / TY + nty * (z / tz));
+  flags[t] = 1;  // benign race: every writer stores 1
 }
 
-// Whole Chebyshev solve of a small coarsest level in one work-group.
-template <typename KT>
-__global__ void __launch_bounds__(1024)
-    k_cheb_block(Lv<KT> L, const float* __restrict__ r, float* xa, float* xb,
-                 float* __restrict__ dv, const float* __restrict__ coef, int steps,
-                 const PcgScalars* sc) {
-  if (sc->done) return;
-  const pfsf_dims d = L.d;
-  const int64_t n = (int64_t)d.lx * d.ly * d.lz;
-  float* xin = xa;
-  float* xout = xb;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) {
-    float di = coef[1] * r[i] * L.dinv[i];
-    dv[i] = di;
-    xin[i] = di;
-  }
-  __syncthreads();
-  for (int s = 1; s < steps; s++) {
-    float c1 = coef[2 * s], c2 = coef[2 * s + 1];
-    for (int64_t i = threadIdx.x; i < n; i += 1024) {
-      float D = L.dg[i];
-      if (!(D > 0.f)) {
-        xout[i] = 0.f;
-        continue;
-      }
-      int x, y, z;
-      unflat(i, d, x, y, z);
-      float xi = xin[i];
-      float ax = D * xi - offsum(L, x, y, z, ldk(L.kap, i), ldk(L.sq, i),
-                                 [&](int64_t j) { return xin[j]; });
-      float di = c1 * dv[i] + c2 * (r[i] - ax) * L.dinv[i];
-      dv[i] = di;
-      xout[i] = xi + di;
-    }
-    __syncthreads();
-    float* t = xin;
-    xin = xout;
-    xout = t;
-  }
-  // result must land in xa
-  if (xin != xa)
-    for (int64_t i = threadIdx.x; i < n; i += 1024) xa[i] = xin[i];
+__global__ void __launch_bounds__(BS) k_fill(int64_t n, float* __restrict__ v, float a) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) v[i] = a;
 }
 
-// ── FP64 iterative refinement ──────────────────────────────────────────
+// ── FP64 residual of the iterate ───────────────────────────────────────
 __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   if (cls == 1) {
     float s = ki + kj;
@@ -604,24 +753,22 @@ __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
 }
 
-// xn = xo + e (free voxels); r = b - A xn in FP64 with the exact coupling
-// rule; r32 = (float) r; partial ||r||^2.  Last block: rel = ||r|| / ||b||.
+// True residual of the FP64 iterate: r = b - A x with the exact coupling rule
+// (FP64), written as r32 = (float) r, plus partial ||r||^2.
 __global__ void __launch_bounds__(BS)
-    k_ir_residual(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
-                  const uint8_t* __restrict__ fr, const float* __restrict__ b,
-                  const double* __restrict__ xo, const float* __restrict__ e,
-                  double* __restrict__ xn, float* __restrict__ r32, IrScalars* ir,
-                  double* partials) {
+    k_true_residual(pfsf_dims d, const uint8_t* __restrict__ type,
+                    const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
+                    const float* __restrict__ b, const double* __restrict__ x,
+                    float* __restrict__ r32, double* partials) {
   __shared__ double sh[1][BS / 64];
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   double acc[1] = {0.0};
   if (i < (int64_t)d.lx * d.ly * d.lz) {
     float ri32 = 0.f;
-    double xi = 0.0;
     if (fr[i]) {
-      xi = xo[i] + (e ? (double)e[i] : 0.0);
-      int x, y, z;
-      unflat(i, d, x, y, z);
+      double xi = x[i];
+      int px, py, pz;
+      unflat(i, d, px, py, pz);
       float ki = kapf(kappa[i]);
       double ax = 0.0;
       for (int dz = -1; dz <= 1; dz++)
@@ -629,50 +776,38 @@ __global__ void __launch_bounds__(BS)
           for (int dx = -1; dx <= 1; dx++) {
             int cls = (dx != 0) + (dy != 0) + (dz != 0);
             if (cls == 0) continue;
-            int nx = x + dx, ny = y + dy, nz = z + dz;
+            int nx = px + dx, ny = py + dy, nz = pz + dz;
             if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
             int64_t j = flat(nx, ny, nz, d);
-            uint8_t tj = type[j];
-            if (tj == 0) continue;
-            double xj = fr[j] ? xo[j] + (e ? (double)e[j] : 0.0) : 0.0;
+            if (type[j] == 0) continue;
+            double xj = fr[j] ? x[j] : 0.0;
             ax += couple_f64(cls, ki, kapf(kappa[j])) * (xi - xj);
           }
       double ri = (double)b[i] - ax;
       ri32 = (float)ri;
       acc[0] = ri * ri;
     }
-    xn[i] = xi;
     r32[i] = ri32;
   }
-  block_sum<1>(acc, sh);
-  if (publish_partials<1>(acc, partials, &ir->counter)) {
-    double t[1];
-    reduce_partials<1>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ir->rr = t[0];
-      ir->rel = ir->bb > 0.0 ? sqrt(t[0] / ir->bb) : sqrt(t[0]);
-      ir->counter = 0;
-    }
-  }
+  write_partials<1>(acc, partials, sh);
+}
+
+// Iterate outside the free set is zero (the caller's guess may not be).
+__global__ void __launch_bounds__(BS)
+    k_mask64(int64_t n, const uint8_t* __restrict__ fr, double* __restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n && !fr[i]) x[i] = 0.0;
 }
 
 // ||b||^2 over the free set.
 __global__ void __launch_bounds__(BS)
-    k_bnorm(int64_t n, const uint8_t* __restrict__ fr, const float* __restrict__ b, IrScalars* ir,
+    k_bnorm(int64_t n, const uint8_t* __restrict__ fr, const float* __restrict__ b,
             double* partials) {
   __shared__ double sh[1][BS / 64];
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   double acc[1] = {0.0};
   if (i < n && fr[i]) acc[0] = (double)b[i] * (double)b[i];
-  block_sum<1>(acc, sh);
-  if (publish_partials<1>(acc, partials, &ir->counter)) {
-    double t[1];
-    reduce_partials<1>(t, partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      ir->bb = t[0];
-      ir->counter = 0;
-    }
-  }
+  write_partials<1>(acc, partials, sh);
 }
 
 __global__ void __launch_bounds__(BS)
@@ -681,26 +816,31 @@ __global__ void __launch_bounds__(BS)
   if (i < n) y[i] = (float)x[i];
 }
 
-// Prepare the inner solve: x32 = 0, and stop threshold from rr0.
-__global__ void k_inner_begin(PcgScalars* sc, const IrScalars* ir, double inner_rel,
-                              int max_iters) {
+__global__ void __launch_bounds__(BS) k_zero64(int64_t n, double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) v[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_copy64(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// (Re)start the PCG recursion from the current true residual: no search
+// direction yet; the recursion stops once its residual norm falls below
+// stop_rel * ||r_true|| (the host then checks the true residual).
+__global__ void k_pcg_begin(PcgScalars* sc, const IrScalars* ir, double stop_rel, int max_iters,
+                            int reset_count) {
   sc->done = 0;
-  sc->iters = 0;
+  if (reset_count) sc->iters = 0;
   sc->have_dir = 0;
   sc->rz = 0.0;
   sc->beta = 0.0;
   sc->alpha = 0.0;
   sc->rr = ir->rr;
-  sc->stop2 = ir->rr * inner_rel * inner_rel;
+  sc->stop2 = ir->rr * stop_rel * stop_rel;
   sc->max_iters = max_iters;
-  for (int k = 0; k < 4; k++) sc->counter[k] = 0;
-}
-
-// Free mask follows the diagonal: voxels whose couplings all vanish leave the solve.
-__global__ void __launch_bounds__(BS)
-    k_sync_free(int64_t n, const float* __restrict__ dg, uint8_t* __restrict__ fr) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) fr[i] = dg[i] > 0.f ? 1 : 0;
 }
 
 // Profiling: keep every kernel live (no convergence stop) for timed replays.
@@ -709,11 +849,6 @@ __global__ void k_profile_begin(PcgScalars* sc) {
   sc->iters = 0;
   sc->stop2 = -1.0;
   sc->max_iters = 0x7fffffff;
-  for (int k = 0; k < 4; k++) sc->counter[k] = 0;
-}
-
-__global__ void k_mark_dir(PcgScalars* sc) {
-  if (!sc->done) sc->have_dir = 1;
 }
 
 }  // namespace
@@ -730,8 +865,7 @@ static inline bool chk(hipError_t e, hipError_t& keep) {
 template <typename T>
 static T* dalloc(int64_t n, std::vector<void*>& owned, hipError_t& err) {
   void* p = nullptr;
-  if (chk(hipMalloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(T)), err))
-    return nullptr;
+  if (chk(hipMalloc(&p, (size_t)std::max<int64_t>(n, 1) * sizeof(T)), err)) return nullptr;
   owned.push_back(p);
   return (T*)p;
 }
@@ -739,6 +873,11 @@ static T* dalloc(int64_t n, std::vector<void*>& owned, hipError_t& err) {
 MgSolver::~MgSolver() { release(); }
 
 void MgSolver::release() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : owned_) (void)hipFree(p);
   owned_.clear();
   if (sc_host_) (void)hipHostFree(sc_host_);
@@ -746,6 +885,8 @@ void MgSolver::release() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   ev0_ = ev1_ = nullptr;
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
   levels_.clear();
 }
 
@@ -754,26 +895,43 @@ static Lv<KT> view(const MgLevel& l) {
   Lv<KT> v;
   v.d = l.d;
   v.kap = (const KT*)l.kap;
-  v.sq = (const KT*)l.sq;
   v.dg = l.dg;
-  v.dinv = l.dinv;
+  v.am = l.am;
   v.scale = l.scale;
   v.omega = l.omega;
   return v;
 }
 
+static TL tl(const MgLevel& l) { return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids}; }
+
+static void fill0(float* p, int64_t n, hipStream_t s) {
+  k_fill<<<blocks_for(n, BS), BS, 0, s>>>(n, p, 0.f);
+}
+
+// Tile depth per level: the deepest of {8, 4} that still gives about eight
+// tiles per CU, else 2 (small levels).
+static int pick_tz(pfsf_dims d) {
+  for (int tz : {8, 4}) {
+    int64_t nt = (int64_t)((d.lx + TX - 1) / TX) * ((d.ly + TY - 1) / TY) * ((d.lz + tz - 1) / tz);
+    if (nt >= 2048) return tz;
+  }
+  return 2;
+}
+
 hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
   hipError_t err = hipSuccess;
   desc_ = desc;
-  stream_ = (hipStream_t)desc.stream;
   d0_ = desc.dims;
   n0_ = nvox(d0_);
   type_ = desc.type;
   kappa_ = desc.kappa;
   bf16_ = desc.precision == PFSF_PREC_BF16_COARSE;
   coarse_sweeps_ = desc.coarse_sweeps > 0 ? desc.coarse_sweeps : 12;
+  // The solver runs on a stream of its own (default priority) so that its
+  // iterations can be captured into a HIP graph; solve() orders it after the
+  // caller's stream and waits for it before returning.
+  if (chk(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), err)) return err;
 
-  // number of levels
   int want = desc.mg_levels;
   if (want <= 0) {
     want = 1;
@@ -785,29 +943,26 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
   }
   nlev_ = std::max(1, std::min(want, 10));
 
-  // workspaces
   sc_ = dalloc<PcgScalars>(1, owned_, err);
   ir_ = dalloc<IrScalars>(1, owned_, err);
-  int64_t nb0 = blocks_for(n0_, BS);
-  partials_ = dalloc<double>(4 * nb0 + 64, owned_, err);
   reach_ = dalloc<uint8_t>(n0_, owned_, err);
   flag_ = dalloc<int>(1, owned_, err);
-  x32_ = dalloc<float>(n0_, owned_, err);
-  r32_ = dalloc<float>(n0_, owned_, err);
-  p_ = dalloc<float>(n0_, owned_, err);
+  x_ = dalloc<double>(n0_, owned_, err);
+  for (int k = 0; k < 2; k++) {
+    r_[k] = dalloc<float>(n0_, owned_, err);
+    p_[k] = dalloc<float>(n0_, owned_, err);
+  }
   q_ = dalloc<float>(n0_, owned_, err);
   z_ = dalloc<float>(n0_, owned_, err);
-  xa_ = dalloc<double>(n0_, owned_, err);
-  xb_ = dalloc<double>(n0_, owned_, err);
   if (err != hipSuccess) return err;
   if (chk(hipHostMalloc((void**)&sc_host_, sizeof(PcgScalars) + sizeof(IrScalars)), err))
     return err;
   chk(hipEventCreate(&ev0_), err);
   chk(hipEventCreate(&ev1_), err);
 
-  // level storage
   levels_.resize(nlev_);
   pfsf_dims d = d0_;
+  int64_t maxblk = blocks_for(n0_, BS);
   for (int l = 0; l < nlev_; l++) {
     MgLevel& L = levels_[l];
     L.d = d;
@@ -815,9 +970,7 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
     L.bf16 = (l > 0) && bf16_;
     size_t ks = L.bf16 ? sizeof(uint16_t) : sizeof(float);
     L.kap = dalloc<uint8_t>(L.n * (int64_t)ks, owned_, err);
-    L.sq = dalloc<uint8_t>(L.n * (int64_t)ks, owned_, err);
     L.dg = dalloc<float>(L.n, owned_, err);
-    L.dinv = dalloc<float>(L.n, owned_, err);
     L.am = dalloc<float>(L.n, owned_, err);
     L.fr = dalloc<uint8_t>(L.n, owned_, err);
     L.scale = (float)(1 << l);
@@ -825,94 +978,125 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
       L.r = dalloc<float>(L.n, owned_, err);
       L.e = dalloc<float>(L.n, owned_, err);
     }
-    L.xs = dalloc<float>(L.n, owned_, err);
     if (l == nlev_ - 1 && nlev_ > 1) {
       L.xb = dalloc<float>(L.n, owned_, err);
       L.dv = dalloc<float>(L.n, owned_, err);
     }
+    L.tiles.tz = pick_tz(d);
+    L.tiles.ntx = (d.lx + TX - 1) / TX;
+    L.tiles.nty = (d.ly + TY - 1) / TY;
+    L.tiles.ntz = (d.lz + L.tiles.tz - 1) / L.tiles.tz;
+    int64_t nt = (int64_t)L.tiles.ntx * L.tiles.nty * L.tiles.ntz;
+    L.tiles.ids = dalloc<int>(nt + 8, owned_, err);
+    maxblk = std::max(maxblk, nt + 8);
     d = {(d.lx + 1) / 2, (d.ly + 1) / 2, (d.lz + 1) / 2};
   }
+  npartials_ = 3 * maxblk + 64;
+  partials_ = dalloc<double>(npartials_, owned_, err);
   coef_ = dalloc<float>(2 * 64, owned_, err);
+  if (err != hipSuccess) return err;
+  // every work vector starts at zero: tiles without free voxels are never
+  // written and must read as zero from their neighbours
+  for (int k = 0; k < 2; k++) {
+    fill0(r_[k], n0_, stream_);
+    fill0(p_[k], n0_, stream_);
+  }
+  fill0(q_, n0_, stream_);
+  fill0(z_, n0_, stream_);
+  k_zero64<<<blocks_for(n0_, BS), BS, 0, stream_>>>(n0_, x_);
+  for (int l = 1; l < nlev_; l++) {
+    fill0(levels_[l].r, levels_[l].n, stream_);
+    fill0(levels_[l].e, levels_[l].n, stream_);
+  }
+  chk(hipGetLastError(), err);
   if (err != hipSuccess) return err;
   return build();
 }
 
-template <typename KT>
-static hipError_t estimate_lmax(const MgLevel& L, float* v, float* w, double* partials,
-                                hipStream_t s, float* out) {
-  Lv<KT> V = view<KT>(L);
-  unsigned nb = blocks_for(L.n, BS);
-  k_power_init<<<nb, BS, 0, s>>>(L.n, L.dg, v);
-  std::vector<double> host(3 * nb);
-  double lam = 0.0;
-  for (int it = 0; it < 12; it++) {
-    k_power<KT><<<nb, BS, 0, s>>>(V, v, w, partials);
-    hipError_t e = hipMemcpyAsync(host.data(), partials, sizeof(double) * 3 * nb,
-                                  hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    double vav = 0, vdv = 0, wdw = 0;
-    for (unsigned b = 0; b < nb; b++) {
-      vav += host[b];
-      vdv += host[nb + b];
-      wdw += host[2 * nb + b];
-    }
-    lam = vdv > 0 ? vav / vdv : 0.0;
-    float sc = wdw > 0 ? (float)(1.0 / sqrt(wdw)) : 0.f;
-    k_scale<<<nb, BS, 0, s>>>(L.n, w, sc);
-    std::swap(v, w);
+// Active-tile list in XCD-grouped launch order: sorted tile ids are split
+// into 8 contiguous chunks and work-group g takes chunk g % 8, entry g / 8.
+hipError_t MgSolver::build_tiles(MgLevel& L) {
+  hipError_t err = hipSuccess;
+  int64_t nt = (int64_t)L.tiles.ntx * L.tiles.nty * L.tiles.ntz;
+  std::vector<int> flags(nt, 0);
+  chk(hipMemsetAsync(L.tiles.ids, 0, sizeof(int) * nt, stream_), err);
+  k_tile_flags<<<blocks_for(L.n, BS), BS, 0, stream_>>>(L.d, L.fr, L.tiles.ntx, L.tiles.nty,
+                                                       L.tiles.tz, L.tiles.ids);
+  chk(hipGetLastError(), err);
+  chk(hipMemcpyAsync(flags.data(), L.tiles.ids, sizeof(int) * nt, hipMemcpyDeviceToHost,
+                     stream_),
+      err);
+  chk(hipStreamSynchronize(stream_), err);
+  if (err != hipSuccess) return err;
+  std::vector<int> act;
+  for (int64_t t = 0; t < nt; t++)
+    if (flags[t]) act.push_back((int)t);
+  if (act.empty()) act.push_back(0);  // keep launches non-empty
+  int T = (int)act.size();
+  int chunk = (T + 7) / 8;
+  std::vector<int> ids(8 * chunk, -1);
+  for (int g = 0; g < 8 * chunk; g++) {
+    int src = (g % 8) * chunk + g / 8;
+    ids[g] = src < T ? act[src] : -1;
   }
-  // Rayleigh quotients approach lambda_max from below; pad for safety.
-  *out = (float)(lam * 1.1);
-  return hipGetLastError();
+  L.tiles.nactive = T;
+  L.tiles.nlaunch = 8 * chunk;
+  chk(hipMemcpyAsync(L.tiles.ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice,
+                     stream_),
+      err);
+  chk(hipStreamSynchronize(stream_), err);
+  return err;
 }
 
 hipError_t MgSolver::build() {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
   chk(anchored_mask(d0_, type_, reach_, flag_, s), err);
   MgLevel& F = levels_[0];
   k_fine_setup<<<blocks_for(n0_, BS), BS, 0, s>>>(d0_, type_, kappa_, reach_, (float*)F.kap,
-                                                  (float*)F.sq, F.am, F.fr);
+                                                  F.am, F.fr);
   for (int l = 1; l < nlev_; l++) {
     MgLevel& A = levels_[l - 1];
     MgLevel& B = levels_[l];
     unsigned nb = blocks_for(B.n, BS);
     if (!A.bf16 && !B.bf16)
       k_coarsen<float, float><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
-                                                (float*)B.kap, (float*)B.sq, B.am, B.fr);
+                                                (float*)B.kap, B.am, B.fr);
     else if (!A.bf16 && B.bf16)
       k_coarsen<float, uint16_t><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
-                                                   (uint16_t*)B.kap, (uint16_t*)B.sq, B.am,
-                                                   B.fr);
+                                                   (uint16_t*)B.kap, B.am, B.fr);
     else
       k_coarsen<uint16_t, uint16_t><<<nb, BS, 0, s>>>(A.d, (const uint16_t*)A.kap, A.am, A.fr,
-                                                      B.d, (uint16_t*)B.kap, (uint16_t*)B.sq,
-                                                      B.am, B.fr);
+                                                      B.d, (uint16_t*)B.kap, B.am, B.fr);
   }
+  // diagonals need every level's free flags: computed after the coarsening
   for (int l = 0; l < nlev_; l++) {
     MgLevel& L = levels_[l];
     unsigned nb = blocks_for(L.n, BS);
-    if (L.bf16)
-      k_level_diag<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(L), L.am, L.fr, L.dg, L.dinv);
-    else
-      k_level_diag<float><<<nb, BS, 0, s>>>(view<float>(L), L.am, L.fr, L.dg, L.dinv);
+    if (L.bf16) {
+      k_level_diag<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(L), L.fr, L.dg);
+      k_sync_free<uint16_t><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (uint16_t*)L.kap);
+    } else {
+      k_level_diag<float><<<nb, BS, 0, s>>>(view<float>(L), L.fr, L.dg);
+      k_sync_free<float><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (float*)L.kap);
+    }
   }
-  k_sync_free<<<blocks_for(n0_, BS), BS, 0, s>>>(n0_, F.dg, F.fr);
   chk(hipGetLastError(), err);
-  // smoother damping / Chebyshev interval per level
+  for (int l = 0; l < nlev_ && err == hipSuccess; l++) chk(build_tiles(levels_[l]), err);
+  // Smoother weights and the Chebyshev interval use the Gershgorin bound
+  // lambda_max(D^-1 A) <= 2 (each diagonal is at least the sum of its row's
+  // off-diagonal magnitudes).  A bound, unlike an eigenvalue estimate, keeps
+  // every smoother contracting and the V-cycle positive definite.
   for (int l = 0; l < nlev_; l++) {
     MgLevel& L = levels_[l];
-    float lm = 2.0f;
-    hipError_t e = L.bf16 ? estimate_lmax<uint16_t>(L, p_, q_, partials_, s, &lm)
-                          : estimate_lmax<float>(L, p_, q_, partials_, s, &lm);
-    chk(e, err);
-    if (!(lm > 0.1f) || !(lm < 4.0f)) lm = 2.0f;
-    L.lmax = lm;
+    L.lmax = 2.0f;
     // damped Jacobi = degree-1 Chebyshev on [0.1 lmax, lmax]: omega = 1/theta
-    L.omega = 1.0f / (0.55f * lm);
+    L.omega = 1.0f / (0.55f * L.lmax);
   }
-  // Chebyshev coefficients for the coarsest level on [lmax/30, lmax]
   {
     MgLevel& C = levels_[nlev_ - 1];
     double lmax = C.lmax, lmin = C.lmax / 30.0;
@@ -920,7 +1104,6 @@ hipError_t MgSolver::build() {
     std::vector<float> cf(2 * 64, 0.f);
     int st = std::min(coarse_sweeps_, 63);
     double rho = 1.0 / sg;
-    cf[0] = 0.f;
     cf[1] = (float)(1.0 / th);
     for (int k = 1; k < st; k++) {
       double rn = 1.0 / (2.0 * sg - rho);
@@ -942,145 +1125,242 @@ hipError_t MgSolver::update(const uint8_t* type, const float* kappa) {
   return build();
 }
 
-// Preconditioner z = M r on the level-0 residual r32_.
+// ── launch helpers: dispatch on tile depth and kappa storage ───────────
+#define PFSF_TZ_DISPATCH(L, CALL) \
+  switch ((L).tiles.tz) {         \
+    case 8: {                     \
+      constexpr int TZV = 8;      \
+      CALL;                       \
+    } break;                      \
+    case 4: {                     \
+      constexpr int TZV = 4;      \
+      CALL;                       \
+    } break;                      \
+    default: {                    \
+      constexpr int TZV = 2;      \
+      CALL;                       \
+    } break;                      \
+  }
+
+template <typename KT>
+static void launch_down_t(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
+                          hipStream_t s) {
+  PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
+                           view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
+                           sc, nullptr)));
+}
+
+static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
+                        hipStream_t s) {
+  if (A.bf16)
+    launch_down_t<uint16_t>(A, B, r, sc, s);
+  else
+    launch_down_t<float>(A, B, r, sc, s);
+}
+
+template <typename KT>
+static void launch_up_t(const MgLevel& A, const MgLevel& B, const float* r, float* out,
+                        PcgScalars* sc, hipStream_t s) {
+  PFSF_TZ_DISPATCH(A, (k_up<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
+                           view<KT>(A), tl(A), B.d, r, B.e, out, sc, nullptr)));
+}
+
+static void launch_up(const MgLevel& A, const MgLevel& B, const float* r, float* out,
+                      PcgScalars* sc, hipStream_t s) {
+  if (A.bf16)
+    launch_up_t<uint16_t>(A, B, r, out, sc, s);
+  else
+    launch_up_t<float>(A, B, r, out, sc, s);
+}
+
+// Coarse part of the V-cycle: down-sweeps from level `from` (whose residual
+// is already in levels_[from].r), the coarsest solve, and up-sweeps back to
+// level `from`.  On return levels_[from].e holds that level's correction.
+hipError_t MgSolver::vcycle_below(int from) {
+  hipStream_t s = stream_;
+  for (int l = from; l < nlev_ - 1; l++)
+    launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
+  MgLevel& C = levels_[nlev_ - 1];
+  int st = std::min(coarse_sweeps_, 63);
+  if (C.n <= 2048) {
+    if (C.bf16)
+      k_cheb_lds<uint16_t><<<1, 1024, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st, sc_);
+    else
+      k_cheb_lds<float><<<1, 1024, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_);
+  } else {
+    float* xin = C.xb;
+    float* xout = C.e;
+    if (st % 2 == 0) std::swap(xin, xout);  // the last step writes C.e
+    for (int k = 0; k < st; k++) {
+      float c1 = coef_host_[2 * k], c2 = coef_host_[2 * k + 1];
+      if (C.bf16) {
+        PFSF_TZ_DISPATCH(C, (k_cheb_tiled<TZV, uint16_t><<<C.tiles.nlaunch, BS, 0, s>>>(
+                                 view<uint16_t>(C), tl(C), C.r, xin, xout, C.dv, c1, c2, k == 0,
+                                 sc_)));
+      } else {
+        PFSF_TZ_DISPATCH(C, (k_cheb_tiled<TZV, float><<<C.tiles.nlaunch, BS, 0, s>>>(
+                                 view<float>(C), tl(C), C.r, xin, xout, C.dv, c1, c2, k == 0,
+                                 sc_)));
+      }
+      std::swap(xin, xout);
+    }
+  }
+  for (int l = nlev_ - 2; l >= from; l--)
+    launch_up(levels_[l], levels_[l + 1], levels_[l].r, levels_[l].e, sc_, s);
+  return hipGetLastError();
+}
+
+// z = M r_[0] and r.z at the start of a solve pass.
 hipError_t MgSolver::precondition() {
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
-  unsigned nb0 = blocks_for(n0_, BS);
   if (nlev_ == 1) {
-    k_jacobi_apply<<<nb0, BS, 0, s>>>(n0_, r32_, F.dinv, z_, sc_, partials_);
+    unsigned nb = blocks_for(n0_, BS);
+    k_jacobi_apply<<<nb, BS, 0, s>>>(n0_, r_[0], F.dg, z_, partials_);
+    k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, nb, sc_, ir_);
     return hipGetLastError();
   }
-  // down-sweep
-  for (int l = 0; l < nlev_ - 1; l++) {
-    MgLevel& A = levels_[l];
-    MgLevel& B = levels_[l + 1];
-    const float* r = (l == 0) ? r32_ : A.r;
-    unsigned nb = blocks_for(B.n, BS);
-    if (A.bf16)
-      k_mg_down<uint16_t, uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(A), B.d, r, A.xs, B.r, sc_);
-    else
-      k_mg_down<float, float><<<nb, BS, 0, s>>>(view<float>(A), B.d, r, A.xs, B.r, sc_);
-  }
-  // coarsest: Chebyshev from zero; result in C.e
-  {
-    MgLevel& C = levels_[nlev_ - 1];
-    int st = std::min(coarse_sweeps_, 63);
-    if (C.n <= 32768) {
-      if (C.bf16)
-        k_cheb_block<uint16_t><<<1, 1024, 0, s>>>(view<uint16_t>(C), C.r, C.e, C.xb, C.dv, coef_,
-                                                  st, sc_);
-      else
-        k_cheb_block<float><<<1, 1024, 0, s>>>(view<float>(C), C.r, C.e, C.xb, C.dv, coef_, st,
-                                               sc_);
-    } else {
-      unsigned nb = blocks_for(C.n, BS);
-      float* xin = C.xb;
-      float* xout = C.e;
-      if (st % 2 == 0) std::swap(xin, xout);  // last write lands in C.e
-      for (int k = 0; k < st; k++) {
-        float c1 = coef_host_[2 * k], c2 = coef_host_[2 * k + 1];
-        if (C.bf16)
-          k_cheb<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(C), C.r, xin, xout, C.dv, c1, c2,
-                                             k == 0, sc_);
-        else
-          k_cheb<float><<<nb, BS, 0, s>>>(view<float>(C), C.r, xin, xout, C.dv, c1, c2, k == 0,
-                                          sc_);
-        std::swap(xin, xout);
-      }
-    }
-  }
-  // up-sweep
-  for (int l = nlev_ - 2; l >= 0; l--) {
-    MgLevel& A = levels_[l];
-    MgLevel& B = levels_[l + 1];
-    const float* r = (l == 0) ? r32_ : A.r;
-    float* out = (l == 0) ? z_ : A.e;
-    unsigned nb = blocks_for(A.n, BS);
-    if (A.bf16)
-      k_mg_up<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(A), B.d, r, A.xs, B.e, out, sc_,
-                                          partials_, l == 0);
-    else
-      k_mg_up<float><<<nb, BS, 0, s>>>(view<float>(A), B.d, r, A.xs, B.e, out, sc_, partials_,
-                                       l == 0);
-  }
+  MgLevel& B = levels_[1];
+  launch_down(F, B, r_[0], sc_, s);
+  (void)vcycle_below(1);
+  PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<F.tiles.nlaunch, BS, 0, s>>>(
+                          view<float>(F), tl(F), B.d, r_[0], B.e, z_, sc_, partials_)));
+  k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, F.tiles.nlaunch, sc_, ir_);
   return hipGetLastError();
 }
 
-hipError_t MgSolver::iterate() {
+// One PCG iteration reading r_[c], p_[c] and writing r_[1-c], p_[1-c]; the
+// FP64 iterate is updated in place.
+hipError_t MgSolver::iteration(int c) {
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
-  unsigned nb0 = blocks_for(n0_, BS);
-  k_pcg_spmv<float><<<nb0, BS, 0, s>>>(view<float>(F), p_, q_, sc_, partials_);
-  k_pcg_update<<<nb0, BS, 0, s>>>(n0_, x32_, r32_, p_, q_, F.dinv, z_, nlev_ == 1, sc_,
-                                  partials_);
-  if (nlev_ > 1) (void)precondition();
-  k_pcg_dir<<<nb0, BS, 0, s>>>(n0_, p_, z_, sc_);
+  const unsigned G = F.tiles.nlaunch;
+  PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
+                          view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, partials_)));
+  k_finalize<<<1, BS, 0, s>>>(FIN_ALPHA, partials_, G, sc_, ir_);
+  if (nlev_ == 1) {
+    unsigned nb = blocks_for(n0_, BS);
+    k_update_jacobi<<<nb, BS, 0, s>>>(n0_, F.dg, x_, r_[c], r_[1 - c], p_[1 - c], q_, z_, sc_,
+                                      partials_);
+    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE_JAC, partials_, nb, sc_, ir_);
+    return hipGetLastError();
+  }
+  MgLevel& B = levels_[1];
+  PFSF_TZ_DISPATCH(F, (k_down<TZV, float, true><<<G, BS, 0, s>>>(
+                          view<float>(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
+                          sc_, partials_)));
+  k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, partials_, G, sc_, ir_);
+  (void)vcycle_below(1);
+  PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<G, BS, 0, s>>>(
+                          view<float>(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, partials_)));
+  k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, G, sc_, ir_);
   return hipGetLastError();
 }
 
+// 2*pairs PCG iterations.  The pair (ping, pong) is captured once into a HIP
+// graph and replayed; the capture holds kernel launches only.
+hipError_t MgSolver::chunk(int pairs) {
+  hipError_t err = hipSuccess;
+  if (use_graph_ && !graph_exec_) {
+    if (!chk(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), err)) {
+      (void)iteration(0);
+      (void)iteration(1);
+      hipGraph_t g = nullptr;
+      chk(hipStreamEndCapture(stream_, &g), err);
+      if (err == hipSuccess && !chk(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), err))
+        graph_ = g;
+      else if (g)
+        (void)hipGraphDestroy(g);
+    }
+    if (err != hipSuccess) {  // fall back to direct launches
+      (void)hipGetLastError();
+      use_graph_ = false;
+      graph_exec_ = nullptr;
+      err = hipSuccess;
+    }
+  }
+  for (int k = 0; k < pairs && err == hipSuccess; k++) {
+    if (use_graph_) {
+      chk(hipGraphLaunch(graph_exec_, stream_), err);
+    } else {
+      chk(iteration(0), err);
+      chk(iteration(1), err);
+    }
+  }
+  return err;
+}
+
+// True FP64 residual of x_ into r_[0]; returns ||r|| / ||b|| (synchronises).
+hipError_t MgSolver::true_residual(const float* b, double* rel) {
+  hipError_t err = hipSuccess;
+  hipStream_t s = stream_;
+  unsigned nb0 = blocks_for(n0_, BS);
+  k_true_residual<<<nb0, BS, 0, s>>>(d0_, type_, kappa_, levels_[0].fr, b, x_, r_[0], partials_);
+  k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, nb0, sc_, ir_);
+  IrScalars* irh = (IrScalars*)(sc_host_ + 1);
+  chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
+  chk(hipStreamSynchronize(s), err);
+  *rel = irh->rel;
+  return err;
+}
+
+// Solve passes: PCG on FP32 vectors with the iterate accumulated in FP64,
+// started from the true FP64 residual; a pass ends when its recursive
+// residual falls below half the target (or the FP32 resolution limit), then
+// the true residual decides whether a further pass (a restart) is needed.
 hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve_info* info) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
+  hipStream_t caller = (hipStream_t)desc_.stream;
   const double tol = desc_.tol > 0 ? desc_.tol : 1e-6;
   const int max_it = desc_.max_iters > 0 ? desc_.max_iters : 1000;
   unsigned nb0 = blocks_for(n0_, BS);
   MgLevel& F = levels_[0];
+  chk(hipEventRecord(ev0_, caller), err);  // order after the caller's stream
+  chk(hipStreamWaitEvent(s, ev0_, 0), err);
   chk(hipEventRecord(ev0_, s), err);
-  double* xc = xa_;
-  double* xn = xb_;
-  if (x64)
-    chk(hipMemcpyAsync(xc, x64, sizeof(double) * n0_, hipMemcpyDeviceToDevice, s), err);
-  else
-    chk(hipMemsetAsync(xc, 0, sizeof(double) * n0_, s), err);
-  chk(hipMemsetAsync(ir_, 0, sizeof(IrScalars), s), err);
-  k_bnorm<<<nb0, BS, 0, s>>>(n0_, F.fr, b, ir_, partials_);
-  k_ir_residual<<<nb0, BS, 0, s>>>(d0_, type_, kappa_, F.fr, b, xc, nullptr, xn, r32_, ir_,
-                                   partials_);
-  std::swap(xc, xn);
-  IrScalars* irh = (IrScalars*)(sc_host_ + 1);
-  chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
-  chk(hipStreamSynchronize(s), err);
-  if (err != hipSuccess) return err;
-  double rel = irh->rel;
-  int total = 0, passes = 0;
-  const int chunk = 8;
-  while (rel > tol && total < max_it && err == hipSuccess) {
-    // inner FP32 PCG on r32_ to a relative reduction the FP32 operator can
-    // resolve; the FP64 residual recomputed afterwards drives the next pass
-    double inner = std::max(0.3 * tol / rel, 2e-4);
-    k_inner_begin<<<1, 1, 0, s>>>(sc_, ir_, inner, max_it - total);
-    chk(hipMemsetAsync(x32_, 0, sizeof(float) * n0_, s), err);
-    chk(precondition(), err);  // z = M r, rz
-    chk(hipMemcpyAsync(p_, z_, sizeof(float) * n0_, hipMemcpyDeviceToDevice, s), err);
-    k_mark_dir<<<1, 1, 0, s>>>(sc_);
+  if (x64) {
+    k_copy64<<<nb0, BS, 0, s>>>(n0_, x64, x_);
+    k_mask64<<<nb0, BS, 0, s>>>(n0_, F.fr, x_);
+  } else {
+    k_zero64<<<nb0, BS, 0, s>>>(n0_, x_);
+  }
+  k_bnorm<<<nb0, BS, 0, s>>>(n0_, F.fr, b, partials_);
+  k_finalize<<<1, BS, 0, s>>>(FIN_BNORM, partials_, nb0, sc_, ir_);
+  double rel = 0.0;
+  chk(true_residual(b, &rel), err);
+  int passes = 0, total = 0;
+  const int pairs_per_check = 4;  // 8 iterations between host checks
+  for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
+    double stop_rel = std::max(0.5 * tol / rel, 2e-6);
+    k_pcg_begin<<<1, 1, 0, s>>>(sc_, ir_, stop_rel, max_it - total, 1);
+    chk(precondition(), err);
     int done = 0;
-    while (!done && err == hipSuccess) {
-      for (int k = 0; k < chunk; k++) chk(iterate(), err);
+    // the device caps the pass at max_it - total iterations; the number of
+    // host checks is capped independently
+    const int max_checks = (max_it - total) / (2 * pairs_per_check) + 2;
+    for (int c = 0; !done && err == hipSuccess && c < max_checks; c++) {
+      chk(chunk(pairs_per_check), err);
       chk(hipMemcpyAsync(sc_host_, sc_, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), err);
       chk(hipStreamSynchronize(s), err);
       done = sc_host_->done;
     }
-    total += sc_host_->iters;
+    const int it = sc_host_->iters;
+    total += it;
     passes++;
-    k_ir_residual<<<nb0, BS, 0, s>>>(d0_, type_, kappa_, F.fr, b, xc, x32_, xn, r32_, ir_,
-                                     partials_);
-    std::swap(xc, xn);
-    chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
-    chk(hipStreamSynchronize(s), err);
-    double nrel = irh->rel;
-    if (sc_host_->done == 2 && nrel >= rel) {
-      rel = nrel;
-      break;  // no progress possible
-    }
+    double nrel = rel;
+    chk(true_residual(b, &nrel), err);
+    bool stalled = (it == 0) || (sc_host_->done == 2 && nrel >= rel);
     rel = nrel;
+    if (stalled) break;
   }
-  if (x64) chk(hipMemcpyAsync(x64, xc, sizeof(double) * n0_, hipMemcpyDeviceToDevice, s), err);
-  if (phi32) k_to_f32<<<nb0, BS, 0, s>>>(n0_, xc, phi32);
+  if (x64) k_copy64<<<nb0, BS, 0, s>>>(n0_, x_, x64);
+  if (phi32) k_to_f32<<<nb0, BS, 0, s>>>(n0_, x_, phi32);
   chk(hipEventRecord(ev1_, s), err);
   chk(hipEventSynchronize(ev1_), err);
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  chk(hipStreamWaitEvent(caller, ev1_, 0), err);
   if (info) {
     info->iterations = total;
     info->refinements = passes;
@@ -1089,45 +1369,49 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
     info->rel_residual = rel;
     info->solve_ms = ms;
   }
-  last_xc_ = xc;
   return err;
 }
 
 // Average device time of one launch of a solver kernel, replayed `reps` times
-// between two events on the solver stream.  which: 0 level-0 SpMV + dot,
-// 1 level-0 vector update, 2 level-0 down-sweep (smooth + residual +
-// restrict), 3 level-0 up-sweep (prolong + smooth + dot), 4 one whole PCG
-// iteration.  Call after a solve so the vectors hold finite data.
+// between two events on the solver stream.  which: 0 finest SpMV + direction
+// + p.q, 1 finest update + down-sweep (or the Jacobi update), 2 second-level
+// down-sweep, 3 finest up-sweep + r.z, 4 one whole PCG iteration.  Call after
+// a solve so the vectors hold finite data.
 hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
-  unsigned nb0 = blocks_for(n0_, BS);
+  const unsigned G = F.tiles.nlaunch;
   if (which < 0 || which > 4 || reps <= 0) return hipErrorInvalidValue;
   if ((which == 2 || which == 3) && nlev_ < 2) return hipErrorInvalidValue;
+  if (which == 2 && nlev_ < 3) return hipErrorInvalidValue;
   auto once = [&]() {
     switch (which) {
       case 0:
-        k_pcg_spmv<float><<<nb0, BS, 0, s>>>(view<float>(F), p_, q_, sc_, partials_);
+        PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
+                                view<float>(F), tl(F), z_, p_[0], p_[1], q_, sc_, partials_)));
         break;
       case 1:
-        k_pcg_update<<<nb0, BS, 0, s>>>(n0_, x32_, r32_, p_, q_, F.dinv, z_, nlev_ == 1, sc_,
-                                        partials_);
+        if (nlev_ == 1) {
+          unsigned nb = blocks_for(n0_, BS);
+          k_update_jacobi<<<nb, BS, 0, s>>>(n0_, F.dg, x_, r_[0], r_[1], p_[1], q_, z_, sc_,
+                                            partials_);
+        } else {
+          PFSF_TZ_DISPATCH(F, (k_down<TZV, float, true><<<G, BS, 0, s>>>(
+                                  view<float>(F), tl(F), levels_[1].d, r_[0], r_[1], q_, p_[1],
+                                  x_, levels_[1].r, sc_, partials_)));
+        }
         break;
-      case 2: {
-        MgLevel& B = levels_[1];
-        k_mg_down<float, float><<<blocks_for(B.n, BS), BS, 0, s>>>(view<float>(F), B.d, r32_,
-                                                                   F.xs, B.r, sc_);
+      case 2:
+        launch_down(levels_[1], levels_[2], levels_[1].r, sc_, s);
         break;
-      }
-      case 3: {
-        MgLevel& B = levels_[1];
-        k_mg_up<float><<<nb0, BS, 0, s>>>(view<float>(F), B.d, r32_, F.xs, B.e, z_, sc_,
-                                          partials_, 1);
+      case 3:
+        PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<G, BS, 0, s>>>(
+                                view<float>(F), tl(F), levels_[1].d, r_[1], levels_[1].e, z_, sc_,
+                                partials_)));
         break;
-      }
       default:
-        (void)iterate();
+        (void)iteration(0);
     }
   };
   k_profile_begin<<<1, 1, 0, s>>>(sc_);

@@ -63,7 +63,8 @@ def test_orphans_and_isolated_voxels_are_excluded(cuda):
 def test_box_union_64_to_1e6(cuda, prec):
     isl = W.box_union(1, 64, 0.6, 0.5, n_boxes=100000, edge=(2, 20))
     info, x, phi = _solve(isl, cuda, precision=prec, tol=1e-6, max_iters=2000)
-    assert info.converged == 1
+    assert info.converged == 1, (info.iterations, info.refinements, info.levels,
+                                 info.rel_residual)
     rel = O.residual(isl, x)
     assert rel <= 1e-6 * 1.01, rel
     ref, it, rr = O.pcg(isl, tol=1e-11)
