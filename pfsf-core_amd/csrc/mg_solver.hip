@@ -34,6 +34,7 @@
 // (fixed 16-step schedule, no tolerance); PCG kernels SH/pcg_*.comp.glsl,
 // NP/src/solver/dispatcher_pcg.cpp:254-297 (FP32 dots, N <= 512^2).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -45,6 +46,12 @@ namespace pfsf {
 namespace {
 
 constexpr int BS = 256;
+// Waves per SIMD the tiled stencil kernels are register-budgeted for (LDS
+// admits six 24 KB tiles per CU).
+#ifndef PFSF_WPE
+#define PFSF_WPE 4
+#endif
+
 constexpr int TX = kTX, TY = kTY;
 constexpr int HX = TX + 2, HY = TY + 2, HXY = HX * HY;
 static_assert(TX * TY == BS, "one thread per (x, y) column of a tile");
@@ -61,6 +68,23 @@ __device__ __forceinline__ void stk(uint16_t* p, int64_t i, float v) {
   p[i] = (uint16_t)(r >> 16);
 }
 
+// Loads and stores at a 32-bit element index.  The byte offset is formed in
+// 32 bits (the solver's arrays stay below 4 GiB, checked at creation), so the
+// access is the scalar kernel-argument base plus one 32-bit vector offset.
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, int i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) +
+                                     (unsigned)i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T* p, int i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (unsigned)i * (unsigned)sizeof(T)) = v;
+}
+__device__ __forceinline__ float ldk32(const float* p, int i) { return ld(p, i); }
+__device__ __forceinline__ float ldk32(const uint16_t* p, int i) {
+  return __uint_as_float(((unsigned)ld(p, i)) << 16);
+}
+
 __device__ __forceinline__ int64_t flat(int x, int y, int z, pfsf_dims d) {
   return (int64_t)x + (int64_t)d.lx * ((int64_t)y + (int64_t)d.ly * z);
 }
@@ -73,6 +97,11 @@ __device__ __forceinline__ void unflat(int64_t i, pfsf_dims d, int& x, int& y, i
 }
 
 __device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
+
+// Hardware reciprocal (1 ulp) for the smoothers' D^-1: every use of a given
+// diagonal entry gets the same value, so pre- and post-smoothing stay
+// transposes of each other.
+__device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
 template <typename KT>
 struct Lv {
@@ -124,12 +153,19 @@ __device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, fl
 }
 
 // ── LDS tile machinery ─────────────────────────────────────────────────
+// Grids hold fewer than 2^31 cells (checked at creation), so the hot kernels
+// index with 32-bit integers.
+__device__ __forceinline__ int flat32(int x, int y, int z, pfsf_dims d) {
+  return x + d.lx * (y + d.ly * z);
+}
+
 template <int TZ>
 struct Tile {
-  static constexpr int HN = HXY * (TZ + 2);
+  static constexpr int HZ = TZ + 2, NROWS = HY * HZ, HN = HXY * HZ;
   float v[HN];  // value (0 outside the free set)
   float k[HN];  // kappa (< 0 outside the free set)
-  float s[HN];  // sqrt kappa (0 outside the free set)
+  float s[HN];  // sqrt kappa (0 outside the free set); hardware square root,
+                // used for both ends of every coupling so A stays symmetric
 };
 
 template <int TZ>
@@ -144,48 +180,78 @@ __device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& 
   return true;
 }
 
-// Stage the halo box of a tile.  val(j, gx, gy, gz) gives the vector value at
-// an in-bounds free voxel j; other cells hold v = 0, k = -1, s = 0.  The loop
-// is fully unrolled so every thread keeps all of its loads in flight.
+// Load one halo cell.  A cell outside the grid is redirected to cell 0 (always
+// in bounds) and its result discarded, so the load needs no branch; val(j, x,
+// y, z) is evaluated unconditionally (it may read garbage or divide by zero
+// outside the free set; the select discards it).
+template <typename KT, typename F>
+__device__ __forceinline__ void load_cell(const Lv<KT>& L, int gx, int gy, int gz, bool ok,
+                                          F& val, float& k, float& v) {
+  const int cx = ok ? gx : 0, cy = ok ? gy : 0, cz = ok ? gz : 0;
+  const int j = flat32(cx, cy, cz, L.d);
+  const float kj = ldk32(L.kap, j);
+  const float vj = val(j, cx, cy, cz);
+  k = ok ? kj : -1.f;
+  v = k >= 0.f ? vj : 0.f;
+}
+
+template <int TZ>
+__device__ __forceinline__ void put_cell(Tile<TZ>& T, int e, float k, float v) {
+  T.v[e] = v;
+  T.k[e] = k;
+  T.s[e] = k > 0.f ? __builtin_amdgcn_sqrtf(k) : 0.f;
+}
+
+// Stage the halo box of a tile row by row: the 32 lanes of a half-wave take
+// the 32 interior columns of one (y, z) row, eight rows per pass; one more
+// pass covers the two halo columns of every row.  Every pass's loads are
+// issued before the first LDS write, so a wave waits for memory once, not
+// once per pass.  Cells outside the grid or the free set hold v = 0, k = -1,
+// s = 0.
 template <int TZ, typename KT, typename F>
 __device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int y0, int z0,
                                       F val) {
-  constexpr int HN = Tile<TZ>::HN;
-  constexpr int IT = (HN + BS - 1) / BS;
+  constexpr int NROWS = Tile<TZ>::NROWS;
+  constexpr int RPP = BS / TX;
+  constexpr int NP = (NROWS + RPP - 1) / RPP;
+  static_assert(2 * NROWS <= BS, "halo columns fit one pass");
   const pfsf_dims d = L.d;
+  const int t = threadIdx.x;
+  const int tx = t % TX, tr = t / TX;
+  const int gx = x0 + tx;
+  const bool inx = gx < d.lx;
+  float kk[NP + 1], vv[NP + 1];
 #pragma unroll
-  for (int it = 0; it < IT; it++) {
-    int e = threadIdx.x + it * BS;
-    if (e < HN) {
-      int ex = e % HX;
-      int r = e / HX;
-      int ey = r % HY;
-      int ez = r / HY;
-      int gx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
-      float v = 0.f, k = -1.f;
-      if (gx >= 0 && gx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz) {
-        int64_t j = flat(gx, gy, gz, d);
-        k = ldk(L.kap, j);
-        if (k >= 0.f) v = val(j, gx, gy, gz);
-      }
-      T.v[e] = v;
-      T.k[e] = k;
-      T.s[e] = k > 0.f ? sqrtf(k) : 0.f;
-    }
+  for (int p = 0; p < NP; p++) {
+    const int row = tr + p * RPP;
+    const int ey = row % HY, ez = row / HY;
+    const int gy = y0 + ey - 1, gz = z0 + ez - 1;
+    const bool ok = row < NROWS && inx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz;
+    load_cell(L, gx, gy, gz, ok, val, kk[p], vv[p]);
   }
-}
-
-__device__ __forceinline__ int tcell(int tx, int ty, int tz) {
-  return (tz + 1) * HXY + (ty + 1) * HX + (tx + 1);
+  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+  {
+    const int ey = hrow % HY, ez = hrow / HY;
+    const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
+    const bool ok = hrow < NROWS && hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 &&
+                    gz < d.lz;
+    load_cell(L, hx, gy, gz, ok, val, kk[NP], vv[NP]);
+  }
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const int row = tr + p * RPP;
+    if (p < NP - 1 || row < NROWS) put_cell<TZ>(T, row * HX + tx + 1, kk[p], vv[p]);
+  }
+  if (hrow < NROWS) put_cell<TZ>(T, hrow * HX + ex, kk[NP], vv[NP]);
 }
 
 // (A v)_i at LDS cell c: difference form over free neighbours + anchor mass.
-template <int TZ>
-__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
-                                            float scale) {
-  const float* v = T.v;
-  const float* k = T.k;
-  const float* s = T.s;
+// Offsets are compile-time constants, so every read is one LDS instruction
+// with an immediate offset from the same base.
+__device__ __forceinline__ float tile_apply(const float* __restrict__ v,
+                                            const float* __restrict__ k,
+                                            const float* __restrict__ s, int c, float ki,
+                                            float mi, float scale) {
   const float vc = v[c];
   float af = 0.f, ae = 0.f, ac = 0.f;
   auto face = [&](int o) {
@@ -221,7 +287,17 @@ __device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, 
   corner(1 - HX + HXY);
   corner(-1 + HX + HXY);
   corner(1 + HX + HXY);
-  return scale * (0.5f * af + sqrtf(ki) * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+  return scale * (0.5f * af + s[c] * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+}
+
+template <int TZ>
+__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
+                                            float scale) {
+  return tile_apply(T.v, T.k, T.s, c, ki, mi, scale);
+}
+
+__device__ __forceinline__ int tcell(int tx, int ty, int tz) {
+  return (tz + 1) * HXY + (ty + 1) * HX + (tx + 1);
 }
 
 // ── FP64 reductions ────────────────────────────────────────────────────
@@ -313,9 +389,13 @@ __global__ void __launch_bounds__(BS)
 }
 
 // ── PCG + V-cycle kernels ──────────────────────────────────────────────
+// The output loops below evaluate the stencil for every cell of the column
+// and select the result (a cell outside the free set yields 0), so no branch
+// depends on the free mask; only the stores are guarded by the grid bounds.
+
 // p_out = z + beta p_in (z alone for the first direction); q = A p_out; p.q.
 template <int TZ, typename KT>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_spmv_dir(Lv<KT> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
                float* __restrict__ pout, float* __restrict__ q, const PcgScalars* sc,
                double* partials) {
@@ -326,31 +406,34 @@ __global__ void __launch_bounds__(BS)
   int x0, y0, z0;
   if (tile_origin<TZ>(M, x0, y0, z0)) {
     const bool first = !sc->have_dir;
-    const float beta = (float)sc->beta;
-    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) {
-      float zj = z[j];
-      return first ? zj : zj + beta * pin[j];
-    });
-    __syncthreads();
+    const float beta = first ? 0.f : (float)sc->beta;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const pfsf_dims d = L.d;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx < d.lx && gy < d.ly) {
+    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+    const bool col = gx < d.lx && gy < d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    bool inb[TZ];
+    float amr[TZ];  // own-voxel operands, loaded before the barrier (cell 0 stands in
+                    // for a cell outside the grid; its value is never used)
 #pragma unroll
-      for (int tz = 0; tz < TZ; tz++) {
-        int gz = z0 + tz;
-        if (gz < d.lz) {
-          int64_t i = flat(gx, gy, gz, d);
-          int c = tcell(tx, ty, tz);
-          float pv = T.v[c], qi = 0.f;
-          float ki = T.k[c];
-          if (ki >= 0.f) {
-            qi = tile_apply<TZ>(T, c, ki, L.am[i], L.scale);
-            acc[0] += (double)pv * qi;
-          }
-          pout[i] = pv;
-          q[i] = qi;
-        }
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      amr[tz] = ld(L.am, inb[tz] ? i0 + tz * sz : 0);
+    }
+    // p_in is finite everywhere (zero-initialised), so beta = 0 drops it
+    stage<TZ>(T, L, x0, y0, z0, [&](int j, int, int, int) { return ld(z, j) + beta * ld(pin, j); });
+    __syncthreads();
+    const int c0 = tcell(tx, ty, 0);
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      const int c = c0 + tz * HXY;
+      const float ki = T.k[c], pv = T.v[c];
+      float qi = tile_apply<TZ>(T, c, ki, amr[tz], L.scale);
+      qi = ki >= 0.f ? qi : 0.f;
+      acc[0] += (double)pv * qi;
+      if (inb[tz]) {
+        st(pout, i0 + tz * sz, pv);
+        st(q, i0 + tz * sz, qi);
       }
     }
   }
@@ -363,13 +446,15 @@ __global__ void __launch_bounds__(BS)
 template <int TZ>
 __device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, int y0, int z0,
                                               int tx, int ty, pfsf_dims c, float* rc) {
+  const int X = (x0 + tx) >> 1, Y = (y0 + ty) >> 1;
+  const bool lead = !(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly;
 #pragma unroll
   for (int k = 0; k < TZ / 2; k++) {
     float a = res[2 * k] + res[2 * k + 1];
     a = a + __shfl_xor(a, 1, 64);
     a = a + __shfl_xor(a, 32, 64);
-    int X = (x0 + tx) >> 1, Y = (y0 + ty) >> 1, Z = (z0 >> 1) + k;
-    if (!(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly && Z < c.lz) rc[flat(X, Y, Z, c)] = a;
+    const int Z = (z0 >> 1) + k;
+    if (lead && Z < c.lz) st(rc, flat32(X, Y, Z, c), a);
   }
 }
 
@@ -378,7 +463,7 @@ __device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, in
 // With UPDATE (finest level inside an iteration): first x64 += alpha p and
 // r_out = r_in - alpha q, then the same on r_out, plus r.r for convergence.
 template <int TZ, typename KT, bool UPDATE>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_down(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ rin, float* __restrict__ rout,
            const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
            float* __restrict__ rc, const PcgScalars* sc, double* partials) {
@@ -392,31 +477,50 @@ __global__ void __launch_bounds__(BS)
   if (tile_origin<TZ>(M, x0, y0, z0)) {
     const float alpha = UPDATE ? (float)sc->alpha : 0.f;
     const double alpha64 = UPDATE ? sc->alpha : 0.0;
-    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) {
-      float rj = UPDATE ? rin[j] - alpha * q[j] : rin[j];
-      return om * rj / L.dg[j];
-    });
-    __syncthreads();
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int gx = x0 + tx, gy = y0 + ty;
+    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+    const bool col = gx < d.lx && gy < d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    // own-voxel operands, loaded before the barrier (cell 0 stands in for a
+    // cell outside the grid; its values are never used)
+    bool inb[TZ];
+    float rr_[TZ], amr[TZ], pr[TZ];
+    double xr[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      const int i = inb[tz] ? i0 + tz * sz : 0;
+      rr_[tz] = UPDATE ? ld(rin, i) - alpha * ld(q, i) : ld(rin, i);
+      amr[tz] = ld(L.am, i);
+      pr[tz] = UPDATE ? ld(p, i) : 0.f;
+      xr[tz] = UPDATE ? ld(x, i) : 0.0;
+    }
+    stage<TZ>(T, L, x0, y0, z0, [&](int j, int, int, int) {
+      const float rj = UPDATE ? ld(rin, j) - alpha * ld(q, j) : ld(rin, j);
+      return om * rj * rcp(ld(L.dg, j));
+    });
+    if (UPDATE) {
+      // p and q are zero outside the free set, so the update leaves x there
+      // unchanged and carries r there over
+#pragma unroll
+      for (int tz = 0; tz < TZ; tz++) {
+        if (inb[tz]) {
+          st(x, i0 + tz * sz, xr[tz] + alpha64 * (double)pr[tz]);
+          st(rout, i0 + tz * sz, rr_[tz]);
+        }
+      }
+    }
+    __syncthreads();
+    const int c0 = tcell(tx, ty, 0);
     float res[TZ];
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
-      res[tz] = 0.f;
-      int gz = z0 + tz;
-      int cc = tcell(tx, ty, tz);
-      float ki = T.k[cc];
-      if (gx < d.lx && gy < d.ly && gz < d.lz && ki >= 0.f) {
-        int64_t i = flat(gx, gy, gz, d);
-        float ri = rin[i];
-        if (UPDATE) {
-          ri = ri - alpha * q[i];
-          x[i] = x[i] + alpha64 * (double)p[i];
-          rout[i] = ri;
-          acc[0] += (double)ri * ri;
-        }
-        res[tz] = ri - tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
-      }
+      const int c = c0 + tz * HXY;
+      const float ki = T.k[c], ri = rr_[tz];
+      const bool fr = ki >= 0.f;
+      const float ax = tile_apply<TZ>(T, c, ki, amr[tz], L.scale);
+      res[tz] = fr ? ri - ax : 0.f;
+      if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
     }
     restrict_tile<TZ>(res, x0, y0, z0, tx, ty, c, rc);
   }
@@ -427,7 +531,7 @@ __global__ void __launch_bounds__(BS)
 // one damped-Jacobi post-smoothing step out = x~ + omega D^-1 (r - A x~).
 // At the finest level it also accumulates r.z.
 template <int TZ, typename KT, bool FINEST>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_up(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ r, const float* __restrict__ ec,
          float* __restrict__ out, const PcgScalars* sc, double* partials) {
   if (sc->done) return;
@@ -438,30 +542,36 @@ __global__ void __launch_bounds__(BS)
   const pfsf_dims d = L.d;
   const float om = L.omega;
   if (tile_origin<TZ>(M, x0, y0, z0)) {
-    stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int gx, int gy, int gz) {
-      return om * r[j] / L.dg[j] + ec[flat(gx >> 1, gy >> 1, gz >> 1, c)];
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+    const bool col = gx < d.lx && gy < d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    // own-voxel operands, loaded before the barrier (cell 0 stands in for a
+    // cell outside the grid; its values are never used)
+    bool inb[TZ];
+    float rr_[TZ], amr[TZ], dgr[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      const int i = inb[tz] ? i0 + tz * sz : 0;
+      rr_[tz] = ld(r, i);
+      amr[tz] = ld(L.am, i);
+      dgr[tz] = ld(L.dg, i);
+    }
+    stage<TZ>(T, L, x0, y0, z0, [&](int j, int gx_, int gy_, int gz_) {
+      return om * ld(r, j) * rcp(ld(L.dg, j)) + ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c));
     });
     __syncthreads();
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx < d.lx && gy < d.ly) {
+    const int c0 = tcell(tx, ty, 0);
 #pragma unroll
-      for (int tz = 0; tz < TZ; tz++) {
-        int gz = z0 + tz;
-        if (gz < d.lz) {
-          int64_t i = flat(gx, gy, gz, d);
-          int cc = tcell(tx, ty, tz);
-          float ki = T.k[cc], zi = 0.f;
-          if (ki >= 0.f) {
-            float xi = T.v[cc];
-            float ri = r[i];
-            float ax = tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
-            zi = xi + om * (ri - ax) / L.dg[i];
-            if (FINEST) acc[0] += (double)ri * zi;
-          }
-          out[i] = zi;
-        }
-      }
+    for (int tz = 0; tz < TZ; tz++) {
+      const int cc = c0 + tz * HXY;
+      const float ki = T.k[cc], xi = T.v[cc], ri = rr_[tz];
+      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
+      float zi = xi + om * (ri - ax) * rcp(dgr[tz]);
+      zi = ki >= 0.f ? zi : 0.f;
+      if (FINEST) acc[0] += ki >= 0.f ? (double)ri * zi : 0.0;
+      if (inb[tz]) st(out, i0 + tz * sz, zi);
     }
   }
   if (FINEST) write_partials<1>(acc, partials, sh);
@@ -521,7 +631,7 @@ __global__ void __launch_bounds__(BS)
   int x0, y0, z0;
   if (!tile_origin<TZ>(M, x0, y0, z0)) return;
   const pfsf_dims d = L.d;
-  stage<TZ>(T, L, x0, y0, z0, [&](int64_t j, int, int, int) { return first ? 0.f : xin[j]; });
+  stage<TZ>(T, L, x0, y0, z0, [&](int j, int, int, int) { return first ? 0.f : xin[j]; });
   __syncthreads();
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int gx = x0 + tx, gy = y0 + ty;
@@ -529,7 +639,7 @@ __global__ void __launch_bounds__(BS)
   for (int tz = 0; tz < TZ; tz++) {
     int gz = z0 + tz;
     if (gz >= d.lz) break;
-    int64_t i = flat(gx, gy, gz, d);
+    const int i = flat32(gx, gy, gz, d);
     int cc = tcell(tx, ty, tz);
     float ki = T.k[cc];
     if (ki < 0.f) {
@@ -539,13 +649,13 @@ __global__ void __launch_bounds__(BS)
     }
     float D = L.dg[i];
     if (first) {
-      float di = c2 * r[i] / D;
+      float di = c2 * r[i] * rcp(D);
       dv[i] = di;
       xout[i] = di;
     } else {
       float xi = T.v[cc];
       float ax = tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
-      float di = c1 * dv[i] + c2 * (r[i] - ax) / D;
+      float di = c1 * dv[i] + c2 * (r[i] - ax) * rcp(D);
       dv[i] = di;
       xout[i] = xi + di;
     }
@@ -553,79 +663,80 @@ __global__ void __launch_bounds__(BS)
 }
 
 // Whole Chebyshev solve of the coarsest level held in one work-group's LDS
-// (n <= kMaxLdsCoarse cells): kappa, sqrt kappa, diagonal, rhs, iterate and
-// direction.
-constexpr int kMaxLdsCoarse = 2048;
+// (n <= kMaxLdsCoarse cells, one cell per thread): kappa, sqrt kappa,
+// diagonal, anchor mass, rhs, iterate and direction live in LDS; coordinates
+// and boundary flags are computed once, outside the step loop.
+constexpr int kMaxLdsCoarse = 1024;
 
 template <typename KT>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kMaxLdsCoarse)
     k_cheb_lds(Lv<KT> L, const float* __restrict__ r, float* __restrict__ out,
                const float* __restrict__ coef, int steps, const PcgScalars* sc) {
   if (sc->done) return;
-  __shared__ float sk[kMaxLdsCoarse], ss[kMaxLdsCoarse], sd[kMaxLdsCoarse];
-  __shared__ float sr[kMaxLdsCoarse], sx[2][kMaxLdsCoarse], sdv[kMaxLdsCoarse];
+  __shared__ float sk[kMaxLdsCoarse], ss[kMaxLdsCoarse], sx[2][kMaxLdsCoarse];
   const pfsf_dims d = L.d;
   const int n = d.lx * d.ly * d.lz;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    float k = ldk(L.kap, i);
-    float D = L.dg[i];
-    sk[i] = k;
-    ss[i] = k > 0.f ? sqrtf(k) : 0.f;
-    sd[i] = D;
-    sr[i] = r[i];
-    float di = k >= 0.f ? coef[1] * r[i] / D : 0.f;
-    sdv[i] = di;
-    sx[0][i] = di;
+  const int i = threadIdx.x;
+  const bool act = i < n;
+  int x = 0, y = 0, z = 0;
+  float ki = -1.f, si = 0.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
+  if (act) {
+    x = i % d.lx;
+    y = (i / d.lx) % d.ly;
+    z = i / (d.lx * d.ly);
+    ki = ldk(L.kap, i);
+    si = ki > 0.f ? sqrtf(ki) : 0.f;
+    if (ki >= 0.f) {
+      D = L.dg[i];
+      m = L.am[i];
+      ri = r[i];
+      di = coef[1] * ri * rcp(D);
+      xi = di;
+    }
+    sk[i] = ki;
+    ss[i] = si;
+    sx[0][i] = xi;
   }
+  const bool xm = x > 0, xp = x + 1 < d.lx, ym = y > 0, yp = y + 1 < d.ly, zm = z > 0,
+             zp = z + 1 < d.lz;
+  const int sy = d.lx, sz = d.lx * d.ly;
   __syncthreads();
   int cur = 0;
   for (int st = 1; st < steps; st++) {
     const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
     const float* xin = sx[cur];
-    float* xo = sx[cur ^ 1];
-    for (int i = threadIdx.x; i < n; i += 1024) {
-      const float ki = sk[i];
-      if (ki < 0.f) {
-        xo[i] = 0.f;
-        continue;
-      }
-      int x, y, z;
-      unflat(i, d, x, y, z);
-      const float xi = xin[i];
+    if (act && ki >= 0.f) {
       float af = 0.f, ae = 0.f, ac = 0.f;
-      for (int dz = -1; dz <= 1; dz++) {
-        int nz = z + dz;
-        if (nz < 0 || nz >= d.lz) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-          int ny = y + dy;
-          if (ny < 0 || ny >= d.ly) continue;
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
           for (int dx = -1; dx <= 1; dx++) {
-            int nx = x + dx;
-            if (nx < 0 || nx >= d.lx) continue;
-            int cls = (dx != 0) + (dy != 0) + (dz != 0);
+            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
             if (cls == 0) continue;
-            int j = nx + d.lx * (ny + d.ly * nz);
+            bool ok = (dx < 0 ? xm : (dx > 0 ? xp : true)) && (dy < 0 ? ym : (dy > 0 ? yp : true)) &&
+                      (dz < 0 ? zm : (dz > 0 ? zp : true));
+            if (!ok) continue;
+            int j = i + dx + dy * sy + dz * sz;
             float kj = sk[j];
-            if (kj < 0.f) continue;
             float dv = xi - xin[j];
             if (cls == 1)
-              af += (ki + kj) * dv;
+              af += (kj >= 0.f ? ki + kj : 0.f) * dv;
             else if (cls == 2)
               ae += ss[j] * dv;
             else
               ac += ss[j] * dv;
           }
-        }
-      }
-      float ax = L.scale * (0.5f * af + ss[i] * (kEdgeP * ae + kCornerP * ac)) + L.am[i] * xi;
-      float di = c1 * sdv[i] + c2 * (sr[i] - ax) / sd[i];
-      sdv[i] = di;
-      xo[i] = xi + di;
+      float ax = L.scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + m * xi;
+      di = c1 * di + c2 * (ri - ax) * rcp(D);
+      xi = xi + di;
     }
+    if (act) sx[cur ^ 1][i] = xi;
     __syncthreads();
     cur ^= 1;
   }
-  for (int i = threadIdx.x; i < n; i += 1024) out[i] = sx[cur][i];
+  if (act) out[i] = ki >= 0.f ? xi : 0.f;
 }
 
 // ── setup kernels ──────────────────────────────────────────────────────
@@ -909,8 +1020,14 @@ static void fill0(float* p, int64_t n, hipStream_t s) {
 }
 
 // Tile depth per level: the deepest of {8, 4} that still gives about eight
-// tiles per CU, else 2 (small levels).
-static int pick_tz(pfsf_dims d) {
+// tiles per CU, else 2 (small levels).  PFSF_TILE_Z=8|4|2 forces the depth of
+// the finest level (tuning runs).
+static int pick_tz(pfsf_dims d, bool finest) {
+  if (finest) {
+    const char* env = getenv("PFSF_TILE_Z");
+    int v = env ? atoi(env) : 0;
+    if (v == 8 || v == 4 || v == 2) return v;
+  }
   for (int tz : {8, 4}) {
     int64_t nt = (int64_t)((d.lx + TX - 1) / TX) * ((d.ly + TY - 1) / TY) * ((d.lz + tz - 1) / tz);
     if (nt >= 2048) return tz;
@@ -982,7 +1099,7 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
       L.xb = dalloc<float>(L.n, owned_, err);
       L.dv = dalloc<float>(L.n, owned_, err);
     }
-    L.tiles.tz = pick_tz(d);
+    L.tiles.tz = pick_tz(d, l == 0);
     L.tiles.ntx = (d.lx + TX - 1) / TX;
     L.tiles.nty = (d.ly + TY - 1) / TY;
     L.tiles.ntz = (d.lz + L.tiles.tz - 1) / L.tiles.tz;
@@ -1182,11 +1299,11 @@ hipError_t MgSolver::vcycle_below(int from) {
     launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
   MgLevel& C = levels_[nlev_ - 1];
   int st = std::min(coarse_sweeps_, 63);
-  if (C.n <= 2048) {
+  if (C.n <= kMaxLdsCoarse) {
     if (C.bf16)
-      k_cheb_lds<uint16_t><<<1, 1024, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st, sc_);
+      k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st, sc_);
     else
-      k_cheb_lds<float><<<1, 1024, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_);
+      k_cheb_lds<float><<<1, kMaxLdsCoarse, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_);
   } else {
     float* xin = C.xb;
     float* xout = C.e;

@@ -8,10 +8,12 @@ or unloadable library raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 PKG_ROOT = pathlib.Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_ROOT / "lib" / "libpfsf.so"
+# PFSF_LIBRARY names another build of libpfsf (e.g. a tuning variant).
+LIB_PATH = pathlib.Path(os.environ.get("PFSF_LIBRARY", PKG_ROOT / "lib" / "libpfsf.so"))
 INCLUDE_DIR = PKG_ROOT.parent / "include" / "pfsf"
 
 # pfsf_result
