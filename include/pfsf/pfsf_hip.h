@@ -143,12 +143,14 @@ PFSF_API pfsf_result pfsf_solver_update(pfsf_solver s, const uint8_t* type, cons
 PFSF_API void pfsf_solver_destroy(pfsf_solver s);
 
 /* Measurement hook: average device time (ms) of one launch of a solver
- * kernel, replayed `reps` times between two hipEvents on the solver's stream
- * with the convergence stop disabled.  which: 0 finest-level SpMV + p.Ap,
- * 1 finest vector update + r.r, 2 finest down-sweep (smooth, residual,
- * restrict), 3 finest up-sweep (prolong, smooth, r.z), 4 one PCG iteration
- * including the whole V-cycle.  Use after pfsf_solver_solve; it overwrites
- * the solver's scratch vectors, not its inputs or results. */
+ * kernel over `reps` launches on the solver's stream with the convergence
+ * stop disabled.  which: 0 finest-level direction update + SpMV + p.Ap,
+ * 1 finest vector update + r.r + down-sweep (smooth, residual, restrict),
+ * 2 second-level down-sweep, 3 finest up-sweep (prolong, smooth, r.z) --
+ * each timed by the kernel's own start/stop event pair -- and 4 one PCG
+ * iteration including the whole V-cycle (events around back-to-back
+ * iterations).  Use after pfsf_solver_solve; it overwrites the solver's
+ * scratch vectors and its internal FP64 iterate, not the caller's arrays. */
 PFSF_API pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps,
                                          double* avg_ms);
 

@@ -10,6 +10,7 @@ namespace pfsf {
 // Device-resident scalars of the inner PCG; written only by kernels.
 struct PcgScalars {
   double rz, pq, alpha, beta, rr, stop2;
+  double rzs[2];  // r.z of the last two iterations (slot = iteration parity)
   int done;  // 1 converged / capped, 2 breakdown
   int iters;
   int max_iters;
@@ -66,7 +67,7 @@ class MgSolver {
   hipError_t precondition();            // z = M r[0], r.z
   hipError_t true_residual(const float* b, double* rel);
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
-  hipError_t vcycle_below(int from);   // levels from..L-1 given level `from` residual
+  hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
   hipError_t chunk(int pairs);         // 2*pairs iterations (graph replay when captured)
   void release();
 
@@ -85,8 +86,11 @@ class MgSolver {
   PcgScalars* sc_ = nullptr;
   IrScalars* ir_ = nullptr;
   PcgScalars* sc_host_ = nullptr;  // pinned: PcgScalars followed by IrScalars
-  double* partials_ = nullptr;
+  double* partials_ = nullptr;  // work-group partial sums; regions below
   int64_t npartials_ = 0;
+  double* ppq_ = nullptr;  // p.q (SpMV)
+  double* prr_ = nullptr;  // r.r (update) [+ r.z of the Jacobi update]
+  double* prz_ = nullptr;  // r.z (preconditioner)
   uint8_t* reach_ = nullptr;
   int* flag_ = nullptr;
   double* x_ = nullptr;  // FP64 iterate

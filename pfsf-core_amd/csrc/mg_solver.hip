@@ -40,12 +40,15 @@
 #include <algorithm>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "mg_solver.h"
 
 namespace pfsf {
 namespace {
 
 constexpr int BS = 256;
+constexpr int kMaxWaves = 16;  // waves of the largest work-group (1024 threads)
 // Waves per SIMD the tiled stencil kernels are register-budgeted for (LDS
 // admits six 24 KB tiles per CU).
 #ifndef PFSF_WPE
@@ -180,78 +183,94 @@ __device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& 
   return true;
 }
 
-// Load one halo cell.  A cell outside the grid is redirected to cell 0 (always
-// in bounds) and its result discarded, so the load needs no branch; val(j, x,
-// y, z) is evaluated unconditionally (it may read garbage or divide by zero
-// outside the free set; the select discards it).
-template <typename KT, typename F>
-__device__ __forceinline__ void load_cell(const Lv<KT>& L, int gx, int gy, int gz, bool ok,
-                                          F& val, float& k, float& v) {
-  const int cx = ok ? gx : 0, cy = ok ? gy : 0, cz = ok ? gz : 0;
-  const int j = flat32(cx, cy, cz, L.d);
-  const float kj = ldk32(L.kap, j);
-  const float vj = val(j, cx, cy, cz);
-  k = ok ? kj : -1.f;
-  v = k >= 0.f ? vj : 0.f;
+// Halo staging in two phases.  stage_load issues every global load of the
+// tile's halo box (kappa plus whatever raw operands ld(j, x, y, z) reads) and
+// returns them in registers; stage_put combines them with fin(raw) and writes
+// LDS, so a wave waits for memory once, after all loads are in flight.
+//
+// Rows of the box: the 32 lanes of a half-wave take the 32 interior columns of
+// one (y, z) row, eight rows per pass; one more pass covers the two halo
+// columns of every row.  A cell outside the grid is redirected to cell 0
+// (always in bounds) and discarded, so no load is behind a branch; fin may
+// see garbage or divide by zero outside the free set, and the select drops
+// it.  Cells outside the grid or the free set hold v = 0, k = -1, s = 0.
+template <int TZ>
+struct Stage {
+  static constexpr int NROWS = Tile<TZ>::NROWS;
+  static constexpr int RPP = BS / TX;
+  static constexpr int NP = (NROWS + RPP - 1) / RPP + 1;  // passes incl. halo columns
+  static_assert(2 * NROWS <= BS, "halo columns fit one pass");
+};
+
+template <int TZ, typename KT, typename Raw, typename LD>
+__device__ __forceinline__ void stage_load(const Lv<KT>& L, int x0, int y0, int z0, LD ld,
+                                           float (&kk)[Stage<TZ>::NP],
+                                           Raw (&raw)[Stage<TZ>::NP]) {
+  constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+  const pfsf_dims d = L.d;
+  const int t = threadIdx.x;
+  auto cell = [&](int p, int gx, int gy, int gz, bool ok) {
+    const int cx = ok ? gx : 0, cy = ok ? gy : 0, cz = ok ? gz : 0;
+    const int j = flat32(cx, cy, cz, d);
+    const float kj = ldk32(L.kap, j);
+    raw[p] = ld(j, cx, cy, cz);
+    kk[p] = ok ? kj : -1.f;
+  };
+  const int tx = t % TX, tr = t / TX;
+  const int gx = x0 + tx;
+  const bool inx = gx < d.lx;
+#pragma unroll
+  for (int p = 0; p < NP - 1; p++) {
+    const int row = tr + p * RPP;
+    const int ey = row % HY, ez = row / HY;
+    const int gy = y0 + ey - 1, gz = z0 + ez - 1;
+    cell(p, gx, gy, gz, row < NROWS && inx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
+  }
+  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+  const int ey = hrow % HY, ez = hrow / HY;
+  const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
+  cell(NP - 1, hx, gy, gz,
+       hrow < NROWS && hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
 }
 
 template <int TZ>
 __device__ __forceinline__ void put_cell(Tile<TZ>& T, int e, float k, float v) {
-  T.v[e] = v;
+  T.v[e] = k >= 0.f ? v : 0.f;
   T.k[e] = k;
   T.s[e] = k > 0.f ? __builtin_amdgcn_sqrtf(k) : 0.f;
 }
 
-// Stage the halo box of a tile row by row: the 32 lanes of a half-wave take
-// the 32 interior columns of one (y, z) row, eight rows per pass; one more
-// pass covers the two halo columns of every row.  Every pass's loads are
-// issued before the first LDS write, so a wave waits for memory once, not
-// once per pass.  Cells outside the grid or the free set hold v = 0, k = -1,
-// s = 0.
+template <int TZ, typename Raw, typename FIN>
+__device__ __forceinline__ void stage_put(Tile<TZ>& T, const float (&kk)[Stage<TZ>::NP],
+                                          const Raw (&raw)[Stage<TZ>::NP], FIN fin) {
+  constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+  const int t = threadIdx.x;
+  const int tx = t % TX, tr = t / TX;
+#pragma unroll
+  for (int p = 0; p < NP - 1; p++) {
+    const int row = tr + p * RPP;
+    if (p < NP - 2 || row < NROWS) put_cell<TZ>(T, row * HX + tx + 1, kk[p], fin(raw[p]));
+  }
+  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+  if (hrow < NROWS) put_cell<TZ>(T, hrow * HX + ex, kk[NP - 1], fin(raw[NP - 1]));
+}
+
+// One-phase staging: val(j, x, y, z) is the staged value itself.
 template <int TZ, typename KT, typename F>
 __device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int y0, int z0,
                                       F val) {
-  constexpr int NROWS = Tile<TZ>::NROWS;
-  constexpr int RPP = BS / TX;
-  constexpr int NP = (NROWS + RPP - 1) / RPP;
-  static_assert(2 * NROWS <= BS, "halo columns fit one pass");
-  const pfsf_dims d = L.d;
-  const int t = threadIdx.x;
-  const int tx = t % TX, tr = t / TX;
-  const int gx = x0 + tx;
-  const bool inx = gx < d.lx;
-  float kk[NP + 1], vv[NP + 1];
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    const int row = tr + p * RPP;
-    const int ey = row % HY, ez = row / HY;
-    const int gy = y0 + ey - 1, gz = z0 + ez - 1;
-    const bool ok = row < NROWS && inx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz;
-    load_cell(L, gx, gy, gz, ok, val, kk[p], vv[p]);
-  }
-  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
-  {
-    const int ey = hrow % HY, ez = hrow / HY;
-    const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
-    const bool ok = hrow < NROWS && hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 &&
-                    gz < d.lz;
-    load_cell(L, hx, gy, gz, ok, val, kk[NP], vv[NP]);
-  }
-#pragma unroll
-  for (int p = 0; p < NP; p++) {
-    const int row = tr + p * RPP;
-    if (p < NP - 1 || row < NROWS) put_cell<TZ>(T, row * HX + tx + 1, kk[p], vv[p]);
-  }
-  if (hrow < NROWS) put_cell<TZ>(T, hrow * HX + ex, kk[NP], vv[NP]);
+  float kk[Stage<TZ>::NP], vv[Stage<TZ>::NP];
+  stage_load<TZ>(L, x0, y0, z0, val, kk, vv);
+  stage_put<TZ>(T, kk, vv, [](float v) { return v; });
 }
 
-// (A v)_i at LDS cell c: difference form over free neighbours + anchor mass.
-// Offsets are compile-time constants, so every read is one LDS instruction
-// with an immediate offset from the same base.
-__device__ __forceinline__ float tile_apply(const float* __restrict__ v,
-                                            const float* __restrict__ k,
-                                            const float* __restrict__ s, int c, float ki,
-                                            float mi, float scale) {
+// (A v)_i at LDS cell c of a box with row stride sy and plane stride sz:
+// difference form over free neighbours + anchor mass.  With constant strides
+// every read is one LDS instruction with an immediate offset from one base.
+__device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
+                                               const float* __restrict__ k,
+                                               const float* __restrict__ s, int c, int sy,
+                                               int sz, float ki, float mi, float scale) {
   const float vc = v[c];
   float af = 0.f, ae = 0.f, ac = 0.f;
   auto face = [&](int o) {
@@ -263,37 +282,37 @@ __device__ __forceinline__ float tile_apply(const float* __restrict__ v,
   auto corner = [&](int o) { ac += s[c + o] * (vc - v[c + o]); };
   face(-1);
   face(1);
-  face(-HX);
-  face(HX);
-  face(-HXY);
-  face(HXY);
-  edge(-1 - HX);
-  edge(1 - HX);
-  edge(-1 + HX);
-  edge(1 + HX);
-  edge(-1 - HXY);
-  edge(1 - HXY);
-  edge(-1 + HXY);
-  edge(1 + HXY);
-  edge(-HX - HXY);
-  edge(HX - HXY);
-  edge(-HX + HXY);
-  edge(HX + HXY);
-  corner(-1 - HX - HXY);
-  corner(1 - HX - HXY);
-  corner(-1 + HX - HXY);
-  corner(1 + HX - HXY);
-  corner(-1 - HX + HXY);
-  corner(1 - HX + HXY);
-  corner(-1 + HX + HXY);
-  corner(1 + HX + HXY);
+  face(-sy);
+  face(sy);
+  face(-sz);
+  face(sz);
+  edge(-1 - sy);
+  edge(1 - sy);
+  edge(-1 + sy);
+  edge(1 + sy);
+  edge(-1 - sz);
+  edge(1 - sz);
+  edge(-1 + sz);
+  edge(1 + sz);
+  edge(-sy - sz);
+  edge(sy - sz);
+  edge(-sy + sz);
+  edge(sy + sz);
+  corner(-1 - sy - sz);
+  corner(1 - sy - sz);
+  corner(-1 + sy - sz);
+  corner(1 + sy - sz);
+  corner(-1 - sy + sz);
+  corner(1 - sy + sz);
+  corner(-1 + sy + sz);
+  corner(1 + sy + sz);
   return scale * (0.5f * af + s[c] * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
 }
 
 template <int TZ>
 __device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
                                             float scale) {
-  return tile_apply(T.v, T.k, T.s, c, ki, mi, scale);
+  return stencil_apply(T.v, T.k, T.s, c, HX, HXY, ki, mi, scale);
 }
 
 __device__ __forceinline__ int tcell(int tx, int ty, int tz) {
@@ -333,31 +352,44 @@ __device__ __forceinline__ void write_partials(double (&v)[N], double* partials,
   }
 }
 
+// Sum of n FP64 partials, returned to every thread of the work-group.  Every
+// work-group adds in the same order, so each one obtains the same value: a
+// consumer kernel can finish a producer's reduction itself instead of
+// waiting for a separate one-block kernel.
+__device__ __forceinline__ double block_total(const double* __restrict__ part, unsigned n) {
+  __shared__ double tot_sh[kMaxWaves];
+  double a = 0.0;
+  for (unsigned b = threadIdx.x; b < n; b += blockDim.x) a += part[b];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) tot_sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < nw; w++) t += tot_sh[w];
+  return t;
+}
+
+// One-work-group reductions that remain separate kernels: the Jacobi
+// (single-level) iteration, a coarsest level too large for LDS, and the
+// FP64 norms of the solve set-up and the refinement passes.
 enum FinMode {
-  FIN_ALPHA = 0,   // p.q -> alpha = rz / pq
-  FIN_UPDATE,      // r.r -> rr, iteration count, convergence
-  FIN_UPDATE_JAC,  // r.r, r.z -> as FIN_UPDATE plus beta, rz (Jacobi M)
-  FIN_BETA,        // r.z -> beta = rz_new / rz, rz
-  FIN_BNORM,       // b.b -> bb
-  FIN_IR,          // r.r -> rr, rel
+  FIN_ALPHA = 0,  // p.q -> alpha = rz / pq
+  FIN_UPDATE,     // r.r -> rr, iteration count, convergence
+  FIN_BNORM,      // b.b -> bb
+  FIN_IR,         // r.r -> rr, rel
 };
 
 __global__ void __launch_bounds__(BS)
     k_finalize(int mode, const double* __restrict__ partials, unsigned nblk, PcgScalars* sc,
                IrScalars* ir) {
-  if (mode <= FIN_BETA && sc->done) return;
-  __shared__ double sh[2][BS / 64];
-  double v[2] = {0.0, 0.0};
-  int nv = (mode == FIN_UPDATE_JAC) ? 2 : 1;
-  for (int k = 0; k < nv; k++)
-    for (unsigned b = threadIdx.x; b < nblk; b += BS) v[k] += partials[(int64_t)k * nblk + b];
-  block_sum<2>(v, sh);
+  if (mode <= FIN_UPDATE && sc->done) return;
+  const double v = block_total(partials, nblk);
   if (threadIdx.x != 0) return;
   switch (mode) {
     case FIN_ALPHA:
-      sc->pq = v[0];
-      if (v[0] > 0.0) {
-        sc->alpha = sc->rz / v[0];
+      sc->pq = v;
+      if (v > 0.0) {
+        sc->alpha = sc->rz / v;
         sc->have_dir = 1;
       } else {
         sc->alpha = 0.0;
@@ -365,25 +397,16 @@ __global__ void __launch_bounds__(BS)
       }
       break;
     case FIN_UPDATE:
-    case FIN_UPDATE_JAC:
-      sc->rr = v[0];
+      sc->rr = v;
       sc->iters += 1;
-      if (mode == FIN_UPDATE_JAC) {
-        sc->beta = (sc->rz > 0.0) ? v[1] / sc->rz : 0.0;
-        sc->rz = v[1];
-      }
-      if (v[0] <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
-      break;
-    case FIN_BETA:
-      sc->beta = (sc->have_dir && sc->rz > 0.0) ? v[0] / sc->rz : 0.0;
-      sc->rz = v[0];
+      if (v <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
       break;
     case FIN_BNORM:
-      ir->bb = v[0];
+      ir->bb = v;
       break;
     case FIN_IR:
-      ir->rr = v[0];
-      ir->rel = ir->bb > 0.0 ? sqrt(v[0] / ir->bb) : sqrt(v[0]);
+      ir->rr = v;
+      ir->rel = ir->bb > 0.0 ? sqrt(v / ir->bb) : sqrt(v);
       break;
   }
 }
@@ -394,34 +417,52 @@ __global__ void __launch_bounds__(BS)
 // depends on the free mask; only the stores are guarded by the grid bounds.
 
 // p_out = z + beta p_in (z alone for the first direction); q = A p_out; p.q.
+// beta = (r.z) / (r.z)_previous is formed here from the r.z partials of the
+// preconditioner (rz_n work-groups); work-group 0 publishes r.z in slot
+// `slot` (the iteration parity) and beta.
 template <int TZ, typename KT>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_spmv_dir(Lv<KT> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
-               float* __restrict__ pout, float* __restrict__ q, const PcgScalars* sc,
-               double* partials) {
+               float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
+               const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
   if (sc->done) return;
   __shared__ Tile<TZ> T;
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
-  int x0, y0, z0;
-  if (tile_origin<TZ>(M, x0, y0, z0)) {
-    const bool first = !sc->have_dir;
-    const float beta = first ? 0.f : (float)sc->beta;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const pfsf_dims d = L.d;
-    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
-    const bool col = gx < d.lx && gy < d.ly;
-    const int i0 = flat32(gx, gy, z0, d);
-    bool inb[TZ];
-    float amr[TZ];  // own-voxel operands, loaded before the barrier (cell 0 stands in
-                    // for a cell outside the grid; its value is never used)
+  const bool first = !sc->have_dir;
+  const double rz_old = sc->rzs[slot ^ 1];
+  const double rz = block_total(rz_part, rz_n);
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const pfsf_dims d = L.d;
+  const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+  const bool col = gx < d.lx && gy < d.ly;
+  const int i0 = flat32(gx, gy, z0, d);
+  bool inb[TZ];
+  float amr[TZ];  // own-voxel operands (cell 0 stands in for a cell outside the grid)
+  float kk[Stage<TZ>::NP];
+  float2 raw[Stage<TZ>::NP];
+  if (have) {
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       inb[tz] = col && z0 + tz < d.lz;
       amr[tz] = ld(L.am, inb[tz] ? i0 + tz * sz : 0);
     }
+    stage_load<TZ>(L, x0, y0, z0,
+                   [&](int j, int, int, int) { return make_float2(ld(z, j), ld(pin, j)); }, kk,
+                   raw);
+  }
+  const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rzs[slot] = rz;
+    sc->rz = rz;
+    sc->beta = beta64;
+  }
+  if (have) {
     // p_in is finite everywhere (zero-initialised), so beta = 0 drops it
-    stage<TZ>(T, L, x0, y0, z0, [&](int j, int, int, int) { return ld(z, j) + beta * ld(pin, j); });
+    const float beta = (float)beta64;
+    stage_put<TZ>(T, kk, raw, [&](float2 v) { return v.x + beta * v.y; });
     __syncthreads();
     const int c0 = tcell(tx, ty, 0);
 #pragma unroll
@@ -462,43 +503,75 @@ __device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, in
 // on the fly, and the coarse right-hand side is rc = P^T (r - A x).
 // With UPDATE (finest level inside an iteration): first x64 += alpha p and
 // r_out = r_in - alpha q, then the same on r_out, plus r.r for convergence.
+// alpha = (r.z) / (p.q) is formed here from the p.q partials of the SpMV
+// (pq_n work-groups); work-group 0 publishes it (or flags a breakdown when
+// p.q <= 0).
 template <int TZ, typename KT, bool UPDATE>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_down(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ rin, float* __restrict__ rout,
            const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
-           float* __restrict__ rc, const PcgScalars* sc, double* partials) {
+           float* __restrict__ rc, PcgScalars* sc, const double* __restrict__ pq_part,
+           unsigned pq_n, double* partials) {
   if (sc->done) return;
   __shared__ Tile<TZ> T;
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
-  int x0, y0, z0;
+  const double rz = UPDATE ? sc->rz : 0.0;
+  const double pq0 = UPDATE ? block_total(pq_part, pq_n) : 0.0;
   const pfsf_dims d = L.d;
   const float om = L.omega;
-  if (tile_origin<TZ>(M, x0, y0, z0)) {
-    const float alpha = UPDATE ? (float)sc->alpha : 0.f;
-    const double alpha64 = UPDATE ? sc->alpha : 0.0;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
-    const bool col = gx < d.lx && gy < d.ly;
-    const int i0 = flat32(gx, gy, z0, d);
-    // own-voxel operands, loaded before the barrier (cell 0 stands in for a
-    // cell outside the grid; its values are never used)
-    bool inb[TZ];
-    float rr_[TZ], amr[TZ], pr[TZ];
-    double xr[TZ];
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+  const bool col = gx < d.lx && gy < d.ly;
+  const int i0 = flat32(gx, gy, z0, d);
+  // own-voxel operands (cell 0 stands in for a cell outside the grid; its
+  // values are never used)
+  bool inb[TZ];
+  float ro[TZ], qo[TZ], amr[TZ], pr[TZ];
+  double xr[TZ];
+  float kk[Stage<TZ>::NP];
+  float3 raw[Stage<TZ>::NP];  // r_in, q, diagonal
+  if (have) {
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       inb[tz] = col && z0 + tz < d.lz;
       const int i = inb[tz] ? i0 + tz * sz : 0;
-      rr_[tz] = UPDATE ? ld(rin, i) - alpha * ld(q, i) : ld(rin, i);
+      ro[tz] = ld(rin, i);
+      qo[tz] = UPDATE ? ld(q, i) : 0.f;
       amr[tz] = ld(L.am, i);
       pr[tz] = UPDATE ? ld(p, i) : 0.f;
       xr[tz] = UPDATE ? ld(x, i) : 0.0;
     }
-    stage<TZ>(T, L, x0, y0, z0, [&](int j, int, int, int) {
-      const float rj = UPDATE ? ld(rin, j) - alpha * ld(q, j) : ld(rin, j);
-      return om * rj * rcp(ld(L.dg, j));
+    stage_load<TZ>(L, x0, y0, z0,
+                   [&](int j, int, int, int) {
+                     return make_float3(ld(rin, j), UPDATE ? ld(q, j) : 0.f, ld(L.dg, j));
+                   },
+                   kk, raw);
+  }
+  double alpha64 = 0.0;
+  if (UPDATE) {
+    const double pq = pq0;
+    const bool ok = pq > 0.0;
+    alpha64 = ok ? rz / pq : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = alpha64;
+      sc->have_dir = 1;
+      if (!ok) sc->done = 2;  // breakdown: the direction carries no energy
+    }
+    if (!ok) return;
+  }
+  if (have) {
+    const float alpha = (float)alpha64;
+    stage_put<TZ>(T, kk, raw, [&](float3 v) {
+      const float rj = UPDATE ? v.x - alpha * v.y : v.x;
+      return om * rj * rcp(v.z);
     });
+    float rr_[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) rr_[tz] = UPDATE ? ro[tz] - alpha * qo[tz] : ro[tz];
     if (UPDATE) {
       // p and q are zero outside the free set, so the update leaves x there
       // unchanged and carries r there over
@@ -515,10 +588,10 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
     float res[TZ];
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
-      const int c = c0 + tz * HXY;
-      const float ki = T.k[c], ri = rr_[tz];
+      const int cc = c0 + tz * HXY;
+      const float ki = T.k[cc], ri = rr_[tz];
       const bool fr = ki >= 0.f;
-      const float ax = tile_apply<TZ>(T, c, ki, amr[tz], L.scale);
+      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
       res[tz] = fr ? ri - ax : 0.f;
       if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
     }
@@ -663,29 +736,58 @@ __global__ void __launch_bounds__(BS)
 }
 
 // Whole Chebyshev solve of the coarsest level held in one work-group's LDS
-// (n <= kMaxLdsCoarse cells, one cell per thread): kappa, sqrt kappa,
-// diagonal, anchor mass, rhs, iterate and direction live in LDS; coordinates
-// and boundary flags are computed once, outside the step loop.
+// (n <= kMaxLdsCoarse cells, one cell per thread).  Kappa, sqrt kappa and the
+// iterate sit in LDS boxes with a one-cell border (kappa -1, value 0), so the
+// 26 neighbour reads need no bounds tests; the diagonal, anchor mass, rhs and
+// direction of a thread's own cell stay in registers.
 constexpr int kMaxLdsCoarse = 1024;
+constexpr int kMaxLdsPadded = 2048;
 
+__host__ __device__ inline int padded_cells(pfsf_dims d) {
+  return (d.lx + 2) * (d.ly + 2) * (d.lz + 2);
+}
+
+//
+// With rr_n > 0 it first finishes the PCG iteration's convergence test from
+// the r.r partials of the finest update (rr_n work-groups): iteration count,
+// r.r, and the stop flag, which also ends this V-cycle.
 template <typename KT>
 __global__ void __launch_bounds__(kMaxLdsCoarse)
     k_cheb_lds(Lv<KT> L, const float* __restrict__ r, float* __restrict__ out,
-               const float* __restrict__ coef, int steps, const PcgScalars* sc) {
+               const float* __restrict__ coef, int steps, PcgScalars* sc,
+               const double* __restrict__ rr_part, unsigned rr_n) {
   if (sc->done) return;
-  __shared__ float sk[kMaxLdsCoarse], ss[kMaxLdsCoarse], sx[2][kMaxLdsCoarse];
+  if (rr_n > 0) {
+    const double rr = block_total(rr_part, rr_n);
+    const int it = sc->iters + 1;
+    const bool stop = rr <= sc->stop2 || it >= sc->max_iters;
+    __syncthreads();  // every thread has read the scalars
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      sc->iters = it;
+      if (stop) sc->done = 1;
+    }
+    if (stop) return;
+  }
+  __shared__ float sk[kMaxLdsPadded], ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
   const pfsf_dims d = L.d;
+  const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
+  for (int e = threadIdx.x; e < np; e += blockDim.x) {
+    sk[e] = -1.f;
+    ss[e] = 0.f;
+    sx[0][e] = 0.f;
+    sx[1][e] = 0.f;
+  }
+  __syncthreads();
   const int n = d.lx * d.ly * d.lz;
   const int i = threadIdx.x;
   const bool act = i < n;
-  int x = 0, y = 0, z = 0;
-  float ki = -1.f, si = 0.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
+  int e = 0;
+  float ki = -1.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
   if (act) {
-    x = i % d.lx;
-    y = (i / d.lx) % d.ly;
-    z = i / (d.lx * d.ly);
+    const int x = i % d.lx, y = (i / d.lx) % d.ly, z = i / (d.lx * d.ly);
+    e = (x + 1) + (y + 1) * sy + (z + 1) * sz;
     ki = ldk(L.kap, i);
-    si = ki > 0.f ? sqrtf(ki) : 0.f;
     if (ki >= 0.f) {
       D = L.dg[i];
       m = L.am[i];
@@ -693,50 +795,25 @@ __global__ void __launch_bounds__(kMaxLdsCoarse)
       di = coef[1] * ri * rcp(D);
       xi = di;
     }
-    sk[i] = ki;
-    ss[i] = si;
-    sx[0][i] = xi;
+    sk[e] = ki;
+    ss[e] = ki > 0.f ? __builtin_amdgcn_sqrtf(ki) : 0.f;
+    sx[0][e] = xi;
   }
-  const bool xm = x > 0, xp = x + 1 < d.lx, ym = y > 0, yp = y + 1 < d.ly, zm = z > 0,
-             zp = z + 1 < d.lz;
-  const int sy = d.lx, sz = d.lx * d.ly;
+  const bool live = act && ki >= 0.f;
   __syncthreads();
   int cur = 0;
   for (int st = 1; st < steps; st++) {
     const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
-    const float* xin = sx[cur];
-    if (act && ki >= 0.f) {
-      float af = 0.f, ae = 0.f, ac = 0.f;
-#pragma unroll
-      for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-          for (int dx = -1; dx <= 1; dx++) {
-            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
-            if (cls == 0) continue;
-            bool ok = (dx < 0 ? xm : (dx > 0 ? xp : true)) && (dy < 0 ? ym : (dy > 0 ? yp : true)) &&
-                      (dz < 0 ? zm : (dz > 0 ? zp : true));
-            if (!ok) continue;
-            int j = i + dx + dy * sy + dz * sz;
-            float kj = sk[j];
-            float dv = xi - xin[j];
-            if (cls == 1)
-              af += (kj >= 0.f ? ki + kj : 0.f) * dv;
-            else if (cls == 2)
-              ae += ss[j] * dv;
-            else
-              ac += ss[j] * dv;
-          }
-      float ax = L.scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + m * xi;
+    if (live) {
+      const float ax = stencil_apply(sx[cur], sk, ss, e, sy, sz, ki, m, L.scale);
       di = c1 * di + c2 * (ri - ax) * rcp(D);
       xi = xi + di;
+      sx[cur ^ 1][e] = xi;
     }
-    if (act) sx[cur ^ 1][i] = xi;
     __syncthreads();
     cur ^= 1;
   }
-  if (act) out[i] = ki >= 0.f ? xi : 0.f;
+  if (act) out[i] = live ? xi : 0.f;
 }
 
 // ── setup kernels ──────────────────────────────────────────────────────
@@ -947,6 +1024,7 @@ __global__ void k_pcg_begin(PcgScalars* sc, const IrScalars* ir, double stop_rel
   if (reset_count) sc->iters = 0;
   sc->have_dir = 0;
   sc->rz = 0.0;
+  sc->rzs[0] = sc->rzs[1] = 0.0;
   sc->beta = 0.0;
   sc->alpha = 0.0;
   sc->rr = ir->rr;
@@ -1108,8 +1186,11 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
     maxblk = std::max(maxblk, nt + 8);
     d = {(d.lx + 1) / 2, (d.ly + 1) / 2, (d.lz + 1) / 2};
   }
-  npartials_ = 3 * maxblk + 64;
+  npartials_ = 4 * maxblk + 64;
   partials_ = dalloc<double>(npartials_, owned_, err);
+  ppq_ = partials_;
+  prr_ = partials_ + maxblk;
+  prz_ = partials_ + 3 * maxblk;
   coef_ = dalloc<float>(2 * 64, owned_, err);
   if (err != hipSuccess) return err;
   // every work vector starts at zero: tiles without free voxels are never
@@ -1264,7 +1345,7 @@ static void launch_down_t(const MgLevel& A, const MgLevel& B, const float* r, Pc
                           hipStream_t s) {
   PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
                            view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
-                           sc, nullptr)));
+                           sc, nullptr, 0u, nullptr)));
 }
 
 static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
@@ -1293,17 +1374,26 @@ static void launch_up(const MgLevel& A, const MgLevel& B, const float* r, float*
 // Coarse part of the V-cycle: down-sweeps from level `from` (whose residual
 // is already in levels_[from].r), the coarsest solve, and up-sweeps back to
 // level `from`.  On return levels_[from].e holds that level's correction.
-hipError_t MgSolver::vcycle_below(int from) {
+// With `check`, the iteration's convergence test (r.r partials of the finest
+// update in prr_) runs first: inside the one-work-group coarsest solve when
+// it fits LDS, else as a finalize kernel.
+hipError_t MgSolver::vcycle_below(int from, bool check) {
   hipStream_t s = stream_;
+  const unsigned G = levels_[0].tiles.nlaunch;
+  MgLevel& C = levels_[nlev_ - 1];
+  const bool lds = C.n <= kMaxLdsCoarse && padded_cells(C.d) <= kMaxLdsPadded;
+  if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_);
   for (int l = from; l < nlev_ - 1; l++)
     launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
-  MgLevel& C = levels_[nlev_ - 1];
   int st = std::min(coarse_sweeps_, 63);
-  if (C.n <= kMaxLdsCoarse) {
+  const unsigned rr_n = (check && lds) ? G : 0u;
+  if (lds) {
     if (C.bf16)
-      k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st, sc_);
+      k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st,
+                                                        sc_, prr_, rr_n);
     else
-      k_cheb_lds<float><<<1, kMaxLdsCoarse, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_);
+      k_cheb_lds<float><<<1, kMaxLdsCoarse, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_,
+                                                     prr_, rr_n);
   } else {
     float* xin = C.xb;
     float* xout = C.e;
@@ -1332,17 +1422,16 @@ hipError_t MgSolver::precondition() {
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
   if (nlev_ == 1) {
+    // r.z goes where the Jacobi update leaves it, behind its r.r partials
     unsigned nb = blocks_for(n0_, BS);
-    k_jacobi_apply<<<nb, BS, 0, s>>>(n0_, r_[0], F.dg, z_, partials_);
-    k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, nb, sc_, ir_);
+    k_jacobi_apply<<<nb, BS, 0, s>>>(n0_, r_[0], F.dg, z_, prr_ + nb);
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
   launch_down(F, B, r_[0], sc_, s);
-  (void)vcycle_below(1);
+  (void)vcycle_below(1, false);
   PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<F.tiles.nlaunch, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[0], B.e, z_, sc_, partials_)));
-  k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, F.tiles.nlaunch, sc_, ir_);
+                          view<float>(F), tl(F), B.d, r_[0], B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
 
@@ -1352,25 +1441,27 @@ hipError_t MgSolver::iteration(int c) {
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
   const unsigned G = F.tiles.nlaunch;
-  PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, partials_)));
-  k_finalize<<<1, BS, 0, s>>>(FIN_ALPHA, partials_, G, sc_, ir_);
   if (nlev_ == 1) {
-    unsigned nb = blocks_for(n0_, BS);
+    const unsigned nb = blocks_for(n0_, BS);
+    PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
+                            view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prr_ + nb, nb,
+                            c, ppq_)));
+    k_finalize<<<1, BS, 0, s>>>(FIN_ALPHA, ppq_, G, sc_, ir_);
     k_update_jacobi<<<nb, BS, 0, s>>>(n0_, F.dg, x_, r_[c], r_[1 - c], p_[1 - c], q_, z_, sc_,
-                                      partials_);
-    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE_JAC, partials_, nb, sc_, ir_);
+                                      prr_);
+    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, nb, sc_, ir_);
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
+  PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
+                          view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prz_, G, c,
+                          ppq_)));
   PFSF_TZ_DISPATCH(F, (k_down<TZV, float, true><<<G, BS, 0, s>>>(
                           view<float>(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
-                          sc_, partials_)));
-  k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, partials_, G, sc_, ir_);
-  (void)vcycle_below(1);
+                          sc_, ppq_, G, prr_)));
+  (void)vcycle_below(1, true);
   PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, partials_)));
-  k_finalize<<<1, BS, 0, s>>>(FIN_BETA, partials_, G, sc_, ir_);
+                          view<float>(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
 
@@ -1489,11 +1580,14 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   return err;
 }
 
-// Average device time of one launch of a solver kernel, replayed `reps` times
-// between two events on the solver stream.  which: 0 finest SpMV + direction
-// + p.q, 1 finest update + down-sweep (or the Jacobi update), 2 second-level
-// down-sweep, 3 finest up-sweep + r.z, 4 one whole PCG iteration.  Call after
-// a solve so the vectors hold finite data.
+// Average device time of one launch of a solver kernel over `reps` launches
+// on the solver stream.  which: 0 finest SpMV + direction + p.q, 1 finest
+// update + down-sweep (or the Jacobi update), 2 second-level down-sweep,
+// 3 finest up-sweep + r.z: each launch is bracketed by the kernel's own
+// start/stop timestamps (hipExtLaunchKernel events), so the figure is the
+// kernel duration alone.  4: one whole PCG iteration, timed between events
+// around `reps` back-to-back iterations (launch gaps included).  Call after a
+// solve so the vectors hold finite data.
 hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
@@ -1502,44 +1596,81 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   if (which < 0 || which > 4 || reps <= 0) return hipErrorInvalidValue;
   if ((which == 2 || which == 3) && nlev_ < 2) return hipErrorInvalidValue;
   if (which == 2 && nlev_ < 3) return hipErrorInvalidValue;
-  auto once = [&]() {
+  k_profile_begin<<<1, 1, 0, s>>>(sc_);
+  if (which == 4) {
+    (void)iteration(0);  // warm
+    chk(hipEventRecord(ev0_, s), err);
+    for (int k = 0; k < reps; k++) (void)iteration(0);
+    chk(hipEventRecord(ev1_, s), err);
+    chk(hipEventSynchronize(ev1_), err);
+    float ms = 0.f;
+    chk(hipEventElapsedTime(&ms, ev0_, ev1_), err);
+    *avg_ms = ms / (float)reps;
+    return err;
+  }
+  auto ext = [&](auto kern, unsigned grid, hipEvent_t a, hipEvent_t b, auto... args) {
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(BS), 0, s, a, b, 0, args...);
+  };
+  auto once = [&](hipEvent_t a, hipEvent_t b) {
     switch (which) {
       case 0:
-        PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
-                                view<float>(F), tl(F), z_, p_[0], p_[1], q_, sc_, partials_)));
+        PFSF_TZ_DISPATCH(F, ext(k_spmv_dir<TZV, float>, G, a, b, view<float>(F), tl(F),
+                                (const float*)z_, (const float*)p_[0], p_[1], q_, sc_,
+                                (const double*)prz_, G, 0, ppq_));
         break;
       case 1:
         if (nlev_ == 1) {
-          unsigned nb = blocks_for(n0_, BS);
-          k_update_jacobi<<<nb, BS, 0, s>>>(n0_, F.dg, x_, r_[0], r_[1], p_[1], q_, z_, sc_,
-                                            partials_);
+          const unsigned nb = blocks_for(n0_, BS);
+          ext(k_update_jacobi, nb, a, b, n0_, (const float*)F.dg, x_, (const float*)r_[0], r_[1],
+              (const float*)p_[1], (const float*)q_, z_, (const PcgScalars*)sc_, prr_);
         } else {
-          PFSF_TZ_DISPATCH(F, (k_down<TZV, float, true><<<G, BS, 0, s>>>(
-                                  view<float>(F), tl(F), levels_[1].d, r_[0], r_[1], q_, p_[1],
-                                  x_, levels_[1].r, sc_, partials_)));
+          PFSF_TZ_DISPATCH(F, ext(k_down<TZV, float, true>, G, a, b, view<float>(F), tl(F),
+                                  levels_[1].d, (const float*)r_[0], r_[1], (const float*)q_,
+                                  (const float*)p_[1], x_, levels_[1].r, sc_,
+                                  (const double*)ppq_, G, prr_));
         }
         break;
-      case 2:
-        launch_down(levels_[1], levels_[2], levels_[1].r, sc_, s);
+      case 2: {
+        const MgLevel& A = levels_[1];
+        const MgLevel& B = levels_[2];
+        if (A.bf16) {
+          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, uint16_t, false>, A.tiles.nlaunch, a, b,
+                                  view<uint16_t>(A), tl(A), B.d, (const float*)A.r,
+                                  (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
+                                  (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
+                                  (double*)nullptr));
+        } else {
+          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, float, false>, A.tiles.nlaunch, a, b,
+                                  view<float>(A), tl(A), B.d, (const float*)A.r,
+                                  (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
+                                  (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
+                                  (double*)nullptr));
+        }
         break;
-      case 3:
-        PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<G, BS, 0, s>>>(
-                                view<float>(F), tl(F), levels_[1].d, r_[1], levels_[1].e, z_, sc_,
-                                partials_)));
-        break;
+      }
       default:
-        (void)iteration(0);
+        PFSF_TZ_DISPATCH(F, ext(k_up<TZV, float, true>, G, a, b, view<float>(F), tl(F),
+                                levels_[1].d, (const float*)r_[1], (const float*)levels_[1].e,
+                                z_, (const PcgScalars*)sc_, prz_));
     }
   };
-  k_profile_begin<<<1, 1, 0, s>>>(sc_);
-  once();  // warm
-  chk(hipEventRecord(ev0_, s), err);
-  for (int k = 0; k < reps; k++) once();
-  chk(hipEventRecord(ev1_, s), err);
-  chk(hipEventSynchronize(ev1_), err);
-  float ms = 0.f;
-  chk(hipEventElapsedTime(&ms, ev0_, ev1_), err);
-  *avg_ms = ms / (float)reps;
+  once(nullptr, nullptr);  // warm
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  for (auto& e : ev) chk(hipEventCreate(&e), err);
+  if (err == hipSuccess) {
+    for (int k = 0; k < reps; k++) once(ev[2 * k], ev[2 * k + 1]);
+    chk(hipGetLastError(), err);
+    chk(hipEventSynchronize(ev.back()), err);
+  }
+  double total = 0.0;
+  for (int k = 0; k < reps && err == hipSuccess; k++) {
+    float ms = 0.f;
+    chk(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), err);
+    total += ms;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  *avg_ms = (float)(total / reps);
   return err;
 }
 
