@@ -33,6 +33,19 @@ METRIC = "time-to-1e-6 residual at 128^3 + batched islands/s vs plain, % of HBM 
 #                 write z 4                                                  = 20.5
 ALG_BYTES = {0: 24.0, 1: 44.5, 3: 20.5}
 KERNEL_NAMES = {0: "k_spmv_dir", 1: "k_down(update)", 3: "k_up(finest)"}
+# HBM bytes per launch of these kernels on this workload, from the committed
+# FETCH_SIZE / WRITE_SIZE counter summary (copy-kernel calibrated).
+TRAFFIC_FILE = ROOT / "profiles" / "solve_128_hbm_traffic.md"
+
+
+def measured_traffic():
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("bench-traffic:"):
+                return {k: int(v) for k, v in (t.split("=") for t in line.split()[1:])}
+    except OSError:
+        pass
+    return {}
 # FP64 Jacobi-PCG iterations the CPU oracle needs on this workload to reach
 # 1e-6 (counted once with oracle.pcg(workload(1), tol=1e-6) on the full solve:
 # 1032 iterations, true relative residual 9.72e-7).
@@ -121,6 +134,7 @@ def main():
     iter_ms = solver.profile(4, reps=30)
     dom = max(prof, key=prof.get)
     ach = ALG_BYTES[dom] * n_act / (prof[dom] * 1e-3)
+    traffic = measured_traffic().get(KERNEL_NAMES[dom])
     # whole-iteration algorithmic rate
     it_bytes = sum(ALG_BYTES.values()) * n_act
     hist = np.bincount(code.cpu().numpy().ravel(), minlength=5).tolist()
@@ -149,8 +163,13 @@ def main():
         "failure_histogram": hist,
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom],
                      "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4), "traffic": None,
-                     "alg_bytes_per_active_voxel": ALG_BYTES[dom]},
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_source": TRAFFIC_FILE.name if traffic else None,
+                     "hbm_GBps_measured_traffic": (round(traffic / (prof[dom] * 1e-3) / 1e9, 1)
+                                                   if traffic else None),
+                     "alg_bytes_per_active_voxel": ALG_BYTES[dom],
+                     "alg_bytes_per_launch": round(ALG_BYTES[dom] * n_act),
+                     "kernel_us": round(prof[dom] * 1e3, 3)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(isl)
