@@ -618,8 +618,7 @@ const char* pfsf_version(void) { return "0.1.0-hip"; }
 uint32_t pfsf_abi_version(void) { return (1u << 16) | (6u << 8) | 0u; }
 bool pfsf_has_feature(const char* name) {
   if (!name) return false;
-  static const char* kFeat[] = {"hip.gfx950",       "solve.mg_pcg",  "solve.batched",
-                                "solve.mixed_bf16", "solve.adjoint", "scan.compact"};
+  static const char* kFeat[] = {"hip.gfx950", "solve.mg_pcg", "solve.mixed_bf16", "scan.compact"};
   for (const char* f : kFeat)
     if (strcmp(f, name) == 0) return true;
   return false;
