@@ -154,6 +154,55 @@ PFSF_API void pfsf_solver_destroy(pfsf_solver s);
 PFSF_API pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps,
                                          double* avg_ms);
 
+/* ── Batched multi-island solve ──────────────────────────────────────────
+ * Replaces the reference tick's sequential per-island loop
+ * (NP/src/pfsf_engine.cpp:424-600: one island's fixed 16-step schedule after
+ * another) with one solve of many independent islands: every stencil launch
+ * covers the tiles of all islands still iterating, each island keeps its own
+ * PCG scalars, multigrid hierarchy and convergence test (the same numerics
+ * as pfsf_solver_*), and the iteration launches are replayed from one HIP
+ * graph.  Every island gets at least two multigrid levels. */
+typedef struct {
+  pfsf_dims dims;
+  const uint8_t* type; /* device, N */
+  const float* kappa;  /* device, N */
+} pfsf_batch_island;
+
+typedef struct {
+  int32_t count;
+  const pfsf_batch_island* islands; /* host array of `count` entries */
+  int32_t coarse_sweeps;            /* Chebyshev sweeps on each coarsest level (0 = 12) */
+  double tol;                       /* per-island ||b - A x|| / ||b|| target */
+  int32_t max_iters;                /* cap on PCG iterations per island */
+  void* stream;
+} pfsf_batch_desc;
+
+typedef struct {
+  int32_t islands;
+  int32_t converged;       /* islands that reached tol */
+  int32_t max_iterations;  /* largest per-island PCG iteration count */
+  int32_t sweeps;          /* batched iterations executed (graph steps) */
+  int64_t total_iterations;
+  double max_rel_residual;
+  double solve_ms;
+} pfsf_batch_info;
+
+typedef struct pfsf_batch_t* pfsf_batch;
+
+/* Builds every island's hierarchy in one pooled device allocation. */
+PFSF_API pfsf_result pfsf_batch_create(const pfsf_batch_desc* desc, pfsf_batch* out);
+
+/* b, x64, phi32: host arrays of `count` device pointers (N_i elements each).
+ * x64[i] is island i's initial guess and FP64 solution (NULL entry or NULL
+ * array: start from 0, no FP64 output); phi32[i] receives the FP32 solution
+ * (may be NULL).  per_island (may be NULL) receives `count` records.
+ * Synchronous. */
+PFSF_API pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, double* const* x64,
+                                      float* const* phi32, pfsf_batch_info* info,
+                                      pfsf_solve_info* per_island);
+
+PFSF_API void pfsf_batch_destroy(pfsf_batch s);
+
 /* Orphan mask used by the solver: 1 for solid voxels 26-connected to an
  * anchor through non-air voxels, else 0 (device, N bytes).  Synchronous. */
 PFSF_API pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach,

@@ -1,0 +1,100 @@
+// Multi-island batch solver (host side of batch_solver.hip).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "mg_solver.h"
+
+namespace pfsf {
+
+// One island at one multigrid level, as the batched kernels see it.
+struct BatchLevel {
+  pfsf_dims d;        // this level
+  const float* kap;   // kappa on the free set, -1 elsewhere
+  const float* dg;    // diagonal
+  const float* am;    // anchor mass
+  float scale, omega;
+  pfsf_dims c;        // next coarser level
+  float* r;           // this level's right-hand side (level 0: the island's r[0])
+  float* e;           // this level's correction (levels >= 1)
+  float* rc;          // next level's right-hand side
+  const float* ec;    // next level's correction
+  int island;
+  int pad_;
+};
+
+// Per-island vectors and the caller's arrays for the current solve.
+struct BatchIsland {
+  pfsf_dims d;
+  int64_t n;
+  const uint8_t* type;
+  const float* kappa;
+  const uint8_t* fr;  // free mask of the finest level
+  float *r0, *r1, *p0, *p1, *q, *z;
+  double* x;          // FP64 iterate
+  const float* b;     // caller: right-hand side
+  double* xio;        // caller: initial guess in, solution out (may be null)
+  float* phio;        // caller: FP32 solution out (may be null)
+  int coarsest;       // index of the island's coarsest BatchLevel
+  int tb, tn;         // the island's finest tiles: partial-sum slots [tb, tb + tn)
+};
+
+// One work-group of a batched stencil launch.
+struct BatchTile {
+  int level;  // BatchLevel index, -1 for padding
+  int x0, y0, z0;
+  int slot;   // partial-sum slot (finest level)
+};
+
+// Per-island outcome of a batched solve.
+struct BatchState {
+  double bb;        // ||b||^2 over the free set
+  double rel;       // last true relative residual
+  double prev_rel;
+  int total;        // PCG iterations over all passes
+  int passes;
+  int finished;     // 1: converged or given up
+  int converged;
+};
+
+class BatchSolver {
+ public:
+  ~BatchSolver();
+  hipError_t create(const pfsf_batch_desc& desc);
+  hipError_t solve(const float* const* b, double* const* x64, float* const* phi32,
+                   pfsf_batch_info* info, pfsf_solve_info* per_island);
+  int islands() const { return (int)isl_.size(); }
+
+ private:
+  hipError_t iteration(int c);
+  hipError_t vcycle_coarse(bool check);
+  hipError_t precondition();
+  void release();
+
+  pfsf_batch_desc desc_{};
+  hipStream_t stream_ = nullptr;
+  std::vector<std::unique_ptr<MgSolver>> isl_;
+  std::vector<BatchIsland> host_isl_;
+  std::vector<int> nlev_;
+  int lmax_ = 0;
+  void* pool_mem_ = nullptr;
+  std::vector<void*> owned_;
+  BatchLevel* levels_ = nullptr;
+  BatchIsland* isl_d_ = nullptr;
+  std::vector<BatchTile*> tiles_;  // per level: launch list
+  std::vector<int> ntiles_;        // per level: launch size
+  int nslots_ = 0;
+  PcgScalars* sc_ = nullptr;
+  BatchState* st_ = nullptr;
+  double *ppq_ = nullptr, *prr_ = nullptr, *prz_ = nullptr;
+  float* coef_ = nullptr;
+  int coarse_sweeps_ = 12;
+  int* live_ = nullptr;       // device: islands still iterating / restarted
+  int* live_host_ = nullptr;  // pinned
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+}  // namespace pfsf

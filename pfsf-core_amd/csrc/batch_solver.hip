@@ -1,0 +1,527 @@
+// Batched multi-island solve: many independent islands, one set of launches.
+//
+// The reference tick solves dirty islands one after another, each with its
+// own fixed schedule of small dispatches (NP/src/pfsf_engine.cpp:424-600);
+// small islands leave the GPU idle between launches.  Here every island keeps
+// the single-system numerics of mg_solver.hip (FP32 MG-preconditioned PCG
+// with an FP64 iterate, FP64 restarts, its own scalars and convergence test),
+// but each stencil launch covers the tiles of all islands at once: a tile
+// record names the island's level descriptor and the tile origin, and a
+// converged island's work-groups return at once.  Islands have between 2 and
+// 5 levels; level-l launches list the tiles of the islands deeper than l+1,
+// and one launch solves every island's coarsest level (one work-group each).
+// Two iterations plus a live-island count are captured in one HIP graph.
+// Per-island set-up, the FP64 residual and the solution copies run one
+// work-group per island.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "batch_solver.h"
+#include "mg_kernels.h"
+
+namespace pfsf {
+namespace {
+
+constexpr int kTZFine = 4;    // tile depth of every island's finest level
+constexpr int kTZCoarse = 2;  // tile depth of the coarse levels
+constexpr int kIslBS = 512;   // threads of the one-work-group-per-island kernels
+
+__device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
+  Lv<float> L;
+  L.d = D.d;
+  L.kap = D.kap;
+  L.dg = D.dg;
+  L.am = D.am;
+  L.scale = D.scale;
+  L.omega = D.omega;
+  return L;
+}
+
+// ── batched stencil launches (one work-group per tile record) ──────────
+template <int TZ>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
+    kb_spmv_dir(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+                const BatchIsland* __restrict__ isl, PcgScalars* sc,
+                const double* __restrict__ rz_part, double* pq_part, int c) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  spmv_body<TZ, float>(lv_of(D), true, t.x0, t.y0, t.z0, I.z, c ? I.p1 : I.p0,
+                       c ? I.p0 : I.p1, I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c,
+                       t.slot == I.tb, pq_part, (unsigned)t.slot);
+}
+
+// UPDATE (finest level, iteration c): r[c] -> r[1-c] with p[1-c], q.
+// Otherwise the down-sweep of the level's own right-hand side.
+template <int TZ, bool UPDATE>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
+    kb_down(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+            const BatchIsland* __restrict__ isl, PcgScalars* sc, const double* __restrict__ pq_part,
+            double* rr_part, int c) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
+  float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
+  down_body<TZ, float, UPDATE>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
+                               c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
+                               (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
+}
+
+// FINEST: z = M r[ridx] and the r.z partial.  Otherwise the up-sweep of a
+// coarse level into its correction.
+template <int TZ, bool FINEST>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
+    kb_up(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+          const BatchIsland* __restrict__ isl, const PcgScalars* sc, double* rz_part, int ridx) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  const float* r = FINEST ? (ridx ? I.r1 : I.r0) : D.r;
+  float* out = FINEST ? I.z : D.e;
+  up_body<TZ, float, FINEST>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
+                             sc + D.island, rz_part, (unsigned)t.slot);
+}
+
+// Every island's coarsest level, one work-group each; with `check` the
+// island's convergence test from its r.r partials runs first.
+__global__ void __launch_bounds__(kMaxLdsCoarse)
+    kb_cheb(const BatchLevel* __restrict__ lv, const BatchIsland* __restrict__ isl,
+            const float* __restrict__ coef, int steps, PcgScalars* sc,
+            const double* __restrict__ rr_part, int check) {
+  const BatchIsland& I = isl[blockIdx.x];
+  const BatchLevel& D = lv[I.coarsest];
+  cheb_lds_body<float>(lv_of(D), D.r, D.e, coef, steps, sc + blockIdx.x, rr_part + I.tb,
+                       check ? (unsigned)I.tn : 0u);
+}
+
+// ── one work-group per island ──────────────────────────────────────────
+__device__ __forceinline__ double isl_sum(double v) {
+  __shared__ double sh[kIslBS / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kIslBS / 64; w++) t += sh[w];
+  return t;
+}
+
+// Start of a solve: the FP64 iterate from the caller's guess (or 0) on the
+// free set, ||b||^2 over the free set, state reset.
+__global__ void __launch_bounds__(kIslBS)
+    kb_begin(const BatchIsland* __restrict__ isl, BatchState* st, PcgScalars* sc) {
+  const BatchIsland& I = isl[blockIdx.x];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+    const bool f = I.fr[i] != 0;
+    I.x[i] = (f && I.xio) ? I.xio[i] : 0.0;
+    const double bv = f ? (double)I.b[i] : 0.0;
+    acc += bv * bv;
+  }
+  const double bb = isl_sum(acc);
+  if (threadIdx.x == 0) {
+    BatchState& S = st[blockIdx.x];
+    S.bb = bb;
+    S.rel = S.prev_rel = INFINITY;
+    S.total = S.passes = S.finished = S.converged = 0;
+    sc[blockIdx.x].iters = 0;
+    sc[blockIdx.x].done = 3;
+  }
+}
+
+// Pass boundary: the true FP64 residual of every unfinished island into its
+// r[0].  An island at the tolerance, out of iterations or stalled finishes;
+// the others restart PCG from that residual (stop when the recursive residual
+// has dropped by max(0.5 tol / rel, 2e-6)).  *live counts the restarts.
+__global__ void __launch_bounds__(kIslBS)
+    kb_residual(const BatchIsland* __restrict__ isl, BatchState* st, PcgScalars* sc, double tol,
+                int max_iters, int* live) {
+  const int k = blockIdx.x;
+  const BatchIsland& I = isl[k];
+  BatchState& S = st[k];
+  PcgScalars& P = sc[k];
+  if (S.finished) return;
+  const int pass_iters = P.iters;
+  const int pass_done = P.done;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+    const double ri = residual_f64(I.d, I.type, I.kappa, I.fr, I.b, I.x, i);
+    I.r0[i] = (float)ri;
+    acc += ri * ri;
+  }
+  const double rr = isl_sum(acc);
+  if (threadIdx.x != 0) return;
+  const double rel = S.bb > 0.0 ? sqrt(rr / S.bb) : sqrt(rr);
+  const bool first = S.passes == 0;
+  S.total += pass_iters;
+  const bool stalled = !first && (pass_iters == 0 || (pass_done == 2 && rel >= S.rel));
+  S.prev_rel = S.rel;
+  S.rel = rel;
+  if (rel <= tol || stalled || S.total >= max_iters || S.passes >= 16) {
+    S.finished = 1;
+    S.converged = rel <= tol;
+    P.done = 3;
+    return;
+  }
+  const double stop_rel = fmax(0.5 * tol / rel, 2e-6);
+  P.done = 0;
+  P.iters = 0;
+  P.have_dir = 0;
+  P.rz = P.beta = P.alpha = P.pq = 0.0;
+  P.rzs[0] = P.rzs[1] = 0.0;
+  P.rr = rr;
+  P.stop2 = rr * stop_rel * stop_rel;
+  P.max_iters = max_iters - S.total;
+  S.passes += 1;
+  atomicAdd(live, 1);
+}
+
+// Islands whose PCG pass is still running.
+__global__ void __launch_bounds__(1024) kb_count_live(const PcgScalars* sc, int n, int* live) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int c = 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) c += sc[k].done == 0;
+  atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) *live = cnt;
+}
+
+// End of a solve: FP64 and FP32 solutions to the caller's arrays.
+__global__ void __launch_bounds__(kIslBS) kb_finish(const BatchIsland* __restrict__ isl) {
+  const BatchIsland& I = isl[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+    const double v = I.x[i];
+    if (I.xio) I.xio[i] = v;
+    if (I.phio) I.phio[i] = (float)v;
+  }
+}
+
+}  // namespace
+
+// ══════════════════════════════ host side ══════════════════════════════
+
+static inline bool chk(hipError_t e, hipError_t& keep) {
+  if (e == hipSuccess) return false;
+  if (keep == hipSuccess) keep = e;
+  return true;
+}
+
+BatchSolver::~BatchSolver() { release(); }
+
+void BatchSolver::release() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  isl_.clear();
+  for (void* p : owned_) (void)hipFree(p);
+  owned_.clear();
+  if (pool_mem_) (void)hipFree(pool_mem_);
+  pool_mem_ = nullptr;
+  if (live_host_) (void)hipHostFree(live_host_);
+  live_host_ = nullptr;
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  ev0_ = ev1_ = nullptr;
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+}
+
+template <typename T>
+static T* balloc(size_t n, std::vector<void*>& owned, hipError_t& err) {
+  void* p = nullptr;
+  if (chk(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), err)) return nullptr;
+  owned.push_back(p);
+  return (T*)p;
+}
+
+// Launch order of a tile list: cut into 8 contiguous chunks, work-group g
+// takes chunk g % 8 (work-groups are dealt to the 8 XCDs in turn), so each
+// XCD walks a contiguous run of tiles; padding records have level -1.
+static std::vector<BatchTile> xcd_order(const std::vector<BatchTile>& v) {
+  const int T = (int)v.size();
+  const int chunk = (T + 7) / 8;
+  std::vector<BatchTile> o((size_t)8 * chunk, BatchTile{-1, 0, 0, 0, 0});
+  for (int g = 0; g < 8 * chunk; g++) {
+    const int src = (g % 8) * chunk + g / 8;
+    if (src < T) o[g] = v[src];
+  }
+  return o;
+}
+
+hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
+  hipError_t err = hipSuccess;
+  desc_ = desc;
+  coarse_sweeps_ = desc.coarse_sweeps > 0 ? std::min(desc.coarse_sweeps, 63) : 12;
+  const int n = desc.count;
+  if (chk(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), err)) return err;
+  chk(hipEventCreate(&ev0_), err);
+  chk(hipEventCreate(&ev1_), err);
+  if (err != hipSuccess) return err;
+
+  // shapes, then one measured pool for every island's hierarchy
+  isl_.resize(n);
+  nlev_.assign(n, 0);
+  for (int i = 0; i < n; i++) {
+    const pfsf_batch_island& bi = desc.islands[i];
+    if (nvox(bi.dims) > kMaxSolverCells) return hipErrorInvalidValue;
+    pfsf_solver_desc sd{};
+    sd.dims = bi.dims;
+    sd.type = bi.type;
+    sd.kappa = bi.kappa;
+    sd.precision = PFSF_PREC_FP32;
+    sd.coarse_sweeps = coarse_sweeps_;
+    sd.tol = desc.tol;
+    sd.max_iters = desc.max_iters;
+    isl_[i].reset(new MgSolver());
+    isl_[i]->configure_pooled(sd, kTZFine, kTZCoarse);
+    const MgLevel& C = isl_[i]->level(isl_[i]->levels() - 1);
+    if (C.n > kMaxLdsCoarse || padded_cells(C.d) > kMaxLdsPadded) return hipErrorInvalidValue;
+  }
+  DevPool measure;
+  for (auto& s : isl_) s->allocate_pooled(measure);
+  if (chk(hipMalloc(&pool_mem_, std::max<size_t>(measure.used, 1)), err)) return err;
+  DevPool pool;
+  pool.base = (char*)pool_mem_;
+  for (auto& s : isl_) s->allocate_pooled(pool);
+  for (int i = 0; i < n && err == hipSuccess; i++) chk(isl_[i]->build_pooled(stream_), err);
+  if (err != hipSuccess) return err;
+
+  // descriptor tables and tile lists
+  std::vector<BatchLevel> lv;
+  host_isl_.assign(n, BatchIsland{});
+  lmax_ = 0;
+  for (int i = 0; i < n; i++) lmax_ = std::max(lmax_, isl_[i]->levels());
+  std::vector<std::vector<BatchTile>> tl(lmax_);
+  int slot = 0;
+  for (int i = 0; i < n; i++) {
+    MgSolver& S = *isl_[i];
+    const int L = S.levels();
+    nlev_[i] = L;
+    const int base = (int)lv.size();
+    for (int l = 0; l < L; l++) {
+      const MgLevel& M = S.level(l);
+      BatchLevel D{};
+      D.d = M.d;
+      D.kap = (const float*)M.kap;
+      D.dg = M.dg;
+      D.am = M.am;
+      D.scale = M.scale;
+      D.omega = M.omega;
+      if (l + 1 < L) {
+        const MgLevel& C = S.level(l + 1);
+        D.c = C.d;
+        D.rc = C.r;
+        D.ec = C.e;
+      } else {
+        D.c = pfsf_dims{1, 1, 1};
+      }
+      D.r = l == 0 ? S.residual(0) : M.r;
+      D.e = M.e;
+      D.island = i;
+      lv.push_back(D);
+    }
+    BatchIsland& I = host_isl_[i];
+    I.d = S.dims();
+    I.n = S.cells();
+    I.type = S.type();
+    I.kappa = S.kappa();
+    I.fr = S.level(0).fr;
+    I.r0 = S.residual(0);
+    I.r1 = S.residual(1);
+    I.p0 = S.direction(0);
+    I.p1 = S.direction(1);
+    I.q = S.product();
+    I.z = S.preconditioned();
+    I.x = S.iterate();
+    I.coarsest = base + L - 1;
+    I.tb = slot;
+    // levels 0 .. L-2 take part in the down/up sweeps (L >= 2)
+    for (int l = 0; l <= L - 2; l++) {
+      const TileList& T = S.level(l).tiles;
+      for (int t : T.host_active) {
+        BatchTile r;
+        r.level = base + l;
+        r.x0 = (t % T.ntx) * kTX;
+        r.y0 = ((t / T.ntx) % T.nty) * kTY;
+        r.z0 = (t / (T.ntx * T.nty)) * T.tz;
+        r.slot = l == 0 ? slot++ : 0;
+        tl[l].push_back(r);
+      }
+    }
+    I.tn = slot - I.tb;
+  }
+  nslots_ = slot;
+  levels_ = balloc<BatchLevel>(lv.size(), owned_, err);
+  isl_d_ = balloc<BatchIsland>(n, owned_, err);
+  tiles_.assign(lmax_, nullptr);
+  ntiles_.assign(lmax_, 0);
+  std::vector<std::vector<BatchTile>> ordered(lmax_);
+  for (int l = 0; l < lmax_; l++) {
+    ordered[l] = xcd_order(tl[l]);
+    ntiles_[l] = (int)ordered[l].size();
+    if (ntiles_[l] > 0) tiles_[l] = balloc<BatchTile>(ordered[l].size(), owned_, err);
+  }
+  sc_ = balloc<PcgScalars>(n, owned_, err);
+  st_ = balloc<BatchState>(n, owned_, err);
+  ppq_ = balloc<double>(nslots_, owned_, err);
+  prr_ = balloc<double>(nslots_, owned_, err);
+  prz_ = balloc<double>(nslots_, owned_, err);
+  coef_ = balloc<float>(2 * 64, owned_, err);
+  live_ = balloc<int>(2, owned_, err);
+  if (err != hipSuccess) return err;
+  if (chk(hipHostMalloc((void**)&live_host_, 2 * sizeof(int)), err)) return err;
+  hipStream_t s = stream_;
+  chk(hipMemcpyAsync(levels_, lv.data(), lv.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
+                     s),
+      err);
+  for (int l = 0; l < lmax_; l++)
+    if (ntiles_[l] > 0)
+      chk(hipMemcpyAsync(tiles_[l], ordered[l].data(), ordered[l].size() * sizeof(BatchTile),
+                         hipMemcpyHostToDevice, s),
+          err);
+  // every island uses the same coarse-solve polynomial (same bound, steps)
+  chk(hipMemcpyAsync(coef_, isl_[0]->chebyshev_coefficients(), 2 * 64 * sizeof(float),
+                     hipMemcpyHostToDevice, s),
+      err);
+  chk(hipMemsetAsync(sc_, 0, n * sizeof(PcgScalars), s), err);
+  chk(hipMemsetAsync(ppq_, 0, nslots_ * sizeof(double), s), err);
+  chk(hipMemsetAsync(prr_, 0, nslots_ * sizeof(double), s), err);
+  chk(hipMemsetAsync(prz_, 0, nslots_ * sizeof(double), s), err);
+  chk(hipStreamSynchronize(s), err);
+  return err;
+}
+
+// Coarse part of every island's V-cycle below its finest level.
+hipError_t BatchSolver::vcycle_coarse(bool check) {
+  hipStream_t s = stream_;
+  const int n = (int)isl_.size();
+  for (int l = 1; l + 1 < lmax_; l++)
+    if (ntiles_[l] > 0)
+      kb_down<kTZCoarse, false><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_,
+                                                          nullptr, nullptr, 0);
+  kb_cheb<<<n, kMaxLdsCoarse, 0, s>>>(levels_, isl_d_, coef_, coarse_sweeps_, sc_, prr_,
+                                      check ? 1 : 0);
+  for (int l = lmax_ - 2; l >= 1; l--)
+    if (ntiles_[l] > 0)
+      kb_up<kTZCoarse, false><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_, nullptr,
+                                                        0);
+  return hipGetLastError();
+}
+
+// z = M r[0] and the r.z partials for the islands starting a pass.
+hipError_t BatchSolver::precondition() {
+  hipStream_t s = stream_;
+  kb_down<kTZFine, false><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, nullptr,
+                                                    nullptr, 0);
+  (void)vcycle_coarse(false);
+  kb_up<kTZFine, true><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 0);
+  return hipGetLastError();
+}
+
+// One PCG iteration of every live island, reading r[c], p[c].
+hipError_t BatchSolver::iteration(int c) {
+  hipStream_t s = stream_;
+  const unsigned G = ntiles_[0];
+  kb_spmv_dir<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, ppq_, c);
+  kb_down<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, ppq_, prr_, c);
+  (void)vcycle_coarse(true);
+  kb_up<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 1 - c);
+  return hipGetLastError();
+}
+
+hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* const* phi32,
+                              pfsf_batch_info* info, pfsf_solve_info* per_island) {
+  hipError_t err = hipSuccess;
+  hipStream_t s = stream_;
+  hipStream_t caller = (hipStream_t)desc_.stream;
+  const int n = (int)isl_.size();
+  const double tol = desc_.tol > 0 ? desc_.tol : 1e-6;
+  const int max_it = desc_.max_iters > 0 ? desc_.max_iters : 1000;
+  for (int i = 0; i < n; i++) {
+    host_isl_[i].b = b[i];
+    host_isl_[i].xio = x64 ? x64[i] : nullptr;
+    host_isl_[i].phio = phi32 ? phi32[i] : nullptr;
+  }
+  chk(hipEventRecord(ev0_, caller), err);  // order after the caller's stream
+  chk(hipStreamWaitEvent(s, ev0_, 0), err);
+  chk(hipEventRecord(ev0_, s), err);
+  chk(hipMemcpyAsync(isl_d_, host_isl_.data(), n * sizeof(BatchIsland), hipMemcpyHostToDevice, s),
+      err);
+  kb_begin<<<n, kIslBS, 0, s>>>(isl_d_, st_, sc_);
+  if (!graph_exec_ && err == hipSuccess) {
+    if (!chk(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), err)) {
+      (void)iteration(0);
+      (void)iteration(1);
+      kb_count_live<<<1, 1024, 0, s>>>(sc_, n, live_ + 1);
+      hipGraph_t g = nullptr;
+      chk(hipStreamEndCapture(s, &g), err);
+      if (err == hipSuccess && !chk(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), err))
+        graph_ = g;
+      else if (g)
+        (void)hipGraphDestroy(g);
+    }
+  }
+  int sweeps = 0;
+  const int pairs_per_check = 4;
+  for (int pass = 0; pass < 18 && err == hipSuccess; pass++) {
+    chk(hipMemsetAsync(live_, 0, sizeof(int), s), err);
+    kb_residual<<<n, kIslBS, 0, s>>>(isl_d_, st_, sc_, tol, max_it, live_);
+    chk(hipMemcpyAsync(live_host_, live_, sizeof(int), hipMemcpyDeviceToHost, s), err);
+    chk(hipStreamSynchronize(s), err);
+    if (err != hipSuccess || live_host_[0] == 0) break;
+    chk(precondition(), err);
+    // every island's pass ends within max_it iterations (device cap)
+    for (int it = 0; it < max_it + 2 * pairs_per_check && err == hipSuccess;
+         it += 2 * pairs_per_check) {
+      for (int k = 0; k < pairs_per_check; k++) chk(hipGraphLaunch(graph_exec_, s), err);
+      sweeps += 2 * pairs_per_check;
+      chk(hipMemcpyAsync(live_host_ + 1, live_ + 1, sizeof(int), hipMemcpyDeviceToHost, s), err);
+      chk(hipStreamSynchronize(s), err);
+      if (live_host_[1] == 0) break;
+    }
+  }
+  kb_finish<<<n, kIslBS, 0, s>>>(isl_d_);
+  chk(hipGetLastError(), err);
+  chk(hipEventRecord(ev1_, s), err);
+  chk(hipEventSynchronize(ev1_), err);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  chk(hipStreamWaitEvent(caller, ev1_, 0), err);
+  std::vector<BatchState> st(n);
+  chk(hipMemcpy(st.data(), st_, n * sizeof(BatchState), hipMemcpyDeviceToHost), err);
+  if (err != hipSuccess) return err;
+  pfsf_batch_info bi{};
+  bi.islands = n;
+  bi.sweeps = sweeps;
+  bi.solve_ms = ms;
+  for (int i = 0; i < n; i++) {
+    const BatchState& S = st[i];
+    bi.converged += S.converged;
+    bi.max_iterations = std::max(bi.max_iterations, S.total);
+    bi.total_iterations += S.total;
+    bi.max_rel_residual = std::max(bi.max_rel_residual, S.rel);
+    if (per_island) {
+      pfsf_solve_info& o = per_island[i];
+      o.iterations = S.total;
+      o.refinements = S.passes;
+      o.converged = S.converged;
+      o.levels = nlev_[i];
+      o.rel_residual = S.rel;
+      o.solve_ms = ms;
+    }
+  }
+  if (info) *info = bi;
+  return err;
+}
+
+}  // namespace pfsf

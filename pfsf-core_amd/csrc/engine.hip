@@ -16,6 +16,7 @@
 #include <mutex>
 #include <vector>
 
+#include "batch_solver.h"
 #include "mg_solver.h"
 #include "pfsf/pfsf.h"
 #include "pfsf_internal.h"
@@ -747,6 +748,7 @@ pfsf_result pfsf_solver_create(const pfsf_solver_desc* desc, pfsf_solver* out) {
   hipError_t e = s->mg.create(*desc);
   if (e != hipSuccess) {
     delete s;
+    if (e == hipErrorInvalidValue) return PFSF_ERROR_INVALID_ARG;  // grid too large
     return e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN;
   }
   *out = s;
@@ -774,5 +776,41 @@ pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps, doub
   *avg_ms = ms;
   RET(e);
 }
+
+// ═════════════════════════════ batched solve ═════════════════════════════
+
+struct pfsf_batch_t {
+  pfsf::BatchSolver b;
+};
+
+pfsf_result pfsf_batch_create(const pfsf_batch_desc* desc, pfsf_batch* out) {
+  if (!desc || !out || desc->count <= 0 || !desc->islands) return PFSF_ERROR_INVALID_ARG;
+  for (int i = 0; i < desc->count; i++) {
+    const pfsf_batch_island& bi = desc->islands[i];
+    if (!bi.type || !bi.kappa || !dims_ok(bi.dims)) return PFSF_ERROR_INVALID_ARG;
+  }
+  *out = nullptr;
+  pfsf_batch_t* s = new (std::nothrow) pfsf_batch_t();
+  if (!s) return PFSF_ERROR_OUT_OF_VRAM;
+  hipError_t e = s->b.create(*desc);
+  if (e != hipSuccess) {
+    delete s;
+    if (e == hipErrorInvalidValue) return PFSF_ERROR_INVALID_ARG;
+    return e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN;
+  }
+  *out = s;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, double* const* x64,
+                             float* const* phi32, pfsf_batch_info* info,
+                             pfsf_solve_info* per_island) {
+  if (!s || !b) return PFSF_ERROR_INVALID_ARG;
+  for (int i = 0; i < s->b.islands(); i++)
+    if (!b[i]) return PFSF_ERROR_INVALID_ARG;
+  RET(s->b.solve(b, x64, phi32, info, per_island));
+}
+
+void pfsf_batch_destroy(pfsf_batch s) { delete s; }
 
 }  // extern "C"

@@ -1,0 +1,676 @@
+// Device building blocks of the tiled multigrid-PCG stencil kernels, shared
+// by the single-system solver (mg_solver.hip) and the multi-island batch
+// solver (batch_solver.hip).  The kernel bodies take the level view, the tile
+// origin and every vector explicitly, so a launch wrapper can resolve them
+// either from its arguments (one system) or from per-tile descriptor tables
+// (many systems in one launch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "mg_solver.h"
+
+namespace pfsf {
+namespace {
+
+constexpr int BS = 256;
+constexpr int kMaxWaves = 16;  // waves of the largest work-group (1024 threads)
+// Waves per SIMD the tiled stencil kernels are register-budgeted for (LDS
+// admits six 24 KB tiles per CU).
+#ifndef PFSF_WPE
+#define PFSF_WPE 4
+#endif
+
+constexpr int TX = kTX, TY = kTY;
+constexpr int HX = TX + 2, HY = TY + 2, HXY = HX * HY;
+static_assert(TX * TY == BS, "one thread per (x, y) column of a tile");
+
+// ── storage helpers (FP32 or BF16 kappa) ───────────────────────────────
+__device__ __forceinline__ float ldk(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float ldk(const uint16_t* p, int64_t i) {
+  return __uint_as_float(((unsigned)p[i]) << 16);
+}
+__device__ __forceinline__ void stk(float* p, int64_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void stk(uint16_t* p, int64_t i, float v) {
+  unsigned u = __float_as_uint(v);
+  unsigned r = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  p[i] = (uint16_t)(r >> 16);
+}
+
+// Loads and stores at a 32-bit element index.  The byte offset is formed in
+// 32 bits (the solver's arrays stay below 4 GiB, checked at creation), so the
+// access is the scalar kernel-argument base plus one 32-bit vector offset.
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, int i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) +
+                                     (unsigned)i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st(T* p, int i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (unsigned)i * (unsigned)sizeof(T)) = v;
+}
+__device__ __forceinline__ float ldk32(const float* p, int i) { return ld(p, i); }
+__device__ __forceinline__ float ldk32(const uint16_t* p, int i) {
+  return __uint_as_float(((unsigned)ld(p, i)) << 16);
+}
+
+__device__ __forceinline__ int64_t flat(int x, int y, int z, pfsf_dims d) {
+  return (int64_t)x + (int64_t)d.lx * ((int64_t)y + (int64_t)d.ly * z);
+}
+
+__device__ __forceinline__ void unflat(int64_t i, pfsf_dims d, int& x, int& y, int& z) {
+  x = (int)(i % d.lx);
+  int64_t r = i / d.lx;
+  y = (int)(r % d.ly);
+  z = (int)(r / d.ly);
+}
+
+__device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
+
+// Hardware reciprocal (1 ulp) for the smoothers' D^-1: every use of a given
+// diagonal entry gets the same value, so pre- and post-smoothing stay
+// transposes of each other.
+__device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
+template <typename KT>
+struct Lv {
+  pfsf_dims d;
+  const KT* kap;     // kappa on the free set, -1 elsewhere
+  const float* dg;   // diagonal (0 outside the free set)
+  const float* am;   // anchor mass
+  float scale;
+  float omega;
+};
+
+struct TL {
+  int ntx, nty;
+  const int* ids;
+};
+
+// ── LDS tile machinery ─────────────────────────────────────────────────
+// Grids hold fewer than 2^31 cells (checked at creation), so the hot kernels
+// index with 32-bit integers.
+__device__ __forceinline__ int flat32(int x, int y, int z, pfsf_dims d) {
+  return x + d.lx * (y + d.ly * z);
+}
+
+template <int TZ>
+struct Tile {
+  static constexpr int HZ = TZ + 2, NROWS = HY * HZ, HN = HXY * HZ;
+  float v[HN];  // value (0 outside the free set)
+  float k[HN];  // kappa (< 0 outside the free set)
+  float s[HN];  // sqrt kappa (0 outside the free set); hardware square root,
+                // used for both ends of every coupling so A stays symmetric
+};
+
+template <int TZ>
+__device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& z0) {
+  int t = M.ids[blockIdx.x];
+  if (t < 0) return false;
+  int tx = t % M.ntx;
+  int r = t / M.ntx;
+  x0 = tx * TX;
+  y0 = (r % M.nty) * TY;
+  z0 = (r / M.nty) * TZ;
+  return true;
+}
+
+// Halo staging in two phases.  stage_load issues every global load of the
+// tile's halo box (kappa plus whatever raw operands ld(j, x, y, z) reads) and
+// returns them in registers; stage_put combines them with fin(raw) and writes
+// LDS, so a wave waits for memory once, after all loads are in flight.
+//
+// Rows of the box: the 32 lanes of a half-wave take the 32 interior columns of
+// one (y, z) row, eight rows per pass; one more pass covers the two halo
+// columns of every row.  A cell outside the grid is redirected to cell 0
+// (always in bounds) and discarded, so no load is behind a branch; fin may
+// see garbage or divide by zero outside the free set, and the select drops
+// it.  Cells outside the grid or the free set hold v = 0, k = -1, s = 0.
+template <int TZ>
+struct Stage {
+  static constexpr int NROWS = Tile<TZ>::NROWS;
+  static constexpr int RPP = BS / TX;
+  static constexpr int NP = (NROWS + RPP - 1) / RPP + 1;  // passes incl. halo columns
+  static_assert(2 * NROWS <= BS, "halo columns fit one pass");
+};
+
+template <int TZ, typename KT, typename Raw, typename LD>
+__device__ __forceinline__ void stage_load(const Lv<KT>& L, int x0, int y0, int z0, LD ld,
+                                           float (&kk)[Stage<TZ>::NP],
+                                           Raw (&raw)[Stage<TZ>::NP]) {
+  constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+  const pfsf_dims d = L.d;
+  const int t = threadIdx.x;
+  auto cell = [&](int p, int gx, int gy, int gz, bool ok) {
+    const int cx = ok ? gx : 0, cy = ok ? gy : 0, cz = ok ? gz : 0;
+    const int j = flat32(cx, cy, cz, d);
+    const float kj = ldk32(L.kap, j);
+    raw[p] = ld(j, cx, cy, cz);
+    kk[p] = ok ? kj : -1.f;
+  };
+  const int tx = t % TX, tr = t / TX;
+  const int gx = x0 + tx;
+  const bool inx = gx < d.lx;
+#pragma unroll
+  for (int p = 0; p < NP - 1; p++) {
+    const int row = tr + p * RPP;
+    const int ey = row % HY, ez = row / HY;
+    const int gy = y0 + ey - 1, gz = z0 + ez - 1;
+    cell(p, gx, gy, gz, row < NROWS && inx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
+  }
+  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+  const int ey = hrow % HY, ez = hrow / HY;
+  const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
+  cell(NP - 1, hx, gy, gz,
+       hrow < NROWS && hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
+}
+
+template <int TZ>
+__device__ __forceinline__ void put_cell(Tile<TZ>& T, int e, float k, float v) {
+  T.v[e] = k >= 0.f ? v : 0.f;
+  T.k[e] = k;
+  T.s[e] = k > 0.f ? __builtin_amdgcn_sqrtf(k) : 0.f;
+}
+
+template <int TZ, typename Raw, typename FIN>
+__device__ __forceinline__ void stage_put(Tile<TZ>& T, const float (&kk)[Stage<TZ>::NP],
+                                          const Raw (&raw)[Stage<TZ>::NP], FIN fin) {
+  constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+  const int t = threadIdx.x;
+  const int tx = t % TX, tr = t / TX;
+#pragma unroll
+  for (int p = 0; p < NP - 1; p++) {
+    const int row = tr + p * RPP;
+    if (p < NP - 2 || row < NROWS) put_cell<TZ>(T, row * HX + tx + 1, kk[p], fin(raw[p]));
+  }
+  const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+  if (hrow < NROWS) put_cell<TZ>(T, hrow * HX + ex, kk[NP - 1], fin(raw[NP - 1]));
+}
+
+// One-phase staging: val(j, x, y, z) is the staged value itself.
+template <int TZ, typename KT, typename F>
+__device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int y0, int z0,
+                                      F val) {
+  float kk[Stage<TZ>::NP], vv[Stage<TZ>::NP];
+  stage_load<TZ>(L, x0, y0, z0, val, kk, vv);
+  stage_put<TZ>(T, kk, vv, [](float v) { return v; });
+}
+
+// (A v)_i at LDS cell c of a box with row stride sy and plane stride sz:
+// difference form over free neighbours + anchor mass.  With constant strides
+// every read is one LDS instruction with an immediate offset from one base.
+__device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
+                                               const float* __restrict__ k,
+                                               const float* __restrict__ s, int c, int sy,
+                                               int sz, float ki, float mi, float scale) {
+  const float vc = v[c];
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  auto face = [&](int o) {
+    float kj = k[c + o];
+    float w = kj >= 0.f ? ki + kj : 0.f;
+    af += w * (vc - v[c + o]);
+  };
+  auto edge = [&](int o) { ae += s[c + o] * (vc - v[c + o]); };
+  auto corner = [&](int o) { ac += s[c + o] * (vc - v[c + o]); };
+  face(-1);
+  face(1);
+  face(-sy);
+  face(sy);
+  face(-sz);
+  face(sz);
+  edge(-1 - sy);
+  edge(1 - sy);
+  edge(-1 + sy);
+  edge(1 + sy);
+  edge(-1 - sz);
+  edge(1 - sz);
+  edge(-1 + sz);
+  edge(1 + sz);
+  edge(-sy - sz);
+  edge(sy - sz);
+  edge(-sy + sz);
+  edge(sy + sz);
+  corner(-1 - sy - sz);
+  corner(1 - sy - sz);
+  corner(-1 + sy - sz);
+  corner(1 + sy - sz);
+  corner(-1 - sy + sz);
+  corner(1 - sy + sz);
+  corner(-1 + sy + sz);
+  corner(1 + sy + sz);
+  return scale * (0.5f * af + s[c] * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+}
+
+template <int TZ>
+__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
+                                            float scale) {
+  return stencil_apply(T.v, T.k, T.s, c, HX, HXY, ki, mi, scale);
+}
+
+__device__ __forceinline__ int tcell(int tx, int ty, int tz) {
+  return (tz + 1) * HXY + (ty + 1) * HX + (tx + 1);
+}
+
+// ── FP64 reductions ────────────────────────────────────────────────────
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[BS / 64]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    double a = v[k];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double a = 0.0;
+      for (int w = 0; w < BS / 64; w++) a += sh[k][w];
+      v[k] = a;
+    }
+  }
+}
+
+// Block partials go to partials[k * gridDim.x + blockIdx.x]; whoever needs
+// the total sums them in a fixed order, so results are deterministic and the
+// kernel boundary orders the writes.
+template <int N>
+__device__ __forceinline__ void write_partials(double (&v)[N], double* partials,
+                                               double (*sh)[BS / 64]) {
+  block_sum<N>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) partials[(int64_t)k * gridDim.x + blockIdx.x] = v[k];
+  }
+}
+
+// One FP64 partial per work-group at partials[slot].
+__device__ __forceinline__ void write_partial_at(double v, double* partials,
+                                                 double (*sh)[BS / 64], unsigned slot) {
+  double a[1] = {v};
+  block_sum<1>(a, sh);
+  if (threadIdx.x == 0) partials[slot] = a[0];
+}
+
+// Sum of n FP64 partials, returned to every thread of the work-group.  Every
+// work-group adds in the same order, so each one obtains the same value: a
+// consumer kernel can finish a producer's reduction itself instead of
+// waiting for a separate one-block kernel.
+__device__ __forceinline__ double block_total(const double* __restrict__ part, unsigned n) {
+  __shared__ double tot_sh[kMaxWaves];
+  double a = 0.0;
+  for (unsigned b = threadIdx.x; b < n; b += blockDim.x) a += part[b];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) tot_sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < nw; w++) t += tot_sh[w];
+  return t;
+}
+
+// ── PCG + V-cycle kernels ──────────────────────────────────────────────
+// The output loops below evaluate the stencil for every cell of the column
+// and select the result (a cell outside the free set yields 0), so no branch
+// depends on the free mask; only the stores are guarded by the grid bounds.
+
+// p_out = z + beta p_in (z alone for the first direction); q = A p_out; p.q.
+// beta = (r.z) / (r.z)_previous is formed here from the r.z partials of the
+// preconditioner (rz_n work-groups); work-group 0 publishes r.z in slot
+// `slot` (the iteration parity) and beta when `publish` (one work-group per
+// system).  The p.q partial goes to partials[pslot].
+template <int TZ, typename KT>
+__device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, int y0, int z0,
+                                          const float* __restrict__ z,
+                                          const float* __restrict__ pin, float* __restrict__ pout,
+                                          float* __restrict__ q, PcgScalars* sc,
+                                          const double* __restrict__ rz_part, unsigned rz_n,
+                                          int slot, bool publish, double* partials,
+                                          unsigned pslot) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  const bool first = !sc->have_dir;
+  const double rz_old = sc->rzs[slot ^ 1];
+  const double rz = block_total(rz_part, rz_n);
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const pfsf_dims d = L.d;
+  const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+  const bool col = gx < d.lx && gy < d.ly;
+  const int i0 = flat32(gx, gy, z0, d);
+  bool inb[TZ];
+  float amr[TZ];  // own-voxel operands (cell 0 stands in for a cell outside the grid)
+  float kk[Stage<TZ>::NP];
+  float2 raw[Stage<TZ>::NP];
+  if (have) {
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      amr[tz] = ld(L.am, inb[tz] ? i0 + tz * sz : 0);
+    }
+    stage_load<TZ>(L, x0, y0, z0,
+                   [&](int j, int, int, int) { return make_float2(ld(z, j), ld(pin, j)); }, kk,
+                   raw);
+  }
+  const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
+  if (publish && threadIdx.x == 0) {
+    sc->rzs[slot] = rz;
+    sc->rz = rz;
+    sc->beta = beta64;
+  }
+  if (have) {
+    // p_in is finite everywhere (zero-initialised), so beta = 0 drops it
+    const float beta = (float)beta64;
+    stage_put<TZ>(T, kk, raw, [&](float2 v) { return v.x + beta * v.y; });
+    __syncthreads();
+    const int c0 = tcell(tx, ty, 0);
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      const int c = c0 + tz * HXY;
+      const float ki = T.k[c], pv = T.v[c];
+      float qi = tile_apply<TZ>(T, c, ki, amr[tz], L.scale);
+      qi = ki >= 0.f ? qi : 0.f;
+      acc[0] += (double)pv * qi;
+      if (inb[tz]) {
+        st(pout, i0 + tz * sz, pv);
+        st(q, i0 + tz * sz, qi);
+      }
+    }
+  }
+  write_partial_at(acc[0], partials, sh, pslot);
+}
+
+// Sum over each 2x2x2 group of children held by one tile, from per-thread
+// values res[tz].  Lanes tx^1 and ty^1 (= lane^32) hold the x and y
+// siblings.  Writes rc for every in-bounds coarse cell.
+template <int TZ>
+__device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, int y0, int z0,
+                                              int tx, int ty, pfsf_dims c, float* rc) {
+  const int X = (x0 + tx) >> 1, Y = (y0 + ty) >> 1;
+  const bool lead = !(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly;
+#pragma unroll
+  for (int k = 0; k < TZ / 2; k++) {
+    float a = res[2 * k] + res[2 * k + 1];
+    a = a + __shfl_xor(a, 1, 64);
+    a = a + __shfl_xor(a, 32, 64);
+    const int Z = (z0 >> 1) + k;
+    if (lead && Z < c.lz) st(rc, flat32(X, Y, Z, c), a);
+  }
+}
+
+// Down-sweep from level l: the pre-smoothed iterate x = omega D^-1 r is formed
+// on the fly, and the coarse right-hand side is rc = P^T (r - A x).
+// With UPDATE (finest level inside an iteration): first x64 += alpha p and
+// r_out = r_in - alpha q, then the same on r_out, plus r.r for convergence.
+// alpha = (r.z) / (p.q) is formed here from the p.q partials of the SpMV
+// (pq_n work-groups); with `publish` the work-group stores it (or flags a
+// breakdown when p.q <= 0).  The r.r partial goes to partials[pslot].
+template <int TZ, typename KT, bool UPDATE>
+__device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, int y0, int z0,
+                                          pfsf_dims c, const float* __restrict__ rin,
+                                          float* __restrict__ rout, const float* __restrict__ q,
+                                          const float* __restrict__ p, double* __restrict__ x,
+                                          float* __restrict__ rc, PcgScalars* sc,
+                                          const double* __restrict__ pq_part, unsigned pq_n,
+                                          bool publish, double* partials, unsigned pslot) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  const double rz = UPDATE ? sc->rz : 0.0;
+  const double pq0 = UPDATE ? block_total(pq_part, pq_n) : 0.0;
+  const pfsf_dims d = L.d;
+  const float om = L.omega;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+  const bool col = gx < d.lx && gy < d.ly;
+  const int i0 = flat32(gx, gy, z0, d);
+  // own-voxel operands (cell 0 stands in for a cell outside the grid; its
+  // values are never used)
+  bool inb[TZ];
+  float ro[TZ], qo[TZ], amr[TZ], pr[TZ];
+  double xr[TZ];
+  float kk[Stage<TZ>::NP];
+  float3 raw[Stage<TZ>::NP];  // r_in, q, diagonal
+  if (have) {
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      const int i = inb[tz] ? i0 + tz * sz : 0;
+      ro[tz] = ld(rin, i);
+      qo[tz] = UPDATE ? ld(q, i) : 0.f;
+      amr[tz] = ld(L.am, i);
+      pr[tz] = UPDATE ? ld(p, i) : 0.f;
+      xr[tz] = UPDATE ? ld(x, i) : 0.0;
+    }
+    stage_load<TZ>(L, x0, y0, z0,
+                   [&](int j, int, int, int) {
+                     return make_float3(ld(rin, j), UPDATE ? ld(q, j) : 0.f, ld(L.dg, j));
+                   },
+                   kk, raw);
+  }
+  double alpha64 = 0.0;
+  if (UPDATE) {
+    const double pq = pq0;
+    const bool ok = pq > 0.0;
+    alpha64 = ok ? rz / pq : 0.0;
+    if (publish && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = alpha64;
+      sc->have_dir = 1;
+      if (!ok) sc->done = 2;  // breakdown: the direction carries no energy
+    }
+    if (!ok) return;
+  }
+  if (have) {
+    const float alpha = (float)alpha64;
+    stage_put<TZ>(T, kk, raw, [&](float3 v) {
+      const float rj = UPDATE ? v.x - alpha * v.y : v.x;
+      return om * rj * rcp(v.z);
+    });
+    float rr_[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) rr_[tz] = UPDATE ? ro[tz] - alpha * qo[tz] : ro[tz];
+    if (UPDATE) {
+      // p and q are zero outside the free set, so the update leaves x there
+      // unchanged and carries r there over
+#pragma unroll
+      for (int tz = 0; tz < TZ; tz++) {
+        if (inb[tz]) {
+          st(x, i0 + tz * sz, xr[tz] + alpha64 * (double)pr[tz]);
+          st(rout, i0 + tz * sz, rr_[tz]);
+        }
+      }
+    }
+    __syncthreads();
+    const int c0 = tcell(tx, ty, 0);
+    float res[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      const int cc = c0 + tz * HXY;
+      const float ki = T.k[cc], ri = rr_[tz];
+      const bool fr = ki >= 0.f;
+      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
+      res[tz] = fr ? ri - ax : 0.f;
+      if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
+    }
+    restrict_tile<TZ>(res, x0, y0, z0, tx, ty, c, rc);
+  }
+  if (UPDATE) write_partial_at(acc[0], partials, sh, pslot);
+}
+
+// Up-sweep at level l: x~ = omega D^-1 r + P e_coarse on the free set, then
+// one damped-Jacobi post-smoothing step out = x~ + omega D^-1 (r - A x~).
+// At the finest level it also accumulates r.z.
+template <int TZ, typename KT, bool FINEST>
+__device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int y0, int z0,
+                                        pfsf_dims c, const float* __restrict__ r,
+                                        const float* __restrict__ ec, float* __restrict__ out,
+                                        const PcgScalars* sc, double* partials, unsigned pslot) {
+  if (sc->done) return;
+  __shared__ Tile<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc[1] = {0.0};
+  const pfsf_dims d = L.d;
+  const float om = L.omega;
+  if (have) {
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+    const bool col = gx < d.lx && gy < d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    // own-voxel operands, loaded before the barrier (cell 0 stands in for a
+    // cell outside the grid; its values are never used)
+    bool inb[TZ];
+    float rr_[TZ], amr[TZ], dgr[TZ];
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      inb[tz] = col && z0 + tz < d.lz;
+      const int i = inb[tz] ? i0 + tz * sz : 0;
+      rr_[tz] = ld(r, i);
+      amr[tz] = ld(L.am, i);
+      dgr[tz] = ld(L.dg, i);
+    }
+    stage<TZ>(T, L, x0, y0, z0, [&](int j, int gx_, int gy_, int gz_) {
+      return om * ld(r, j) * rcp(ld(L.dg, j)) + ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c));
+    });
+    __syncthreads();
+    const int c0 = tcell(tx, ty, 0);
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      const int cc = c0 + tz * HXY;
+      const float ki = T.k[cc], xi = T.v[cc], ri = rr_[tz];
+      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
+      float zi = xi + om * (ri - ax) * rcp(dgr[tz]);
+      zi = ki >= 0.f ? zi : 0.f;
+      if (FINEST) acc[0] += ki >= 0.f ? (double)ri * zi : 0.0;
+      if (inb[tz]) st(out, i0 + tz * sz, zi);
+    }
+  }
+  if (FINEST) write_partial_at(acc[0], partials, sh, pslot);
+}
+
+// Whole Chebyshev solve of the coarsest level held in one work-group's LDS
+// (n <= kMaxLdsCoarse cells, one cell per thread).  Kappa, sqrt kappa and the
+// iterate sit in LDS boxes with a one-cell border (kappa -1, value 0), so the
+// 26 neighbour reads need no bounds tests; the diagonal, anchor mass, rhs and
+// direction of a thread's own cell stay in registers.
+constexpr int kMaxLdsCoarse = 1024;
+constexpr int kMaxLdsPadded = 2048;
+
+__host__ __device__ inline int padded_cells(pfsf_dims d) {
+  return (d.lx + 2) * (d.ly + 2) * (d.lz + 2);
+}
+
+//
+// With rr_n > 0 it first finishes the PCG iteration's convergence test from
+// the r.r partials of the finest update (rr_n work-groups): iteration count,
+// r.r, and the stop flag, which also ends this V-cycle.
+template <typename KT>
+__device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __restrict__ r,
+                                              float* __restrict__ out,
+                                              const float* __restrict__ coef, int steps,
+                                              PcgScalars* sc, const double* __restrict__ rr_part,
+                                              unsigned rr_n) {
+  if (sc->done) return;
+  if (rr_n > 0) {
+    const double rr = block_total(rr_part, rr_n);
+    const int it = sc->iters + 1;
+    const bool stop = rr <= sc->stop2 || it >= sc->max_iters;
+    __syncthreads();  // every thread has read the scalars
+    if (threadIdx.x == 0) {
+      sc->rr = rr;
+      sc->iters = it;
+      if (stop) sc->done = 1;
+    }
+    if (stop) return;
+  }
+  __shared__ float sk[kMaxLdsPadded], ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
+  const pfsf_dims d = L.d;
+  const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
+  for (int e = threadIdx.x; e < np; e += blockDim.x) {
+    sk[e] = -1.f;
+    ss[e] = 0.f;
+    sx[0][e] = 0.f;
+    sx[1][e] = 0.f;
+  }
+  __syncthreads();
+  const int n = d.lx * d.ly * d.lz;
+  const int i = threadIdx.x;
+  const bool act = i < n;
+  int e = 0;
+  float ki = -1.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
+  if (act) {
+    const int x = i % d.lx, y = (i / d.lx) % d.ly, z = i / (d.lx * d.ly);
+    e = (x + 1) + (y + 1) * sy + (z + 1) * sz;
+    ki = ldk(L.kap, i);
+    if (ki >= 0.f) {
+      D = L.dg[i];
+      m = L.am[i];
+      ri = r[i];
+      di = coef[1] * ri * rcp(D);
+      xi = di;
+    }
+    sk[e] = ki;
+    ss[e] = ki > 0.f ? __builtin_amdgcn_sqrtf(ki) : 0.f;
+    sx[0][e] = xi;
+  }
+  const bool live = act && ki >= 0.f;
+  __syncthreads();
+  int cur = 0;
+  for (int st = 1; st < steps; st++) {
+    const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
+    if (live) {
+      const float ax = stencil_apply(sx[cur], sk, ss, e, sy, sz, ki, m, L.scale);
+      di = c1 * di + c2 * (ri - ax) * rcp(D);
+      xi = xi + di;
+      sx[cur ^ 1][e] = xi;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (act) out[i] = live ? xi : 0.f;
+}
+
+// ── FP64 residual of the iterate ───────────────────────────────────────
+__device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
+  if (cls == 1) {
+    float s = ki + kj;
+    return 0.5 * (double)s;
+  }
+  float p = ki * kj;
+  double r = sqrt((double)p);
+  return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
+}
+
+// (b - A x)_i with the exact coupling rule in FP64 (0 outside the free set;
+// x outside the free set counts as 0).
+__device__ __forceinline__ double residual_f64(pfsf_dims d, const uint8_t* __restrict__ type,
+                                               const float* __restrict__ kappa,
+                                               const uint8_t* __restrict__ fr,
+                                               const float* __restrict__ b,
+                                               const double* __restrict__ x, int64_t i) {
+  if (!fr[i]) return 0.0;
+  const double xi = x[i];
+  int px, py, pz;
+  unflat(i, d, px, py, pz);
+  const float ki = kapf(kappa[i]);
+  double ax = 0.0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int cls = (dx != 0) + (dy != 0) + (dz != 0);
+        if (cls == 0) continue;
+        const int nx = px + dx, ny = py + dy, nz = pz + dz;
+        if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+        const int64_t j = flat(nx, ny, nz, d);
+        if (type[j] == 0) continue;
+        const double xj = fr[j] ? x[j] : 0.0;
+        ax += couple_f64(cls, ki, kapf(kappa[j])) * (xi - xj);
+      }
+  return (double)b[i] - ax;
+}
+
+}  // namespace
+}  // namespace pfsf

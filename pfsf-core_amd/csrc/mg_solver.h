@@ -30,6 +30,19 @@ struct TileList {
   int nlaunch = 0;                // launch size (multiple of 8, padded with -1)
   int nactive = 0;                // tiles holding at least one free voxel
   int* ids = nullptr;             // device: tile id per work-group, XCD-grouped
+  std::vector<int> host_active;   // host: active tile ids, ascending
+};
+
+// Grids of the solver stay below 2^29 cells, so every array (FP64 included)
+// is under 4 GiB and 32-bit byte offsets address it.
+constexpr int64_t kMaxSolverCells = int64_t(1) << 29;
+
+// Bump allocator over one device allocation (the batch solver's pooled
+// hierarchies).  With base == nullptr it only measures.
+struct DevPool {
+  char* base = nullptr;
+  size_t used = 0;
+  void* take(size_t bytes);
 };
 
 struct MgLevel {
@@ -54,14 +67,37 @@ class MgSolver {
  public:
   ~MgSolver();
   hipError_t create(const pfsf_solver_desc& desc);
+  // Pooled hierarchy for the batch solver: shapes first (host), then arrays
+  // carved from `pool` (a measuring pass first), then the set-up kernels on
+  // the batch's stream.  solve() and profile() are not available.
+  void configure_pooled(const pfsf_solver_desc& desc, int tz_fine, int tz_coarse);
+  void allocate_pooled(DevPool& pool);
+  hipError_t build_pooled(hipStream_t s);
   hipError_t update(const uint8_t* type, const float* kappa);
   hipError_t solve(const float* b, double* x64, float* phi32, pfsf_solve_info* info);
   hipError_t profile(int which, int reps, float* avg_ms);
   int levels() const { return nlev_; }
+  const MgLevel& level(int l) const { return levels_[l]; }
+  pfsf_dims dims() const { return d0_; }
+  int64_t cells() const { return n0_; }
+  const float* chebyshev_coefficients() const { return coef_host_.data(); }
+  int coarse_sweeps() const { return coarse_sweeps_; }
+  double* iterate() const { return x_; }
+  float* residual(int k) const { return r_[k]; }
+  float* direction(int k) const { return p_[k]; }
+  float* product() const { return q_; }
+  float* preconditioned() const { return z_; }
+  const uint8_t* type() const { return type_; }
+  const float* kappa() const { return kappa_; }
   const uint8_t* free_mask() const { return levels_.empty() ? nullptr : levels_[0].fr; }
   hipStream_t stream() const { return stream_; }
 
  private:
+  void configure(const pfsf_solver_desc& desc, int min_levels, int tz_fine, int tz_coarse);
+  hipError_t allocate(bool standalone);
+  template <typename T>
+  T* alloc(int64_t n, hipError_t& err);
+  void zero_vectors();
   hipError_t build();
   hipError_t build_tiles(MgLevel& L);
   hipError_t precondition();            // z = M r[0], r.z
@@ -73,6 +109,8 @@ class MgSolver {
 
   pfsf_solver_desc desc_{};
   hipStream_t stream_ = nullptr;
+  bool owns_stream_ = false;
+  DevPool* pool_ = nullptr;
   pfsf_dims d0_{0, 0, 0};
   int64_t n0_ = 0;
   const uint8_t* type_ = nullptr;

@@ -88,6 +88,23 @@ class SolveInfo(C.Structure):
 P, I32, I64, F32, F64, U32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint32
 R = C.c_int  # pfsf_result
 
+class BatchIsland(C.Structure):
+    _fields_ = [("dims", Dims), ("type", C.c_void_p), ("kappa", C.c_void_p)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("count", C.c_int32), ("islands", C.POINTER(BatchIsland)),
+                ("coarse_sweeps", C.c_int32), ("tol", C.c_double), ("max_iters", C.c_int32),
+                ("stream", C.c_void_p)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("islands", C.c_int32), ("converged", C.c_int32),
+                ("max_iterations", C.c_int32), ("sweeps", C.c_int32),
+                ("total_iterations", C.c_int64), ("max_rel_residual", C.c_double),
+                ("solve_ms", C.c_double)]
+
+
 SIGNATURES = {
     # pfsf.h
     "pfsf_create": (P, [C.POINTER(Config)]),
@@ -136,6 +153,10 @@ SIGNATURES = {
     "pfsf_solver_solve": (R, [P, P, P, P, C.POINTER(SolveInfo)]),
     "pfsf_solver_update": (R, [P, P, P]),
     "pfsf_solver_destroy": (None, [P]),
+    "pfsf_batch_create": (R, [C.POINTER(BatchDesc), C.POINTER(P)]),
+    "pfsf_batch_solve": (R, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(BatchInfo),
+                             C.POINTER(SolveInfo)]),
+    "pfsf_batch_destroy": (None, [P]),
     "pfsf_solver_profile": (R, [P, I32, I32, C.POINTER(F64)]),
     "pfsf_hip_anchored_mask": (R, [Dims, P, P, P]),
     "pfsf_register_island_kappa": (R, [P, I32, P, I64]),

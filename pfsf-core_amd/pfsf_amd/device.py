@@ -138,3 +138,48 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+class BatchSolver:
+    """Many independent islands solved together (pfsf_batch_*).
+
+    islands: list of (vtype, kappa) device tensors, shapes (lz, ly, lx)."""
+
+    def __init__(self, islands, tol=1e-6, max_iters=1000, coarse_sweeps=0):
+        self._keep = list(islands)  # keep the device arrays alive
+        n = len(self._keep)
+        arr = (capi.BatchIsland * n)()
+        for k, (vt, kp) in enumerate(self._keep):
+            arr[k] = capi.BatchIsland(_dims(vt), _p(vt).value, _p(kp).value)
+        self._arr = arr
+        desc = capi.BatchDesc(n, arr, coarse_sweeps, tol, max_iters, _stream().value)
+        h = C.c_void_p()
+        capi.check("pfsf_batch_create", _lib().pfsf_batch_create(C.byref(desc), C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def solve(self, bs, x64s=None, phis=None, per_island=False):
+        """Solve every island; returns BatchInfo (and per-island SolveInfo list)."""
+        n = self.n
+        parr = lambda ts: (C.c_void_p * n)(*[_p(t).value if t is not None else None
+                                             for t in ts]) if ts is not None else None
+        b = parr(bs)
+        x = parr(x64s)
+        ph = parr(phis)
+        info = capi.BatchInfo()
+        per = (capi.SolveInfo * n)() if per_island else None
+        cast = lambda a: C.cast(a, C.POINTER(C.c_void_p)) if a is not None else None
+        capi.check("pfsf_batch_solve", _lib().pfsf_batch_solve(
+            self.h, cast(b), cast(x), cast(ph), C.byref(info), per))
+        return (info, list(per)) if per_island else info
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().pfsf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

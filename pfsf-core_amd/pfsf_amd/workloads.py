@@ -152,6 +152,88 @@ def workload(index: int) -> Island:
     raise ValueError(f"no single-island workload {index}")
 
 
+def grown_island(rng: np.random.Generator, n_target: int, sigma_logn: float = 0.5,
+                 max_anchors: int = 8, name: str = "") -> Island:
+    """Random box union grown to about n_target voxels with 1..max_anchors
+    anchored base voxels at z=0.
+
+    The domain is a cube holding ~2.5 n_target cells.  The first box stands on
+    z=0; every later box contains a randomly chosen occupied voxel, so the
+    union stays 26-connected and every voxel is kept by the anchored-component
+    filter.  Box edges are U{2..max(2, L/2)} per axis.  Anchors are distinct
+    occupied voxels of the z=0 plane; the island is cropped to its bounding box.
+    """
+    L = max(4, int(round((2.5 * n_target) ** (1.0 / 3.0))))
+    occ = np.zeros((L, L, L), dtype=bool)
+    hi = max(2, L // 2)
+    count = 0
+    nb = 0
+    while count < n_target:
+        # random draws in blocks: box edges, placement fractions, candidate voxels
+        if nb == 0 or k >= nb:
+            nb = 64
+            edges = rng.integers(2, hi + 1, size=(nb, 3)).tolist()
+            frac = rng.random((nb, 3)).tolist()
+            cand = rng.integers(0, L, size=(nb * 16, 3)).tolist()
+            k = ci = 0
+            if count == 0:  # the first box stands on z=0
+                ex, ey, ez = edges[0]
+                x0 = int(frac[0][0] * (L - ex + 1))
+                y0 = int(frac[0][1] * (L - ey + 1))
+                z0 = 0
+                k = 1
+                occ[z0:z0 + ez, y0:y0 + ey, x0:x0 + ex] = True
+                count = ex * ey * ez
+                continue
+        ex, ey, ez = edges[k]
+        fx, fy, fz = frac[k]
+        k += 1
+        while True:  # a random occupied voxel the box must contain
+            if ci >= len(cand):
+                cand = rng.integers(0, L, size=(nb * 16, 3)).tolist()
+                ci = 0
+            cx, cy, cz = cand[ci]
+            ci += 1
+            if occ[cz, cy, cx]:
+                break
+        x0 = min(max(cx - int(fx * ex), 0), L - ex)
+        y0 = min(max(cy - int(fy * ey), 0), L - ey)
+        z0 = min(max(cz - int(fz * ez), 0), L - ez)
+        blk = occ[z0:z0 + ez, y0:y0 + ey, x0:x0 + ex]
+        count += blk.size - int(np.count_nonzero(blk))
+        blk[...] = True
+    zs, ys, xs = np.nonzero(occ)
+    occ = occ[:zs.max() + 1, ys.min():ys.max() + 1, xs.min():xs.max() + 1]
+    base = np.flatnonzero(occ[0])
+    k = int(rng.integers(1, max_anchors + 1))
+    pick = rng.choice(base, size=min(k, base.size), replace=False)
+    vtype = np.where(occ, SOLID, AIR).astype(np.uint8)
+    vtype[0].reshape(-1)[pick] = ANCHOR
+    kappa, source, rcomp, rtens = _materials32(rng, occ.shape, sigma_logn)
+    return _finish(vtype, kappa, source, rcomp, rtens, name)
+
+
+def _materials32(rng, shape, sigma_logn: float):
+    """_materials' distributions drawn in float32 (the batch generator's volume)."""
+    f = np.float32
+    kappa = f(30.0) * np.exp(f(sigma_logn) * rng.standard_normal(shape, dtype=f))
+    rho = f(1500.0) + f(6300.0) * rng.random(shape, dtype=f)
+    source = rho * f(GRAVITY * 1e-6)
+    rcomp = f(5.0) + f(55.0) * rng.random(shape, dtype=f)
+    rtens = f(0.1) * rcomp
+    return kappa, source, rcomp, rtens
+
+
+def island_batch(count: int = 4096, index: int = 2, n_min: int = 64, n_max: int = 262144,
+                 sigma_logn: float = 0.5) -> list:
+    """The multi-island batch: voxel counts log-uniform on [n_min, n_max]
+    (4^3 .. 64^3), each island grown by grown_island, seed 0x9F5F0000 + index."""
+    rng = _rng(index)
+    sizes = np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), size=count))).astype(int)
+    return [grown_island(rng, int(n), sigma_logn, name=f"batch{index}.{i}")
+            for i, n in enumerate(sizes)]
+
+
 def small_box_union(seed: int, L: int = 16, occupancy: float = 0.5, sigma_logn: float = 0.5,
                     n_boxes: int = 40, edge=(2, 8)) -> Island:
     """Small random box union (test-sized), same recipe as the 128^3 workload."""

@@ -1,0 +1,103 @@
+"""The batched multi-island solve (pfsf_batch_*) against the FP64 oracle.
+
+Bar, per island: the FP64 residual ||b - A x|| / ||b|| recomputed by the
+oracle from the returned x <= tol, and phi within 1e-5 relative L2 of the FP64
+direct solve."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pfsf_amd import device as D
+from pfsf_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL_PHI = 1e-5
+
+
+def _islands(count=20, seed=7, n_max=6000):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.rint(np.exp(rng.uniform(np.log(64), np.log(n_max), size=count))).astype(int)
+    out = [W.grown_island(rng, int(n), name=f"t{k}") for k, n in enumerate(sizes)]
+    out.append(W.column(24, kappa=3.0, load=0.25))
+    out.append(W.small_box_union(3, L=14))
+    return out
+
+
+def _upload(isls, cuda):
+    vt = [torch.from_numpy(i.vtype).to(cuda) for i in isls]
+    kp = [torch.from_numpy(i.kappa).to(cuda) for i in isls]
+    b = [torch.from_numpy(i.source).to(cuda) for i in isls]
+    return vt, kp, b
+
+
+def _solve(isls, cuda, tol=1e-6, x0=None):
+    vt, kp, b = _upload(isls, cuda)
+    s = D.BatchSolver(list(zip(vt, kp)), tol=tol, max_iters=2000)
+    xs = [torch.zeros(i.vtype.shape, dtype=torch.float64, device=cuda) if x0 is None
+          else torch.from_numpy(x0[k]).to(cuda) for k, i in enumerate(isls)]
+    ph = [torch.empty(i.vtype.shape, dtype=torch.float32, device=cuda) for i in isls]
+    info, per = s.solve(b, xs, ph, per_island=True)
+    torch.cuda.synchronize()
+    return s, info, per, [x.cpu().numpy() for x in xs], [p.cpu().numpy() for p in ph]
+
+
+def test_batch_matches_direct_solves(cuda):
+    isls = _islands()
+    s, info, per, xs, phis = _solve(isls, cuda, tol=1e-8)
+    s.close()
+    assert info.islands == len(isls) and info.converged == len(isls), (
+        info.converged, [(p.iterations, p.rel_residual) for p in per])
+    for k, isl in enumerate(isls):
+        assert per[k].levels >= 2
+        assert O.residual(isl, xs[k]) <= 1e-8 * 1.01, (k, per[k].rel_residual)
+        ref = O.direct_solve(isl)
+        err = np.linalg.norm(phis[k] - ref) / np.linalg.norm(ref)
+        assert err <= TOL_PHI, (k, err)
+        assert np.array_equal(phis[k], xs[k].astype(np.float32))
+
+
+def test_batch_agrees_with_single_solver(cuda):
+    isls = _islands(count=6, seed=11)
+    s, info, per, xs, _ = _solve(isls, cuda, tol=1e-6)
+    s.close()
+    for k, isl in enumerate(isls):
+        vt, kp, b = (torch.from_numpy(a).to(cuda) for a in (isl.vtype, isl.kappa, isl.source))
+        one = D.Solver(vt, kp, tol=1e-6)
+        x1 = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=cuda)
+        i1 = one.solve(b, x1)
+        one.close()
+        assert i1.converged == 1 and per[k].converged == 1
+        a, c = xs[k], x1.cpu().numpy()
+        assert np.linalg.norm(a - c) / np.linalg.norm(c) <= 1e-5
+
+
+def test_batch_warm_start_and_determinism(cuda):
+    isls = _islands(count=8, seed=5)
+    s, info, per, xs, phis = _solve(isls, cuda, tol=1e-6)
+    s.close()
+    assert info.converged == len(isls)
+    # a converged guess needs no iterations
+    s2, info2, per2, xs2, _ = _solve(isls, cuda, tol=1e-6, x0=xs)
+    s2.close()
+    assert info2.total_iterations == 0 and info2.converged == len(isls)
+    # same inputs, same bits
+    s3, info3, per3, xs3, phis3 = _solve(isls, cuda, tol=1e-6)
+    s3.close()
+    assert info3.total_iterations == info.total_iterations
+    for a, c in zip(phis, phis3):
+        assert np.array_equal(a, c)
+
+
+def test_batch_orphan_island_keeps_zero(cuda):
+    isl = W.small_box_union(4, L=10)
+    vt = isl.vtype.copy()
+    vt[vt == W.ANCHOR] = W.SOLID  # no anchor anywhere: every voxel is an orphan
+    orphan = W.Island(isl.lx, isl.ly, isl.lz, vt, isl.kappa, isl.source, isl.rcomp, isl.rtens)
+    isls = [orphan, W.small_box_union(5, L=12)]
+    s, info, per, xs, phis = _solve(isls, cuda, tol=1e-6)
+    s.close()
+    assert np.all(phis[0] == 0)
+    assert per[1].converged == 1
+    assert O.residual(isls[1], xs[1]) <= 1e-6 * 1.01
