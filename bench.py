@@ -52,6 +52,10 @@ def measured_traffic():
 CPU_PCG_ITERS_TO_1E6 = 1032
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(isl, sample_iters=40):
     """The oracle's FP64 Jacobi-PCG (C, one thread) on a bounded sample."""
     from oracle import oracle as O
@@ -67,6 +71,77 @@ def cpu_baseline(isl, sample_iters=40):
                       f"{iters} iterations that solver needs for 1e-6"}
 
 
+def batch_bench(torch, D, W, steps=3, plain_stride=16):
+    """Batched solve of the 4,096-island batch against plain per-island solves.
+
+    Islands/s = 4096 / (batched solve time); the plain rate comes from solving
+    every plain_stride-th island (by size) one at a time with its own solver,
+    the summed solve times scaled by plain_stride."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    isls = W.island_batch(4096)
+    gen_s = time.perf_counter() - t0
+    log(f"batch: generated {len(isls)} islands in {gen_s:.1f} s")
+    dev = torch.device("cuda:0")
+    vt = [torch.from_numpy(i.vtype).to(dev) for i in isls]
+    kp = [torch.from_numpy(i.kappa).to(dev) for i in isls]
+    bs = [torch.from_numpy(i.source).to(dev) for i in isls]
+    phis = [torch.empty(i.vtype.shape, dtype=torch.float32, device=dev) for i in isls]
+    n_act = int(sum(i.n_active for i in isls))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bat = D.BatchSolver(list(zip(vt, kp)), tol=1e-6, max_iters=2000)
+    setup_s = time.perf_counter() - t0
+    log(f"batch: hierarchies built in {setup_s:.1f} s")
+    bat.solve(bs, None, phis)  # warm-up (graph capture)
+    times, infos = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos.append(bat.solve(bs, None, phis))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"batch: solve {times[-1] * 1e3:.1f} ms, max iterations {infos[-1].max_iterations}")
+    bat.close()
+    t_batch = float(np.median(times))
+    info = infos[-1]
+    # plain: one solver per island, islands one after another
+    order = np.argsort([i.n_active for i in isls], kind="stable")
+    sample = order[::plain_stride]
+    t_plain, it_plain = 0.0, 0
+    for k in sample:
+        s = D.Solver(vt[k], kp[k], tol=1e-6, max_iters=2000)
+        x = torch.zeros(isls[k].vtype.shape, dtype=torch.float64, device=dev)
+        s.solve(bs[k], x)  # warm-up (graph capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x.zero_()
+        inf = s.solve(bs[k], x)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        it_plain += inf.iterations
+        s.close()
+    log(f"plain: {len(sample)} islands in {t_plain * 1e3:.1f} ms")
+    t_plain_all = t_plain * plain_stride
+    return {
+        "islands": len(isls), "active_voxels": n_act, "generate_s": round(gen_s, 2),
+        "setup_s": round(setup_s, 2), "batch_ms": round(t_batch * 1e3, 2),
+        "islands_per_s": round(len(isls) / t_batch, 1),
+        "plain_islands_per_s": round(len(isls) / t_plain_all, 1),
+        "speedup_vs_plain": round(t_plain_all / t_batch, 2),
+        "voxels_per_s": round(n_act / t_batch, 1),
+        "converged": info.converged, "max_iterations": info.max_iterations,
+        "mean_iterations": round(info.total_iterations / info.islands, 1),
+        "graph_sweeps": info.sweeps, "max_rel_residual": info.max_rel_residual,
+        "device_ms": round(info.solve_ms, 2),
+        "plain_sample": f"{len(sample)} islands (every {plain_stride}th by voxel count), "
+                        f"{t_plain * 1e3:.1f} ms summed, {it_plain} PCG iterations, scaled "
+                        f"x{plain_stride}",
+        "tol": 1e-6, "workload": "4096 grown box-union islands, voxel counts log-uniform "
+                                 "64..262144, 1-8 anchors each, cold start"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +150,7 @@ def main():
     ap.add_argument("--levels", type=int, default=0, help="MG levels (0 = auto hierarchy)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with 3 plain 256 MiB device copies (reference kernel for "
                          "hardware byte counters)")
@@ -174,6 +250,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(isl)
     solver.close()
+    if not args.no_batch:
+        out["batched"] = batch_bench(torch, D, W)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
