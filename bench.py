@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 4096-island batch and print its record")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with 3 plain 256 MiB device copies (reference kernel for "
                          "hardware byte counters)")
@@ -165,6 +167,10 @@ def main():
 
     from pfsf_amd import device as D
     from pfsf_amd import workloads as W
+
+    if args.batch_only:
+        print(json.dumps({"batched": batch_bench(torch, D, W, steps=args.steps)}))
+        return
 
     isl = W.workload(1)
     n_act = isl.n_active
