@@ -142,6 +142,50 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16):
                                  "64..262144, 1-8 anchors each, cold start"}
 
 
+def mixed_precision_bench(torch, D, W, steps=3):
+    """The 256^3 island (kappa lognormal sigma 1.5, ~70 % occupied): time to
+    1e-6 with FP32 coarse levels and with BF16 coarse kappa; phi against a
+    1e-11 solve of the same system, and the failure codes of both."""
+    import numpy as np
+
+    isl = W.workload(3)
+    dev = torch.device("cuda:0")
+    vt = torch.from_numpy(isl.vtype).to(dev)
+    kp = torch.from_numpy(isl.kappa).to(dev)
+    b = torch.from_numpy(isl.source).to(dev)
+    rc = torch.from_numpy(isl.rcomp).to(dev)
+    rt = torch.from_numpy(isl.rtens).to(dev)
+    ref = D.Solver(vt, kp, tol=1e-11, max_iters=4000)
+    xr = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=dev)
+    pr = torch.empty(isl.vtype.shape, dtype=torch.float32, device=dev)
+    iref = ref.solve(b, xr, pr)
+    ref.close()
+    code_ref = torch.empty_like(vt)
+    D.failure_scan(vt, kp, pr, rc, rt, code_ref)
+    out = {"grid": [isl.lx, isl.ly, isl.lz], "active_voxels": isl.n_active, "tol": 1e-6,
+           "reference": f"same system solved to {iref.rel_residual:.2e} (FP64 residual)"}
+    for prec, name in ((0, "fp32"), (1, "bf16_coarse")):
+        s = D.Solver(vt, kp, precision=prec, tol=1e-6, max_iters=2000)
+        x = torch.zeros_like(xr)
+        ph = torch.empty_like(pr)
+        s.solve(b, x, ph)  # warm-up (graph capture)
+        ms, info = [], None
+        for _ in range(steps):
+            x.zero_()
+            info = s.solve(b, x, ph)
+            ms.append(info.solve_ms)
+        s.close()
+        code = torch.empty_like(vt)
+        D.failure_scan(vt, kp, ph, rc, rt, code)
+        err = float(torch.linalg.norm((ph - pr).double()) / torch.linalg.norm(pr.double()))
+        out[name] = {"solve_ms": round(float(np.median(ms)), 3), "iterations": info.iterations,
+                     "passes": info.refinements, "levels": info.levels,
+                     "rel_residual": info.rel_residual, "phi_rel_l2_vs_reference": err,
+                     "failure_codes_differing": int((code != code_ref).sum())}
+        log(f"256^3 {name}: {out[name]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +195,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
+    ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 4096-island batch and print its record")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -258,6 +303,8 @@ def main():
     solver.close()
     if not args.no_batch:
         out["batched"] = batch_bench(torch, D, W)
+    if not args.no_256:
+        out["mixed_precision_256"] = mixed_precision_bench(torch, D, W)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

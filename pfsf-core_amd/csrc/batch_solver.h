@@ -38,6 +38,13 @@ struct BatchIsland {
   float* phio;        // caller: FP32 solution out (may be null)
   int coarsest;       // index of the island's coarsest BatchLevel
   int tb, tn;         // the island's finest tiles: partial-sum slots [tb, tb + tn)
+  int cb, cn;         // the island's cell chunks [cb, cb + cn)
+};
+
+// A run of cells of one island (the per-cell FP64 passes of a solve).
+struct BatchChunk {
+  int island;
+  int begin, end;
 };
 
 // One work-group of a batched stencil launch.
@@ -68,6 +75,8 @@ class BatchSolver {
 
  private:
   hipError_t iteration(int c);
+  hipError_t capture();
+  hipError_t select_islands(const int* live);  // tile lists of the live islands only
   hipError_t vcycle_coarse(bool check);
   hipError_t precondition();
   void release();
@@ -82,8 +91,17 @@ class BatchSolver {
   std::vector<void*> owned_;
   BatchLevel* levels_ = nullptr;
   BatchIsland* isl_d_ = nullptr;
-  std::vector<BatchTile*> tiles_;  // per level: launch list
+  std::vector<BatchTile*> tiles_;  // per level: launch list (live islands)
   std::vector<int> ntiles_;        // per level: launch size
+  std::vector<std::vector<BatchTile>> all_tiles_;  // per level: every island's tiles, by island
+  std::vector<std::vector<int>> tile_begin_;       // per level: island -> first index (n + 1)
+  std::vector<int> listed_;        // islands in the current launch lists
+  int listed_tiles_ = 0;           // their finest tiles
+  BatchChunk* chunks_ = nullptr;
+  int nchunks_ = 0;
+  double* pcell_ = nullptr;        // per-chunk partial sums
+  int* flags_ = nullptr;           // device: per island 1 while its PCG pass runs
+  std::vector<int> flags_host_;
   int nslots_ = 0;
   PcgScalars* sc_ = nullptr;
   BatchState* st_ = nullptr;

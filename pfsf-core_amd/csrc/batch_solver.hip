@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kTZFine = 4;    // tile depth of every island's finest level
 constexpr int kTZCoarse = 2;  // tile depth of the coarse levels
-constexpr int kIslBS = 512;   // threads of the one-work-group-per-island kernels
+constexpr int kChunk = 4096;  // cells per work-group of the per-cell passes
 
 __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
   Lv<float> L;
@@ -101,30 +101,31 @@ __global__ void __launch_bounds__(kMaxLdsCoarse)
                        check ? (unsigned)I.tn : 0u);
 }
 
-// ── one work-group per island ──────────────────────────────────────────
-__device__ __forceinline__ double isl_sum(double v) {
-  __shared__ double sh[kIslBS / 64];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int w = 0; w < kIslBS / 64; w++) t += sh[w];
-  return t;
-}
-
-// Start of a solve: the FP64 iterate from the caller's guess (or 0) on the
-// free set, ||b||^2 over the free set, state reset.
-__global__ void __launch_bounds__(kIslBS)
-    kb_begin(const BatchIsland* __restrict__ isl, BatchState* st, PcgScalars* sc) {
-  const BatchIsland& I = isl[blockIdx.x];
+// ── per-cell passes over chunks of every island ────────────────────────
+// ||b||^2 partials over the free set and the FP64 iterate from the caller's
+// guess (or 0) on the free set.
+__global__ void __launch_bounds__(BS)
+    kb_begin_part(const BatchChunk* __restrict__ ch, const BatchIsland* __restrict__ isl,
+                  double* part) {
+  __shared__ double sh[1][BS / 64];
+  const BatchChunk c = ch[blockIdx.x];
+  const BatchIsland& I = isl[c.island];
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+  for (int i = c.begin + (int)threadIdx.x; i < c.end; i += BS) {
     const bool f = I.fr[i] != 0;
     I.x[i] = (f && I.xio) ? I.xio[i] : 0.0;
     const double bv = f ? (double)I.b[i] : 0.0;
     acc += bv * bv;
   }
-  const double bb = isl_sum(acc);
+  write_partial_at(acc, part, sh, blockIdx.x);
+}
+
+// Per island: ||b||^2 from its chunk partials; state reset.
+__global__ void __launch_bounds__(BS)
+    kb_begin_fin(const BatchIsland* __restrict__ isl, const double* __restrict__ part,
+                 BatchState* st, PcgScalars* sc) {
+  const BatchIsland& I = isl[blockIdx.x];
+  const double bb = block_total(part + I.cb, (unsigned)I.cn);
   if (threadIdx.x == 0) {
     BatchState& S = st[blockIdx.x];
     S.bb = bb;
@@ -135,28 +136,40 @@ __global__ void __launch_bounds__(kIslBS)
   }
 }
 
-// Pass boundary: the true FP64 residual of every unfinished island into its
-// r[0].  An island at the tolerance, out of iterations or stalled finishes;
-// the others restart PCG from that residual (stop when the recursive residual
-// has dropped by max(0.5 tol / rel, 2e-6)).  *live counts the restarts.
-__global__ void __launch_bounds__(kIslBS)
-    kb_residual(const BatchIsland* __restrict__ isl, BatchState* st, PcgScalars* sc, double tol,
-                int max_iters, int* live) {
-  const int k = blockIdx.x;
-  const BatchIsland& I = isl[k];
-  BatchState& S = st[k];
-  PcgScalars& P = sc[k];
-  if (S.finished) return;
-  const int pass_iters = P.iters;
-  const int pass_done = P.done;
+// True FP64 residual of the unfinished islands into their r[0], with
+// ||r||^2 partials.
+__global__ void __launch_bounds__(BS)
+    kb_residual_part(const BatchChunk* __restrict__ ch, const BatchIsland* __restrict__ isl,
+                     const BatchState* __restrict__ st, double* part) {
+  __shared__ double sh[1][BS / 64];
+  const BatchChunk c = ch[blockIdx.x];
+  if (st[c.island].finished) return;
+  const BatchIsland& I = isl[c.island];
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+  for (int i = c.begin + (int)threadIdx.x; i < c.end; i += BS) {
     const double ri = residual_f64(I.d, I.type, I.kappa, I.fr, I.b, I.x, i);
     I.r0[i] = (float)ri;
     acc += ri * ri;
   }
-  const double rr = isl_sum(acc);
+  write_partial_at(acc, part, sh, blockIdx.x);
+}
+
+// Pass boundary, per island: an island at the tolerance, out of iterations
+// or stalled finishes; the others restart PCG from the new residual (the pass
+// stops when the recursive residual has dropped by max(0.5 tol / rel, 2e-6)).
+// *live counts the restarts.
+__global__ void __launch_bounds__(BS)
+    kb_residual_fin(const BatchIsland* __restrict__ isl, const double* __restrict__ part,
+                    BatchState* st, PcgScalars* sc, double tol, int max_iters, int* live) {
+  const int k = blockIdx.x;
+  const BatchIsland& I = isl[k];
+  BatchState& S = st[k];
+  if (S.finished) return;
+  const double rr = block_total(part + I.cb, (unsigned)I.cn);
   if (threadIdx.x != 0) return;
+  PcgScalars& P = sc[k];
+  const int pass_iters = P.iters;
+  const int pass_done = P.done;
   const double rel = S.bb > 0.0 ? sqrt(rr / S.bb) : sqrt(rr);
   const bool first = S.passes == 0;
   S.total += pass_iters;
@@ -182,22 +195,29 @@ __global__ void __launch_bounds__(kIslBS)
   atomicAdd(live, 1);
 }
 
-// Islands whose PCG pass is still running.
-__global__ void __launch_bounds__(1024) kb_count_live(const PcgScalars* sc, int n, int* live) {
+// Islands whose PCG pass is still running: flags[k] and their count.
+__global__ void __launch_bounds__(1024)
+    kb_live(const PcgScalars* __restrict__ sc, int n, int* flags, int* live) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   int c = 0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) c += sc[k].done == 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int f = sc[k].done == 0;
+    flags[k] = f;
+    c += f;
+  }
   atomicAdd(&cnt, c);
   __syncthreads();
   if (threadIdx.x == 0) *live = cnt;
 }
 
 // End of a solve: FP64 and FP32 solutions to the caller's arrays.
-__global__ void __launch_bounds__(kIslBS) kb_finish(const BatchIsland* __restrict__ isl) {
-  const BatchIsland& I = isl[blockIdx.x];
-  for (int64_t i = threadIdx.x; i < I.n; i += kIslBS) {
+__global__ void __launch_bounds__(BS)
+    kb_finish_part(const BatchChunk* __restrict__ ch, const BatchIsland* __restrict__ isl) {
+  const BatchChunk c = ch[blockIdx.x];
+  const BatchIsland& I = isl[c.island];
+  for (int i = c.begin + (int)threadIdx.x; i < c.end; i += BS) {
     const double v = I.x[i];
     if (I.xio) I.xio[i] = v;
     if (I.phio) I.phio[i] = (float)v;
@@ -301,7 +321,9 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
   host_isl_.assign(n, BatchIsland{});
   lmax_ = 0;
   for (int i = 0; i < n; i++) lmax_ = std::max(lmax_, isl_[i]->levels());
-  std::vector<std::vector<BatchTile>> tl(lmax_);
+  all_tiles_.assign(lmax_, {});
+  tile_begin_.assign(lmax_, std::vector<int>(n + 1, 0));
+  std::vector<BatchChunk> chunks;
   int slot = 0;
   for (int i = 0; i < n; i++) {
     MgSolver& S = *isl_[i];
@@ -355,22 +377,28 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
         r.y0 = ((t / T.ntx) % T.nty) * kTY;
         r.z0 = (t / (T.ntx * T.nty)) * T.tz;
         r.slot = l == 0 ? slot++ : 0;
-        tl[l].push_back(r);
+        all_tiles_[l].push_back(r);
       }
     }
+    for (int l = 0; l < lmax_; l++) tile_begin_[l][i + 1] = (int)all_tiles_[l].size();
     I.tn = slot - I.tb;
+    I.cb = (int)chunks.size();
+    for (int64_t c0 = 0; c0 < I.n; c0 += kChunk)
+      chunks.push_back(BatchChunk{i, (int)c0, (int)std::min<int64_t>(I.n, c0 + kChunk)});
+    I.cn = (int)chunks.size() - I.cb;
   }
   nslots_ = slot;
+  nchunks_ = (int)chunks.size();
   levels_ = balloc<BatchLevel>(lv.size(), owned_, err);
   isl_d_ = balloc<BatchIsland>(n, owned_, err);
   tiles_.assign(lmax_, nullptr);
   ntiles_.assign(lmax_, 0);
-  std::vector<std::vector<BatchTile>> ordered(lmax_);
-  for (int l = 0; l < lmax_; l++) {
-    ordered[l] = xcd_order(tl[l]);
-    ntiles_[l] = (int)ordered[l].size();
-    if (ntiles_[l] > 0) tiles_[l] = balloc<BatchTile>(ordered[l].size(), owned_, err);
-  }
+  for (int l = 0; l < lmax_; l++)
+    if (!all_tiles_[l].empty())
+      tiles_[l] = balloc<BatchTile>(all_tiles_[l].size() + 8, owned_, err);
+  chunks_ = balloc<BatchChunk>(nchunks_, owned_, err);
+  pcell_ = balloc<double>(nchunks_, owned_, err);
+  flags_ = balloc<int>(n, owned_, err);
   sc_ = balloc<PcgScalars>(n, owned_, err);
   st_ = balloc<BatchState>(n, owned_, err);
   ppq_ = balloc<double>(nslots_, owned_, err);
@@ -380,15 +408,14 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
   live_ = balloc<int>(2, owned_, err);
   if (err != hipSuccess) return err;
   if (chk(hipHostMalloc((void**)&live_host_, 2 * sizeof(int)), err)) return err;
+  flags_host_.assign(n, 1);
   hipStream_t s = stream_;
   chk(hipMemcpyAsync(levels_, lv.data(), lv.size() * sizeof(BatchLevel), hipMemcpyHostToDevice,
                      s),
       err);
-  for (int l = 0; l < lmax_; l++)
-    if (ntiles_[l] > 0)
-      chk(hipMemcpyAsync(tiles_[l], ordered[l].data(), ordered[l].size() * sizeof(BatchTile),
-                         hipMemcpyHostToDevice, s),
-          err);
+  chk(hipMemcpyAsync(chunks_, chunks.data(), chunks.size() * sizeof(BatchChunk),
+                     hipMemcpyHostToDevice, s),
+      err);
   // every island uses the same coarse-solve polynomial (same bound, steps)
   chk(hipMemcpyAsync(coef_, isl_[0]->chebyshev_coefficients(), 2 * 64 * sizeof(float),
                      hipMemcpyHostToDevice, s),
@@ -398,6 +425,55 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
   chk(hipMemsetAsync(prr_, 0, nslots_ * sizeof(double), s), err);
   chk(hipMemsetAsync(prz_, 0, nslots_ * sizeof(double), s), err);
   chk(hipStreamSynchronize(s), err);
+  if (err != hipSuccess) return err;
+  return select_islands(flags_host_.data());
+}
+
+// Launch lists holding only the tiles of islands with live[i] != 0, in
+// XCD-grouped order; the graph is re-captured at its next use.
+hipError_t BatchSolver::select_islands(const int* live) {
+  hipError_t err = hipSuccess;
+  const int n = (int)isl_.size();
+  listed_.assign(live, live + n);
+  listed_tiles_ = 0;
+  std::vector<std::vector<BatchTile>> order(lmax_);  // alive until the copies are done
+  for (int l = 0; l < lmax_; l++) {
+    std::vector<BatchTile> v;
+    for (int i = 0; i < n; i++)
+      if (live[i])
+        v.insert(v.end(), all_tiles_[l].begin() + tile_begin_[l][i],
+                 all_tiles_[l].begin() + tile_begin_[l][i + 1]);
+    if (l == 0) listed_tiles_ = (int)v.size();
+    order[l] = xcd_order(v);
+    ntiles_[l] = (int)order[l].size();
+    if (!order[l].empty())
+      chk(hipMemcpyAsync(tiles_[l], order[l].data(), order[l].size() * sizeof(BatchTile),
+                         hipMemcpyHostToDevice, stream_),
+          err);
+  }
+  chk(hipStreamSynchronize(stream_), err);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  return err;
+}
+
+// Two iterations plus the live-island flags, captured for replay.
+hipError_t BatchSolver::capture() {
+  hipError_t err = hipSuccess;
+  hipStream_t s = stream_;
+  const int n = (int)isl_.size();
+  if (chk(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), err)) return err;
+  (void)iteration(0);
+  (void)iteration(1);
+  kb_live<<<1, 1024, 0, s>>>(sc_, n, flags_, live_ + 1);
+  hipGraph_t g = nullptr;
+  chk(hipStreamEndCapture(s, &g), err);
+  if (err == hipSuccess && !chk(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), err))
+    graph_ = g;
+  else if (g)
+    (void)hipGraphDestroy(g);
   return err;
 }
 
@@ -421,6 +497,7 @@ hipError_t BatchSolver::vcycle_coarse(bool check) {
 // z = M r[0] and the r.z partials for the islands starting a pass.
 hipError_t BatchSolver::precondition() {
   hipStream_t s = stream_;
+  if (ntiles_[0] == 0) return hipSuccess;
   kb_down<kTZFine, false><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, nullptr,
                                                     nullptr, 0);
   (void)vcycle_coarse(false);
@@ -431,6 +508,7 @@ hipError_t BatchSolver::precondition() {
 // One PCG iteration of every live island, reading r[c], p[c].
 hipError_t BatchSolver::iteration(int c) {
   hipStream_t s = stream_;
+  if (ntiles_[0] == 0) return hipSuccess;
   const unsigned G = ntiles_[0];
   kb_spmv_dir<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, ppq_, c);
   kb_down<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, ppq_, prr_, c);
@@ -457,40 +535,42 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
   chk(hipEventRecord(ev0_, s), err);
   chk(hipMemcpyAsync(isl_d_, host_isl_.data(), n * sizeof(BatchIsland), hipMemcpyHostToDevice, s),
       err);
-  kb_begin<<<n, kIslBS, 0, s>>>(isl_d_, st_, sc_);
-  if (!graph_exec_ && err == hipSuccess) {
-    if (!chk(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), err)) {
-      (void)iteration(0);
-      (void)iteration(1);
-      kb_count_live<<<1, 1024, 0, s>>>(sc_, n, live_ + 1);
-      hipGraph_t g = nullptr;
-      chk(hipStreamEndCapture(s, &g), err);
-      if (err == hipSuccess && !chk(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), err))
-        graph_ = g;
-      else if (g)
-        (void)hipGraphDestroy(g);
-    }
-  }
+  kb_begin_part<<<nchunks_, BS, 0, s>>>(chunks_, isl_d_, pcell_);
+  kb_begin_fin<<<n, BS, 0, s>>>(isl_d_, pcell_, st_, sc_);
   int sweeps = 0;
   const int pairs_per_check = 4;
   for (int pass = 0; pass < 18 && err == hipSuccess; pass++) {
     chk(hipMemsetAsync(live_, 0, sizeof(int), s), err);
-    kb_residual<<<n, kIslBS, 0, s>>>(isl_d_, st_, sc_, tol, max_it, live_);
-    chk(hipMemcpyAsync(live_host_, live_, sizeof(int), hipMemcpyDeviceToHost, s), err);
+    kb_residual_part<<<nchunks_, BS, 0, s>>>(chunks_, isl_d_, st_, pcell_);
+    kb_residual_fin<<<n, BS, 0, s>>>(isl_d_, pcell_, st_, sc_, tol, max_it, live_);
+    kb_live<<<1, 1024, 0, s>>>(sc_, n, flags_, live_ + 1);
+    chk(hipMemcpyAsync(flags_host_.data(), flags_, n * sizeof(int), hipMemcpyDeviceToHost, s),
+        err);
+    chk(hipMemcpyAsync(live_host_, live_, 2 * sizeof(int), hipMemcpyDeviceToHost, s), err);
     chk(hipStreamSynchronize(s), err);
     if (err != hipSuccess || live_host_[0] == 0) break;
+    if (flags_host_ != listed_) chk(select_islands(flags_host_.data()), err);
     chk(precondition(), err);
     // every island's pass ends within max_it iterations (device cap)
     for (int it = 0; it < max_it + 2 * pairs_per_check && err == hipSuccess;
          it += 2 * pairs_per_check) {
-      for (int k = 0; k < pairs_per_check; k++) chk(hipGraphLaunch(graph_exec_, s), err);
+      if (!graph_exec_) chk(capture(), err);
+      for (int k = 0; k < pairs_per_check && err == hipSuccess; k++)
+        chk(hipGraphLaunch(graph_exec_, s), err);
       sweeps += 2 * pairs_per_check;
+      chk(hipMemcpyAsync(flags_host_.data(), flags_, n * sizeof(int), hipMemcpyDeviceToHost, s),
+          err);
       chk(hipMemcpyAsync(live_host_ + 1, live_ + 1, sizeof(int), hipMemcpyDeviceToHost, s), err);
       chk(hipStreamSynchronize(s), err);
       if (live_host_[1] == 0) break;
+      // drop finished islands from the launches once they hold a quarter of the tiles
+      int live_tiles = 0;
+      for (int i = 0; i < n; i++)
+        if (flags_host_[i]) live_tiles += host_isl_[i].tn;
+      if (4 * live_tiles < 3 * listed_tiles_) chk(select_islands(flags_host_.data()), err);
     }
   }
-  kb_finish<<<n, kIslBS, 0, s>>>(isl_d_);
+  kb_finish_part<<<nchunks_, BS, 0, s>>>(chunks_, isl_d_);
   chk(hipGetLastError(), err);
   chk(hipEventRecord(ev1_, s), err);
   chk(hipEventSynchronize(ev1_), err);

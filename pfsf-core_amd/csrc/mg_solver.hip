@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(BS)
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   double acc[1] = {0.0};
   if (i < (int64_t)d.lx * d.ly * d.lz) {
-    const double ri = residual_f64(d, type, kappa, fr, b, x, i);
+    const double ri = residual_f64(d, type, kappa, fr, b, x, (int)i);
     r32[i] = (float)ri;
     acc[0] = ri * ri;
   }
