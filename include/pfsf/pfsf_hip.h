@@ -203,6 +203,46 @@ PFSF_API pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, doubl
 
 PFSF_API void pfsf_batch_destroy(pfsf_batch s);
 
+/* ── Load-path loss and its adjoint gradient dL/dkappa ───────────────────
+ * No counterpart in the reference (it has no adjoint); the loss is built on
+ * the failure scan's crushing flux (SH/failure_scan.comp.glsl:116-158).
+ * Load steps b_t = (t/steps) * source, t = 1..steps, each solved with the
+ * multigrid PCG solver to `tol`;
+ *   L = sum_t sum_{free i} softplus(f_i(phi_t) - rcomp_i),
+ *   f_i = sum_j a_ij (phi_j - phi_i)^+ over the non-air neighbours of i.
+ * The gradient uses one adjoint solve per step with the same (symmetric)
+ * operator; states are kept every `checkpoint_every` steps and each segment
+ * is re-solved from its checkpoint during the backward pass. */
+typedef struct {
+  pfsf_dims dims;
+  const uint8_t* type;  /* device, N */
+  const float* kappa;   /* device, N */
+  const float* source;  /* device, N: the full load b */
+  const float* rcomp;   /* device, N */
+  int32_t steps;            /* 0 = 50 */
+  int32_t checkpoint_every; /* 0 = 10 */
+  double tol;               /* per-solve relative residual (0 = 1e-6) */
+  int32_t max_iters;        /* per solve (0 = 2000) */
+  void* stream;
+} pfsf_load_path_desc;
+
+typedef struct {
+  double loss;
+  int32_t forward_solves, recompute_solves, adjoint_solves, checkpoints;
+  int64_t pcg_iterations;
+  double ms; /* device time of the whole evaluation */
+} pfsf_load_path_info;
+
+/* Loss only.  fp64 = 0: FP32 flux and softplus (the gradient's forward);
+ * fp64 = 1: FP64 throughout (for finite-difference checks).  Synchronous. */
+PFSF_API pfsf_result pfsf_load_path_loss(const pfsf_load_path_desc* desc, int32_t fp64,
+                                         pfsf_load_path_info* info);
+
+/* Loss (FP32 forward) and dL/dkappa into grad (device, N doubles; voxels
+ * whose kappa does not enter the operator get 0).  Synchronous. */
+PFSF_API pfsf_result pfsf_load_path_grad(const pfsf_load_path_desc* desc, double* grad,
+                                         pfsf_load_path_info* info);
+
 /* Orphan mask used by the solver: 1 for solid voxels 26-connected to an
  * anchor through non-air voxels, else 0 (device, N bytes).  Synchronous. */
 PFSF_API pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach,

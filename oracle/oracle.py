@@ -71,6 +71,10 @@ def lib():
         L.or_chebyshev_omega.restype = C.c_float
         L.or_downsample_2to1.argtypes = [P, P, I, I, I, P, P]
         L.or_energy_grad_kappa.argtypes = [I, I, I, P, P, P, P, P]
+        L.or_flux_loss.argtypes = [I, I, I, P, P, P, P, P, P]
+        L.or_flux_loss.restype = C.c_double
+        L.or_flux_grad_phi.argtypes = [I, I, I, P, P, P, P, P, P]
+        L.or_flux_grad_kappa.argtypes = [I, I, I, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -262,6 +266,66 @@ def direct_solve(isl, b=None):
     phi = np.zeros(isl.n, np.float64)
     phi[sidx] = sol
     return phi.reshape(isl.vtype.shape)
+
+
+def free_mask(isl):
+    """Unknowns of the solve: solid voxels 26-connected to an anchor (u8)."""
+    reach, _ = anchored(isl)
+    return ((isl.vtype == 1) & (reach != 0)).astype(np.uint8)
+
+
+def energy_grad_kappa(isl, u, v):
+    """sum over couplings of d a_ij / d kappa_i (u_i - u_j)(v_i - v_j) per voxel i."""
+    g = np.empty(isl.vtype.shape, np.float64)
+    lib().or_energy_grad_kappa(*_dims(isl), _p(isl.vtype), _p(isl.kappa),
+                               _p(np.ascontiguousarray(u, np.float64)),
+                               _p(np.ascontiguousarray(v, np.float64)), _p(g))
+    return g
+
+
+def flux_loss(isl, fr, phi, rcomp=None):
+    """(L, w) of the load-path loss for one load state phi (FP64)."""
+    rc = isl.rcomp if rcomp is None else rcomp
+    w = np.empty(isl.vtype.shape, np.float64)
+    L = lib().or_flux_loss(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(fr),
+                           _p(np.ascontiguousarray(phi, np.float64)), _p(rc), _p(w))
+    return L, w
+
+
+def flux_grad_phi(isl, fr, phi, w):
+    g = np.empty(isl.vtype.shape, np.float64)
+    lib().or_flux_grad_phi(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(fr),
+                           _p(np.ascontiguousarray(phi, np.float64)), _p(w), _p(g))
+    return g
+
+
+def flux_grad_kappa(isl, fr, phi, w):
+    g = np.empty(isl.vtype.shape, np.float64)
+    lib().or_flux_grad_kappa(*_dims(isl), _p(isl.vtype), _p(isl.kappa), _p(fr),
+                             _p(np.ascontiguousarray(phi, np.float64)), _p(w), _p(g))
+    return g
+
+
+def load_path(isl, steps=50, b=None):
+    """Load path b_t = (t/steps) b, t = 1..steps, solved exactly (sparse direct):
+    returns (L, dL/dkappa) in FP64.  A phi_t = b_t is linear in the load, so
+    phi_t = s_t phi_1 (s_t = t/steps) and the adjoint states combine into one
+    solve: lambda = A^-1 sum_t s_t dL_t/dphi_t."""
+    fr = free_mask(isl)
+    b = isl.source if b is None else b
+    phi1 = direct_solve(isl, b)
+    L = 0.0
+    gexp = np.zeros(isl.vtype.shape, np.float64)
+    G = np.zeros(isl.vtype.shape, np.float64)
+    for t in range(1, steps + 1):
+        s = t / steps
+        ph = s * phi1
+        Lt, w = flux_loss(isl, fr, ph)
+        L += Lt
+        G += s * flux_grad_phi(isl, fr, ph, w)
+        gexp += flux_grad_kappa(isl, fr, ph, w)
+    lam = direct_solve(isl, G)
+    return L, gexp - energy_grad_kappa(isl, lam, phi1)
 
 
 def energy(isl_all_solid_kappa, phi, lx, ly, lz):

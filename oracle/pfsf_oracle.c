@@ -510,3 +510,100 @@ void or_energy_grad_kappa(int lx, int ly, int lz, const uint8_t* type, const flo
         g[i] = acc;
       }
 }
+
+/* ── Load-path loss (adjoint workload) ─────────────────────────────────────
+ * Not a reference function: the reference has no adjoint.  The loss is
+ * defined on the reference's crushing flux (the failure scan's flux_in,
+ * SH/failure_scan.comp.glsl:116-158 restated above):
+ *   f_i = sum_j a_ij (phi_j - phi_i)^+   over non-air neighbours j of a free i
+ *   L   = sum_i softplus(f_i - rcomp_i)  over free voxels i
+ * phi is 0 outside the free set (mask `fr`).  FP64 throughout; the pieces
+ * below are the exact partial derivatives of L, checked against finite
+ * differences in the tests. */
+static inline double softplus(double u) { return (u > 0 ? u : 0.0) + log1p(exp(-fabs(u))); }
+
+double or_flux_loss(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                    const uint8_t* fr, const double* phi, const float* rcomp, double* w) {
+  double L = 0.0;
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        if (w) w[i] = 0.0;
+        if (!fr[i]) continue;
+        float ki = kap(kappa, i);
+        double pi = phi[i], f = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double pj = fr[j] ? phi[j] : 0.0;
+          if (pj > pi) f += coupling(k, ki, kap(kappa, j)) * (pj - pi);
+        }
+        double u = f - (double)rcomp[i];
+        L += softplus(u);
+        if (w) w[i] = 1.0 / (1.0 + exp(-u));
+      }
+  return L;
+}
+
+/* g_k = dL/dphi_k = -w_k sum_j a_kj [phi_j > phi_k] + sum_j w_j a_kj [phi_k > phi_j]
+ * (free k; 0 elsewhere; w = 0 off the free set). */
+void or_flux_grad_phi(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                      const uint8_t* fr, const double* phi, const double* w, double* g) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        g[i] = 0.0;
+        if (!fr[i]) continue;
+        float ki = kap(kappa, i);
+        double pi = phi[i], acc = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double a = coupling(k, ki, kap(kappa, j));
+          double pj = fr[j] ? phi[j] : 0.0;
+          if (pj > pi) acc -= w[i] * a;
+          if (pi > pj && fr[j]) acc += w[j] * a;
+        }
+        g[i] = acc;
+      }
+}
+
+/* Explicit part of dL/dkappa_m at fixed phi:
+ *   sum_j da_mj/dkappa_m [w_m (phi_j - phi_m)^+ + w_j (phi_m - phi_j)^+]
+ * over non-air neighbours (da as in or_energy_grad_kappa). */
+void or_flux_grad_kappa(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                        const uint8_t* fr, const double* phi, const double* w, double* g) {
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        g[i] = 0.0;
+        if (type[i] == T_AIR) continue;
+        double ki = (double)kap(kappa, i);
+        double pi = fr[i] ? phi[i] : 0.0, wi = fr[i] ? w[i] : 0.0, acc = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double kj = (double)kap(kappa, j);
+          double da;
+          if (k < 6)
+            da = 0.5;
+          else {
+            double c = (k < 18) ? (double)EDGE_P : (double)CORNER_P;
+            da = (ki > 0.0) ? c * 0.5 * sqrt(kj / ki) : 0.0;
+          }
+          double pj = fr[j] ? phi[j] : 0.0, wj = fr[j] ? w[j] : 0.0;
+          double up = pj > pi ? pj - pi : 0.0, dn = pi > pj ? pi - pj : 0.0;
+          acc += da * (wi * up + wj * dn);
+        }
+        g[i] = acc;
+      }
+}

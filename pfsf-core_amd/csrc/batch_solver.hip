@@ -27,7 +27,7 @@ namespace {
 
 constexpr int kTZFine = 4;    // tile depth of every island's finest level
 constexpr int kTZCoarse = 2;  // tile depth of the coarse levels
-constexpr int kChunk = 4096;  // cells per work-group of the per-cell passes
+constexpr int kChunk = 1024;  // cells per work-group of the per-cell passes
 
 __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
   Lv<float> L;

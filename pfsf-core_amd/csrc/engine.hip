@@ -813,4 +813,27 @@ pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, double* const*
 
 void pfsf_batch_destroy(pfsf_batch s) { delete s; }
 
+// ═════════════════════════════ load path / adjoint ═══════════════════════
+
+static bool load_path_ok(const pfsf_load_path_desc* d) {
+  return d && d->type && d->kappa && d->source && d->rcomp && dims_ok(d->dims) &&
+         nvox(d->dims) <= pfsf::kMaxSolverCells && d->steps >= 0 && d->checkpoint_every >= 0;
+}
+
+pfsf_result pfsf_load_path_loss(const pfsf_load_path_desc* desc, int32_t fp64,
+                                pfsf_load_path_info* info) {
+  if (!load_path_ok(desc) || !info) return PFSF_ERROR_INVALID_ARG;
+  hipError_t e = pfsf::load_path_run(*desc, fp64 != 0, nullptr, info);
+  return e == hipSuccess ? PFSF_OK
+                         : (e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN);
+}
+
+pfsf_result pfsf_load_path_grad(const pfsf_load_path_desc* desc, double* grad,
+                                pfsf_load_path_info* info) {
+  if (!load_path_ok(desc) || !grad || !info) return PFSF_ERROR_INVALID_ARG;
+  hipError_t e = pfsf::load_path_run(*desc, false, grad, info);
+  return e == hipSuccess ? PFSF_OK
+                         : (e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN);
+}
+
 }  // extern "C"

@@ -142,4 +142,8 @@ class MgSolver {
   bool use_graph_ = true;
 };
 
+// Load-path loss and adjoint gradient (adjoint.hip).
+hipError_t load_path_run(const pfsf_load_path_desc& desc, bool fp64, double* grad,
+                         pfsf_load_path_info* info);
+
 }  // namespace pfsf

@@ -105,6 +105,19 @@ class BatchInfo(C.Structure):
                 ("solve_ms", C.c_double)]
 
 
+class LoadPathDesc(C.Structure):
+    _fields_ = [("dims", Dims), ("type", C.c_void_p), ("kappa", C.c_void_p),
+                ("source", C.c_void_p), ("rcomp", C.c_void_p), ("steps", C.c_int32),
+                ("checkpoint_every", C.c_int32), ("tol", C.c_double), ("max_iters", C.c_int32),
+                ("stream", C.c_void_p)]
+
+
+class LoadPathInfo(C.Structure):
+    _fields_ = [("loss", C.c_double), ("forward_solves", C.c_int32),
+                ("recompute_solves", C.c_int32), ("adjoint_solves", C.c_int32),
+                ("checkpoints", C.c_int32), ("pcg_iterations", C.c_int64), ("ms", C.c_double)]
+
+
 SIGNATURES = {
     # pfsf.h
     "pfsf_create": (P, [C.POINTER(Config)]),
@@ -157,6 +170,8 @@ SIGNATURES = {
     "pfsf_batch_solve": (R, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(BatchInfo),
                              C.POINTER(SolveInfo)]),
     "pfsf_batch_destroy": (None, [P]),
+    "pfsf_load_path_loss": (R, [C.POINTER(LoadPathDesc), C.c_int32, C.POINTER(LoadPathInfo)]),
+    "pfsf_load_path_grad": (R, [C.POINTER(LoadPathDesc), P, C.POINTER(LoadPathInfo)]),
     "pfsf_solver_profile": (R, [P, I32, I32, C.POINTER(F64)]),
     "pfsf_hip_anchored_mask": (R, [Dims, P, P, P]),
     "pfsf_register_island_kappa": (R, [P, I32, P, I64]),

@@ -290,3 +290,35 @@ def test_workloads_are_seeded_and_shaped():
     b = W.dense_block(0, 32, 0.5)
     assert np.array_equal(a.kappa, b.kappa)
     assert 20.0 < np.median(a.kappa) < 45.0
+
+
+def _loaded(seed, L, load=40.0):
+    """Tiny island whose crushing flux is near rcomp (so the loss is not flat)."""
+    isl = W.small_box_union(seed, L=L)
+    isl.source = np.ascontiguousarray(isl.source * np.float32(load))
+    return isl
+
+
+def test_load_path_gradient_matches_finite_differences():
+    """dL/dkappa of the load-path loss (flux_loss, flux_grad_phi,
+    flux_grad_kappa, energy_grad_kappa, one combined adjoint solve) against
+    central differences of L itself, FP64, exact solves."""
+    isl = _loaded(21, 7)
+    L, g = O.load_path(isl, steps=10)
+    assert L > 0
+    rng = np.random.Generator(np.random.PCG64(3))
+    cells = np.flatnonzero(isl.vtype.ravel() != 0)
+    for m in rng.choice(cells, size=12, replace=False):
+        k0 = isl.kappa.ravel()[m]
+        vals, ks = [], []
+        for sgn in (1, -1):
+            # the couplings round kappa products to float32 (relative 6e-8), so
+            # the step must be large against that: 3e-3 keeps the rounding
+            # noise near 2e-5 and the truncation error near 2e-6
+            isl.kappa.ravel()[m] = k0 * (1 + sgn * 3e-3)
+            ks.append(float(isl.kappa.ravel()[m]))  # the float32 value actually used
+            vals.append(O.load_path(isl, steps=10)[0])
+        isl.kappa.ravel()[m] = k0
+        fd = (vals[0] - vals[1]) / (ks[0] - ks[1])
+        assert abs(fd - g.ravel()[m]) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g).max()), (m, fd,
+                                                                                   g.ravel()[m])
