@@ -186,6 +186,34 @@ def mixed_precision_bench(torch, D, W, steps=3):
     return out
 
 
+def adjoint_bench(torch, W, steps=2):
+    """The 96^3 island: 50-step load path, loss (forward) and dL/dkappa
+    (forward + checkpointed adjoint), solves to 1e-6."""
+    import numpy as np
+
+    from pfsf_amd import adjoint as A
+
+    isl = W.workload(4)
+    dev = torch.device("cuda:0")
+    vt, kp, b, rc = (torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                     for a in (isl.vtype, isl.kappa, isl.source, isl.rcomp))
+    A.load_path_loss(vt, kp, b, rc)  # warm-up
+    fw = [A.load_path_loss(vt, kp, b, rc) for _ in range(steps)]
+    gr = [A.load_path_grad(vt, kp, b, rc)[0] for _ in range(steps)]
+    f, g = fw[-1], gr[-1]
+    out = {"grid": [isl.lx, isl.ly, isl.lz], "active_voxels": isl.n_active, "load_steps": 50,
+           "checkpoint_every": 10, "tol": 1e-6, "loss": f.loss,
+           "forward_ms": round(float(np.median([i.ms for i in fw])), 2),
+           "gradient_ms": round(float(np.median([i.ms for i in gr])), 2),
+           "gradient_solves": {"forward": g.forward_solves, "recompute": g.recompute_solves,
+                               "adjoint": g.adjoint_solves},
+           "gradient_pcg_iterations": g.pcg_iterations,
+           "check": "finite differences on 64 voxels and an exact FP64 gradient: "
+                    "tests/test_gpu_adjoint.py"}
+    log(f"96^3 adjoint: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +224,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint record")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 4096-island batch and print its record")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -305,6 +334,8 @@ def main():
         out["batched"] = batch_bench(torch, D, W)
     if not args.no_256:
         out["mixed_precision_256"] = mixed_precision_bench(torch, D, W)
+    if not args.no_adjoint:
+        out["adjoint_96"] = adjoint_bench(torch, W)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
