@@ -99,10 +99,18 @@ template <int TZ>
 struct Tile {
   static constexpr int HZ = TZ + 2, NROWS = HY * HZ, HN = HXY * HZ;
   float v[HN];  // value (0 outside the free set)
-  float k[HN];  // kappa (< 0 outside the free set)
-  float s[HN];  // sqrt kappa (0 outside the free set); hardware square root,
-                // used for both ends of every coupling so A stays symmetric
+  float s[HN];  // sqrt kappa on the free set, -0.0 elsewhere (see free_s)
 };
+
+// LDS tiles hold sqrt(kappa) only: kappa = s^2 for the face weights, and a
+// cell outside the free set holds s = -0.0, so its edge and corner products
+// vanish and its sign bit (set by no free cell, not even one with kappa = 0)
+// marks it for the face test.  The hardware square root is used for both
+// ends of every coupling, so the operator stays symmetric.
+__device__ __forceinline__ float free_s(float k) {
+  return k >= 0.f ? __builtin_amdgcn_sqrtf(k) : -0.0f;
+}
+__device__ __forceinline__ bool is_free_s(float s) { return __float_as_int(s) >= 0; }
 
 template <int TZ>
 __device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& z0) {
@@ -169,8 +177,7 @@ __device__ __forceinline__ void stage_load(const Lv<KT>& L, int x0, int y0, int 
 template <int TZ>
 __device__ __forceinline__ void put_cell(Tile<TZ>& T, int e, float k, float v) {
   T.v[e] = k >= 0.f ? v : 0.f;
-  T.k[e] = k;
-  T.s[e] = k > 0.f ? __builtin_amdgcn_sqrtf(k) : 0.f;
+  T.s[e] = free_s(k);
 }
 
 template <int TZ, typename Raw, typename FIN>
@@ -198,17 +205,18 @@ __device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int 
 }
 
 // (A v)_i at LDS cell c of a box with row stride sy and plane stride sz:
-// difference form over free neighbours + anchor mass.  With constant strides
-// every read is one LDS instruction with an immediate offset from one base.
+// difference form over free neighbours + anchor mass.  v holds values (0 off
+// the free set), s the free_s coding of kappa.  With constant strides every
+// read is one LDS instruction with an immediate offset from one base.
 __device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
-                                               const float* __restrict__ k,
                                                const float* __restrict__ s, int c, int sy,
-                                               int sz, float ki, float mi, float scale) {
-  const float vc = v[c];
+                                               int sz, float mi, float scale) {
+  const float vc = v[c], si = s[c];
+  const float ki = si * si;
   float af = 0.f, ae = 0.f, ac = 0.f;
   auto face = [&](int o) {
-    float kj = k[c + o];
-    float w = kj >= 0.f ? ki + kj : 0.f;
+    const float sj = s[c + o];
+    const float w = is_free_s(sj) ? fmaf(sj, sj, ki) : 0.f;
     af += w * (vc - v[c + o]);
   };
   auto edge = [&](int o) { ae += s[c + o] * (vc - v[c + o]); };
@@ -239,13 +247,12 @@ __device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
   corner(1 - sy + sz);
   corner(-1 + sy + sz);
   corner(1 + sy + sz);
-  return scale * (0.5f * af + s[c] * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
 }
 
 template <int TZ>
-__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float ki, float mi,
-                                            float scale) {
-  return stencil_apply(T.v, T.k, T.s, c, HX, HXY, ki, mi, scale);
+__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float mi, float scale) {
+  return stencil_apply(T.v, T.s, c, HX, HXY, mi, scale);
 }
 
 __device__ __forceinline__ int tcell(int tx, int ty, int tz) {
@@ -369,9 +376,9 @@ __device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, in
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       const int c = c0 + tz * HXY;
-      const float ki = T.k[c], pv = T.v[c];
-      float qi = tile_apply<TZ>(T, c, ki, amr[tz], L.scale);
-      qi = ki >= 0.f ? qi : 0.f;
+      const float pv = T.v[c];
+      float qi = tile_apply<TZ>(T, c, amr[tz], L.scale);
+      qi = is_free_s(T.s[c]) ? qi : 0.f;
       acc[0] += (double)pv * qi;
       if (inb[tz]) {
         st(pout, i0 + tz * sz, pv);
@@ -490,9 +497,9 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       const int cc = c0 + tz * HXY;
-      const float ki = T.k[cc], ri = rr_[tz];
-      const bool fr = ki >= 0.f;
-      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
+      const float ri = rr_[tz];
+      const bool fr = is_free_s(T.s[cc]);
+      const float ax = tile_apply<TZ>(T, cc, amr[tz], L.scale);
       res[tz] = fr ? ri - ax : 0.f;
       if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
     }
@@ -540,11 +547,12 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       const int cc = c0 + tz * HXY;
-      const float ki = T.k[cc], xi = T.v[cc], ri = rr_[tz];
-      const float ax = tile_apply<TZ>(T, cc, ki, amr[tz], L.scale);
+      const float xi = T.v[cc], ri = rr_[tz];
+      const bool fr = is_free_s(T.s[cc]);
+      const float ax = tile_apply<TZ>(T, cc, amr[tz], L.scale);
       float zi = xi + om * (ri - ax) * rcp(dgr[tz]);
-      zi = ki >= 0.f ? zi : 0.f;
-      if (FINEST) acc[0] += ki >= 0.f ? (double)ri * zi : 0.0;
+      zi = fr ? zi : 0.f;
+      if (FINEST) acc[0] += fr ? (double)ri * zi : 0.0;
       if (inb[tz]) st(out, i0 + tz * sz, zi);
     }
   }
@@ -552,8 +560,9 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
 }
 
 // Whole Chebyshev solve of the coarsest level held in one work-group's LDS
-// (n <= kMaxLdsCoarse cells, one cell per thread).  Kappa, sqrt kappa and the
-// iterate sit in LDS boxes with a one-cell border (kappa -1, value 0), so the
+// (n <= kMaxLdsCoarse cells, one cell per thread).  The free_s coding of kappa
+// and the iterate sit in LDS boxes with a one-cell border (outside the free
+// set, value 0), so the
 // 26 neighbour reads need no bounds tests; the diagonal, anchor mass, rhs and
 // direction of a thread's own cell stay in registers.
 constexpr int kMaxLdsCoarse = 1024;
@@ -586,12 +595,11 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
     }
     if (stop) return;
   }
-  __shared__ float sk[kMaxLdsPadded], ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
+  __shared__ float ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
   const pfsf_dims d = L.d;
   const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
   for (int e = threadIdx.x; e < np; e += blockDim.x) {
-    sk[e] = -1.f;
-    ss[e] = 0.f;
+    ss[e] = -0.0f;
     sx[0][e] = 0.f;
     sx[1][e] = 0.f;
   }
@@ -612,8 +620,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
       di = coef[1] * ri * rcp(D);
       xi = di;
     }
-    sk[e] = ki;
-    ss[e] = ki > 0.f ? __builtin_amdgcn_sqrtf(ki) : 0.f;
+    ss[e] = free_s(ki);
     sx[0][e] = xi;
   }
   const bool live = act && ki >= 0.f;
@@ -622,7 +629,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   for (int st = 1; st < steps; st++) {
     const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
     if (live) {
-      const float ax = stencil_apply(sx[cur], sk, ss, e, sy, sz, ki, m, L.scale);
+      const float ax = stencil_apply(sx[cur], ss, e, sy, sz, m, L.scale);
       di = c1 * di + c2 * (ri - ax) * rcp(D);
       xi = xi + di;
       sx[cur ^ 1][e] = xi;

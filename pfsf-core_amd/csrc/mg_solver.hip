@@ -231,8 +231,7 @@ __global__ void __launch_bounds__(BS)
     if (gz >= d.lz) break;
     const int i = flat32(gx, gy, gz, d);
     int cc = tcell(tx, ty, tz);
-    float ki = T.k[cc];
-    if (ki < 0.f) {
+    if (!is_free_s(T.s[cc])) {
       xout[i] = 0.f;
       if (first) dv[i] = 0.f;
       continue;
@@ -244,7 +243,7 @@ __global__ void __launch_bounds__(BS)
       xout[i] = di;
     } else {
       float xi = T.v[cc];
-      float ax = tile_apply<TZ>(T, cc, ki, L.am[i], L.scale);
+      float ax = tile_apply<TZ>(T, cc, L.am[i], L.scale);
       float di = c1 * dv[i] + c2 * (r[i] - ax) * rcp(D);
       dv[i] = di;
       xout[i] = xi + di;
