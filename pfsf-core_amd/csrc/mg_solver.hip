@@ -1000,20 +1000,33 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   double rel = 0.0;
   chk(true_residual(b, &rel), err);
   int passes = 0, total = 0;
-  const int pairs_per_check = 4;  // 8 iterations between host checks
+  const int pairs_first = 4;  // 8 iterations before the first host check
+  const IrScalars* irh = (const IrScalars*)(sc_host_ + 1);
   for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
     double stop_rel = std::max(0.5 * tol / rel, 2e-6);
     k_pcg_begin<<<1, 1, 0, s>>>(sc_, ir_, stop_rel, max_it - total, 1);
     chk(precondition(), err);
-    int done = 0;
-    // the device caps the pass at max_it - total iterations; the number of
-    // host checks is capped independently
-    const int max_checks = (max_it - total) / (2 * pairs_per_check) + 2;
-    for (int c = 0; !done && err == hipSuccess && c < max_checks; c++) {
-      chk(chunk(pairs_per_check), err);
+    const double rr0 = irh->rr;  // the pass starts from the true residual
+    int done = 0, pairs = pairs_first, run = 0;
+    // the device caps the pass at max_it - total iterations; the host replays
+    // at most that many iterations plus one check's worth
+    while (!done && err == hipSuccess && run < max_it - total + 2 * pairs_first) {
+      chk(chunk(pairs), err);
+      run += 2 * pairs;
       chk(hipMemcpyAsync(sc_host_, sc_, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), err);
       chk(hipStreamSynchronize(s), err);
       done = sc_host_->done;
+      // next check where the observed contraction rate puts the end of the
+      // pass: few host round trips, few replays past the stop (which return
+      // at once but still cost their launches)
+      const int it = sc_host_->iters;
+      const double rr = sc_host_->rr, stop2 = sc_host_->stop2;
+      pairs = pairs_first;
+      if (!done && it > 0 && rr > stop2 && rr < rr0) {
+        const double rho = pow(rr / rr0, 1.0 / it);
+        const double m = ceil(log(stop2 / rr) / log(rho));
+        pairs = (int)std::min(8.0, std::max(1.0, ceil(0.5 * m)));
+      }
     }
     const int it = sc_host_->iters;
     total += it;
