@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--levels", type=int, default=0, help="MG levels (0 = auto hierarchy)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
+    ap.add_argument("--coarse-sweeps", type=int, default=0,
+                    help="Chebyshev steps on the coarsest level (0 = solver default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
@@ -259,7 +261,7 @@ def main():
     H = torch.zeros_like(phi)
     code = torch.zeros_like(vt)
     solver = D.Solver(vt, kp, mg_levels=args.levels, precision=args.precision, tol=1e-6,
-                      max_iters=2000)
+                      max_iters=2000, coarse_sweeps=args.coarse_sweeps)
 
     def step():
         x64.zero_()

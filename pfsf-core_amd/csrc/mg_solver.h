@@ -24,6 +24,11 @@ struct IrScalars {
 // Stencil tiles: 32 x 8 x tz voxels per 256-thread work-group.
 constexpr int kTX = 32, kTY = 8;
 
+// Chebyshev steps of the coarsest-level solve.  Measured on the 128^3 island:
+// 6 steps give the same PCG iteration count as 12 (52) at half the coarse
+// cost; 2-5 and 7-10 steps need 53-63 iterations.
+constexpr int kCoarseSweeps = 6;
+
 struct TileList {
   int tz = 8;                     // tile depth (8, 4 or 2) chosen per level
   int ntx = 0, nty = 0, ntz = 0;  // tiles per axis

@@ -527,7 +527,7 @@ void MgSolver::configure(const pfsf_solver_desc& desc, int min_levels, int tz_fi
   type_ = desc.type;
   kappa_ = desc.kappa;
   bf16_ = desc.precision == PFSF_PREC_BF16_COARSE;
-  coarse_sweeps_ = desc.coarse_sweeps > 0 ? desc.coarse_sweeps : 12;
+  coarse_sweeps_ = desc.coarse_sweeps > 0 ? desc.coarse_sweeps : kCoarseSweeps;
   int want = desc.mg_levels;
   if (want <= 0) {
     want = 1;
