@@ -27,12 +27,12 @@ namespace {
 
 constexpr int kTZFine = 4;    // tile depth of every island's finest level
 constexpr int kTZCoarse = 2;  // tile depth of the coarse levels
-constexpr int kBatchCoarseSweeps = 12;
 constexpr int kChunk = 1024;  // cells per work-group of the per-cell passes
 // Coarsest-level Chebyshev steps in a batch: a batched sweep costs ~5 ms of
 // fine-level work against ~50 us for the coarse solve, so the steps that
 // cut the iteration count of the slowest island pay (12 steps: 35 iterations
 // at most on the 4,096-island batch; 6 steps: 37).
+constexpr int kBatchCoarseSweeps = 12;
 
 __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
   Lv<float> L;
