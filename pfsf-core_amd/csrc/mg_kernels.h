@@ -683,5 +683,87 @@ __device__ __forceinline__ double residual_f64(pfsf_dims d, const uint8_t* __res
   return (double)b[i] - ax;
 }
 
+// True residual over the tiles of the finest level, FP64 and exact: the tile
+// stages the iterate (0 off the free set) and the coupling kappa (-1 on air)
+// with a one-voxel halo, then each thread forms b - A x for its column from
+// LDS.  Cells of tiles without free voxels are never written and stay 0.
+template <int TZ>
+struct TileR {
+  static constexpr int HN = Tile<TZ>::HN;
+  double x[HN];
+  float k[HN];
+};
+
+template <int TZ>
+__device__ __forceinline__ void true_residual_tile(
+    pfsf_dims d, bool have, int x0, int y0, int z0, const uint8_t* __restrict__ type,
+    const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
+    const float* __restrict__ b, const double* __restrict__ x, float* __restrict__ r32,
+    double* partials, unsigned pslot) {
+  __shared__ TileR<TZ> T;
+  __shared__ double sh[1][BS / 64];
+  double acc = 0.0;
+  if (have) {
+    constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+    const int t = threadIdx.x;
+    auto cell = [&](int e, int gx, int gy, int gz, bool ok) {
+      const int j = ok ? flat32(gx, gy, gz, d) : 0;
+      const uint8_t tj = type[j];
+      const float kj = kappa[j];
+      const uint8_t fj = fr[j];
+      const double xj = x[j];
+      T.k[e] = (ok && tj != 0) ? kapf(kj) : -1.f;
+      T.x[e] = (ok && fj) ? xj : 0.0;
+    };
+    const int tx = t % TX, tr = t / TX;
+    const int gx = x0 + tx;
+#pragma unroll
+    for (int p = 0; p < NP - 1; p++) {
+      const int row = tr + p * RPP;
+      const int ey = row % HY, ez = row / HY;
+      const int gy = y0 + ey - 1, gz = z0 + ez - 1;
+      if (p < NP - 2 || row < NROWS)
+        cell(row * HX + tx + 1, gx, gy, gz,
+             gx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
+    }
+    const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
+    if (hrow < NROWS) {
+      const int ey = hrow % HY, ez = hrow / HY;
+      const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
+      cell(hrow * HX + ex, hx, gy, gz,
+           hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
+    }
+    __syncthreads();
+    const int ty = t / TX, gy = y0 + ty, sz = d.lx * d.ly;
+    const bool col = gx < d.lx && gy < d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    for (int tz = 0; tz < TZ; tz++) {
+      if (!col || z0 + tz >= d.lz) break;
+      const int i = i0 + tz * sz;
+      if (!fr[i]) continue;
+      const int c = tcell(tx, ty, tz);
+      const float ki = T.k[c];
+      const double xi = T.x[c];
+      double ax = 0.0;
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
+            if (cls == 0) continue;
+            const int o = dx + dy * HX + dz * HXY;
+            const float kj = T.k[c + o];
+            if (kj >= 0.f) ax += couple_f64(cls, ki, kj) * (xi - T.x[c + o]);
+          }
+      const double ri = (double)b[i] - ax;
+      r32[i] = (float)ri;
+      acc += ri * ri;
+    }
+  }
+  write_partial_at(acc, partials, sh, pslot);
+}
+
 }  // namespace
 }  // namespace pfsf

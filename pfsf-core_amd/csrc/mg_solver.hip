@@ -366,21 +366,16 @@ __global__ void __launch_bounds__(BS) k_fill(int64_t n, float* __restrict__ v, f
 }
 
 // True residual of the FP64 iterate: r = b - A x with the exact coupling rule
-// (FP64), written as r32 = (float) r, plus partial ||r||^2.
+// (FP64), written as r32 = (float) r, plus partial ||r||^2 per tile.
+template <int TZ>
 __global__ void __launch_bounds__(BS)
-    k_true_residual(pfsf_dims d, const uint8_t* __restrict__ type,
-                    const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
-                    const float* __restrict__ b, const double* __restrict__ x,
-                    float* __restrict__ r32, double* partials) {
-  __shared__ double sh[1][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double acc[1] = {0.0};
-  if (i < (int64_t)d.lx * d.ly * d.lz) {
-    const double ri = residual_f64(d, type, kappa, fr, b, x, (int)i);
-    r32[i] = (float)ri;
-    acc[0] = ri * ri;
-  }
-  write_partials<1>(acc, partials, sh);
+    k_true_residual_tiles(pfsf_dims d, TL M, const uint8_t* __restrict__ type,
+                          const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
+                          const float* __restrict__ b, const double* __restrict__ x,
+                          float* __restrict__ r32, double* partials) {
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  true_residual_tile<TZ>(d, have, x0, y0, z0, type, kappa, fr, b, x, r32, partials, blockIdx.x);
 }
 
 // Iterate outside the free set is zero (the caller's guess may not be).
@@ -964,9 +959,10 @@ hipError_t MgSolver::chunk(int pairs) {
 hipError_t MgSolver::true_residual(const float* b, double* rel) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
-  unsigned nb0 = blocks_for(n0_, BS);
-  k_true_residual<<<nb0, BS, 0, s>>>(d0_, type_, kappa_, levels_[0].fr, b, x_, r_[0], partials_);
-  k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, nb0, sc_, ir_);
+  const MgLevel& F = levels_[0];
+  PFSF_TZ_DISPATCH(F, (k_true_residual_tiles<TZV><<<F.tiles.nlaunch, BS, 0, s>>>(
+                          d0_, tl(F), type_, kappa_, F.fr, b, x_, r_[0], partials_)));
+  k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, F.tiles.nlaunch, sc_, ir_);
   IrScalars* irh = (IrScalars*)(sc_host_ + 1);
   chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
   chk(hipStreamSynchronize(s), err);
