@@ -41,7 +41,7 @@ struct BatchIsland {
   int cb, cn;         // the island's cell chunks [cb, cb + cn)
 };
 
-// A run of cells of one island (the per-cell FP64 passes of a solve).
+// A run of cells of one island (the per-cell set-up and copy-out passes).
 struct BatchChunk {
   int island;
   int begin, end;
@@ -100,6 +100,9 @@ class BatchSolver {
   BatchChunk* chunks_ = nullptr;
   int nchunks_ = 0;
   double* pcell_ = nullptr;        // per-chunk partial sums
+  double* pres_ = nullptr;         // per-tile partial sums of the FP64 residual
+  BatchTile* all0_ = nullptr;      // every island's finest tiles, XCD order
+  int nall0_ = 0;
   int* flags_ = nullptr;           // device: per island 1 while its PCG pass runs
   std::vector<int> flags_host_;
   int nslots_ = 0;

@@ -141,22 +141,20 @@ __global__ void __launch_bounds__(BS)
   }
 }
 
-// True FP64 residual of the unfinished islands into their r[0], with
-// ||r||^2 partials.
+// True FP64 residual of the unfinished islands into their r[0], one finest
+// tile per work-group (every island's tiles, in XCD order), with the ||r||^2
+// partial in the tile's slot.
 __global__ void __launch_bounds__(BS)
-    kb_residual_part(const BatchChunk* __restrict__ ch, const BatchIsland* __restrict__ isl,
-                     const BatchState* __restrict__ st, double* part) {
-  __shared__ double sh[1][BS / 64];
-  const BatchChunk c = ch[blockIdx.x];
-  if (st[c.island].finished) return;
-  const BatchIsland& I = isl[c.island];
-  double acc = 0.0;
-  for (int i = c.begin + (int)threadIdx.x; i < c.end; i += BS) {
-    const double ri = residual_f64(I.d, I.type, I.kappa, I.fr, I.b, I.x, i);
-    I.r0[i] = (float)ri;
-    acc += ri * ri;
-  }
-  write_partial_at(acc, part, sh, blockIdx.x);
+    kb_residual_tiles(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+                      const BatchIsland* __restrict__ isl, const BatchState* __restrict__ st,
+                      double* part) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  if (st[D.island].finished) return;
+  const BatchIsland& I = isl[D.island];
+  true_residual_tile<kTZFine>(I.d, true, t.x0, t.y0, t.z0, I.type, I.kappa, I.fr, I.b, I.x,
+                              I.r0, part, (unsigned)t.slot);
 }
 
 // Pass boundary, per island: an island at the tolerance, out of iterations
@@ -170,7 +168,7 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[k];
   BatchState& S = st[k];
   if (S.finished) return;
-  const double rr = block_total(part + I.cb, (unsigned)I.cn);
+  const double rr = block_total(part + I.tb, (unsigned)I.tn);
   if (threadIdx.x != 0) return;
   PcgScalars& P = sc[k];
   const int pass_iters = P.iters;
@@ -403,6 +401,14 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
       tiles_[l] = balloc<BatchTile>(all_tiles_[l].size() + 8, owned_, err);
   chunks_ = balloc<BatchChunk>(nchunks_, owned_, err);
   pcell_ = balloc<double>(nchunks_, owned_, err);
+  pres_ = balloc<double>(nslots_, owned_, err);
+  {
+    const std::vector<BatchTile> o = xcd_order(all_tiles_[0]);
+    nall0_ = (int)o.size();
+    all0_ = balloc<BatchTile>(o.size(), owned_, err);
+    if (err == hipSuccess)
+      chk(hipMemcpy(all0_, o.data(), o.size() * sizeof(BatchTile), hipMemcpyHostToDevice), err);
+  }
   flags_ = balloc<int>(n, owned_, err);
   sc_ = balloc<PcgScalars>(n, owned_, err);
   st_ = balloc<BatchState>(n, owned_, err);
@@ -546,8 +552,8 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
   const int pairs_per_check = 4;
   for (int pass = 0; pass < 18 && err == hipSuccess; pass++) {
     chk(hipMemsetAsync(live_, 0, sizeof(int), s), err);
-    kb_residual_part<<<nchunks_, BS, 0, s>>>(chunks_, isl_d_, st_, pcell_);
-    kb_residual_fin<<<n, BS, 0, s>>>(isl_d_, pcell_, st_, sc_, tol, max_it, live_);
+    kb_residual_tiles<<<nall0_, BS, 0, s>>>(all0_, levels_, isl_d_, st_, pres_);
+    kb_residual_fin<<<n, BS, 0, s>>>(isl_d_, pres_, st_, sc_, tol, max_it, live_);
     kb_live<<<1, 1024, 0, s>>>(sc_, n, flags_, live_ + 1);
     chk(hipMemcpyAsync(flags_host_.data(), flags_, n * sizeof(int), hipMemcpyDeviceToHost, s),
         err);

@@ -651,38 +651,6 @@ __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
 }
 
-// (b - A x)_i with the exact coupling rule in FP64 (0 outside the free set;
-// x outside the free set counts as 0).  32-bit cell index (grids below 2^29).
-__device__ __forceinline__ double residual_f64(pfsf_dims d, const uint8_t* __restrict__ type,
-                                               const float* __restrict__ kappa,
-                                               const uint8_t* __restrict__ fr,
-                                               const float* __restrict__ b,
-                                               const double* __restrict__ x, int i) {
-  if (!fr[i]) return 0.0;
-  const double xi = x[i];
-  const int px = i % d.lx, r = i / d.lx;
-  const int py = r % d.ly, pz = r / d.ly;
-  const int sz = d.lx * d.ly;
-  const float ki = kapf(kappa[i]);
-  double ax = 0.0;
-#pragma unroll
-  for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-      for (int dx = -1; dx <= 1; dx++) {
-        const int cls = (dx != 0) + (dy != 0) + (dz != 0);
-        if (cls == 0) continue;
-        const int nx = px + dx, ny = py + dy, nz = pz + dz;
-        if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
-        const int j = i + dx + dy * d.lx + dz * sz;
-        if (type[j] == 0) continue;
-        const double xj = fr[j] ? x[j] : 0.0;
-        ax += couple_f64(cls, ki, kapf(kappa[j])) * (xi - xj);
-      }
-  return (double)b[i] - ax;
-}
-
 // True residual over the tiles of the finest level, FP64 and exact: the tile
 // stages the iterate (0 off the free set) and the coupling kappa (-1 on air)
 // with a one-voxel halo, then each thread forms b - A x for its column from
