@@ -69,11 +69,17 @@ class BatchSolver {
  public:
   ~BatchSolver();
   hipError_t create(const pfsf_batch_desc& desc);
+  // A batch over solvers owned by the caller (each built with two or more
+  // levels and the batch tile depths, see MgSolver::create(desc, true)); the
+  // solvers must outlive the batch, and their vectors are the batch's.
+  hipError_t attach(const std::vector<MgSolver*>& solvers, double tol, int max_iters,
+                    hipStream_t caller);
   hipError_t solve(const float* const* b, double* const* x64, float* const* phi32,
                    pfsf_batch_info* info, pfsf_solve_info* per_island);
-  int islands() const { return (int)isl_.size(); }
+  int islands() const { return (int)sol_.size(); }
 
  private:
+  hipError_t build_tables();
   hipError_t iteration(int c);
   hipError_t capture();
   hipError_t select_islands(const int* live);  // tile lists of the live islands only
@@ -83,7 +89,8 @@ class BatchSolver {
 
   pfsf_batch_desc desc_{};
   hipStream_t stream_ = nullptr;
-  std::vector<std::unique_ptr<MgSolver>> isl_;
+  std::vector<std::unique_ptr<MgSolver>> isl_;  // owned hierarchies (create)
+  std::vector<MgSolver*> sol_;                   // the islands, owned or attached
   std::vector<BatchIsland> host_isl_;
   std::vector<int> nlev_;
   int lmax_ = 0;

@@ -25,8 +25,8 @@
 namespace pfsf {
 namespace {
 
-constexpr int kTZFine = 4;    // tile depth of every island's finest level
-constexpr int kTZCoarse = 2;  // tile depth of the coarse levels
+constexpr int kTZFine = kBatchTZFine;      // tile depth of every island's finest level
+constexpr int kTZCoarse = kBatchTZCoarse;  // tile depth of the coarse levels
 constexpr int kChunk = 1024;  // cells per work-group of the per-cell passes
 // Coarsest-level Chebyshev steps in a batch: a batched sweep costs ~5 ms of
 // fine-level work against ~50 us for the coarse solve, so the steps that
@@ -245,6 +245,7 @@ void BatchSolver::release() {
   graph_exec_ = nullptr;
   graph_ = nullptr;
   if (stream_) (void)hipStreamSynchronize(stream_);
+  sol_.clear();
   isl_.clear();
   for (void* p : owned_) (void)hipFree(p);
   owned_.clear();
@@ -318,18 +319,55 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
   for (auto& s : isl_) s->allocate_pooled(pool);
   for (int i = 0; i < n && err == hipSuccess; i++) chk(isl_[i]->build_pooled(stream_), err);
   if (err != hipSuccess) return err;
+  sol_.resize(n);
+  for (int i = 0; i < n; i++) sol_[i] = isl_[i].get();
+  return build_tables();
+}
+
+hipError_t BatchSolver::attach(const std::vector<MgSolver*>& solvers, double tol, int max_iters,
+                               hipStream_t caller) {
+  hipError_t err = hipSuccess;
+  desc_ = pfsf_batch_desc{};
+  desc_.count = (int32_t)solvers.size();
+  desc_.tol = tol;
+  desc_.max_iters = max_iters;
+  desc_.stream = caller;
+  if (solvers.empty()) return hipErrorInvalidValue;
+  coarse_sweeps_ = std::min(solvers[0]->coarse_sweeps(), 63);
+  for (MgSolver* m : solvers) {
+    if (m->levels() < 2 || m->coarse_sweeps() != solvers[0]->coarse_sweeps() ||
+        m->level(0).tiles.tz != kTZFine)
+      return hipErrorInvalidValue;
+    for (int l = 1; l < m->levels(); l++)
+      if (m->level(l).tiles.tz != kTZCoarse) return hipErrorInvalidValue;
+    const MgLevel& C = m->level(m->levels() - 1);
+    if (C.n > kMaxLdsCoarse || padded_cells(C.d) > kMaxLdsPadded) return hipErrorInvalidValue;
+  }
+  if (chk(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), err)) return err;
+  chk(hipEventCreate(&ev0_), err);
+  chk(hipEventCreate(&ev1_), err);
+  if (err != hipSuccess) return err;
+  sol_ = solvers;
+  return build_tables();
+}
+
+// Descriptor tables, tile lists, chunks and scalars over sol_.
+hipError_t BatchSolver::build_tables() {
+  hipError_t err = hipSuccess;
+  const int n = (int)sol_.size();
+  nlev_.assign(n, 0);
 
   // descriptor tables and tile lists
   std::vector<BatchLevel> lv;
   host_isl_.assign(n, BatchIsland{});
   lmax_ = 0;
-  for (int i = 0; i < n; i++) lmax_ = std::max(lmax_, isl_[i]->levels());
+  for (int i = 0; i < n; i++) lmax_ = std::max(lmax_, sol_[i]->levels());
   all_tiles_.assign(lmax_, {});
   tile_begin_.assign(lmax_, std::vector<int>(n + 1, 0));
   std::vector<BatchChunk> chunks;
   int slot = 0;
   for (int i = 0; i < n; i++) {
-    MgSolver& S = *isl_[i];
+    MgSolver& S = *sol_[i];
     const int L = S.levels();
     nlev_[i] = L;
     const int base = (int)lv.size();
@@ -428,7 +466,7 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
                      hipMemcpyHostToDevice, s),
       err);
   // every island uses the same coarse-solve polynomial (same bound, steps)
-  chk(hipMemcpyAsync(coef_, isl_[0]->chebyshev_coefficients(), 2 * 64 * sizeof(float),
+  chk(hipMemcpyAsync(coef_, sol_[0]->chebyshev_coefficients(), 2 * 64 * sizeof(float),
                      hipMemcpyHostToDevice, s),
       err);
   chk(hipMemsetAsync(sc_, 0, n * sizeof(PcgScalars), s), err);
@@ -444,7 +482,7 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
 // XCD-grouped order; the graph is re-captured at its next use.
 hipError_t BatchSolver::select_islands(const int* live) {
   hipError_t err = hipSuccess;
-  const int n = (int)isl_.size();
+  const int n = (int)sol_.size();
   listed_.assign(live, live + n);
   listed_tiles_ = 0;
   std::vector<std::vector<BatchTile>> order(lmax_);  // alive until the copies are done
@@ -474,7 +512,7 @@ hipError_t BatchSolver::select_islands(const int* live) {
 hipError_t BatchSolver::capture() {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
-  const int n = (int)isl_.size();
+  const int n = (int)sol_.size();
   if (chk(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), err)) return err;
   (void)iteration(0);
   (void)iteration(1);
@@ -491,7 +529,7 @@ hipError_t BatchSolver::capture() {
 // Coarse part of every island's V-cycle below its finest level.
 hipError_t BatchSolver::vcycle_coarse(bool check) {
   hipStream_t s = stream_;
-  const int n = (int)isl_.size();
+  const int n = (int)sol_.size();
   for (int l = 1; l + 1 < lmax_; l++)
     if (ntiles_[l] > 0)
       kb_down<kTZCoarse, false><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_,
@@ -533,7 +571,7 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   hipStream_t caller = (hipStream_t)desc_.stream;
-  const int n = (int)isl_.size();
+  const int n = (int)sol_.size();
   const double tol = desc_.tol > 0 ? desc_.tol : 1e-6;
   const int max_it = desc_.max_iters > 0 ? desc_.max_iters : 1000;
   for (int i = 0; i < n; i++) {

@@ -10,6 +10,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <memory>
@@ -195,28 +196,49 @@ bool upload(pfsf_engine_t* E, Island& is) {
   return hipGetLastError() == hipSuccess;
 }
 
+// The island's solver, created or rebuilt when its voxels changed.  With
+// multigrid on, solvers are batchable so that several dirty islands can be
+// solved in one batched solve.
+pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
+  if (is.solver && !is.topo_dirty) return PFSF_OK;
+  if (!is.solver) {
+    pfsf_solver_desc d{};
+    d.dims = is.dims;
+    d.type = is.type;
+    d.kappa = is.kappa;
+    d.mg_levels = E->cfg.enable_multigrid ? 0 : 1;
+    d.precision = PFSF_PREC_FP32;
+    d.tol = E->tol;
+    d.max_iters = E->max_iters;
+    d.stream = E->stream;
+    auto sv = std::make_unique<MgSolver>();
+    if (sv->create(d, E->cfg.enable_multigrid != 0) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
+    is.solver = std::move(sv);
+  } else if (is.solver->update(is.type, is.kappa) != hipSuccess) {
+    return PFSF_ERROR_VULKAN;
+  }
+  is.topo_dirty = false;
+  return PFSF_OK;
+}
+
+// History and failure scan of a solved island.
+pfsf_result post_solve(pfsf_engine_t* E, Island& is) {
+  hipStream_t s = E->stream;
+  if (E->cfg.enable_phase_field &&
+      launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens, kPhiOrphan,
+                          is.code, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  is.have_phi = true;
+  return PFSF_OK;
+}
+
 pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
   hipStream_t s = E->stream;
   if (E->pcg) {
-    if (!is.solver || is.topo_dirty) {
-      if (!is.solver) {
-        pfsf_solver_desc d{};
-        d.dims = is.dims;
-        d.type = is.type;
-        d.kappa = is.kappa;
-        d.mg_levels = E->cfg.enable_multigrid ? 0 : 1;
-        d.precision = PFSF_PREC_FP32;
-        d.tol = E->tol;
-        d.max_iters = E->max_iters;
-        d.stream = s;
-        auto sv = std::make_unique<MgSolver>();
-        if (sv->create(d) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
-        is.solver = std::move(sv);
-      } else if (is.solver->update(is.type, is.kappa) != hipSuccess) {
-        return PFSF_ERROR_VULKAN;
-      }
-      is.topo_dirty = false;
-    }
+    pfsf_result r = ensure_solver(E, is);
+    if (r != PFSF_OK) return r;
     pfsf_solve_info info{};
     if (is.solver->solve(is.source, is.x64, is.phi, &info) != hipSuccess) return PFSF_ERROR_VULKAN;
   } else {
@@ -225,13 +247,32 @@ pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
       return PFSF_ERROR_VULKAN;
     k_widen<<<(unsigned)((is.n + 255) / 256), 256, 0, s>>>(is.n, is.phi, is.x64);
   }
-  if (E->cfg.enable_phase_field &&
-      launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s) != hipSuccess)
+  return post_solve(E, is);
+}
+
+// Several dirty islands in one batched solve (same numerics per island).
+pfsf_result solve_batch(pfsf_engine_t* E, const std::vector<Island*>& todo) {
+  std::vector<MgSolver*> sv;
+  std::vector<const float*> b;
+  std::vector<double*> x;
+  std::vector<float*> phi;
+  for (Island* is : todo) {
+    pfsf_result r = ensure_solver(E, *is);
+    if (r != PFSF_OK) return r;
+    sv.push_back(is->solver.get());
+    b.push_back(is->source);
+    x.push_back(is->x64);
+    phi.push_back(is->phi);
+  }
+  BatchSolver batch;
+  if (batch.attach(sv, E->tol, E->max_iters, E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
+  pfsf_batch_info info{};
+  if (batch.solve(b.data(), x.data(), phi.data(), &info, nullptr) != hipSuccess)
     return PFSF_ERROR_VULKAN;
-  if (launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens, kPhiOrphan,
-                          is.code, s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
-  is.have_phi = true;
+  for (Island* is : todo) {
+    pfsf_result r = post_solve(E, *is);
+    if (r != PFSF_OK) return r;
+  }
   return PFSF_OK;
 }
 
@@ -259,44 +300,68 @@ pfsf_result drain_failures(pfsf_engine_t* E, Island& is, int32_t* header, int32_
   return PFSF_OK;
 }
 
+// Failure records and stress readback of a solved island.
+pfsf_result report(pfsf_engine_t* E, Island& is, pfsf_tick_result* result, void* fail_addr,
+                   int64_t fail_bytes) {
+  if (fail_addr && fail_bytes >= 20) {
+    int32_t cap = (int32_t)((fail_bytes - 4) / 16);
+    int32_t* hdr = (int32_t*)fail_addr;
+    pfsf_result r = drain_failures(E, is, hdr, hdr + 1, cap);
+    if (r != PFSF_OK) return r;
+  }
+  if (result && result->events && result->count < result->capacity) {
+    int32_t hdr = result->count;
+    pfsf_result r = drain_failures(E, is, &hdr, (int32_t*)result->events, result->capacity);
+    if (r != PFSF_OK) return r;
+    result->count = hdr;
+  }
+  if (is.host.stress && is.host.stress_bytes > 0) {
+    int64_t cnt = is.host.stress_bytes / 4;
+    if (cnt > is.n) cnt = is.n;
+    if (hipMemcpyAsync(is.host.stress, is.phi, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(E->stream) != hipSuccess)
+      return PFSF_ERROR_VULKAN;
+  }
+  is.dirty = false;
+  return PFSF_OK;
+}
+
+// The reference walks the dirty islands one at a time, uploading and solving
+// each within the tick budget (NP/src/pfsf_engine.cpp:424-600).  Here the
+// islands that fit the budget are uploaded first; with PCG and multigrid on,
+// two or more of them are then solved in one batched solve, else one by one;
+// each then gets its history update, failure scan and reports.
 pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count,
                       pfsf_tick_result* result, void* fail_addr, int64_t fail_bytes) {
   if (!E->available) return PFSF_ERROR_NOT_INIT;
   if (count > 0 && !ids) return PFSF_ERROR_INVALID_ARG;
   auto t0 = std::chrono::steady_clock::now();
   if (result) result->count = 0;
-  const bool drain = fail_addr && fail_bytes >= 20;
+  std::vector<Island*> todo;
   for (int32_t k = 0; k < count; k++) {
     Island* is = find(E, ids[k]);
     if (!is || !is->dirty) continue;
-    if (is->host.registered && !upload(E, *is)) return PFSF_ERROR_VULKAN;
+    if (std::find(todo.begin(), todo.end(), is) != todo.end()) continue;
     auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
                   std::chrono::steady_clock::now() - t0)
                   .count();
-    if (ms >= E->cfg.tick_budget_ms) break;  // island stays dirty
-    pfsf_result r = solve_island(E, *is);
+    if (ms >= E->cfg.tick_budget_ms) break;  // the rest stay dirty
+    if (is->host.registered && !upload(E, *is)) return PFSF_ERROR_VULKAN;
+    todo.push_back(is);
+  }
+  if (E->pcg && E->cfg.enable_multigrid && todo.size() >= 2) {
+    pfsf_result r = solve_batch(E, todo);
     if (r != PFSF_OK) return r;
-    if (drain) {
-      int32_t cap = (int32_t)((fail_bytes - 4) / 16);
-      int32_t* hdr = (int32_t*)fail_addr;
-      r = drain_failures(E, *is, hdr, hdr + 1, cap);
+  } else {
+    for (Island* is : todo) {
+      pfsf_result r = solve_island(E, *is);
       if (r != PFSF_OK) return r;
     }
-    if (result && result->events && result->count < result->capacity) {
-      int32_t hdr = result->count;
-      r = drain_failures(E, *is, &hdr, (int32_t*)result->events, result->capacity);
-      if (r != PFSF_OK) return r;
-      result->count = hdr;
-    }
-    if (is->host.stress && is->host.stress_bytes > 0) {
-      int64_t cnt = is->host.stress_bytes / 4;
-      if (cnt > is->n) cnt = is->n;
-      if (hipMemcpyAsync(is->host.stress, is->phi, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
-              hipSuccess ||
-          hipStreamSynchronize(E->stream) != hipSuccess)
-        return PFSF_ERROR_VULKAN;
-    }
-    is->dirty = false;
+  }
+  for (Island* is : todo) {
+    pfsf_result r = report(E, *is, result, fail_addr, fail_bytes);
+    if (r != PFSF_OK) return r;
   }
   if (hipStreamSynchronize(E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
   std::lock_guard<std::mutex> lk(E->stats_mtx);

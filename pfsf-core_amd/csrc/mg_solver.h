@@ -29,6 +29,9 @@ constexpr int kTX = 32, kTY = 8;
 // cost; 2-5 and 7-10 steps need 53-63 iterations.
 constexpr int kCoarseSweeps = 6;
 
+// Tile depths of a batchable hierarchy (finest level, coarse levels).
+constexpr int kBatchTZFine = 4, kBatchTZCoarse = 2;
+
 struct TileList {
   int tz = 8;                     // tile depth (8, 4 or 2) chosen per level
   int ntx = 0, nty = 0, ntz = 0;  // tiles per axis
@@ -71,7 +74,9 @@ struct MgLevel {
 class MgSolver {
  public:
   ~MgSolver();
-  hipError_t create(const pfsf_solver_desc& desc);
+  // batchable: at least two levels and the batch solver's tile depths (4 on
+  // the finest level, 2 below), so the solver can join a BatchSolver::attach.
+  hipError_t create(const pfsf_solver_desc& desc, bool batchable = false);
   // Pooled hierarchy for the batch solver: shapes first (host), then arrays
   // carved from `pool` (a measuring pass first), then the set-up kernels on
   // the batch's stream.  solve() and profile() are not available.

@@ -624,10 +624,13 @@ void MgSolver::zero_vectors() {
   }
 }
 
-hipError_t MgSolver::create(const pfsf_solver_desc& desc) {
+hipError_t MgSolver::create(const pfsf_solver_desc& desc, bool batchable) {
   hipError_t err = hipSuccess;
   if (nvox(desc.dims) > kMaxSolverCells) return hipErrorInvalidValue;
-  configure(desc, 1, 0, 0);
+  if (batchable)
+    configure(desc, 2, kBatchTZFine, kBatchTZCoarse);
+  else
+    configure(desc, 1, 0, 0);
   // The solver runs on a stream of its own (default priority) so that its
   // iterations can be captured into a HIP graph; solve() orders it after the
   // caller's stream and waits for it before returning.
