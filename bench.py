@@ -214,6 +214,39 @@ def adjoint_bench(torch, W, steps=2):
     return out
 
 
+def host_tick_bench(isl, steps=3):
+    """The same 128^3 island through the host-buffer C ABI, as the Java runtime
+    drives it: pfsf_register_island_buffers + pfsf_tick_dbb per tick (PCIe
+    upload of every registered array, hierarchy rebuild, solve, history, scan,
+    failure compaction, phi readback).  Cold: phi zeroed before each tick;
+    warm: the previous tick's phi is the start (an unchanged island)."""
+    import numpy as np
+
+    from pfsf_amd import capi
+    from pfsf_amd.engine import Engine
+
+    out = {}
+    with Engine(max_island_size=isl.n, tick_budget_ms=1 << 30) as e:
+        if e.init() != capi.OK or e.add_island(1, isl.lx, isl.ly, isl.lz) != capi.OK:
+            return {"error": "engine init"}
+        e.register(1, isl, kappa=isl.kappa)
+        e.tick_dbb([1], capacity=1 << 16)  # warm-up
+        for mode in ("cold", "warm"):
+            ts = []
+            for _ in range(steps):
+                if mode == "cold":
+                    e.phi(1)[...] = 0
+                e.lib.pfsf_mark_full_rebuild(e.h, 1)
+                t0 = time.perf_counter()
+                r, _ = e.tick_dbb([1], capacity=1 << 16)
+                ts.append(time.perf_counter() - t0)
+            out[f"{mode}_tick_ms"] = round(float(np.median(ts)) * 1e3, 3)
+    out["note"] = ("pfsf_tick_dbb on host buffers: PCIe traffic included, so this is not the "
+                   "bench value")
+    log(f"host-buffer tick 128^3: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +260,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint record")
+    ap.add_argument("--no-host-tick", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) tick record")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 4096-island batch and print its record")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -332,6 +367,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(isl)
     solver.close()
+    if not args.no_host_tick:
+        out["host_buffer_tick_128"] = host_tick_bench(isl)
     if not args.no_batch:
         out["batched"] = batch_bench(torch, D, W)
     if not args.no_256:
