@@ -244,9 +244,34 @@ PFSF_API pfsf_result pfsf_load_path_grad(const pfsf_load_path_desc* desc, double
                                          pfsf_load_path_info* info);
 
 /* Orphan mask used by the solver: 1 for solid voxels 26-connected to an
- * anchor through non-air voxels, else 0 (device, N bytes).  Synchronous. */
+ * anchor through non-air voxels, else 0 (device, N bytes).  Synchronous.
+ * N <= 2^30.  Computed from the same union-find components as
+ * pfsf_hip_label_components, over non-air voxels. */
 PFSF_API pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach,
                                             void* stream);
+
+/* Connected components of the live voxels (type 1 solid or 2 anchor), with
+ * connectivity 26 (PFSFStencil offsets) or 6 (faces).  Replaces the
+ * label-propagation pipeline label_prop_{init,iterate,summarise_alloc,
+ * summarise_aggregate}.comp.glsl (PFSFLabelPropRecorder.java:75-190) and its
+ * CPU oracle LabelPropagation.shiloachVishkin (LabelPropagation.java:199-270):
+ *   island_id (device, N uint32): 1 + the smallest flat index of the voxel's
+ *     component; 0xFFFFFFFF for every other type.  This is the fixed point of
+ *     the reference's hook/jump iteration, reached here in one union-find pass
+ *     whatever the component diameter (the reference stops after 12 rounds).
+ *   components (device, max_components x 12 uint32): one record per root in
+ *     ascending root order, {label, voxel count, anchored 0/1, 0, min x,
+ *     min y, min z, 0, max x, max y, max z, 0}; slots past the last record
+ *     are zero (PFSFLabelPropCpuSimulator.java:164-238 layout).
+ *   *num_components = records written (<= max_components); *overflow = 1 when
+ *     the grid has more components than max_components.
+ * 1 <= max_components <= 1024 (the reference uses 64); N <= 2^30.
+ * Synchronous (the counts are returned to host memory). */
+PFSF_API pfsf_result pfsf_hip_label_components(pfsf_dims d, const uint8_t* type,
+                                               int32_t connectivity, uint32_t* island_id,
+                                               uint32_t* components, int32_t max_components,
+                                               int32_t* num_components, int32_t* overflow,
+                                               void* stream);
 
 /* ── Engine extension: exact per-voxel kappa ────────────────────────────
  * By default a registered island's kappa is the per-voxel maximum of its

@@ -75,6 +75,12 @@ def lib():
         L.or_flux_loss.restype = C.c_double
         L.or_flux_grad_phi.argtypes = [I, I, I, P, P, P, P, P, P]
         L.or_flux_grad_kappa.argtypes = [I, I, I, P, P, P, P, P, P]
+        L.or_label_sv.argtypes = [I, I, I, P, I, P]
+        L.or_label_sv.restype = I
+        L.or_label_bfs.argtypes = [I, I, I, P, I, P]
+        L.or_label_bfs.restype = C.c_int64
+        L.or_label_summarise.argtypes = [I, I, I, P, P, I, P, C.POINTER(I)]
+        L.or_label_summarise.restype = I
         _lib = L
     return _lib
 
@@ -340,3 +346,43 @@ def energy(isl_all_solid_kappa, phi, lx, ly, lz):
 
 
 __all__ = [n for n in dir() if not n.startswith("_")]
+
+
+# ------------------------------------------------------ connected components
+
+LABEL_NONE = np.uint32(0xFFFFFFFF)
+
+
+def _grid_dims(vtype):
+    lz, ly, lx = vtype.shape
+    return lx, ly, lz
+
+
+def label_sv(vtype, conn=26):
+    """LabelPropagation.shiloachVishkin to its fixed point (uint32 labels,
+    air = 0xFFFFFFFF).  Returns (labels, rounds)."""
+    t = np.ascontiguousarray(vtype, np.uint8)
+    ids = np.empty(t.shape, np.uint32)
+    rounds = lib().or_label_sv(*_grid_dims(t), _p(t), conn, _p(ids))
+    return ids, int(rounds)
+
+
+def label_bfs(vtype, conn=26):
+    """Breadth-first partition labelled 1 + smallest flat index per component.
+    Returns (labels, number of components)."""
+    t = np.ascontiguousarray(vtype, np.uint8)
+    ids = np.empty(t.shape, np.uint32)
+    comps = lib().or_label_bfs(*_grid_dims(t), _p(t), conn, _p(ids))
+    return ids, int(comps)
+
+
+def label_summarise(vtype, ids, max_components=64):
+    """Component records (max_components x 12 uint32) of converged labels,
+    the number of allocated slots and the overflow flag."""
+    t = np.ascontiguousarray(vtype, np.uint8)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    rec = np.empty((max_components, 12), np.uint32)
+    ov = C.c_int(0)
+    num = lib().or_label_summarise(*_grid_dims(t), _p(t), _p(ids), max_components, _p(rec),
+                                   C.byref(ov))
+    return rec, int(num), bool(ov.value)

@@ -607,3 +607,142 @@ void or_flux_grad_kappa(int lx, int ly, int lz, const uint8_t* type, const float
         g[i] = acc;
       }
 }
+
+/* ── Connected components (label propagation) ──────────────────────────
+ * Live voxels are solid (1) or anchor (2); other types are not labelled.
+ * conn = 26 uses the 26 offsets of PFSFStencil order, conn = 6 the faces
+ * only (LabelPropagation.java:277-288). */
+#define LBL_NONE 0xFFFFFFFFu
+
+static inline int lbl_live(uint8_t t) { return t == 1 || t == T_ANCHOR; }
+
+/* LabelPropagation.shiloachVishkin (LabelPropagation.java:199-270): labels
+ * start at i+1 (air LBL_NONE); each round is a hook-to-min sweep in z,y,x
+ * order that reads the array it is updating, then one pointer jump that
+ * reads the pre-jump labels; rounds repeat until neither changes a label
+ * (cap max(64, 2n)).  Returns the number of rounds. */
+int or_label_sv(int lx, int ly, int lz, const uint8_t* type, int conn, uint32_t* id) {
+  int64_t n = (int64_t)lx * ly * lz;
+  uint32_t* prev = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) id[i] = lbl_live(type[i]) ? (uint32_t)(i + 1) : LBL_NONE;
+  int64_t cap = 2 * n > 64 ? 2 * n : 64;
+  int rounds = 0, changed = 1;
+  while (changed && rounds < cap) {
+    rounds++;
+    changed = 0;
+    for (int z = 0; z < lz; z++)
+      for (int y = 0; y < ly; y++)
+        for (int x = 0; x < lx; x++) {
+          int64_t i = IDX(x, y, z, lx, ly);
+          uint32_t me = id[i];
+          if (me == LBL_NONE) continue;
+          uint32_t mn = me;
+          for (int k = 0; k < (conn == 6 ? 6 : 26); k++) {
+            int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+            if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+            uint32_t nl = id[IDX(nx, ny, nz, lx, ly)];
+            if (nl != LBL_NONE && nl < mn) mn = nl;
+          }
+          if (mn < me) {
+            id[i] = mn;
+            changed = 1;
+          }
+        }
+    memcpy(prev, id, sizeof(uint32_t) * (size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+      uint32_t l = prev[i];
+      if (l == LBL_NONE) continue;
+      int64_t p = (int64_t)l - 1;
+      if (p < 0 || p >= n) continue;
+      uint32_t pl = prev[p];
+      if (pl != LBL_NONE && pl < l) {
+        id[i] = pl;
+        changed = 1;
+      }
+    }
+  }
+  free(prev);
+  return rounds;
+}
+
+/* Breadth-first partition (LabelPropagation.bfsComponents,
+ * LabelPropagation.java:121-168, on the flat grid): every live voxel gets
+ * 1 + the smallest flat index of its component.  Returns the number of
+ * components; anchored[c] is not produced here (see or_label_summarise). */
+int64_t or_label_bfs(int lx, int ly, int lz, const uint8_t* type, int conn, uint32_t* id) {
+  int64_t n = (int64_t)lx * ly * lz, comps = 0;
+  int64_t* q = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) id[i] = LBL_NONE;
+  for (int64_t s = 0; s < n; s++) {
+    if (!lbl_live(type[s]) || id[s] != LBL_NONE) continue;
+    /* s is the first unvisited voxel of its component in flat order, so it
+     * is the component's smallest index */
+    uint32_t lab = (uint32_t)(s + 1);
+    int64_t head = 0, tail = 0;
+    q[tail++] = s;
+    id[s] = lab;
+    comps++;
+    while (head < tail) {
+      int64_t i = q[head++];
+      int x = (int)(i % lx), y = (int)((i / lx) % ly), z = (int)(i / ((int64_t)lx * ly));
+      for (int k = 0; k < (conn == 6 ? 6 : 26); k++) {
+        int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+        if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+        int64_t j = IDX(nx, ny, nz, lx, ly);
+        if (!lbl_live(type[j]) || id[j] != LBL_NONE) continue;
+        id[j] = lab;
+        q[tail++] = j;
+      }
+    }
+  }
+  free(q);
+  return comps;
+}
+
+/* Component records of converged labels: label_prop_summarise_alloc +
+ * _aggregate as replayed by PFSFLabelPropCpuSimulator
+ * (PFSFLabelPropCpuSimulator.java:164-238).  Roots (id[i] == i+1) take
+ * slots in ascending flat order; the first maxc get a 12-word record
+ * {root label, count, anchored, 0, min x, min y, min z, 0, max x, max y,
+ * max z, 0}; later roots set *overflow.  rec (maxc*12 words) is zeroed first,
+ * as the recorder clears it.  Returns the number of allocated slots. */
+int or_label_summarise(int lx, int ly, int lz, const uint8_t* type, const uint32_t* id,
+                       int maxc, uint32_t* rec, int* overflow) {
+  int64_t n = (int64_t)lx * ly * lz;
+  uint32_t* slot = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  memset(rec, 0, sizeof(uint32_t) * 12 * (size_t)maxc);
+  int num = 0;
+  *overflow = 0;
+  for (int64_t i = 0; i < n; i++) {
+    slot[i] = LBL_NONE;
+    if (id[i] == LBL_NONE || id[i] != (uint32_t)(i + 1)) continue;
+    if (num >= maxc) {
+      *overflow = 1;
+      continue;
+    }
+    slot[i] = (uint32_t)num;
+    uint32_t* r = rec + 12 * (size_t)num;
+    r[0] = id[i];
+    r[4] = r[5] = r[6] = 0xFFFFFFFFu;
+    num++;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t l = id[i];
+    if (l == LBL_NONE) continue;
+    int64_t root = (int64_t)l - 1;
+    if (root < 0 || root >= n || slot[root] == LBL_NONE) continue;
+    uint32_t* r = rec + 12 * (size_t)slot[root];
+    uint32_t x = (uint32_t)(i % lx), y = (uint32_t)((i / lx) % ly),
+             z = (uint32_t)(i / ((int64_t)lx * ly));
+    r[1]++;
+    if (type[i] == T_ANCHOR) r[2] |= 1u;
+    if (x < r[4]) r[4] = x;
+    if (y < r[5]) r[5] = y;
+    if (z < r[6]) r[6] = z;
+    if (x > r[8]) r[8] = x;
+    if (y > r[9]) r[9] = y;
+    if (z > r[10]) r[10] = z;
+  }
+  free(slot);
+  return num;
+}

@@ -787,12 +787,37 @@ pfsf_result pfsf_hip_anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* re
                                    void* stream) {
   DIMS_OR_FAIL(d);
   if (!type || !reach) return PFSF_ERROR_INVALID_ARG;
+  if (nvox(d) > kMaxLabelCells) return PFSF_ERROR_INVALID_ARG;
   hipStream_t s = (hipStream_t)stream;
-  int* flag = nullptr;
-  if (hipMalloc((void**)&flag, sizeof(int)) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
-  hipError_t e = anchored_mask(d, type, reach, flag, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(flag);
+  uint32_t* scratch = nullptr;
+  if (hipMalloc((void**)&scratch, sizeof(uint32_t) * (size_t)(nvox(d) + 1)) != hipSuccess)
+    return PFSF_ERROR_OUT_OF_VRAM;
+  hipError_t e = anchored_mask(d, type, reach, scratch, (int*)(scratch + nvox(d)), s);
+  (void)hipFree(scratch);
+  RET(e);
+}
+
+pfsf_result pfsf_hip_label_components(pfsf_dims d, const uint8_t* type, int32_t connectivity,
+                                      uint32_t* island_id, uint32_t* components,
+                                      int32_t max_components, int32_t* num_components,
+                                      int32_t* overflow, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !island_id || !components || (connectivity != 26 && connectivity != 6) ||
+      max_components < 1 || max_components > kMaxLabelComponents || nvox(d) > kMaxLabelCells)
+    return PFSF_ERROR_INVALID_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* scratch = nullptr;
+  if (hipMalloc((void**)&scratch, sizeof(uint32_t) * (size_t)label_scratch_words(d)) !=
+      hipSuccess)
+    return PFSF_ERROR_OUT_OF_VRAM;
+  uint32_t hdr[4] = {0, 0, 0, 0};
+  hipError_t e = label_components(d, type, connectivity, island_id, components, max_components,
+                                  scratch, hdr, s);
+  (void)hipFree(scratch);
+  if (e == hipSuccess) {
+    if (num_components) *num_components = (int32_t)hdr[0];
+    if (overflow) *overflow = (int32_t)hdr[1];
+  }
   RET(e);
 }
 

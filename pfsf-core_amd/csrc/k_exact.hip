@@ -379,46 +379,6 @@ __global__ void __launch_bounds__(BS)
   }
 }
 
-// Reachability from anchors: monotone relaxation reach_i |= OR_j reach_j over
-// 26 non-air neighbours.  Writes only 0 -> 1, so concurrent updates are benign.
-__global__ void __launch_bounds__(BS)
-    k_reach_init(int64_t n, const uint8_t* __restrict__ type, uint8_t* __restrict__ reach) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) reach[i] = type[i] == 2 ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(BS)
-    k_reach_step(pfsf_dims d, const uint8_t* __restrict__ type, uint8_t* reach, int* changed) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i >= nvox(d)) return;
-  if (type[i] == 0 || reach[i]) return;
-  XYZ p = unflat(i, d);
-  for (int dz = -1; dz <= 1; dz++) {
-    int nz = p.z + dz;
-    if (nz < 0 || nz >= d.lz) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      int ny = p.y + dy;
-      if (ny < 0 || ny >= d.ly) continue;
-      for (int dx = -1; dx <= 1; dx++) {
-        int nx = p.x + dx;
-        if (nx < 0 || nx >= d.lx) continue;
-        int64_t j = flat(nx, ny, nz, d);
-        if (reach[j] && type[j] != 0) {
-          reach[i] = 1;
-          *changed = 1;
-          return;
-        }
-      }
-    }
-  }
-}
-
-__global__ void __launch_bounds__(BS)
-    k_reach_final(int64_t n, const uint8_t* __restrict__ type, uint8_t* __restrict__ reach) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) reach[i] = (reach[i] && is_solid(type[i])) ? 1 : 0;
-}
-
 // Sparse voxel re-upload: 48-byte records {flat, source, type, max_phi, rcomp,
 // rtens, cond[6]} scattered into the island's device arrays
 // (SH/sparse_scatter.comp.glsl:21-67).  Out-of-range indices are skipped.
@@ -527,27 +487,6 @@ hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
   unsigned nb = blocks_for(6 * n, BS);
   if (nb > 4096) nb = 4096;
   k_scale3<<<nb, BS, 0, s>>>(a, b, c, cond, n, max_bits);
-  return hipGetLastError();
-}
-
-hipError_t anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach, int* dflag,
-                         hipStream_t s) {
-  int64_t n = nvox(d);
-  k_reach_init<<<blocks_for(n, BS), BS, 0, s>>>(n, type, reach);
-  // Every relaxation pass extends the reached set by at least one voxel along
-  // each shortest path, so n passes always suffice; the flag ends it early.
-  int hflag = 1;
-  for (int64_t pass = 0; pass < n && hflag; pass += 8) {
-    hipError_t e = hipMemsetAsync(dflag, 0, sizeof(int), s);
-    if (e != hipSuccess) return e;
-    for (int k = 0; k < 8; k++)
-      k_reach_step<<<blocks_for(n, BS), BS, 0, s>>>(d, type, reach, dflag);
-    e = hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-  }
-  k_reach_final<<<blocks_for(n, BS), BS, 0, s>>>(n, type, reach);
   return hipGetLastError();
 }
 

@@ -712,7 +712,10 @@ hipError_t MgSolver::build() {
   if (graph_) (void)hipGraphDestroy(graph_);
   graph_exec_ = nullptr;
   graph_ = nullptr;
-  chk(anchored_mask(d0_, type_, reach_, flag_, s), err);
+  // q_ is free until the first solve: it carries the component parents and
+  // is cleared again afterwards (every work vector starts at zero)
+  chk(anchored_mask(d0_, type_, reach_, (uint32_t*)q_, flag_, s), err);
+  fill0(q_, n0_, s);
   MgLevel& F = levels_[0];
   k_fine_setup<<<blocks_for(n0_, BS), BS, 0, s>>>(d0_, type_, kappa_, reach_, (float*)F.kap,
                                                   F.am, F.fr);

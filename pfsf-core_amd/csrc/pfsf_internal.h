@@ -52,8 +52,20 @@ hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
 hipError_t launch_sparse_scatter(const uint32_t* rec, int count, int64_t n, float* source,
                                  float* cond, uint8_t* type, float* max_phi, float* rcomp,
                                  float* rtens, hipStream_t s);
-// Reachability from anchors (26-connected over non-air).  reach: u8 N.
-hipError_t anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach, int* dflag,
-                         hipStream_t s);
+// ── labels.hip: connected components by union-find ──────────────────────
+constexpr int64_t kMaxLabelCells = int64_t(1) << 30;
+constexpr int kMaxLabelComponents = 1024;
+// Words of scratch label_components needs (device, uint32).
+int64_t label_scratch_words(pfsf_dims d);
+// Labels (1 + smallest flat index of the component, air 0xFFFFFFFF) of the
+// solid/anchor voxels, conn 26 or 6, and maxc 12-word component records;
+// hdr_host[0..3] = {records, overflow, components, error}.  Synchronous.
+hipError_t label_components(pfsf_dims d, const uint8_t* type, int conn, uint32_t* id,
+                            uint32_t* rec, int maxc, uint32_t* scratch, uint32_t* hdr_host,
+                            hipStream_t s);
+// Solid voxels 26-connected through non-air voxels to an anchor: reach u8 N.
+// scratch: N uint32 (overwritten); dflag: one int.  Synchronous.
+hipError_t anchored_mask(pfsf_dims d, const uint8_t* type, uint8_t* reach, uint32_t* scratch,
+                         int* dflag, hipStream_t s);
 
 }  // namespace pfsf

@@ -98,6 +98,26 @@ def anchored_mask(vtype, reach):
         _dims(vtype), _p(vtype), _p(reach), _stream()))
 
 
+def label_components(vtype, connectivity=26, max_components=64, island_id=None,
+                     components=None):
+    """Connected components of the solid/anchor voxels of a (lz, ly, lx) uint8
+    grid: island_id (int32 view of the uint32 labels, -1 = air), the component
+    records (max_components x 12, int32 view of uint32), the number of records
+    and the overflow flag (LabelPropagation.java:199-270 fixed point,
+    PFSFLabelPropCpuSimulator.java:164-238 records)."""
+    import torch
+
+    if island_id is None:
+        island_id = torch.empty(vtype.shape, dtype=torch.int32, device=vtype.device)
+    if components is None:
+        components = torch.empty((max_components, 12), dtype=torch.int32, device=vtype.device)
+    num, ov = C.c_int32(0), C.c_int32(0)
+    capi.check("pfsf_hip_label_components", _lib().pfsf_hip_label_components(
+        _dims(vtype), _p(vtype), connectivity, _p(island_id), _p(components), max_components,
+        C.byref(num), C.byref(ov), _stream()))
+    return island_id, components, num.value, bool(ov.value)
+
+
 class Solver:
     """MG-preconditioned PCG to an FP64 relative-residual tolerance."""
 
