@@ -214,6 +214,53 @@ def adjoint_bench(torch, W, steps=2):
     return out
 
 
+def label_bench(torch, D, W, isl, steps=10):
+    """Connected-component labelling (pfsf_hip_label_components, 26-connected,
+    64 records) of the 128^3 island and of a fragmented 256^3 grid (random
+    voxels at 12 % density: ~10^5 components), and the solver set-up that
+    builds on the same union-find (anchored mask + hierarchy), median ms."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0x9F5F0000 + 5)
+    r = rng.random((256, 256, 256))
+    frag = np.where(r < 0.11, 1, np.where(r < 0.12, 2, 0)).astype(np.uint8)
+    out = {}
+    for name, t in (("island_128", isl.vtype), ("fragmented_256", frag)):
+        tg = torch.from_numpy(np.ascontiguousarray(t)).to(dev)
+        ids = torch.empty(tg.shape, dtype=torch.int32, device=dev)
+        rec = torch.empty((64, 12), dtype=torch.int32, device=dev)
+        D.label_components(tg, 26, 64, ids, rec)  # warm-up
+        ts = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, _, num, ov = D.label_components(tg, 26, 64, ids, rec)
+            ts.append(time.perf_counter() - t0)
+        roots = int(torch.count_nonzero(ids == torch.arange(1, tg.numel() + 1, device=dev,
+                                                            dtype=torch.int32).view(tg.shape)))
+        ms = float(np.median(ts)) * 1e3
+        out[name] = {"ms": round(ms, 3), "voxels": int(tg.numel()),
+                     "voxels_per_s": round(tg.numel() / (ms * 1e-3), 1), "components": roots,
+                     "records": num, "overflow": ov}
+        del tg, ids
+    vt = torch.from_numpy(isl.vtype).to(dev)
+    kp = torch.from_numpy(isl.kappa).to(dev)
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = D.Solver(vt, kp, tol=1e-6, max_iters=2000)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+        s.close()
+    out["solver_setup_128_ms"] = round(float(np.median(ts)) * 1e3, 2)
+    out["note"] = ("synchronous C-ABI calls timed on the host: scratch allocation and the "
+                   "count readback included")
+    log(f"labels: {out}")
+    return out
+
+
 def host_tick_bench(isl, steps=3):
     """The same 128^3 island through the host-buffer C ABI, as the Java runtime
     drives it: pfsf_register_island_buffers + pfsf_tick_dbb per tick (PCIe
@@ -260,6 +307,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint record")
+    ap.add_argument("--no-labels", action="store_true", help="skip the labelling record")
     ap.add_argument("--no-host-tick", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) tick record")
     ap.add_argument("--batch-only", action="store_true",
@@ -375,6 +423,8 @@ def main():
         out["mixed_precision_256"] = mixed_precision_bench(torch, D, W)
     if not args.no_adjoint:
         out["adjoint_96"] = adjoint_bench(torch, W)
+    if not args.no_labels:
+        out["labels"] = label_bench(torch, D, W, isl)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

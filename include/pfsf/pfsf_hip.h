@@ -284,6 +284,17 @@ PFSF_API pfsf_result pfsf_register_island_kappa(pfsf_engine engine, int32_t isla
  * PCG iterations per island per tick (default 500). */
 PFSF_API void pfsf_set_solve_tolerance(pfsf_engine engine, double tol, int32_t max_iters);
 
+/* Connected components of a registered island: its registered voxel-type
+ * buffer is uploaded, labelled as by pfsf_hip_label_components (26-connected)
+ * and the summary returned to host memory -- the work of
+ * PFSFLabelPropRecorder.recordLabelProp + readbackAfterFence
+ * (PFSFLabelPropRecorder.java:75-266) in one call.  header[0] = records,
+ * header[1] = overflow; records: max_components x 12 uint32 (host);
+ * labels_out: N uint32 (host) or NULL.  1 <= max_components <= 1024. */
+PFSF_API pfsf_result pfsf_island_components(pfsf_engine engine, int32_t island_id,
+                                            int32_t max_components, uint32_t* header,
+                                            uint32_t* records, uint32_t* labels_out);
+
 /* Device history field of an island (N floats), for inspection. */
 PFSF_API pfsf_result pfsf_read_history(pfsf_engine engine, int32_t island_id, float* out,
                                        int32_t capacity, int32_t* out_count);

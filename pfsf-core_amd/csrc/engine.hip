@@ -72,6 +72,9 @@ struct Island {
   int64_t scratch_elems = 0;
   uint32_t* sparse_dev = nullptr;
   void* sparse_host = nullptr;
+  uint32_t* labels = nullptr;     // component labels (N), allocated on first use
+  uint32_t* lab_rec = nullptr;    // kMaxLabelComponents records of 12 words
+  uint32_t* lab_scratch = nullptr;
   std::unique_ptr<MgSolver> solver;
   std::vector<void*> owned;
 };
@@ -635,6 +638,38 @@ pfsf_result pfsf_read_history(pfsf_engine E, int32_t id, float* out, int32_t cap
   return PFSF_OK;
 }
 
+pfsf_result pfsf_island_components(pfsf_engine E, int32_t id, int32_t max_components,
+                                   uint32_t* header, uint32_t* records, uint32_t* labels_out) {
+  CHECK_ENGINE(E);
+  if (!header || !records || max_components < 1 || max_components > kMaxLabelComponents)
+    return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is || !is->host.registered) return PFSF_ERROR_INVALID_ARG;
+  if (!is->labels) {
+    if (!dev_alloc(*is, &is->labels, is->n) ||
+        !dev_alloc(*is, &is->lab_rec, 12 * (int64_t)kMaxLabelComponents) ||
+        !dev_alloc(*is, &is->lab_scratch, label_scratch_words(is->dims)))
+      return PFSF_ERROR_OUT_OF_VRAM;
+  }
+  hipStream_t s = E->stream;
+  // the registered host buffer is the source of truth, as at every tick
+  if (hipMemcpyAsync(is->type, is->host.type, is->n, hipMemcpyHostToDevice, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  uint32_t hdr[4] = {0, 0, 0, 0};
+  if (label_components(is->dims, is->type, 26, is->labels, is->lab_rec, max_components,
+                       is->lab_scratch, hdr, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (hipMemcpyAsync(records, is->lab_rec, sizeof(uint32_t) * 12 * (size_t)max_components,
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      (labels_out && hipMemcpyAsync(labels_out, is->labels, sizeof(uint32_t) * (size_t)is->n,
+                                    hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  header[0] = hdr[0];
+  header[1] = hdr[1];
+  return PFSF_OK;
+}
+
 int32_t pfsf_drain_callbacks(pfsf_engine E, int32_t* out, int32_t cap) {
   (void)E;
   (void)out;
@@ -684,7 +719,8 @@ const char* pfsf_version(void) { return "0.1.0-hip"; }
 uint32_t pfsf_abi_version(void) { return (1u << 16) | (6u << 8) | 0u; }
 bool pfsf_has_feature(const char* name) {
   if (!name) return false;
-  static const char* kFeat[] = {"hip.gfx950", "solve.mg_pcg", "solve.mixed_bf16", "scan.compact"};
+  static const char* kFeat[] = {"hip.gfx950", "solve.mg_pcg", "solve.mixed_bf16", "scan.compact",
+                                "labels.union_find"};
   for (const char* f : kFeat)
     if (strcmp(f, name) == 0) return true;
   return false;

@@ -176,6 +176,7 @@ SIGNATURES = {
     "pfsf_hip_anchored_mask": (R, [Dims, P, P, P]),
     "pfsf_hip_label_components": (R, [Dims, P, I32, P, P, I32, C.POINTER(I32), C.POINTER(I32),
                                       P]),
+    "pfsf_island_components": (R, [P, I32, I32, P, P, P]),
     "pfsf_register_island_kappa": (R, [P, I32, P, I64]),
     "pfsf_set_solve_tolerance": (None, [P, F64, I32]),
     "pfsf_read_history": (R, [P, I32, P, I32, C.POINTER(I32)]),

@@ -108,6 +108,19 @@ class Engine:
         r = self.lib.pfsf_read_history(self.h, island_id, _addr(out), n, C.byref(cnt))
         return r, out[:cnt.value]
 
+    def components(self, island_id, max_components=64, labels=False):
+        """pfsf_island_components: (result, records [max_components x 12] uint32 of
+        which the first `num` are filled, num, overflow, labels or None)."""
+        hdr = np.zeros(2, np.uint32)
+        rec = np.zeros((max_components, 12), np.uint32)
+        keep = self._keep.get(island_id)
+        out = None
+        if labels and keep is not None:
+            out = np.zeros(keep["vt"].size, np.uint32)
+        r = self.lib.pfsf_island_components(self.h, island_id, max_components, _addr(hdr),
+                                            _addr(rec), _addr(out) if out is not None else None)
+        return r, rec, int(hdr[0]), bool(hdr[1]), out
+
     def stats(self):
         s = capi.Stats()
         r = self.lib.pfsf_get_stats(self.h, C.byref(s))

@@ -142,3 +142,29 @@ def test_multi_island_tick_is_batched_and_matches_oracle(cuda):
             assert np.array_equal(H, O.psi(isl, phi).ravel())
             want.append(_expected_events(isl, phi, o))
         assert np.array_equal(ev, np.concatenate(want))
+
+
+def test_island_components_match_oracle_and_follow_host_types(cuda):
+    # pillar + detached cluster: one anchored component, one orphan
+    isl = domains.pillar_and_cluster()
+    with Engine(max_island_size=10**6, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(3, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(3, isl) == capi.OK
+        r, rec, num, ov, lab = e.components(3, 64, labels=True)
+        assert r == capi.OK
+        want_ids, _ = O.label_sv(isl.vtype, 26)
+        wrec, wnum, wov = O.label_summarise(isl.vtype, want_ids, 64)
+        assert np.array_equal(lab.reshape(isl.vtype.shape), want_ids)
+        assert (num, ov) == (wnum, wov) == (2, False)
+        assert np.array_equal(rec, wrec)
+        assert sorted(rec[:num, 2].tolist()) == [0, 1]
+        # the registered host buffer is re-read: cut the pillar off the ground
+        vt = e._keep[3]["vt"].reshape(isl.vtype.shape)
+        vt[1, 1, 1] = 0
+        r, rec2, num2, _, _ = e.components(3, 64)
+        assert r == capi.OK
+        t2 = vt.copy()
+        wrec2, wnum2, _ = O.label_summarise(t2, O.label_sv(t2, 26)[0], 64)
+        assert num2 == wnum2 and np.array_equal(rec2, wrec2)
+        assert e.lib.pfsf_island_components(e.h, 99, 64, None, None, None) != capi.OK
