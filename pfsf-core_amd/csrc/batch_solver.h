@@ -14,6 +14,7 @@ struct BatchLevel {
   const float* kap;   // kappa on the free set, -1 elsewhere
   const float* dg;    // diagonal
   const float* am;    // anchor mass
+  const uint32_t* msk;  // coupling mask (levels >= 1), null on the finest level
   float scale, omega;
   pfsf_dims c;        // next coarser level
   float* r;           // this level's right-hand side (level 0: the island's r[0])

@@ -40,6 +40,7 @@ __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
   L.kap = D.kap;
   L.dg = D.dg;
   L.am = D.am;
+  L.msk = D.msk;
   L.scale = D.scale;
   L.omega = D.omega;
   return L;
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
 
 // UPDATE (finest level, iteration c): r[c] -> r[1-c] with p[1-c], q.
 // Otherwise the down-sweep of the level's own right-hand side.
-template <int TZ, bool UPDATE>
+template <int TZ, bool UPDATE, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     kb_down(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
             const BatchIsland* __restrict__ isl, PcgScalars* sc, const double* __restrict__ pq_part,
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_body<TZ, float, UPDATE>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
+  down_body<TZ, float, UPDATE, MASK>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
                                c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
                                (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
 }
@@ -90,8 +91,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
   const BatchIsland& I = isl[D.island];
   const float* r = FINEST ? (ridx ? I.r1 : I.r0) : D.r;
   float* out = FINEST ? I.z : D.e;
-  up_body<TZ, float, FINEST>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
-                             sc + D.island, rz_part, (unsigned)t.slot);
+  // every level below the finest carries a coupling mask
+  up_body<TZ, float, FINEST, !FINEST>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
+                                      sc + D.island, rz_part, (unsigned)t.slot);
 }
 
 // Every island's coarsest level, one work-group each; with `check` the
@@ -378,6 +380,7 @@ hipError_t BatchSolver::build_tables() {
       D.kap = (const float*)M.kap;
       D.dg = M.dg;
       D.am = M.am;
+      D.msk = M.msk;
       D.scale = M.scale;
       D.omega = M.omega;
       if (l + 1 < L) {
@@ -532,8 +535,8 @@ hipError_t BatchSolver::vcycle_coarse(bool check) {
   const int n = (int)sol_.size();
   for (int l = 1; l + 1 < lmax_; l++)
     if (ntiles_[l] > 0)
-      kb_down<kTZCoarse, false><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_,
-                                                          nullptr, nullptr, 0);
+      kb_down<kTZCoarse, false, true><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_,
+                                                                nullptr, nullptr, 0);
   kb_cheb<<<n, kMaxLdsCoarse, 0, s>>>(levels_, isl_d_, coef_, coarse_sweeps_, sc_, prr_,
                                       check ? 1 : 0);
   for (int l = lmax_ - 2; l >= 1; l--)

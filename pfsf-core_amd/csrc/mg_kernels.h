@@ -76,12 +76,20 @@ __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v);
 template <typename KT>
 struct Lv {
   pfsf_dims d;
-  const KT* kap;     // kappa on the free set, -1 elsewhere
-  const float* dg;   // diagonal (0 outside the free set)
-  const float* am;   // anchor mass
+  const KT* kap;        // kappa on the free set, -1 elsewhere
+  const float* dg;      // diagonal (0 outside the free set)
+  const float* am;      // anchor mass
+  const uint32_t* msk;  // coarse levels: coupling mask (nbit order); null = all
   float scale;
   float omega;
 };
+
+// Bit of neighbour offset (dx, dy, dz) in a coupling mask; the opposite
+// offset has bit 26 - nbit.
+__host__ __device__ constexpr int nbit(int dx, int dy, int dz) {
+  return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+}
+constexpr uint32_t kAllCouplings = 0x07FFFFFFu;
 
 struct TL {
   int ntx, nty;
@@ -208,51 +216,61 @@ __device__ __forceinline__ void stage(Tile<TZ>& T, const Lv<KT>& L, int x0, int 
 // difference form over free neighbours + anchor mass.  v holds values (0 off
 // the free set), s the free_s coding of kappa.  With constant strides every
 // read is one LDS instruction with an immediate offset from one base.
+template <bool MASK = false>
 __device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
                                                const float* __restrict__ s, int c, int sy,
-                                               int sz, float mi, float scale) {
+                                               int sz, float mi, float scale,
+                                               uint32_t mk = kAllCouplings) {
   const float vc = v[c], si = s[c];
   const float ki = si * si;
   float af = 0.f, ae = 0.f, ac = 0.f;
-  auto face = [&](int o) {
+  // A masked-off neighbour contributes exactly like one outside the free set.
+  // The mask is applied as a bitwise AND on values that are always loaded, so
+  // no neighbour access becomes a branch.
+  auto keep = [&](float x, int b) {
+    if (!MASK) return x;
+    return __uint_as_float(__float_as_uint(x) & (0u - ((mk >> b) & 1u)));
+  };
+  auto face = [&](int o, int b) {
     const float sj = s[c + o];
     const float w = is_free_s(sj) ? fmaf(sj, sj, ki) : 0.f;
-    af += w * (vc - v[c + o]);
+    af += keep(w, b) * (vc - v[c + o]);
   };
-  auto edge = [&](int o) { ae += s[c + o] * (vc - v[c + o]); };
-  auto corner = [&](int o) { ac += s[c + o] * (vc - v[c + o]); };
-  face(-1);
-  face(1);
-  face(-sy);
-  face(sy);
-  face(-sz);
-  face(sz);
-  edge(-1 - sy);
-  edge(1 - sy);
-  edge(-1 + sy);
-  edge(1 + sy);
-  edge(-1 - sz);
-  edge(1 - sz);
-  edge(-1 + sz);
-  edge(1 + sz);
-  edge(-sy - sz);
-  edge(sy - sz);
-  edge(-sy + sz);
-  edge(sy + sz);
-  corner(-1 - sy - sz);
-  corner(1 - sy - sz);
-  corner(-1 + sy - sz);
-  corner(1 + sy - sz);
-  corner(-1 - sy + sz);
-  corner(1 - sy + sz);
-  corner(-1 + sy + sz);
-  corner(1 + sy + sz);
+  auto edge = [&](int o, int b) { ae += keep(s[c + o], b) * (vc - v[c + o]); };
+  auto corner = [&](int o, int b) { ac += keep(s[c + o], b) * (vc - v[c + o]); };
+  face(-1, nbit(-1, 0, 0));
+  face(1, nbit(1, 0, 0));
+  face(-sy, nbit(0, -1, 0));
+  face(sy, nbit(0, 1, 0));
+  face(-sz, nbit(0, 0, -1));
+  face(sz, nbit(0, 0, 1));
+  edge(-1 - sy, nbit(-1, -1, 0));
+  edge(1 - sy, nbit(1, -1, 0));
+  edge(-1 + sy, nbit(-1, 1, 0));
+  edge(1 + sy, nbit(1, 1, 0));
+  edge(-1 - sz, nbit(-1, 0, -1));
+  edge(1 - sz, nbit(1, 0, -1));
+  edge(-1 + sz, nbit(-1, 0, 1));
+  edge(1 + sz, nbit(1, 0, 1));
+  edge(-sy - sz, nbit(0, -1, -1));
+  edge(sy - sz, nbit(0, 1, -1));
+  edge(-sy + sz, nbit(0, -1, 1));
+  edge(sy + sz, nbit(0, 1, 1));
+  corner(-1 - sy - sz, nbit(-1, -1, -1));
+  corner(1 - sy - sz, nbit(1, -1, -1));
+  corner(-1 + sy - sz, nbit(-1, 1, -1));
+  corner(1 + sy - sz, nbit(1, 1, -1));
+  corner(-1 - sy + sz, nbit(-1, -1, 1));
+  corner(1 - sy + sz, nbit(1, -1, 1));
+  corner(-1 + sy + sz, nbit(-1, 1, 1));
+  corner(1 + sy + sz, nbit(1, 1, 1));
   return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
 }
 
-template <int TZ>
-__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float mi, float scale) {
-  return stencil_apply(T.v, T.s, c, HX, HXY, mi, scale);
+template <int TZ, bool MASK = false>
+__device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float mi, float scale,
+                                            uint32_t mk = kAllCouplings) {
+  return stencil_apply<MASK>(T.v, T.s, c, HX, HXY, mi, scale, mk);
 }
 
 __device__ __forceinline__ int tcell(int tx, int ty, int tz) {
@@ -414,7 +432,7 @@ __device__ __forceinline__ void restrict_tile(const float (&res)[TZ], int x0, in
 // alpha = (r.z) / (p.q) is formed here from the p.q partials of the SpMV
 // (pq_n work-groups); with `publish` the work-group stores it (or flags a
 // breakdown when p.q <= 0).  The r.r partial goes to partials[pslot].
-template <int TZ, typename KT, bool UPDATE>
+template <int TZ, typename KT, bool UPDATE, bool MASK = false>
 __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, int y0, int z0,
                                           pfsf_dims c, const float* __restrict__ rin,
                                           float* __restrict__ rout, const float* __restrict__ q,
@@ -439,6 +457,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   bool inb[TZ];
   float ro[TZ], qo[TZ], amr[TZ], pr[TZ];
   double xr[TZ];
+  uint32_t mkr[TZ];
   float kk[Stage<TZ>::NP];
   float3 raw[Stage<TZ>::NP];  // r_in, q, diagonal
   if (have) {
@@ -451,6 +470,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
       amr[tz] = ld(L.am, i);
       pr[tz] = UPDATE ? ld(p, i) : 0.f;
       xr[tz] = UPDATE ? ld(x, i) : 0.0;
+      mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
     }
     stage_load<TZ>(L, x0, y0, z0,
                    [&](int j, int, int, int) {
@@ -499,7 +519,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
       const int cc = c0 + tz * HXY;
       const float ri = rr_[tz];
       const bool fr = is_free_s(T.s[cc]);
-      const float ax = tile_apply<TZ>(T, cc, amr[tz], L.scale);
+      const float ax = tile_apply<TZ, MASK>(T, cc, amr[tz], L.scale, mkr[tz]);
       res[tz] = fr ? ri - ax : 0.f;
       if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
     }
@@ -511,7 +531,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
 // Up-sweep at level l: x~ = omega D^-1 r + P e_coarse on the free set, then
 // one damped-Jacobi post-smoothing step out = x~ + omega D^-1 (r - A x~).
 // At the finest level it also accumulates r.z.
-template <int TZ, typename KT, bool FINEST>
+template <int TZ, typename KT, bool FINEST, bool MASK = false>
 __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int y0, int z0,
                                         pfsf_dims c, const float* __restrict__ r,
                                         const float* __restrict__ ec, float* __restrict__ out,
@@ -531,6 +551,7 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
     // cell outside the grid; its values are never used)
     bool inb[TZ];
     float rr_[TZ], amr[TZ], dgr[TZ];
+    uint32_t mkr[TZ];
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       inb[tz] = col && z0 + tz < d.lz;
@@ -538,6 +559,7 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       rr_[tz] = ld(r, i);
       amr[tz] = ld(L.am, i);
       dgr[tz] = ld(L.dg, i);
+      mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
     }
     stage<TZ>(T, L, x0, y0, z0, [&](int j, int gx_, int gy_, int gz_) {
       return om * ld(r, j) * rcp(ld(L.dg, j)) + ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c));
@@ -549,7 +571,7 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       const int cc = c0 + tz * HXY;
       const float xi = T.v[cc], ri = rr_[tz];
       const bool fr = is_free_s(T.s[cc]);
-      const float ax = tile_apply<TZ>(T, cc, amr[tz], L.scale);
+      const float ax = tile_apply<TZ, MASK>(T, cc, amr[tz], L.scale, mkr[tz]);
       float zi = xi + om * (ri - ax) * rcp(dgr[tz]);
       zi = fr ? zi : 0.f;
       if (FINEST) acc[0] += fr ? (double)ri * zi : 0.0;
@@ -609,6 +631,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   const bool act = i < n;
   int e = 0;
   float ki = -1.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
+  uint32_t mk = kAllCouplings;
   if (act) {
     const int x = i % d.lx, y = (i / d.lx) % d.ly, z = i / (d.lx * d.ly);
     e = (x + 1) + (y + 1) * sy + (z + 1) * sz;
@@ -616,6 +639,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
     if (ki >= 0.f) {
       D = L.dg[i];
       m = L.am[i];
+      if (L.msk) mk = L.msk[i];
       ri = r[i];
       di = coef[1] * ri * rcp(D);
       xi = di;
@@ -629,7 +653,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   for (int st = 1; st < steps; st++) {
     const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
     if (live) {
-      const float ax = stencil_apply(sx[cur], ss, e, sy, sz, m, L.scale);
+      const float ax = stencil_apply<true>(sx[cur], ss, e, sy, sz, m, L.scale, mk);
       di = c1 * di + c2 * (ri - ax) * rcp(D);
       xi = xi + di;
       sx[cur ^ 1][e] = xi;

@@ -61,6 +61,7 @@ struct MgLevel {
   float* dg = nullptr;    // diagonal (0 outside the free set)
   float* am = nullptr;    // anchor mass (couplings to Dirichlet voxels)
   uint8_t* fr = nullptr;  // free mask
+  uint32_t* msk = nullptr;  // coupling mask (levels > 0; bit nbit(dx,dy,dz))
   float* r = nullptr;     // restricted residual (levels > 0)
   float* e = nullptr;     // correction returned to the finer level (levels > 0)
   float* xb = nullptr;    // Chebyshev ping-pong (coarsest)

@@ -49,9 +49,10 @@ namespace pfsf {
 namespace {
 
 // ── gather form of the operator (setup) ────────────────────────────────
-// sum over free neighbours of w_ij * v(j).
+// sum over free neighbours (within the coupling mask mk) of w_ij * v(j).
 template <typename KT, typename F>
-__device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, float ki, F v) {
+__device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, float ki,
+                                        uint32_t mk, F v) {
   const pfsf_dims d = L.d;
   const float si = sqrtf(ki);
   float af = 0.f, ae = 0.f, ac = 0.f;
@@ -65,7 +66,7 @@ __device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, fl
         int nx = x + dx;
         if (nx < 0 || nx >= d.lx) continue;
         int cls = (dx != 0) + (dy != 0) + (dz != 0);
-        if (cls == 0) continue;
+        if (cls == 0 || !((mk >> nbit(dx, dy, dz)) & 1u)) continue;
         int64_t j = flat(nx, ny, nz, d);
         float kj = ldk(L.kap, j);
         if (kj < 0.f) continue;
@@ -138,7 +139,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
                     blockIdx.x == 0, partials, blockIdx.x);
 }
 
-template <int TZ, typename KT, bool UPDATE>
+// MASK: the level carries a coupling mask (every level below the finest).
+template <int TZ, typename KT, bool UPDATE, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_down(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ rin, float* __restrict__ rout,
            const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
@@ -146,17 +148,17 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
            unsigned pq_n, double* partials) {
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  down_body<TZ, KT, UPDATE>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n,
-                            blockIdx.x == 0, partials, blockIdx.x);
+  down_body<TZ, KT, UPDATE, MASK>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part,
+                                  pq_n, blockIdx.x == 0, partials, blockIdx.x);
 }
 
-template <int TZ, typename KT, bool FINEST>
+template <int TZ, typename KT, bool FINEST, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_up(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ r, const float* __restrict__ ec,
          float* __restrict__ out, const PcgScalars* sc, double* partials) {
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  up_body<TZ, KT, FINEST>(L, have, x0, y0, z0, c, r, ec, out, sc, partials, blockIdx.x);
+  up_body<TZ, KT, FINEST, MASK>(L, have, x0, y0, z0, c, r, ec, out, sc, partials, blockIdx.x);
 }
 
 template <typename KT>
@@ -243,7 +245,7 @@ __global__ void __launch_bounds__(BS)
       xout[i] = di;
     } else {
       float xi = T.v[cc];
-      float ax = tile_apply<TZ>(T, cc, L.am[i], L.scale);
+      float ax = tile_apply<TZ, true>(T, cc, L.am[i], L.scale, L.msk ? L.msk[i] : kAllCouplings);
       float di = c1 * dv[i] + c2 * (r[i] - ax) * rcp(D);
       dv[i] = di;
       xout[i] = xi + di;
@@ -317,6 +319,45 @@ __global__ void __launch_bounds__(BS)
   freec[ci] = nf > 0 ? 1 : 0;
 }
 
+// Coupling mask of a coarse level: bit nbit(D) of coarse cell I is set when
+// some free child i of I is coupled at the finer level (free neighbour j,
+// within the finer level's own mask) to a child j of the coarse neighbour
+// I + D.  Rediscretised couplings are kept only where the finer level has a
+// coupling, so structures separated by air at the finer level stay separate
+// (the relation is symmetric, so the operator stays symmetric).
+__global__ void __launch_bounds__(BS)
+    k_coarse_mask(pfsf_dims f, const uint8_t* __restrict__ freef, const uint32_t* __restrict__ mf,
+                  pfsf_dims c, const uint8_t* __restrict__ freec, uint32_t* __restrict__ mc) {
+  int64_t ci = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (ci >= nvox(c)) return;
+  uint32_t m = 0;
+  if (freec[ci]) {
+    int X, Y, Z;
+    unflat(ci, c, X, Y, Z);
+    for (int cz = 0; cz < 2; cz++)
+      for (int cy = 0; cy < 2; cy++)
+        for (int cx = 0; cx < 2; cx++) {
+          const int x = 2 * X + cx, y = 2 * Y + cy, z = 2 * Z + cz;
+          if (x >= f.lx || y >= f.ly || z >= f.lz) continue;
+          const int64_t i = flat(x, y, z, f);
+          if (!freef[i]) continue;
+          const uint32_t mi = mf ? mf[i] : kAllCouplings;
+          for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+              for (int dx = -1; dx <= 1; dx++) {
+                const int nx = x + dx, ny = y + dy, nz = z + dz;
+                if ((dx | dy | dz) == 0 || !((mi >> nbit(dx, dy, dz)) & 1u)) continue;
+                if (nx < 0 || nx >= f.lx || ny < 0 || ny >= f.ly || nz < 0 || nz >= f.lz)
+                  continue;
+                if (!freef[flat(nx, ny, nz, f)]) continue;
+                const int DX = (nx >> 1) - X, DY = (ny >> 1) - Y, DZ = (nz >> 1) - Z;
+                if ((DX | DY | DZ) != 0) m |= 1u << nbit(DX, DY, DZ);
+              }
+        }
+  }
+  mc[ci] = m;
+}
+
 // Diagonal of a level: scale * sum over free neighbours of w + anchor mass.
 template <typename KT>
 __global__ void __launch_bounds__(BS)
@@ -330,7 +371,8 @@ __global__ void __launch_bounds__(BS)
   }
   int x, y, z;
   unflat(i, d, x, y, z);
-  float s = offsum(L, x, y, z, ldk(L.kap, i), [&](int64_t) { return 1.0f; });
+  const uint32_t mk = L.msk ? L.msk[i] : kAllCouplings;
+  float s = offsum(L, x, y, z, ldk(L.kap, i), mk, [&](int64_t) { return 1.0f; });
   float D = s + L.am[i];
   dg[i] = D > 0.f ? D : 0.f;
 }
@@ -484,6 +526,7 @@ static Lv<KT> view(const MgLevel& l) {
   v.kap = (const KT*)l.kap;
   v.dg = l.dg;
   v.am = l.am;
+  v.msk = l.msk;
   v.scale = l.scale;
   v.omega = l.omega;
   return v;
@@ -542,7 +585,8 @@ void MgSolver::configure(const pfsf_solver_desc& desc, int min_levels, int tz_fi
     L.bf16 = (l > 0) && bf16_;
     L.scale = (float)(1 << l);
     const int tzf = l == 0 ? tz_fine : tz_coarse;
-    L.tiles.tz = tzf > 0 ? tzf : pick_tz(d, l == 0);
+    // masked coarse stencils need more registers: coarse tiles stay 2 deep
+    L.tiles.tz = tzf > 0 ? tzf : (l == 0 ? pick_tz(d, true) : 2);
     L.tiles.ntx = (d.lx + TX - 1) / TX;
     L.tiles.nty = (d.ly + TY - 1) / TY;
     L.tiles.ntz = (d.lz + L.tiles.tz - 1) / L.tiles.tz;
@@ -586,6 +630,7 @@ hipError_t MgSolver::allocate(bool standalone) {
     if (l > 0) {
       L.r = alloc<float>(L.n, err);
       L.e = alloc<float>(L.n, err);
+      L.msk = alloc<uint32_t>(L.n, err);
     }
     if (l == nlev_ - 1 && nlev_ > 1) {
       L.xb = alloc<float>(L.n, err);
@@ -733,7 +778,12 @@ hipError_t MgSolver::build() {
       k_coarsen<uint16_t, uint16_t><<<nb, BS, 0, s>>>(A.d, (const uint16_t*)A.kap, A.am, A.fr,
                                                       B.d, (uint16_t*)B.kap, B.am, B.fr);
   }
-  // diagonals need every level's free flags: computed after the coarsening
+  for (int l = 1; l < nlev_; l++) {
+    const MgLevel& A = levels_[l - 1];
+    MgLevel& B = levels_[l];
+    k_coarse_mask<<<blocks_for(B.n, BS), BS, 0, s>>>(A.d, A.fr, A.msk, B.d, B.fr, B.msk);
+  }
+  // diagonals need every level's free flags and masks: computed after both
   for (int l = 0; l < nlev_; l++) {
     MgLevel& L = levels_[l];
     unsigned nb = blocks_for(L.n, BS);
@@ -806,9 +856,15 @@ hipError_t MgSolver::update(const uint8_t* type, const float* kappa) {
 template <typename KT>
 static void launch_down_t(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
                           hipStream_t s) {
-  PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
-                           view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
-                           sc, nullptr, 0u, nullptr)));
+  if (A.msk) {
+    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false, true><<<A.tiles.nlaunch, BS, 0, s>>>(
+                             view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
+                             sc, nullptr, 0u, nullptr)));
+  } else {
+    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
+                             view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
+                             sc, nullptr, 0u, nullptr)));
+  }
 }
 
 static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
@@ -822,8 +878,13 @@ static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgS
 template <typename KT>
 static void launch_up_t(const MgLevel& A, const MgLevel& B, const float* r, float* out,
                         PcgScalars* sc, hipStream_t s) {
-  PFSF_TZ_DISPATCH(A, (k_up<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
-                           view<KT>(A), tl(A), B.d, r, B.e, out, sc, nullptr)));
+  if (A.msk) {
+    PFSF_TZ_DISPATCH(A, (k_up<TZV, KT, false, true><<<A.tiles.nlaunch, BS, 0, s>>>(
+                             view<KT>(A), tl(A), B.d, r, B.e, out, sc, nullptr)));
+  } else {
+    PFSF_TZ_DISPATCH(A, (k_up<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
+                             view<KT>(A), tl(A), B.d, r, B.e, out, sc, nullptr)));
+  }
 }
 
 static void launch_up(const MgLevel& A, const MgLevel& B, const float* r, float* out,
@@ -1111,13 +1172,13 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
         const MgLevel& A = levels_[1];
         const MgLevel& B = levels_[2];
         if (A.bf16) {
-          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, uint16_t, false>, A.tiles.nlaunch, a, b,
+          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, uint16_t, false, true>, A.tiles.nlaunch, a, b,
                                   view<uint16_t>(A), tl(A), B.d, (const float*)A.r,
                                   (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
                                   (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
                                   (double*)nullptr));
         } else {
-          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, float, false>, A.tiles.nlaunch, a, b,
+          PFSF_TZ_DISPATCH(A, ext(k_down<TZV, float, false, true>, A.tiles.nlaunch, a, b,
                                   view<float>(A), tl(A), B.d, (const float*)A.r,
                                   (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
                                   (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
