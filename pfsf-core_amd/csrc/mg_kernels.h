@@ -359,7 +359,6 @@ __device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, in
   double acc[1] = {0.0};
   const bool first = !sc->have_dir;
   const double rz_old = sc->rzs[slot ^ 1];
-  const double rz = block_total(rz_part, rz_n);
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const pfsf_dims d = L.d;
   const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
@@ -379,6 +378,8 @@ __device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, in
                    [&](int j, int, int, int) { return make_float2(ld(z, j), ld(pin, j)); }, kk,
                    raw);
   }
+  // the r.z partials are summed while the tile's loads are in flight
+  const double rz = block_total(rz_part, rz_n);
   const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
   if (publish && threadIdx.x == 0) {
     sc->rzs[slot] = rz;
@@ -445,7 +446,6 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
   const double rz = UPDATE ? sc->rz : 0.0;
-  const double pq0 = UPDATE ? block_total(pq_part, pq_n) : 0.0;
   const pfsf_dims d = L.d;
   const float om = L.omega;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -480,7 +480,8 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   }
   double alpha64 = 0.0;
   if (UPDATE) {
-    const double pq = pq0;
+    // the p.q partials are summed while the tile's loads are in flight
+    const double pq = block_total(pq_part, pq_n);
     const bool ok = pq > 0.0;
     alpha64 = ok ? rz / pq : 0.0;
     if (publish && threadIdx.x == 0) {
@@ -597,13 +598,40 @@ __host__ __device__ inline int padded_cells(pfsf_dims d) {
 //
 // With rr_n > 0 it first finishes the PCG iteration's convergence test from
 // the r.r partials of the finest update (rr_n work-groups): iteration count,
-// r.r, and the stop flag, which also ends this V-cycle.
+// r.r, and the stop flag, which also ends this V-cycle.  With `mirror` the
+// test also stores {(iterations << 8) | stop, r.r} into those host-visible
+// words, so a host can follow the iteration without copying scalars back.
+//
+// Every global operand (the cell's kappa, diagonal, anchor mass, mask, rhs
+// and the step coefficients) is requested before the partials are summed, so
+// the two memory round trips overlap.
+__device__ __forceinline__ void publish_progress(unsigned long long* mirror, int iters, int done,
+                                                 double rr) {
+  // system scope, no release: each word carries its information by itself
+  __hip_atomic_store(mirror + 1, (unsigned long long)__double_as_longlong(rr), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(mirror, ((unsigned long long)(unsigned)iters << 8) | (unsigned)done,
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename KT>
 __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __restrict__ r,
                                               float* __restrict__ out,
                                               const float* __restrict__ coef, int steps,
                                               PcgScalars* sc, const double* __restrict__ rr_part,
-                                              unsigned rr_n) {
+                                              unsigned rr_n,
+                                              unsigned long long* mirror = nullptr) {
+  __shared__ float ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
+  __shared__ float cf[128];
+  const pfsf_dims d = L.d;
+  const int n = d.lx * d.ly * d.lz;
+  const int i = threadIdx.x;
+  const bool act = i < n;
+  const int ic = act ? i : 0;  // cell 0 stands in for an idle thread
+  const float k_ld = ldk(L.kap, ic);
+  const float D_ld = L.dg[ic], m_ld = L.am[ic], r_ld = r[ic];
+  const uint32_t mk_ld = L.msk ? L.msk[ic] : kAllCouplings;
+  const float c_ld = i < 2 * steps ? coef[i] : 0.f;
   if (sc->done) return;
   if (rr_n > 0) {
     const double rr = block_total(rr_part, rr_n);
@@ -614,34 +642,31 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
       sc->rr = rr;
       sc->iters = it;
       if (stop) sc->done = 1;
+      if (mirror) publish_progress(mirror, it, stop ? 1 : 0, rr);
     }
     if (stop) return;
   }
-  __shared__ float ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
-  const pfsf_dims d = L.d;
   const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
   for (int e = threadIdx.x; e < np; e += blockDim.x) {
     ss[e] = -0.0f;
     sx[0][e] = 0.f;
     sx[1][e] = 0.f;
   }
+  if (i < 2 * steps) cf[i] = c_ld;
   __syncthreads();
-  const int n = d.lx * d.ly * d.lz;
-  const int i = threadIdx.x;
-  const bool act = i < n;
   int e = 0;
   float ki = -1.f, D = 1.f, m = 0.f, ri = 0.f, xi = 0.f, di = 0.f;
   uint32_t mk = kAllCouplings;
   if (act) {
     const int x = i % d.lx, y = (i / d.lx) % d.ly, z = i / (d.lx * d.ly);
     e = (x + 1) + (y + 1) * sy + (z + 1) * sz;
-    ki = ldk(L.kap, i);
+    ki = k_ld;
     if (ki >= 0.f) {
-      D = L.dg[i];
-      m = L.am[i];
-      if (L.msk) mk = L.msk[i];
-      ri = r[i];
-      di = coef[1] * ri * rcp(D);
+      D = D_ld;
+      m = m_ld;
+      mk = mk_ld;
+      ri = r_ld;
+      di = cf[1] * ri * rcp(D);
       xi = di;
     }
     ss[e] = free_s(ki);
@@ -651,7 +676,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   __syncthreads();
   int cur = 0;
   for (int st = 1; st < steps; st++) {
-    const float c1 = coef[2 * st], c2 = coef[2 * st + 1];
+    const float c1 = cf[2 * st], c2 = cf[2 * st + 1];
     if (live) {
       const float ax = stencil_apply<true>(sx[cur], ss, e, sy, sz, m, L.scale, mk);
       di = c1 * di + c2 * (ri - ax) * rcp(D);

@@ -115,7 +115,11 @@ class MgSolver {
   hipError_t true_residual(const float* b, double* rel);
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
   hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
-  hipError_t chunk(int pairs);         // 2*pairs iterations (graph replay when captured)
+  hipError_t launch_chunk(int k);       // 2 << k iterations (graph replay once captured)
+  // Spins until the mirrored progress shows `target` convergence tests or a
+  // stop; a drained stream ends the wait with the device scalars.
+  hipError_t wait_progress(int target, int* iters, int* done, double* rr);
+  void drop_graphs();
   void release();
 
   pfsf_solver_desc desc_{};
@@ -135,6 +139,9 @@ class MgSolver {
   PcgScalars* sc_ = nullptr;
   IrScalars* ir_ = nullptr;
   PcgScalars* sc_host_ = nullptr;  // pinned: PcgScalars followed by IrScalars
+  // pinned, coherent: {(iterations << 8) | done, r.r bits}, stored by the
+  // convergence test of every iteration
+  unsigned long long* mirror_ = nullptr;
   double* partials_ = nullptr;  // work-group partial sums; regions below
   int64_t npartials_ = 0;
   double* ppq_ = nullptr;  // p.q (SpMV)
@@ -148,8 +155,8 @@ class MgSolver {
   float *q_ = nullptr, *z_ = nullptr;
   float* coef_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_[2] = {nullptr, nullptr};  // 2 and 4 PCG iterations
+  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
   bool use_graph_ = true;
 };
 

@@ -38,6 +38,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -95,7 +96,7 @@ enum FinMode {
 
 __global__ void __launch_bounds__(BS)
     k_finalize(int mode, const double* __restrict__ partials, unsigned nblk, PcgScalars* sc,
-               IrScalars* ir) {
+               IrScalars* ir, unsigned long long* mirror) {
   if (mode <= FIN_UPDATE && sc->done) return;
   const double v = block_total(partials, nblk);
   if (threadIdx.x != 0) return;
@@ -114,6 +115,7 @@ __global__ void __launch_bounds__(BS)
       sc->rr = v;
       sc->iters += 1;
       if (v <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
+      if (mirror) publish_progress(mirror, sc->iters, sc->done, v);
       break;
     case FIN_BNORM:
       ir->bb = v;
@@ -165,8 +167,8 @@ template <typename KT>
 __global__ void __launch_bounds__(kMaxLdsCoarse)
     k_cheb_lds(Lv<KT> L, const float* __restrict__ r, float* __restrict__ out,
                const float* __restrict__ coef, int steps, PcgScalars* sc,
-               const double* __restrict__ rr_part, unsigned rr_n) {
-  cheb_lds_body<KT>(L, r, out, coef, steps, sc, rr_part, rr_n);
+               const double* __restrict__ rr_part, unsigned rr_n, unsigned long long* mirror) {
+  cheb_lds_body<KT>(L, r, out, coef, steps, sc, rr_part, rr_n, mirror);
 }
 
 // Jacobi-preconditioned variant of the update (single-level solve):
@@ -501,16 +503,24 @@ static T* dalloc(int64_t n, std::vector<void*>& owned, hipError_t& err) {
 
 MgSolver::~MgSolver() { release(); }
 
+void MgSolver::drop_graphs() {
+  for (int k = 0; k < 2; k++) {
+    if (graph_exec_[k]) (void)hipGraphExecDestroy(graph_exec_[k]);
+    if (graph_[k]) (void)hipGraphDestroy(graph_[k]);
+    graph_exec_[k] = nullptr;
+    graph_[k] = nullptr;
+  }
+}
+
 void MgSolver::release() {
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  graph_exec_ = nullptr;
-  graph_ = nullptr;
+  drop_graphs();
   if (stream_ && owns_stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : owned_) (void)hipFree(p);
   owned_.clear();
   if (sc_host_) (void)hipHostFree(sc_host_);
   sc_host_ = nullptr;
+  if (mirror_) (void)hipHostFree(mirror_);
+  mirror_ = nullptr;
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   ev0_ = ev1_ = nullptr;
@@ -684,6 +694,12 @@ hipError_t MgSolver::create(const pfsf_solver_desc& desc, bool batchable) {
   if (chk(allocate(true), err)) return err;
   if (chk(hipHostMalloc((void**)&sc_host_, sizeof(PcgScalars) + sizeof(IrScalars)), err))
     return err;
+  // device stores must reach the host while a graph is still running
+  if (chk(hipHostMalloc((void**)&mirror_, 2 * sizeof(unsigned long long),
+                        hipHostMallocMapped | hipHostMallocCoherent),
+          err))
+    return err;
+  mirror_[0] = mirror_[1] = 0;
   chk(hipEventCreate(&ev0_), err);
   chk(hipEventCreate(&ev1_), err);
   if (err != hipSuccess) return err;
@@ -753,10 +769,7 @@ hipError_t MgSolver::build_tiles(MgLevel& L) {
 hipError_t MgSolver::build() {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
-  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
-  graph_exec_ = nullptr;
-  graph_ = nullptr;
+  drop_graphs();
   // q_ is free until the first solve: it carries the component parents and
   // is cleared again afterwards (every work vector starts at zero)
   chk(anchored_mask(d0_, type_, reach_, (uint32_t*)q_, flag_, s), err);
@@ -906,7 +919,7 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   const unsigned G = levels_[0].tiles.nlaunch;
   MgLevel& C = levels_[nlev_ - 1];
   const bool lds = C.n <= kMaxLdsCoarse && padded_cells(C.d) <= kMaxLdsPadded;
-  if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_);
+  if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
   for (int l = from; l < nlev_ - 1; l++)
     launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
   int st = std::min(coarse_sweeps_, 63);
@@ -914,10 +927,10 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   if (lds) {
     if (C.bf16)
       k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st,
-                                                        sc_, prr_, rr_n);
+                                                        sc_, prr_, rr_n, mirror_);
     else
       k_cheb_lds<float><<<1, kMaxLdsCoarse, 0, s>>>(view<float>(C), C.r, C.e, coef_, st, sc_,
-                                                     prr_, rr_n);
+                                                     prr_, rr_n, mirror_);
   } else {
     float* xin = C.xb;
     float* xout = C.e;
@@ -970,10 +983,10 @@ hipError_t MgSolver::iteration(int c) {
     PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
                             view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prr_ + nb, nb,
                             c, ppq_)));
-    k_finalize<<<1, BS, 0, s>>>(FIN_ALPHA, ppq_, G, sc_, ir_);
+    k_finalize<<<1, BS, 0, s>>>(FIN_ALPHA, ppq_, G, sc_, ir_, nullptr);
     k_update_jacobi<<<nb, BS, 0, s>>>(n0_, F.dg, x_, r_[c], r_[1 - c], p_[1 - c], q_, z_, sc_,
                                       prr_);
-    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, nb, sc_, ir_);
+    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, nb, sc_, ir_, mirror_);
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
@@ -989,37 +1002,83 @@ hipError_t MgSolver::iteration(int c) {
   return hipGetLastError();
 }
 
-// 2*pairs PCG iterations.  The pair (ping, pong) is captured once into a HIP
-// graph and replayed; the capture holds kernel launches only.
-hipError_t MgSolver::chunk(int pairs) {
+// 2 << k PCG iterations (k = 0: one ping-pong pair, k = 1: two pairs).
+// Each length is captured once into a HIP graph and replayed; the capture
+// holds kernel launches only.  Without graph support the kernels are
+// launched directly.
+hipError_t MgSolver::launch_chunk(int k) {
   hipError_t err = hipSuccess;
-  if (use_graph_ && !graph_exec_) {
+  const int pairs = 1 << k;
+  if (use_graph_ && !graph_exec_[k]) {
     if (!chk(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), err)) {
-      (void)iteration(0);
-      (void)iteration(1);
+      for (int i = 0; i < pairs; i++) {
+        (void)iteration(0);
+        (void)iteration(1);
+      }
       hipGraph_t g = nullptr;
       chk(hipStreamEndCapture(stream_, &g), err);
-      if (err == hipSuccess && !chk(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0), err))
-        graph_ = g;
+      if (err == hipSuccess &&
+          !chk(hipGraphInstantiate(&graph_exec_[k], g, nullptr, nullptr, 0), err))
+        graph_[k] = g;
       else if (g)
         (void)hipGraphDestroy(g);
     }
     if (err != hipSuccess) {  // fall back to direct launches
       (void)hipGetLastError();
+      drop_graphs();
       use_graph_ = false;
-      graph_exec_ = nullptr;
       err = hipSuccess;
     }
   }
-  for (int k = 0; k < pairs && err == hipSuccess; k++) {
-    if (use_graph_) {
-      chk(hipGraphLaunch(graph_exec_, stream_), err);
-    } else {
-      chk(iteration(0), err);
-      chk(iteration(1), err);
-    }
+  if (use_graph_) return hipGraphLaunch(graph_exec_[k], stream_);
+  for (int i = 0; i < pairs && err == hipSuccess; i++) {
+    chk(iteration(0), err);
+    chk(iteration(1), err);
   }
   return err;
+}
+
+// Waits until the mirrored progress word shows at least `target` convergence
+// tests of the running pass, or a stop.  The word is stored by the test
+// itself, so the host follows the iteration without a copy on the stream.
+// A breakdown (p.q <= 0) stops the pass without a convergence test: once the
+// stream has drained, the device scalars are copied back and decide.
+hipError_t MgSolver::wait_progress(int target, int* iters, int* done, double* rr) {
+  // The stream is queried only after 2 ms without a new word: a query can
+  // place a marker on the queue, which would cost every iteration a packet.
+  using clk = std::chrono::steady_clock;
+  unsigned long long last = ~0ull;
+  clk::time_point since = clk::now();
+  for (unsigned spin = 0;; spin++) {
+    const unsigned long long w = __atomic_load_n(mirror_, __ATOMIC_ACQUIRE);
+    *iters = (int)(w >> 8);
+    *done = (int)(w & 0xffu);
+    if (*done || *iters >= target) {
+      const unsigned long long rb = __atomic_load_n(mirror_ + 1, __ATOMIC_ACQUIRE);
+      memcpy(rr, &rb, sizeof(double));
+      return hipSuccess;
+    }
+    if (w != last) {
+      last = w;
+      since = clk::now();
+      continue;
+    }
+    if ((spin & 255u) != 255u || clk::now() - since < std::chrono::milliseconds(2)) continue;
+    const hipError_t q = hipStreamQuery(stream_);
+    if (q == hipErrorNotReady) {
+      since = clk::now();
+      continue;
+    }
+    if (q != hipSuccess) return q;
+    hipError_t err = hipSuccess;
+    chk(hipMemcpyAsync(sc_host_, sc_, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream_), err);
+    chk(hipStreamSynchronize(stream_), err);
+    *iters = sc_host_->iters;
+    *done = sc_host_->done;
+    *rr = sc_host_->rr;
+    if (!*done && *iters < target) *done = 3;  // drained short of the target: stop
+    return err;
+  }
 }
 
 // True FP64 residual of x_ into r_[0]; returns ||r|| / ||b|| (synchronises).
@@ -1029,7 +1088,7 @@ hipError_t MgSolver::true_residual(const float* b, double* rel) {
   const MgLevel& F = levels_[0];
   PFSF_TZ_DISPATCH(F, (k_true_residual_tiles<TZV><<<F.tiles.nlaunch, BS, 0, s>>>(
                           d0_, tl(F), type_, kappa_, F.fr, b, x_, r_[0], partials_)));
-  k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, F.tiles.nlaunch, sc_, ir_);
+  k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, F.tiles.nlaunch, sc_, ir_, nullptr);
   IrScalars* irh = (IrScalars*)(sc_host_ + 1);
   chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
   chk(hipStreamSynchronize(s), err);
@@ -1059,39 +1118,50 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
     k_zero64<<<nb0, BS, 0, s>>>(n0_, x_);
   }
   k_bnorm<<<nb0, BS, 0, s>>>(n0_, F.fr, b, partials_);
-  k_finalize<<<1, BS, 0, s>>>(FIN_BNORM, partials_, nb0, sc_, ir_);
+  k_finalize<<<1, BS, 0, s>>>(FIN_BNORM, partials_, nb0, sc_, ir_, nullptr);
   double rel = 0.0;
   chk(true_residual(b, &rel), err);
   int passes = 0, total = 0;
-  const int pairs_first = 4;  // 8 iterations before the first host check
   const IrScalars* irh = (const IrScalars*)(sc_host_ + 1);
   for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
     double stop_rel = std::max(0.5 * tol / rel, 2e-6);
+    mirror_[0] = mirror_[1] = 0;  // the stream is idle here (true_residual synchronised)
     k_pcg_begin<<<1, 1, 0, s>>>(sc_, ir_, stop_rel, max_it - total, 1);
     chk(precondition(), err);
-    const double rr0 = irh->rr;  // the pass starts from the true residual
-    int done = 0, pairs = pairs_first, run = 0;
-    // the device caps the pass at max_it - total iterations; the host replays
-    // at most that many iterations plus one check's worth
-    while (!done && err == hipSuccess && run < max_it - total + 2 * pairs_first) {
-      chk(chunk(pairs), err);
-      run += 2 * pairs;
-      chk(hipMemcpyAsync(sc_host_, sc_, sizeof(PcgScalars), hipMemcpyDeviceToHost, s), err);
-      chk(hipStreamSynchronize(s), err);
-      done = sc_host_->done;
-      // next check where the observed contraction rate puts the end of the
-      // pass: few host round trips, few replays past the stop (which return
-      // at once but still cost their launches)
-      const int it = sc_host_->iters;
-      const double rr = sc_host_->rr, stop2 = sc_host_->stop2;
-      pairs = pairs_first;
-      if (!done && it > 0 && rr > stop2 && rr < rr0) {
-        const double rho = pow(rr / rr0, 1.0 / it);
-        const double m = ceil(log(stop2 / rr) / log(rho));
-        pairs = (int)std::min(8.0, std::max(1.0, ceil(0.5 * m)));
+    // The device stops the pass at its tolerance or after `cap` iterations;
+    // the host only decides how much work to queue.  It follows every
+    // convergence test through the mirror and tops the queue up while at
+    // most one untested iteration is left, unless the contraction rate of
+    // the last tests puts the stop within the queued iterations.  Queued
+    // iterations past the stop still cost their launches (their kernels
+    // return at once), so the queue is kept short near the predicted end.
+    const int cap = max_it - total;
+    const double stop2 = irh->rr * stop_rel * stop_rel;
+    int launched = 0, it = 0, done = 0;
+    double rr = 0.0;
+    std::vector<double> hist;  // r.r after each test
+    auto launch = [&](int k) {
+      chk(launch_chunk(k), err);
+      launched += 2 << k;
+    };
+    launch(cap > 2 ? 1 : 0);
+    while (!done && err == hipSuccess) {
+      chk(wait_progress(it + 1, &it, &done, &rr), err);
+      if (done || err != hipSuccess) break;
+      hist.resize(it, rr);  // tests the host did not see take the latest r.r
+      hist[it - 1] = rr;
+      const int pending = launched - it;  // iterations not yet tested
+      if (pending > 1 || launched >= cap) continue;
+      int more = 8;  // iterations still needed (unknown: a full chunk)
+      if (it >= 4 && rr > stop2 && hist[it - 4] > rr) {
+        const double rho = pow(rr / hist[it - 4], 1.0 / 3.0);
+        if (rho > 0.0 && rho < 1.0) more = (int)ceil(log(stop2 / rr) / log(rho));
       }
+      if (more > pending) launch(more - pending > 2 ? 1 : 0);
     }
-    const int it = sc_host_->iters;
+    chk(hipStreamSynchronize(s), err);
+    sc_host_->done = done;
+    sc_host_->iters = it;
     total += it;
     passes++;
     double nrel = rel;
