@@ -359,6 +359,7 @@ __device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, in
   double acc[1] = {0.0};
   const bool first = !sc->have_dir;
   const double rz_old = sc->rzs[slot ^ 1];
+  const double rz = block_total(rz_part, rz_n);
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const pfsf_dims d = L.d;
   const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
@@ -378,8 +379,6 @@ __device__ __forceinline__ void spmv_body(const Lv<KT>& L, bool have, int x0, in
                    [&](int j, int, int, int) { return make_float2(ld(z, j), ld(pin, j)); }, kk,
                    raw);
   }
-  // the r.z partials are summed while the tile's loads are in flight
-  const double rz = block_total(rz_part, rz_n);
   const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
   if (publish && threadIdx.x == 0) {
     sc->rzs[slot] = rz;
@@ -446,6 +445,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
   const double rz = UPDATE ? sc->rz : 0.0;
+  const double pq0 = UPDATE ? block_total(pq_part, pq_n) : 0.0;
   const pfsf_dims d = L.d;
   const float om = L.omega;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
@@ -480,8 +480,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   }
   double alpha64 = 0.0;
   if (UPDATE) {
-    // the p.q partials are summed while the tile's loads are in flight
-    const double pq = block_total(pq_part, pq_n);
+    const double pq = pq0;
     const bool ok = pq > 0.0;
     alpha64 = ok ? rz / pq : 0.0;
     if (publish && threadIdx.x == 0) {
@@ -614,6 +613,25 @@ __device__ __forceinline__ void publish_progress(unsigned long long* mirror, int
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// The PCG iteration's convergence test from the r.r partials of the finest
+// update (rr_n work-groups): iteration count, r.r and the stop flag (also
+// mirrored to the host words when given).  Returns the stop decision, the
+// same in every thread of the work-group.
+__device__ __forceinline__ bool convergence_test(PcgScalars* sc, const double* __restrict__ rr_part,
+                                                 unsigned rr_n, unsigned long long* mirror) {
+  const double rr = block_total(rr_part, rr_n);
+  const int it = sc->iters + 1;
+  const bool stop = rr <= sc->stop2 || it >= sc->max_iters;
+  __syncthreads();  // every thread has read the scalars
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    sc->iters = it;
+    if (stop) sc->done = 1;
+    if (mirror) publish_progress(mirror, it, stop ? 1 : 0, rr);
+  }
+  return stop;
+}
+
 template <typename KT>
 __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __restrict__ r,
                                               float* __restrict__ out,
@@ -633,19 +651,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   const uint32_t mk_ld = L.msk ? L.msk[ic] : kAllCouplings;
   const float c_ld = i < 2 * steps ? coef[i] : 0.f;
   if (sc->done) return;
-  if (rr_n > 0) {
-    const double rr = block_total(rr_part, rr_n);
-    const int it = sc->iters + 1;
-    const bool stop = rr <= sc->stop2 || it >= sc->max_iters;
-    __syncthreads();  // every thread has read the scalars
-    if (threadIdx.x == 0) {
-      sc->rr = rr;
-      sc->iters = it;
-      if (stop) sc->done = 1;
-      if (mirror) publish_progress(mirror, it, stop ? 1 : 0, rr);
-    }
-    if (stop) return;
-  }
+  if (rr_n > 0 && convergence_test(sc, rr_part, rr_n, mirror)) return;
   const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
   for (int e = threadIdx.x; e < np; e += blockDim.x) {
     ss[e] = -0.0f;

@@ -919,11 +919,11 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   const unsigned G = levels_[0].tiles.nlaunch;
   MgLevel& C = levels_[nlev_ - 1];
   const bool lds = C.n <= kMaxLdsCoarse && padded_cells(C.d) <= kMaxLdsPadded;
+  int st = std::min(coarse_sweeps_, 63);
+  const unsigned rr_n = (check && lds) ? G : 0u;
   if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
   for (int l = from; l < nlev_ - 1; l++)
     launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
-  int st = std::min(coarse_sweeps_, 63);
-  const unsigned rr_n = (check && lds) ? G : 0u;
   if (lds) {
     if (C.bf16)
       k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st,
