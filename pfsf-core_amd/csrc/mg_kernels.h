@@ -706,7 +706,26 @@ __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
 }
 
-// True residual over the tiles of the finest level, FP64 and exact: the tile
+// The same coupling with the square root of the FP32 product formed from the
+// hardware reciprocal square root and one FP64 Newton step (relative error
+// about 2e-14 instead of a correctly rounded FP64 square root, at a fraction
+// of its cost): the convergence check needs the residual to far better than
+// its 1e-6 target, not to the last bit.
+__device__ __forceinline__ double couple_f64_fast(int cls, float ki, float kj) {
+  if (cls == 1) {
+    float s = ki + kj;
+    return 0.5 * (double)s;
+  }
+  const float p = ki * kj;
+  const double d = (double)p;
+  double r = (double)__builtin_amdgcn_rsqf(p);
+  r = r * (1.5 - 0.5 * d * r * r);
+  const double sq = p > 0.f ? d * r : 0.0;
+  return sq * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
+}
+
+// True residual over the tiles of the finest level in FP64 (couplings of
+// couple_f64_fast): the tile
 // stages the iterate (0 off the free set) and the coupling kappa (-1 on air)
 // with a one-voxel halo, then each thread forms b - A x for its column from
 // LDS.  Cells of tiles without free voxels are never written and stay 0.
@@ -756,14 +775,24 @@ __device__ __forceinline__ void true_residual_tile(
       cell(hrow * HX + ex, hx, gy, gz,
            hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
     }
-    __syncthreads();
     const int ty = t / TX, gy = y0 + ty, sz = d.lx * d.ly;
     const bool col = gx < d.lx && gy < d.ly;
     const int i0 = flat32(gx, gy, z0, d);
+    // own cells' free flags and loads, requested with the halo
+    bool fown[TZ];
+    float bown[TZ];
+#pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
-      if (!col || z0 + tz >= d.lz) break;
+      const bool in = col && z0 + tz < d.lz;
+      const int i = in ? i0 + tz * sz : 0;
+      fown[tz] = in && fr[i];
+      bown[tz] = b[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tz = 0; tz < TZ; tz++) {
+      if (!fown[tz]) continue;
       const int i = i0 + tz * sz;
-      if (!fr[i]) continue;
       const int c = tcell(tx, ty, tz);
       const float ki = T.k[c];
       const double xi = T.x[c];
@@ -778,9 +807,9 @@ __device__ __forceinline__ void true_residual_tile(
             if (cls == 0) continue;
             const int o = dx + dy * HX + dz * HXY;
             const float kj = T.k[c + o];
-            if (kj >= 0.f) ax += couple_f64(cls, ki, kj) * (xi - T.x[c + o]);
+            if (kj >= 0.f) ax += couple_f64_fast(cls, ki, kj) * (xi - T.x[c + o]);
           }
-      const double ri = (double)b[i] - ax;
+      const double ri = (double)bown[tz] - ax;
       r32[i] = (float)ri;
       acc += ri * ri;
     }

@@ -115,7 +115,8 @@ class MgSolver {
   hipError_t true_residual(const float* b, double* rel);
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
   hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
-  hipError_t launch_chunk(int k);       // 2 << k iterations (graph replay once captured)
+  // n = 1, 2 or 4 iterations starting at parity c (graph replay once captured)
+  hipError_t launch_iters(int n, int c);
   // Spins until the mirrored progress shows `target` convergence tests or a
   // stop; a drained stream ends the wait with the device scalars.
   hipError_t wait_progress(int target, int* iters, int* done, double* rr);
@@ -155,8 +156,10 @@ class MgSolver {
   float *q_ = nullptr, *z_ = nullptr;
   float* coef_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipGraph_t graph_[2] = {nullptr, nullptr};  // 2 and 4 PCG iterations
-  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+  // PCG iteration sequences: index 2 * log2(length) + starting parity
+  hipGraph_t graph_[6] = {};
+  hipGraphExec_t graph_exec_[6] = {};
+  double rho_pass_[4] = {};  // per-iteration r.r contraction of each pass of the last solve
   bool use_graph_ = true;
 };
 

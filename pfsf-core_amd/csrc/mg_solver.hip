@@ -45,6 +45,7 @@
 
 #include "mg_kernels.h"
 #include "mg_solver.h"
+#include "mg_stream.h"
 
 namespace pfsf {
 namespace {
@@ -504,7 +505,7 @@ static T* dalloc(int64_t n, std::vector<void*>& owned, hipError_t& err) {
 MgSolver::~MgSolver() { release(); }
 
 void MgSolver::drop_graphs() {
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < 6; k++) {
     if (graph_exec_[k]) (void)hipGraphExecDestroy(graph_exec_[k]);
     if (graph_[k]) (void)hipGraphDestroy(graph_[k]);
     graph_exec_[k] = nullptr;
@@ -557,9 +558,12 @@ static int pick_tz(pfsf_dims d, bool finest) {
     int v = env ? atoi(env) : 0;
     if (v == 8 || v == 4 || v == 2) return v;
   }
+  // the finest level streams its columns plane by plane: deeper columns
+  // load fewer halo planes, and about two columns per CU keep it busy
+  const int64_t want = finest ? 512 : 2048;
   for (int tz : {8, 4}) {
     int64_t nt = (int64_t)((d.lx + TX - 1) / TX) * ((d.ly + TY - 1) / TY) * ((d.lz + tz - 1) / tz);
-    if (nt >= 2048) return tz;
+    if (nt >= want) return tz;
   }
   return 2;
 }
@@ -965,10 +969,13 @@ hipError_t MgSolver::precondition() {
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
-  launch_down(F, B, r_[0], sc_, s);
+  const unsigned G = F.tiles.nlaunch;
+  PFSF_TZ_DISPATCH(F, (ks_down<TZV, false><<<G, BS, 0, s>>>(
+                          view<float>(F), tl(F), B.d, r_[0], nullptr, nullptr, nullptr, nullptr,
+                          B.r, sc_, nullptr, 0u, nullptr)));
   (void)vcycle_below(1, false);
-  PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<F.tiles.nlaunch, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[0], B.e, z_, sc_, prz_)));
+  PFSF_TZ_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(view<float>(F), tl(F), B.d, r_[0], B.e,
+                                                          z_, sc_, prz_)));
   return hipGetLastError();
 }
 
@@ -990,31 +997,26 @@ hipError_t MgSolver::iteration(int c) {
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
-  PFSF_TZ_DISPATCH(F, (k_spmv_dir<TZV, float><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prz_, G, c,
-                          ppq_)));
-  PFSF_TZ_DISPATCH(F, (k_down<TZV, float, true><<<G, BS, 0, s>>>(
+  PFSF_TZ_DISPATCH(F, (ks_spmv<TZV><<<G, BS, 0, s>>>(view<float>(F), tl(F), z_, p_[c],
+                                                      p_[1 - c], q_, sc_, prz_, G, c, ppq_)));
+  PFSF_TZ_DISPATCH(F, (ks_down<TZV, true><<<G, BS, 0, s>>>(
                           view<float>(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
                           sc_, ppq_, G, prr_)));
   (void)vcycle_below(1, true);
-  PFSF_TZ_DISPATCH(F, (k_up<TZV, float, true><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, prz_)));
+  PFSF_TZ_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(view<float>(F), tl(F), B.d, r_[1 - c],
+                                                          B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
 
-// 2 << k PCG iterations (k = 0: one ping-pong pair, k = 1: two pairs).
-// Each length is captured once into a HIP graph and replayed; the capture
-// holds kernel launches only.  Without graph support the kernels are
-// launched directly.
-hipError_t MgSolver::launch_chunk(int k) {
+// n = 1, 2 or 4 PCG iterations, the first of parity c.  Each sequence is
+// captured once into a HIP graph and replayed; the capture holds kernel
+// launches only.  Without graph support the kernels are launched directly.
+hipError_t MgSolver::launch_iters(int n, int c) {
   hipError_t err = hipSuccess;
-  const int pairs = 1 << k;
+  const int k = 2 * (n == 4 ? 2 : n == 2 ? 1 : 0) + c;
   if (use_graph_ && !graph_exec_[k]) {
     if (!chk(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed), err)) {
-      for (int i = 0; i < pairs; i++) {
-        (void)iteration(0);
-        (void)iteration(1);
-      }
+      for (int i = 0; i < n; i++) (void)iteration((c + i) & 1);
       hipGraph_t g = nullptr;
       chk(hipStreamEndCapture(stream_, &g), err);
       if (err == hipSuccess &&
@@ -1031,10 +1033,7 @@ hipError_t MgSolver::launch_chunk(int k) {
     }
   }
   if (use_graph_) return hipGraphLaunch(graph_exec_[k], stream_);
-  for (int i = 0; i < pairs && err == hipSuccess; i++) {
-    chk(iteration(0), err);
-    chk(iteration(1), err);
-  }
+  for (int i = 0; i < n && err == hipSuccess; i++) chk(iteration((c + i) & 1), err);
   return err;
 }
 
@@ -1131,34 +1130,46 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
     // The device stops the pass at its tolerance or after `cap` iterations;
     // the host only decides how much work to queue.  It follows every
     // convergence test through the mirror and tops the queue up while at
-    // most one untested iteration is left, unless the contraction rate of
-    // the last tests puts the stop within the queued iterations.  Queued
-    // iterations past the stop still cost their launches (their kernels
-    // return at once), so the queue is kept short near the predicted end.
+    // most one untested iteration is left, by the iterations that the
+    // contraction rate still predicts (before this pass has a rate of its
+    // own, the rate of the same pass of the previous solve).  Queued iterations past the stop still cost their
+    // launches (their kernels return at once), so the queue holds no more
+    // than the prediction asks for.
     const int cap = max_it - total;
-    const double stop2 = irh->rr * stop_rel * stop_rel;
+    const double rr0 = irh->rr, stop2 = rr0 * stop_rel * stop_rel;
     int launched = 0, it = 0, done = 0;
-    double rr = 0.0;
+    double rr = rr0;
     std::vector<double> hist;  // r.r after each test
-    auto launch = [&](int k) {
-      chk(launch_chunk(k), err);
-      launched += 2 << k;
+    auto predict = [&](double rho) {  // iterations from r.r = rr to the stop
+      if (!(rho > 0.0 && rho < 1.0) || rr <= stop2) return 8;
+      return std::max(1, (int)ceil(log(stop2 / rr) / log(rho)));
     };
-    launch(cap > 2 ? 1 : 0);
+    auto launch = [&](int want) {  // queue `want` iterations (1, 2 or 4 per graph)
+      want = std::max(1, std::min(want, cap + 1 - launched));
+      while (want > 0 && err == hipSuccess) {
+        const int n = want >= 4 ? 4 : want >= 2 ? 2 : 1;
+        chk(launch_iters(n, launched & 1), err);
+        launched += n;
+        want -= n;
+      }
+    };
+    const double rho_last = rho_pass_[std::min(passes, 3)];
+    launch(std::min(4, predict(rho_last)));
     while (!done && err == hipSuccess) {
       chk(wait_progress(it + 1, &it, &done, &rr), err);
       if (done || err != hipSuccess) break;
       hist.resize(it, rr);  // tests the host did not see take the latest r.r
       hist[it - 1] = rr;
       const int pending = launched - it;  // iterations not yet tested
-      if (pending > 1 || launched >= cap) continue;
-      int more = 8;  // iterations still needed (unknown: a full chunk)
-      if (it >= 4 && rr > stop2 && hist[it - 4] > rr) {
-        const double rho = pow(rr / hist[it - 4], 1.0 / 3.0);
-        if (rho > 0.0 && rho < 1.0) more = (int)ceil(log(stop2 / rr) / log(rho));
-      }
-      if (more > pending) launch(more - pending > 2 ? 1 : 0);
+      if (pending > 1 || launched > cap) continue;
+      // rate over the last three tests, else since the pass started
+      double rho = it >= 4 && hist[it - 4] > rr ? pow(rr / hist[it - 4], 1.0 / 3.0)
+                                                : pow(rr / rr0, 1.0 / it);
+      if (it < 4 && rho_last > 0.0) rho = std::min(rho, rho_last);
+      const int more = predict(rho);
+      if (more > pending) launch(std::min(more - pending, 8));
     }
+    if (it > 0 && rr > 0.0 && rr < rr0) rho_pass_[std::min(passes, 3)] = pow(rr / rr0, 1.0 / it);
     chk(hipStreamSynchronize(s), err);
     sc_host_->done = done;
     sc_host_->iters = it;
@@ -1222,9 +1233,9 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   auto once = [&](hipEvent_t a, hipEvent_t b) {
     switch (which) {
       case 0:
-        PFSF_TZ_DISPATCH(F, ext(k_spmv_dir<TZV, float>, G, a, b, view<float>(F), tl(F),
-                                (const float*)z_, (const float*)p_[0], p_[1], q_, sc_,
-                                (const double*)prz_, G, 0, ppq_));
+        PFSF_TZ_DISPATCH(F, ext(ks_spmv<TZV>, G, a, b, view<float>(F), tl(F), (const float*)z_,
+                                (const float*)p_[0], p_[1], q_, sc_, (const double*)prz_, G, 0,
+                                ppq_));
         break;
       case 1:
         if (nlev_ == 1) {
@@ -1232,7 +1243,7 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
           ext(k_update_jacobi, nb, a, b, n0_, (const float*)F.dg, x_, (const float*)r_[0], r_[1],
               (const float*)p_[1], (const float*)q_, z_, (const PcgScalars*)sc_, prr_);
         } else {
-          PFSF_TZ_DISPATCH(F, ext(k_down<TZV, float, true>, G, a, b, view<float>(F), tl(F),
+          PFSF_TZ_DISPATCH(F, ext(ks_down<TZV, true>, G, a, b, view<float>(F), tl(F),
                                   levels_[1].d, (const float*)r_[0], r_[1], (const float*)q_,
                                   (const float*)p_[1], x_, levels_[1].r, sc_,
                                   (const double*)ppq_, G, prr_));
@@ -1257,9 +1268,9 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
         break;
       }
       default:
-        PFSF_TZ_DISPATCH(F, ext(k_up<TZV, float, true>, G, a, b, view<float>(F), tl(F),
-                                levels_[1].d, (const float*)r_[1], (const float*)levels_[1].e,
-                                z_, (const PcgScalars*)sc_, prz_));
+        PFSF_TZ_DISPATCH(F, ext(ks_up<TZV, true>, G, a, b, view<float>(F), tl(F), levels_[1].d,
+                                (const float*)r_[1], (const float*)levels_[1].e, z_,
+                                (const PcgScalars*)sc_, prz_));
     }
   };
   once(nullptr, nullptr);  // warm

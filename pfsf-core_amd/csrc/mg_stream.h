@@ -1,0 +1,464 @@
+// Plane-streaming kernels of the finest level (single-system solver).
+//
+// A work-group owns a 32 x 8 column of the finest grid over TZ planes and
+// walks it plane by plane.  The staged value and the free_s coding of kappa
+// of each plane (with its one-cell halo ring, 34 x 10 cells) go into a ring
+// of four LDS planes; the loads of the next PD planes are in flight while the
+// current plane's stencil runs, so memory traffic, LDS work and stores
+// overlap inside one work-group instead of running as one load phase, one
+// compute phase and one store phase over the whole grid.  Each z-halo plane
+// is loaded once per column rather than once per 32 x 8 x TZ tile.
+//
+// The arithmetic is that of spmv_body / down_body / up_body (same operand
+// order in the stencil), so the vector outputs equal the tiled kernels' bit
+// for bit; only the per-work-group partial sums cover other cell sets.
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "mg_kernels.h"
+
+namespace pfsf {
+namespace {
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>): the plane
+// loop is unrolled by construction, so every ring slot, register buffer and
+// parity test is a compile-time constant (no indexed private arrays).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+constexpr int kRing = 4;   // LDS planes: z-1, z, z+1 being read, one being written
+// planes whose loads are in flight ahead of the staged one
+#ifndef PFSF_PD
+#define PFSF_PD 2
+#endif
+constexpr int kPD = PFSF_PD;
+constexpr int kHaloCells = 2 * HX + 2 * TY;  // halo ring of one plane (84 cells)
+static_assert(kHaloCells <= BS, "one halo cell per thread at most");
+
+struct PlaneRing {
+  float v[kRing][HXY];  // staged value (0 off the free set)
+  float s[kRing][HXY];  // free_s coding of kappa
+};
+
+// Thread geometry of a plane: the interior cell (tx, ty) and, for threads
+// t < 84, one cell of the halo ring (top row, bottom row, left, right).
+struct PlaneGeom {
+  int gx, gy, hx, hy;
+  bool inxy, hown, hxy;
+  int li, lh;  // LDS offsets inside a plane
+};
+
+__device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
+  PlaneGeom g;
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  g.gx = x0 + tx;
+  g.gy = y0 + ty;
+  g.inxy = g.gx < d.lx && g.gy < d.ly;
+  g.li = (ty + 1) * HX + tx + 1;
+  int ex, ey;
+  if (t < HX) {
+    ex = t;
+    ey = 0;
+  } else if (t < 2 * HX) {
+    ex = t - HX;
+    ey = HY - 1;
+  } else if (t < 2 * HX + TY) {
+    ex = 0;
+    ey = t - 2 * HX + 1;
+  } else {
+    ex = HX - 1;
+    ey = t - 2 * HX - TY + 1;
+  }
+  g.hown = t < kHaloCells;
+  g.hx = x0 + ex - 1;
+  g.hy = y0 + ey - 1;
+  g.hxy = g.hown && g.hx >= 0 && g.hx < d.lx && g.hy >= 0 && g.hy < d.ly;
+  g.lh = ey * HX + ex;
+  return g;
+}
+
+// (A v) at offset c of plane slot s0 with neighbour planes sm (z-1) and sp
+// (z+1): the operand order of stencil_apply, so the FP32 result is the same.
+__device__ __forceinline__ float stencil3(const PlaneRing& R, int sm, int s0, int sp, int c,
+                                          float mi, float scale) {
+  const float* vm = R.v[sm];
+  const float* v0 = R.v[s0];
+  const float* vp = R.v[sp];
+  const float* km = R.s[sm];
+  const float* k0 = R.s[s0];
+  const float* kp = R.s[sp];
+  const float vc = v0[c], si = k0[c];
+  const float ki = si * si;
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  auto face = [&](const float* V, const float* S, int o) {
+    const float sj = S[c + o];
+    const float w = is_free_s(sj) ? fmaf(sj, sj, ki) : 0.f;
+    af += w * (vc - V[c + o]);
+  };
+  auto edge = [&](const float* V, const float* S, int o) { ae += S[c + o] * (vc - V[c + o]); };
+  auto corner = [&](const float* V, const float* S, int o) { ac += S[c + o] * (vc - V[c + o]); };
+  face(v0, k0, -1);
+  face(v0, k0, 1);
+  face(v0, k0, -HX);
+  face(v0, k0, HX);
+  face(vm, km, 0);
+  face(vp, kp, 0);
+  edge(v0, k0, -1 - HX);
+  edge(v0, k0, 1 - HX);
+  edge(v0, k0, -1 + HX);
+  edge(v0, k0, 1 + HX);
+  edge(vm, km, -1);
+  edge(vm, km, 1);
+  edge(vp, kp, -1);
+  edge(vp, kp, 1);
+  edge(vm, km, -HX);
+  edge(vm, km, HX);
+  edge(vp, kp, -HX);
+  edge(vp, kp, HX);
+  corner(vm, km, -1 - HX);
+  corner(vm, km, 1 - HX);
+  corner(vm, km, -1 + HX);
+  corner(vm, km, 1 + HX);
+  corner(vp, kp, -1 - HX);
+  corner(vp, kp, 1 - HX);
+  corner(vp, kp, -1 + HX);
+  corner(vp, kp, 1 + HX);
+  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+}
+
+// Flat indices of a plane's two cells (cell 0 stands in for a cell outside
+// the grid; its values are never used).
+struct PlaneIdx {
+  int i, h;
+  bool iv, hv;
+};
+__device__ __forceinline__ PlaneIdx plane_idx(const PlaneGeom& g, int z, pfsf_dims d) {
+  PlaneIdx p;
+  const bool zin = z >= 0 && z < d.lz;
+  p.iv = g.inxy && zin;
+  p.hv = g.hxy && zin;
+  p.i = p.iv ? flat32(g.gx, g.gy, z, d) : 0;
+  p.h = p.hv ? flat32(g.hx, g.hy, z, d) : 0;
+  return p;
+}
+
+__device__ __forceinline__ void ring_put(PlaneRing& R, int slot, int off, float k, float v) {
+  R.v[slot][off] = k >= 0.f ? v : 0.f;
+  R.s[slot][off] = free_s(k);
+}
+
+// Restriction over the z-pair of planes (zo even, zo + 1) held by one thread
+// in `a`, then over the x and y siblings (lanes tx^1 and lane^32).
+__device__ __forceinline__ void restrict_pair(float a, int tx, int ty, int gx, int gy, int zo,
+                                              pfsf_dims c, float* rc) {
+  a = a + __shfl_xor(a, 1, 64);
+  a = a + __shfl_xor(a, 32, 64);
+  const int X = gx >> 1, Y = gy >> 1, Z = zo >> 1;
+  if (!(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly && Z < c.lz) st(rc, flat32(X, Y, Z, c), a);
+}
+
+// ── p_out = z + beta p_in; q = A p_out; p.q (see spmv_body) ─────────────
+struct SpRaw {
+  PlaneIdx ix;
+  float k, z, p, am, hk, hz, hp;
+};
+
+template <int TZ>
+__device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
+                                        const float* z, const float* pin) {
+  R.ix = plane_idx(g, zpl, L.d);
+  R.k = ld(L.kap, R.ix.i);
+  R.z = ld(z, R.ix.i);
+  R.p = ld(pin, R.ix.i);
+  R.am = ld(L.am, R.ix.i);
+  float hk = -1.f, hz = 0.f, hp = 0.f;  // locals keep the struct in registers
+  if (g.hown) {
+    hk = ld(L.kap, R.ix.h);
+    hz = ld(z, R.ix.h);
+    hp = ld(pin, R.ix.h);
+  }
+  R.hk = hk;
+  R.hz = hz;
+  R.hp = hp;
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(BS)
+    ks_spmv(Lv<float> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
+            float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
+            const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
+  if (sc->done) return;
+  __shared__ PlaneRing R;
+  __shared__ double sh[1][BS / 64];
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  const bool first = !sc->have_dir;
+  const double rz_old = sc->rzs[slot ^ 1];
+  const double rz = block_total(rz_part, rz_n);
+  const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rzs[slot] = rz;
+    sc->rz = rz;
+    sc->beta = beta64;
+  }
+  double acc = 0.0;
+  if (have) {
+    const float beta = (float)beta64;
+    const PlaneGeom g = plane_geom(x0, y0, L.d);
+    SpRaw buf[kPD];
+    static_for<kPD>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
+    });
+    float pv_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
+    bool fr_own[2] = {false, false};
+    PlaneIdx ix_own[2];
+    static_for<TZ + 2>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      const SpRaw cur = buf[s % kPD];
+      if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, z, pin);
+      const int sl = s % kRing;
+      const float ki = cur.ix.iv ? cur.k : -1.f;
+      const float pv = ki >= 0.f ? cur.z + beta * cur.p : 0.f;
+      ring_put(R, sl, g.li, ki, pv);
+      if (g.hown) {
+        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        ring_put(R, sl, g.lh, kh, cur.hz + beta * cur.hp);
+      }
+      if constexpr (s >= 1 && s <= TZ) {
+        if (cur.ix.iv) st(pout, cur.ix.i, pv);
+        pv_own[s & 1] = pv;
+        am_own[s & 1] = cur.am;
+        fr_own[s & 1] = ki >= 0.f;
+        ix_own[s & 1] = cur.ix;
+      }
+      __syncthreads();
+      if constexpr (s >= 2) {
+        const int o = (s - 1) & 1;
+        float qi = stencil3(R, (s - 2) % kRing, (s - 1) % kRing, sl, g.li, am_own[o], L.scale);
+        qi = fr_own[o] ? qi : 0.f;
+        acc += (double)pv_own[o] * qi;
+        if (ix_own[o].iv) st(q, ix_own[o].i, qi);
+      }
+    });
+  }
+  write_partial_at(acc, partials, sh, blockIdx.x);
+}
+
+// ── down-sweep of the finest level (see down_body) ─────────────────────
+// With UPDATE: x64 += alpha p, r_out = r_in - alpha q (stored as each owned
+// plane is staged), r.r partials; then x = omega D^-1 r_out and the coarse
+// rhs P^T (r_out - A x).
+struct DnRaw {
+  PlaneIdx ix;
+  float k, r, q, dg, am, p, hk, hr, hq, hdg;
+  double x;
+};
+
+template <int TZ, bool UPDATE>
+__device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
+                                        const float* rin, const float* q, const float* p,
+                                        const double* x) {
+  R.ix = plane_idx(g, zpl, L.d);
+  R.k = ld(L.kap, R.ix.i);
+  R.r = ld(rin, R.ix.i);
+  R.q = UPDATE ? ld(q, R.ix.i) : 0.f;
+  R.dg = ld(L.dg, R.ix.i);
+  R.am = ld(L.am, R.ix.i);
+  R.p = UPDATE ? ld(p, R.ix.i) : 0.f;
+  R.x = UPDATE ? ld(x, R.ix.i) : 0.0;
+  float hk = -1.f, hr = 0.f, hq = 0.f, hdg = 1.f;  // locals keep the struct in registers
+  if (g.hown) {
+    hk = ld(L.kap, R.ix.h);
+    hr = ld(rin, R.ix.h);
+    hq = UPDATE ? ld(q, R.ix.h) : 0.f;
+    hdg = ld(L.dg, R.ix.h);
+  }
+  R.hk = hk;
+  R.hr = hr;
+  R.hq = hq;
+  R.hdg = hdg;
+}
+
+template <int TZ, bool UPDATE>
+__global__ void __launch_bounds__(BS)
+    ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
+            float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
+            double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
+            const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
+  if (sc->done) return;
+  __shared__ PlaneRing R;
+  __shared__ double sh[1][BS / 64];
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  double alpha64 = 0.0;
+  if (UPDATE) {
+    const double rz = sc->rz;
+    const double pq = block_total(pq_part, pq_n);
+    const bool ok = pq > 0.0;
+    alpha64 = ok ? rz / pq : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = alpha64;
+      sc->have_dir = 1;
+      if (!ok) sc->done = 2;  // breakdown: the direction carries no energy
+    }
+    if (!ok) return;
+  }
+  double acc = 0.0;
+  if (have) {
+    const float alpha = (float)alpha64, om = L.omega;
+    const PlaneGeom g = plane_geom(x0, y0, L.d);
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    DnRaw buf[kPD];
+    static_for<kPD>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
+    });
+    float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
+    bool fr_own[2] = {false, false};
+    float pair = 0.f;
+    static_for<TZ + 2>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      const DnRaw cur = buf[s % kPD];
+      if constexpr (s + kPD < TZ + 2)
+        dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, L, rin, q, p, x);
+      const int sl = s % kRing;
+      const float ki = cur.ix.iv ? cur.k : -1.f;
+      const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
+      ring_put(R, sl, g.li, ki, om * ri * rcp(cur.dg));
+      if (g.hown) {
+        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        const float rh = UPDATE ? cur.hr - alpha * cur.hq : cur.hr;
+        ring_put(R, sl, g.lh, kh, om * rh * rcp(cur.hdg));
+      }
+      if constexpr (s >= 1 && s <= TZ) {
+        const bool fr = ki >= 0.f;
+        if (UPDATE) {
+          // p and q are zero outside the free set, so the update leaves x
+          // there unchanged and carries r there over
+          if (cur.ix.iv) {
+            st(x, cur.ix.i, cur.x + alpha64 * (double)cur.p);
+            st(rout, cur.ix.i, ri);
+          }
+          acc += fr ? (double)ri * ri : 0.0;
+        }
+        r_own[s & 1] = ri;
+        am_own[s & 1] = cur.am;
+        fr_own[s & 1] = fr;
+      }
+      __syncthreads();
+      if constexpr (s >= 2) {
+        const int o = (s - 1) & 1;
+        const float ax =
+            stencil3(R, (s - 2) % kRing, (s - 1) % kRing, sl, g.li, am_own[o], L.scale);
+        const float res = fr_own[o] ? r_own[o] - ax : 0.f;
+        const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
+        if constexpr ((s & 1) == 0) {
+          pair = res;
+        } else {
+          restrict_pair(pair + res, tx, ty, g.gx, g.gy, zo - 1, c, rc);
+        }
+      }
+    });
+  }
+  if (UPDATE) write_partial_at(acc, partials, sh, blockIdx.x);
+}
+
+// ── up-sweep of the finest level (see up_body) ─────────────────────────
+// x~ = omega D^-1 r + P e_coarse on the free set; out = x~ + omega D^-1
+// (r - A x~); with FINEST the r.z partials.
+struct UpRaw {
+  PlaneIdx ix;
+  float k, r, dg, am, ec, hk, hr, hdg, hec;
+};
+
+template <int TZ>
+__device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
+                                        pfsf_dims c, const float* r, const float* ec) {
+  R.ix = plane_idx(g, zpl, L.d);
+  const int zc = (R.ix.iv || R.ix.hv) ? zpl >> 1 : 0;
+  R.k = ld(L.kap, R.ix.i);
+  R.r = ld(r, R.ix.i);
+  R.dg = ld(L.dg, R.ix.i);
+  R.am = ld(L.am, R.ix.i);
+  R.ec = ld(ec, R.ix.iv ? flat32(g.gx >> 1, g.gy >> 1, zc, c) : 0);
+  float hk = -1.f, hr = 0.f, hdg = 1.f, hec = 0.f;  // locals keep the struct in registers
+  if (g.hown) {
+    hk = ld(L.kap, R.ix.h);
+    hr = ld(r, R.ix.h);
+    hdg = ld(L.dg, R.ix.h);
+    hec = ld(ec, R.ix.hv ? flat32(g.hx >> 1, g.hy >> 1, zc, c) : 0);
+  }
+  R.hk = hk;
+  R.hr = hr;
+  R.hdg = hdg;
+  R.hec = hec;
+}
+
+template <int TZ, bool FINEST>
+__global__ void __launch_bounds__(BS)
+    ks_up(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ r,
+          const float* __restrict__ ec, float* __restrict__ out, const PcgScalars* sc,
+          double* partials) {
+  if (sc->done) return;
+  __shared__ PlaneRing R;
+  __shared__ double sh[1][BS / 64];
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  double acc = 0.0;
+  if (have) {
+    const float om = L.omega;
+    const PlaneGeom g = plane_geom(x0, y0, L.d);
+    UpRaw buf[kPD];
+    static_for<kPD>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
+    });
+    float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
+    bool fr_own[2] = {false, false};
+    PlaneIdx ix_own[2];
+    static_for<TZ + 2>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      const UpRaw cur = buf[s % kPD];
+      if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, c, r, ec);
+      const int sl = s % kRing;
+      const float ki = cur.ix.iv ? cur.k : -1.f;
+      ring_put(R, sl, g.li, ki, om * cur.r * rcp(cur.dg) + cur.ec);
+      if (g.hown) {
+        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        ring_put(R, sl, g.lh, kh, om * cur.hr * rcp(cur.hdg) + cur.hec);
+      }
+      if constexpr (s >= 1 && s <= TZ) {
+        r_own[s & 1] = cur.r;
+        am_own[s & 1] = cur.am;
+        dg_own[s & 1] = cur.dg;
+        fr_own[s & 1] = ki >= 0.f;
+        ix_own[s & 1] = cur.ix;
+      }
+      __syncthreads();
+      if constexpr (s >= 2) {
+        const int o = (s - 1) & 1;
+        const int s0 = (s - 1) % kRing;
+        const float ax = stencil3(R, (s - 2) % kRing, s0, sl, g.li, am_own[o], L.scale);
+        const float xi = R.v[s0][g.li];
+        float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
+        zi = fr_own[o] ? zi : 0.f;
+        if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
+        if (ix_own[o].iv) st(out, ix_own[o].i, zi);
+      }
+    });
+  }
+  if (FINEST) write_partial_at(acc, partials, sh, blockIdx.x);
+}
+
+}  // namespace
+}  // namespace pfsf
