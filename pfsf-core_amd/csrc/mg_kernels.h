@@ -746,7 +746,8 @@ __device__ __forceinline__ void true_residual_tile(
   __shared__ double sh[1][BS / 64];
   double acc = 0.0;
   if (have) {
-    constexpr int NROWS = Stage<TZ>::NROWS, RPP = Stage<TZ>::RPP, NP = Stage<TZ>::NP;
+    constexpr int NROWS = Tile<TZ>::NROWS, RPP = BS / TX;
+    constexpr int NP = (NROWS + RPP - 1) / RPP + 1;
     const int t = threadIdx.x;
     auto cell = [&](int e, int gx, int gy, int gz, bool ok) {
       const int j = ok ? flat32(gx, gy, gz, d) : 0;
@@ -768,8 +769,8 @@ __device__ __forceinline__ void true_residual_tile(
         cell(row * HX + tx + 1, gx, gy, gz,
              gx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
     }
-    const int hrow = t >> 1, ex = (t & 1) ? HX - 1 : 0;
-    if (hrow < NROWS) {
+    const int ex = (t & 1) ? HX - 1 : 0;
+    for (int hrow = t >> 1; hrow < NROWS; hrow += BS / 2) {  // the two halo columns
       const int ey = hrow % HY, ez = hrow / HY;
       const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
       cell(hrow * HX + ex, hx, gy, gz,

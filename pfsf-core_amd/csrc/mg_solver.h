@@ -111,8 +111,8 @@ class MgSolver {
   void zero_vectors();
   hipError_t build();
   hipError_t build_tiles(MgLevel& L);
-  hipError_t precondition();            // z = M r[0], r.z
-  hipError_t true_residual(const float* b, double* rel);
+  hipError_t precondition(int c);       // z = M r[c], r.z
+  hipError_t true_residual(const float* b, double* rel, int c);  // b - A x into r[c]
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
   hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
   // n = 1, 2 or 4 iterations starting at parity c (graph replay once captured)

@@ -2,12 +2,15 @@
 //
 // A work-group owns a 32 x 8 column of the finest grid over TZ planes and
 // walks it plane by plane.  The staged value and the free_s coding of kappa
-// of each plane (with its one-cell halo ring, 34 x 10 cells) go into a ring
-// of four LDS planes; the loads of the next PD planes are in flight while the
-// current plane's stencil runs, so memory traffic, LDS work and stores
-// overlap inside one work-group instead of running as one load phase, one
-// compute phase and one store phase over the whole grid.  Each z-halo plane
-// is loaded once per column rather than once per 32 x 8 x TZ tile.
+// of each plane (with its one-cell halo ring, 34 x 10 cells) go into one of
+// two LDS plane slots; every thread then copies its 3 x 3 neighbourhood of
+// that plane into a register window holding the last three planes, so each
+// output reads 18 LDS words instead of 52.  The loads of the next PD planes
+// are in flight while the current plane's stencil runs, so memory traffic,
+// LDS work and stores overlap inside one work-group instead of running as
+// one load phase, one compute phase and one store phase over the whole grid.
+// Each z-halo plane is loaded once per column rather than once per
+// 32 x 8 x TZ tile.
 //
 // The arithmetic is that of spmv_body / down_body / up_body (same operand
 // order in the stencil), so the vector outputs equal the tiled kernels' bit
@@ -34,7 +37,7 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-constexpr int kRing = 4;   // LDS planes: z-1, z, z+1 being read, one being written
+constexpr int kRing = 2;   // LDS plane slots: one read into registers, one written
 // planes whose loads are in flight ahead of the staged one
 #ifndef PFSF_PD
 #define PFSF_PD 2
@@ -85,52 +88,61 @@ __device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
   return g;
 }
 
-// (A v) at offset c of plane slot s0 with neighbour planes sm (z-1) and sp
-// (z+1): the operand order of stencil_apply, so the FP32 result is the same.
-__device__ __forceinline__ float stencil3(const PlaneRing& R, int sm, int s0, int sp, int c,
-                                          float mi, float scale) {
-  const float* vm = R.v[sm];
-  const float* v0 = R.v[s0];
-  const float* vp = R.v[sp];
-  const float* km = R.s[sm];
-  const float* k0 = R.s[s0];
-  const float* kp = R.s[sp];
-  const float vc = v0[c], si = k0[c];
+// A thread's 3 x 3 neighbourhood of one plane, index (dy + 1) * 3 + (dx + 1).
+struct Win {
+  float v[9], s[9];
+};
+
+__device__ __forceinline__ void read_win(const PlaneRing& R, int slot, int c, Win& w) {
+#pragma unroll
+  for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+      w.v[(dy + 1) * 3 + dx + 1] = R.v[slot][c + dy * HX + dx];
+      w.s[(dy + 1) * 3 + dx + 1] = R.s[slot][c + dy * HX + dx];
+    }
+}
+
+// (A v) at the centre of plane window W0 with neighbour planes Wm (z-1) and
+// Wp (z+1): the operand order of stencil_apply, so the FP32 result is the same.
+__device__ __forceinline__ float stencil_win(const Win& Wm, const Win& W0, const Win& Wp, float mi,
+                                             float scale) {
+  const float vc = W0.v[4], si = W0.s[4];
   const float ki = si * si;
   float af = 0.f, ae = 0.f, ac = 0.f;
-  auto face = [&](const float* V, const float* S, int o) {
-    const float sj = S[c + o];
+  auto face = [&](const Win& W, int k) {
+    const float sj = W.s[k];
     const float w = is_free_s(sj) ? fmaf(sj, sj, ki) : 0.f;
-    af += w * (vc - V[c + o]);
+    af += w * (vc - W.v[k]);
   };
-  auto edge = [&](const float* V, const float* S, int o) { ae += S[c + o] * (vc - V[c + o]); };
-  auto corner = [&](const float* V, const float* S, int o) { ac += S[c + o] * (vc - V[c + o]); };
-  face(v0, k0, -1);
-  face(v0, k0, 1);
-  face(v0, k0, -HX);
-  face(v0, k0, HX);
-  face(vm, km, 0);
-  face(vp, kp, 0);
-  edge(v0, k0, -1 - HX);
-  edge(v0, k0, 1 - HX);
-  edge(v0, k0, -1 + HX);
-  edge(v0, k0, 1 + HX);
-  edge(vm, km, -1);
-  edge(vm, km, 1);
-  edge(vp, kp, -1);
-  edge(vp, kp, 1);
-  edge(vm, km, -HX);
-  edge(vm, km, HX);
-  edge(vp, kp, -HX);
-  edge(vp, kp, HX);
-  corner(vm, km, -1 - HX);
-  corner(vm, km, 1 - HX);
-  corner(vm, km, -1 + HX);
-  corner(vm, km, 1 + HX);
-  corner(vp, kp, -1 - HX);
-  corner(vp, kp, 1 - HX);
-  corner(vp, kp, -1 + HX);
-  corner(vp, kp, 1 + HX);
+  auto edge = [&](const Win& W, int k) { ae += W.s[k] * (vc - W.v[k]); };
+  auto corner = [&](const Win& W, int k) { ac += W.s[k] * (vc - W.v[k]); };
+  face(W0, 3);  // -x
+  face(W0, 5);  // +x
+  face(W0, 1);  // -y
+  face(W0, 7);  // +y
+  face(Wm, 4);  // -z
+  face(Wp, 4);  // +z
+  edge(W0, 0);
+  edge(W0, 2);
+  edge(W0, 6);
+  edge(W0, 8);
+  edge(Wm, 3);
+  edge(Wm, 5);
+  edge(Wp, 3);
+  edge(Wp, 5);
+  edge(Wm, 1);
+  edge(Wm, 7);
+  edge(Wp, 1);
+  edge(Wp, 7);
+  corner(Wm, 0);
+  corner(Wm, 2);
+  corner(Wm, 6);
+  corner(Wm, 8);
+  corner(Wp, 0);
+  corner(Wp, 2);
+  corner(Wp, 6);
+  corner(Wp, 8);
   return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
 }
 
@@ -221,6 +233,7 @@ __global__ void __launch_bounds__(BS)
     float pv_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
+    Win W[3];
     static_for<TZ + 2>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       const SpRaw cur = buf[s % kPD];
@@ -241,9 +254,10 @@ __global__ void __launch_bounds__(BS)
         ix_own[s & 1] = cur.ix;
       }
       __syncthreads();
+      read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        float qi = stencil3(R, (s - 2) % kRing, (s - 1) % kRing, sl, g.li, am_own[o], L.scale);
+        float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
         qi = fr_own[o] ? qi : 0.f;
         acc += (double)pv_own[o] * qi;
         if (ix_own[o].iv) st(q, ix_own[o].i, qi);
@@ -326,6 +340,7 @@ __global__ void __launch_bounds__(BS)
     float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     float pair = 0.f;
+    Win W[3];
     static_for<TZ + 2>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       const DnRaw cur = buf[s % kPD];
@@ -356,10 +371,10 @@ __global__ void __launch_bounds__(BS)
         fr_own[s & 1] = fr;
       }
       __syncthreads();
+      read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        const float ax =
-            stencil3(R, (s - 2) % kRing, (s - 1) % kRing, sl, g.li, am_own[o], L.scale);
+        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
         const float res = fr_own[o] ? r_own[o] - ax : 0.f;
         const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
         if constexpr ((s & 1) == 0) {
@@ -426,6 +441,7 @@ __global__ void __launch_bounds__(BS)
     float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
+    Win W[3];
     static_for<TZ + 2>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       const UpRaw cur = buf[s % kPD];
@@ -445,11 +461,11 @@ __global__ void __launch_bounds__(BS)
         ix_own[s & 1] = cur.ix;
       }
       __syncthreads();
+      read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        const int s0 = (s - 1) % kRing;
-        const float ax = stencil3(R, (s - 2) % kRing, s0, sl, g.li, am_own[o], L.scale);
-        const float xi = R.v[s0][g.li];
+        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
+        const float xi = W[(s - 1) % 3].v[4];
         float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
         zi = fr_own[o] ? zi : 0.f;
         if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
