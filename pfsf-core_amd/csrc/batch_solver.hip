@@ -21,6 +21,7 @@
 
 #include "batch_solver.h"
 #include "mg_kernels.h"
+#include "mg_stream.h"
 
 namespace pfsf {
 namespace {
@@ -47,20 +48,6 @@ __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
 }
 
 // ── batched stencil launches (one work-group per tile record) ──────────
-template <int TZ>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
-    kb_spmv_dir(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
-                const BatchIsland* __restrict__ isl, PcgScalars* sc,
-                const double* __restrict__ rz_part, double* pq_part, int c) {
-  const BatchTile t = tiles[blockIdx.x];
-  if (t.level < 0) return;
-  const BatchLevel& D = lv[t.level];
-  const BatchIsland& I = isl[D.island];
-  spmv_body<TZ, float>(lv_of(D), true, t.x0, t.y0, t.z0, I.z, c ? I.p1 : I.p0,
-                       c ? I.p0 : I.p1, I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c,
-                       t.slot == I.tb, pq_part, (unsigned)t.slot);
-}
-
 // UPDATE (finest level, iteration c): r[c] -> r[1-c] with p[1-c], q.
 // Otherwise the down-sweep of the level's own right-hand side.
 template <int TZ, bool UPDATE, bool MASK = false>
@@ -94,6 +81,51 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
   // every level below the finest carries a coupling mask
   up_body<TZ, float, FINEST, !FINEST>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
                                       sc + D.island, rz_part, (unsigned)t.slot);
+}
+
+// Finest level: the plane-streaming bodies (one 32 x 8 x kTZFine column per
+// tile record): the SpMV with the new direction, the update + down-sweep
+// and the up-sweep with r.z.
+template <int TZ>
+__global__ void __launch_bounds__(BS)
+    kbs_spmv(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+             const BatchIsland* __restrict__ isl, PcgScalars* sc,
+             const double* __restrict__ rz_part, double* pq_part, int c) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  spmv_stream_body<TZ>(lv_of(D), true, t.x0, t.y0, t.z0, I.z, c ? I.p1 : I.p0, c ? I.p0 : I.p1,
+                       I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c, t.slot == I.tb,
+                       pq_part, (unsigned)t.slot);
+}
+
+template <int TZ, bool UPDATE>
+__global__ void __launch_bounds__(BS)
+    kbs_down(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+             const BatchIsland* __restrict__ isl, PcgScalars* sc,
+             const double* __restrict__ pq_part, double* rr_part, int c) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
+  float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
+  down_stream_body<TZ, UPDATE>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
+                               c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
+                               (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(BS)
+    kbs_up(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+           const BatchIsland* __restrict__ isl, const PcgScalars* sc, double* rz_part, int ridx) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const BatchLevel& D = lv[t.level];
+  const BatchIsland& I = isl[D.island];
+  up_stream_body<TZ, true>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, ridx ? I.r1 : I.r0, D.ec, I.z,
+                           sc + D.island, rz_part, (unsigned)t.slot);
 }
 
 // Every island's coarsest level, one work-group each; with `check` the
@@ -550,10 +582,10 @@ hipError_t BatchSolver::vcycle_coarse(bool check) {
 hipError_t BatchSolver::precondition() {
   hipStream_t s = stream_;
   if (ntiles_[0] == 0) return hipSuccess;
-  kb_down<kTZFine, false><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, nullptr,
-                                                    nullptr, 0);
+  kbs_down<kTZFine, false><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, nullptr,
+                                                     nullptr, 0);
   (void)vcycle_coarse(false);
-  kb_up<kTZFine, true><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 0);
+  kbs_up<kTZFine><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 0);
   return hipGetLastError();
 }
 
@@ -562,10 +594,10 @@ hipError_t BatchSolver::iteration(int c) {
   hipStream_t s = stream_;
   if (ntiles_[0] == 0) return hipSuccess;
   const unsigned G = ntiles_[0];
-  kb_spmv_dir<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, ppq_, c);
-  kb_down<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, ppq_, prr_, c);
+  kbs_spmv<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, ppq_, c);
+  kbs_down<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, ppq_, prr_, c);
   (void)vcycle_coarse(true);
-  kb_up<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 1 - c);
+  kbs_up<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 1 - c);
   return hipGetLastError();
 }
 

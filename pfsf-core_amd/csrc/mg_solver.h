@@ -29,8 +29,9 @@ constexpr int kTX = 32, kTY = 8;
 // cost; 2-5 and 7-10 steps need 53-63 iterations.
 constexpr int kCoarseSweeps = 6;
 
-// Tile depths of a batchable hierarchy (finest level, coarse levels).
-constexpr int kBatchTZFine = 4, kBatchTZCoarse = 2;
+// Tile depths of a batchable hierarchy (finest level: 8-plane streaming
+// columns; coarse levels).
+constexpr int kBatchTZFine = 8, kBatchTZCoarse = 2;
 
 struct TileList {
   int tz = 8;                     // tile depth (8, 4 or 2) chosen per level

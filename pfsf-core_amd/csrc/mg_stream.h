@@ -1,4 +1,4 @@
-// Plane-streaming kernels of the finest level (single-system solver).
+// Plane-streaming kernels of the finest level (single-system and batched solver).
 //
 // A work-group owns a 32 x 8 column of the finest grid over TZ planes and
 // walks it plane by plane.  The staged value and the free_s coding of kappa
@@ -202,21 +202,22 @@ __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, c
   R.hp = hp;
 }
 
+// Column (x0, y0, z0) of one system; `publish` (one work-group per system)
+// stores the PCG scalars, the p.q partial goes to partials[pslot].
 template <int TZ>
-__global__ void __launch_bounds__(BS)
-    ks_spmv(Lv<float> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
-            float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
-            const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
+__device__ __forceinline__ void spmv_stream_body(
+    const Lv<float>& L, bool have, int x0, int y0, int z0, const float* __restrict__ z,
+    const float* __restrict__ pin, float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
+    const double* __restrict__ rz_part, unsigned rz_n, int slot, bool publish, double* partials,
+    unsigned pslot) {
   if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
   const bool first = !sc->have_dir;
   const double rz_old = sc->rzs[slot ^ 1];
   const double rz = block_total(rz_part, rz_n);
   const double beta64 = (!first && rz_old > 0.0) ? rz / rz_old : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (publish && threadIdx.x == 0) {
     sc->rzs[slot] = rz;
     sc->rz = rz;
     sc->beta = beta64;
@@ -264,7 +265,18 @@ __global__ void __launch_bounds__(BS)
       }
     });
   }
-  write_partial_at(acc, partials, sh, blockIdx.x);
+  write_partial_at(acc, partials, sh, pslot);
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(BS)
+    ks_spmv(Lv<float> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
+            float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
+            const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  spmv_stream_body<TZ>(L, have, x0, y0, z0, z, pin, pout, q, sc, rz_part, rz_n, slot,
+                       blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── down-sweep of the finest level (see down_body) ─────────────────────
@@ -303,23 +315,22 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, c
 }
 
 template <int TZ, bool UPDATE>
-__global__ void __launch_bounds__(BS)
-    ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
-            float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
-            double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
-            const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
+__device__ __forceinline__ void down_stream_body(
+    const Lv<float>& L, bool have, int x0, int y0, int z0, pfsf_dims c,
+    const float* __restrict__ rin, float* __restrict__ rout, const float* __restrict__ q,
+    const float* __restrict__ p, double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
+    const double* __restrict__ pq_part, unsigned pq_n, bool publish, double* partials,
+    unsigned pslot) {
   if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
   double alpha64 = 0.0;
   if (UPDATE) {
     const double rz = sc->rz;
     const double pq = block_total(pq_part, pq_n);
     const bool ok = pq > 0.0;
     alpha64 = ok ? rz / pq : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (publish && threadIdx.x == 0) {
       sc->pq = pq;
       sc->alpha = alpha64;
       sc->have_dir = 1;
@@ -385,7 +396,19 @@ __global__ void __launch_bounds__(BS)
       }
     });
   }
-  if (UPDATE) write_partial_at(acc, partials, sh, blockIdx.x);
+  if (UPDATE) write_partial_at(acc, partials, sh, pslot);
+}
+
+template <int TZ, bool UPDATE>
+__global__ void __launch_bounds__(BS)
+    ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
+            float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
+            double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
+            const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  down_stream_body<TZ, UPDATE>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n,
+                               blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── up-sweep of the finest level (see up_body) ─────────────────────────
@@ -420,15 +443,14 @@ __device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, c
 }
 
 template <int TZ, bool FINEST>
-__global__ void __launch_bounds__(BS)
-    ks_up(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ r,
-          const float* __restrict__ ec, float* __restrict__ out, const PcgScalars* sc,
-          double* partials) {
+__device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, int x0, int y0,
+                                               int z0, pfsf_dims c, const float* __restrict__ r,
+                                               const float* __restrict__ ec,
+                                               float* __restrict__ out, const PcgScalars* sc,
+                                               double* partials, unsigned pslot) {
   if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
   double acc = 0.0;
   if (have) {
     const float om = L.omega;
@@ -473,7 +495,17 @@ __global__ void __launch_bounds__(BS)
       }
     });
   }
-  if (FINEST) write_partial_at(acc, partials, sh, blockIdx.x);
+  if (FINEST) write_partial_at(acc, partials, sh, pslot);
+}
+
+template <int TZ, bool FINEST>
+__global__ void __launch_bounds__(BS)
+    ks_up(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ r,
+          const float* __restrict__ ec, float* __restrict__ out, const PcgScalars* sc,
+          double* partials) {
+  int x0 = 0, y0 = 0, z0 = 0;
+  const bool have = tile_origin<TZ>(M, x0, y0, z0);
+  up_stream_body<TZ, FINEST>(L, have, x0, y0, z0, c, r, ec, out, sc, partials, blockIdx.x);
 }
 
 }  // namespace
