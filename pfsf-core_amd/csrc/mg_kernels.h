@@ -736,16 +736,32 @@ struct TileR {
   float k[HN];
 };
 
+// With xzero (the iterate is known to be 0 everywhere) the residual is b on
+// the free set: the same values, without the stencil.
 template <int TZ>
 __device__ __forceinline__ void true_residual_tile(
     pfsf_dims d, bool have, int x0, int y0, int z0, const uint8_t* __restrict__ type,
     const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
     const float* __restrict__ b, const double* __restrict__ x, float* __restrict__ r32,
-    double* partials, unsigned pslot) {
+    double* partials, unsigned pslot, bool xzero = false) {
   __shared__ TileR<TZ> T;
   __shared__ double sh[1][BS / 64];
   double acc = 0.0;
-  if (have) {
+  if (have && xzero) {
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
+    const int i0 = flat32(gx, gy, z0, d);
+    if (gx < d.lx && gy < d.ly) {
+      for (int tz = 0; tz < TZ; tz++) {
+        if (z0 + tz >= d.lz) break;
+        const int i = i0 + tz * sz;
+        if (!fr[i]) continue;
+        const double ri = (double)b[i];
+        r32[i] = (float)ri;
+        acc += ri * ri;
+      }
+    }
+  } else if (have) {
     constexpr int NROWS = Tile<TZ>::NROWS, RPP = BS / TX;
     constexpr int NP = (NROWS + RPP - 1) / RPP + 1;
     const int t = threadIdx.x;

@@ -113,7 +113,7 @@ class MgSolver {
   hipError_t build();
   hipError_t build_tiles(MgLevel& L);
   hipError_t precondition(int c);       // z = M r[c], r.z
-  hipError_t true_residual(const float* b, double* rel, int c);  // b - A x into r[c]
+  hipError_t true_residual(const float* b, double* rel, int c, int seq = 0);  // b - A x into r[c]
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
   hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
   // n = 1, 2 or 4 iterations starting at parity c (graph replay once captured)
@@ -156,6 +156,8 @@ class MgSolver {
   float* p_[2] = {nullptr, nullptr};
   float *q_ = nullptr, *z_ = nullptr;
   float* coef_ = nullptr;
+  int* nz_flag_ = nullptr;  // last solve (seq) whose initial guess held a nonzero
+  int solve_seq_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // PCG iteration sequences: index 2 * log2(length) + starting parity
   hipGraph_t graph_[6] = {};

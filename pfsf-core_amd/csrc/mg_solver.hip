@@ -417,13 +417,30 @@ __global__ void __launch_bounds__(BS)
     k_true_residual_tiles(pfsf_dims d, TL M, const uint8_t* __restrict__ type,
                           const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
                           const float* __restrict__ b, const double* __restrict__ x,
-                          float* __restrict__ r32, double* partials) {
+                          float* __restrict__ r32, double* partials, const int* nz, int seq) {
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  true_residual_tile<TZ>(d, have, x0, y0, z0, type, kappa, fr, b, x, r32, partials, blockIdx.x);
+  // nz holds the number of the last solve whose initial guess had a nonzero
+  const bool xzero = nz && *nz != seq;
+  true_residual_tile<TZ>(d, have, x0, y0, z0, type, kappa, fr, b, x, r32, partials, blockIdx.x,
+                         xzero);
 }
 
 // Iterate outside the free set is zero (the caller's guess may not be).
+// The FP64 iterate from the caller's guess, 0 off the free set; a wave that
+// loads a nonzero value raises *nz to seq (the number of this solve).
+__global__ void __launch_bounds__(BS)
+    k_load_guess(int64_t n, const double* __restrict__ xin, const uint8_t* __restrict__ fr,
+                 double* __restrict__ x, int* nz, int seq) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    v = fr[i] ? xin[i] : 0.0;
+    x[i] = v;
+  }
+  if (__ballot(v != 0.0) != 0 && (threadIdx.x & 63) == 0) atomicMax(nz, seq);
+}
+
 __global__ void __launch_bounds__(BS)
     k_mask64(int64_t n, const uint8_t* __restrict__ fr, double* __restrict__ x) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
@@ -663,6 +680,8 @@ hipError_t MgSolver::allocate(bool standalone) {
     prr_ = partials_ + maxblk;
     prz_ = partials_ + 3 * maxblk;
     coef_ = alloc<float>(2 * 64, err);
+    nz_flag_ = alloc<int>(1, err);
+    if (nz_flag_) chk(hipMemsetAsync(nz_flag_, 0, sizeof(int), stream_), err);
   }
   return err;
 }
@@ -1089,12 +1108,15 @@ hipError_t MgSolver::wait_progress(int target, int* iters, int* done, double* rr
 }
 
 // True FP64 residual of x_ into r_[c]; returns ||r|| / ||b|| (synchronises).
-hipError_t MgSolver::true_residual(const float* b, double* rel, int c) {
+// With seq > 0 (the solve's first residual) an all-zero initial guess, as
+// flagged by k_load_guess, takes the residual b without the stencil.
+hipError_t MgSolver::true_residual(const float* b, double* rel, int c, int seq) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   const MgLevel& F = levels_[0];
+  const int* nz = seq > 0 ? nz_flag_ : nullptr;
   PFSF_TZF_DISPATCH(F, (k_true_residual_tiles<TZV><<<F.tiles.nlaunch, BS, 0, s>>>(
-                          d0_, tl(F), type_, kappa_, F.fr, b, x_, r_[c], partials_)));
+                          d0_, tl(F), type_, kappa_, F.fr, b, x_, r_[c], partials_, nz, seq)));
   k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, F.tiles.nlaunch, sc_, ir_, nullptr);
   IrScalars* irh = (IrScalars*)(sc_host_ + 1);
   chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);
@@ -1118,16 +1140,16 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   chk(hipEventRecord(ev0_, caller), err);  // order after the caller's stream
   chk(hipStreamWaitEvent(s, ev0_, 0), err);
   chk(hipEventRecord(ev0_, s), err);
-  if (x64) {
-    k_copy64<<<nb0, BS, 0, s>>>(n0_, x64, x_);
-    k_mask64<<<nb0, BS, 0, s>>>(n0_, F.fr, x_);
-  } else {
+  solve_seq_ = solve_seq_ % 0x7ffffffe + 1;  // never 0, the flag's initial value
+  const int seq = solve_seq_;
+  if (x64)
+    k_load_guess<<<nb0, BS, 0, s>>>(n0_, x64, F.fr, x_, nz_flag_, seq);
+  else
     k_zero64<<<nb0, BS, 0, s>>>(n0_, x_);
-  }
   k_bnorm<<<nb0, BS, 0, s>>>(n0_, F.fr, b, partials_);
   k_finalize<<<1, BS, 0, s>>>(FIN_BNORM, partials_, nb0, sc_, ir_, nullptr);
   double rel = 0.0;
-  chk(true_residual(b, &rel, 0), err);
+  chk(true_residual(b, &rel, 0, seq), err);
   int passes = 0, total = 0;
   const IrScalars* irh = (const IrScalars*)(sc_host_ + 1);
   for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
@@ -1178,7 +1200,7 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
       if (more > pending) launch(std::min(more - pending, 8));
     }
     if (it > 0 && rr > 0.0 && rr < rr0) rho_pass_[std::min(passes, 3)] = pow(rr / rr0, 1.0 / it);
-    chk(hipStreamSynchronize(s), err);
+    // the true residual is queued behind any iterations still in flight
     sc_host_->done = done;
     sc_host_->iters = it;
     total += it;
