@@ -117,43 +117,78 @@ __global__ void __launch_bounds__(BS)
   phi[i] = jacobi_voxel(i, p, d, type, kappa, src, phi, 1.0f);
 }
 
+// Failure scan over 32 x 8 x 4 tiles of the whole grid, one 256-thread
+// work-group each.  The tile and its one-voxel halo are staged in LDS (the
+// coupling kappa, -1 for air and cells outside the grid, and phi), then each
+// thread classifies its column of four voxels with the neighbour loop of the
+// per-voxel form: offsets in PFSFStencil order, the same FP64 operations in
+// the same order, so the codes are bit-identical to the restatement; only the
+// 26 neighbour loads come from LDS instead of global memory.
+constexpr int kFsTX = 32, kFsTY = 8, kFsTZ = 4;
+constexpr int kFsHX = kFsTX + 2, kFsHY = kFsTY + 2, kFsHZ = kFsTZ + 2;
+constexpr int kFsHN = kFsHX * kFsHY * kFsHZ;
+static_assert(kFsTX * kFsTY == BS, "one thread per column of a tile");
+
 __global__ void __launch_bounds__(BS)
     k_failure_scan(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
                    const float* __restrict__ phi, const float* __restrict__ rcomp,
                    const float* __restrict__ rtens, float phi_orphan, uint8_t* __restrict__ code) {
   constexpr int O[26][3] = PFSF_OFFSETS;
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i >= nvox(d)) return;
-  if (!is_solid(type[i])) {
-    code[i] = 0;
-    return;
+  __shared__ float K[kFsHN], P[kFsHN];
+  const int ntx = (d.lx + kFsTX - 1) / kFsTX, nty = (d.ly + kFsTY - 1) / kFsTY;
+  const int bt = (int)blockIdx.x;
+  const int x0 = (bt % ntx) * kFsTX, y0 = ((bt / ntx) % nty) * kFsTY,
+            z0 = (bt / (ntx * nty)) * kFsTZ;
+  for (int e = threadIdx.x; e < kFsHN; e += BS) {
+    const int ex = e % kFsHX, ey = (e / kFsHX) % kFsHY, ez = e / (kFsHX * kFsHY);
+    const int x = x0 + ex - 1, y = y0 + ey - 1, z = z0 + ez - 1;
+    const bool in = x >= 0 && x < d.lx && y >= 0 && y < d.ly && z >= 0 && z < d.lz;
+    const int64_t j = in ? flat(x, y, z, d) : 0;
+    const uint8_t tj = type[j];
+    const float kj = kappa[j];
+    const float pj = phi[j];
+    K[e] = (in && tj != 0) ? kapf(kj) : -1.0f;
+    P[e] = in ? pj : 0.0f;
   }
-  XYZ p = unflat(i, d);
-  float ki = kapf(kappa[i]);
-  double pi = (double)phi[i];
-  double diag = 0.0, fin = 0.0, fout = 0.0;
+  __syncthreads();
+  const int tx = threadIdx.x % kFsTX, ty = threadIdx.x / kFsTX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= d.lx || y >= d.ly) return;
+  for (int tz = 0; tz < kFsTZ; tz++) {
+    const int z = z0 + tz;
+    if (z >= d.lz) break;
+    const int64_t i = flat(x, y, z, d);
+    if (!is_solid(type[i])) {
+      code[i] = 0;
+      continue;
+    }
+    const int c = (tx + 1) + kFsHX * ((ty + 1) + kFsHY * (tz + 1));
+    const float ki = K[c];
+    const float phii = P[c];
+    const double pi = (double)phii;
+    double diag = 0.0, fin = 0.0, fout = 0.0;
 #pragma unroll
-  for (int k = 0; k < 26; k++) {
-    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
-    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
-    int64_t j = flat(nx, ny, nz, d);
-    if (type[j] == 0) continue;
-    double a = couple_exact(k, ki, kapf(kappa[j]));
-    diag += a;
-    double dp = (double)phi[j] - pi;
-    if (dp > 0.0)
-      fin += a * dp;
-    else if (dp < 0.0)
-      fout += a * (-dp);
+    for (int k = 0; k < 26; k++) {
+      const int o = O[k][0] + kFsHX * (O[k][1] + kFsHY * O[k][2]);
+      const float kj = K[c + o];
+      if (kj < 0.0f) continue;
+      double a = couple_exact(k, ki, kj);
+      diag += a;
+      double dp = (double)P[c + o] - pi;
+      if (dp > 0.0)
+        fin += a * dp;
+      else if (dp < 0.0)
+        fout += a * (-dp);
+    }
+    uint8_t cd = 0;
+    if (diag == 0.0 || phii > phi_orphan)
+      cd = 3;
+    else if (rcomp[i] > 0.0f && fin > (double)rcomp[i])
+      cd = 2;
+    else if (rtens[i] > 0.0f && fout > (double)rtens[i])
+      cd = 4;
+    code[i] = cd;
   }
-  uint8_t c = 0;
-  if (diag == 0.0 || phi[i] > phi_orphan)
-    c = 3;
-  else if (rcomp[i] > 0.0f && fin > (double)rcomp[i])
-    c = 2;
-  else if (rtens[i] > 0.0f && fout > (double)rtens[i])
-    c = 4;
-  code[i] = c;
 }
 
 __device__ __forceinline__ float axis_grad(const uint8_t* __restrict__ type,
@@ -433,8 +468,12 @@ hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kapp
 hipError_t launch_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
                                const float* phi, const float* rcomp, const float* rtens,
                                float phi_orphan, uint8_t* code, hipStream_t s) {
-  k_failure_scan<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, phi, rcomp, rtens,
-                                                       phi_orphan, code);
+  const int64_t tiles = (int64_t)((d.lx + kFsTX - 1) / kFsTX) * ((d.ly + kFsTY - 1) / kFsTY) *
+                        ((d.lz + kFsTZ - 1) / kFsTZ);
+  if (tiles == 0) return hipSuccess;
+  if (tiles > 0x7fffffff) return hipErrorInvalidValue;
+  k_failure_scan<<<(unsigned)tiles, BS, 0, s>>>(d, type, kappa, phi, rcomp, rtens, phi_orphan,
+                                                code);
   return hipGetLastError();
 }
 
