@@ -24,9 +24,9 @@ struct IrScalars {
 // Stencil tiles: 32 x 8 x tz voxels per 256-thread work-group.
 constexpr int kTX = 32, kTY = 8;
 
-// Chebyshev steps of the coarsest-level solve.  Measured on the 128^3 island:
-// 6 steps give the same PCG iteration count as 12 (52) at half the coarse
-// cost; 2-5 and 7-10 steps need 53-63 iterations.
+// Chebyshev steps of the coarsest-level solve (interval [lmax / 100, lmax]).
+// Measured on the 128^3 island: 6 steps need 40 PCG iterations, 8 need 41 and
+// 12 need 39 at ~0.9 us more per step and iteration.
 constexpr int kCoarseSweeps = 6;
 
 // Tile depths of a batchable hierarchy (finest level: 8-plane streaming
@@ -157,6 +157,7 @@ class MgSolver {
   float *q_ = nullptr, *z_ = nullptr;
   float* coef_ = nullptr;
   int* nz_flag_ = nullptr;  // last solve (seq) whose initial guess held a nonzero
+  int* guess_flags_ = nullptr;  // per block of the guess load: a nonzero seen
   int solve_seq_ = 0;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // PCG iteration sequences: index 2 * log2(length) + starting parity
