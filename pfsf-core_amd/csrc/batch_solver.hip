@@ -185,10 +185,13 @@ __global__ void __launch_bounds__(BS)
   const BatchTile t = tiles[blockIdx.x];
   if (t.level < 0) return;
   const BatchLevel& D = lv[t.level];
-  if (st[D.island].finished) return;
+  const BatchState& S = st[D.island];
+  if (S.finished) return;
   const BatchIsland& I = isl[D.island];
+  // before the first pass an island without an initial guess has x = 0
+  const bool xzero = S.passes == 0 && !I.xio;
   true_residual_tile<kTZFine>(I.d, true, t.x0, t.y0, t.z0, I.type, I.kappa, I.fr, I.b, I.x,
-                              I.r0, part, (unsigned)t.slot);
+                              I.r0, part, (unsigned)t.slot, xzero);
 }
 
 // Pass boundary, per island: an island at the tolerance, out of iterations
