@@ -806,40 +806,27 @@ __device__ __forceinline__ void true_residual_tile(
       bown[tz] = b[i];
     }
     __syncthreads();
-    // The column's TZ voxels advance together, neighbour by neighbour, so the
-    // FP64 accumulations form TZ independent chains instead of one long
-    // dependent chain per voxel; each voxel still sums its neighbours in the
-    // same order.  Voxels off the free set are computed and discarded.
-    double ax[TZ], xi[TZ];
-    float ki[TZ];
-    const int c0 = tcell(tx, ty, 0);
-#pragma unroll
-    for (int tz = 0; tz < TZ; tz++) {
-      ax[tz] = 0.0;
-      ki[tz] = T.k[c0 + tz * HXY];
-      xi[tz] = T.x[c0 + tz * HXY];
-    }
-    // the neighbour loop stays rolled (wave-uniform offset and class), so only
-    // one neighbour's TZ operands are live at a time
-#pragma unroll 1
-    for (int nb = 0; nb < 27; nb++) {
-      const int dx = nb % 3 - 1, dy = (nb / 3) % 3 - 1, dz = nb / 9 - 1;
-      const int cls = (dx != 0) + (dy != 0) + (dz != 0);
-      if (cls == 0) continue;
-      const int o = dx + dy * HX + dz * HXY;
-#pragma unroll
-      for (int tz = 0; tz < TZ; tz++) {
-        const int c = c0 + tz * HXY + o;
-        const float kj = T.k[c];
-        const double w = kj >= 0.f ? couple_f64_fast(cls, ki[tz], kj) : 0.0;
-        ax[tz] += w * (xi[tz] - T.x[c]);
-      }
-    }
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       if (!fown[tz]) continue;
       const int i = i0 + tz * sz;
-      const double ri = (double)bown[tz] - ax[tz];
+      const int c = tcell(tx, ty, tz);
+      const float ki = T.k[c];
+      const double xi = T.x[c];
+      double ax = 0.0;
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
+            if (cls == 0) continue;
+            const int o = dx + dy * HX + dz * HXY;
+            const float kj = T.k[c + o];
+            if (kj >= 0.f) ax += couple_f64_fast(cls, ki, kj) * (xi - T.x[c + o]);
+          }
+      const double ri = (double)bown[tz] - ax;
       r32[i] = (float)ri;
       acc += ri * ri;
     }

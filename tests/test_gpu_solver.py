@@ -129,6 +129,26 @@ def test_warm_start_converged_input_needs_no_iterations(cuda):
     s = D.Solver(vt, kp, tol=1e-7)
     x64 = torch.zeros(isl.vtype.shape, dtype=torch.float64, device=cuda)
     i1 = s.solve(b, x64)
+    x1 = x64.cpu().numpy()
     i2 = s.solve(b, x64)
+    # a zero guess again (its first residual is b without the stencil) must
+    # repeat the cold solve exactly
+    x64.zero_()
+    i3 = s.solve(b, x64)
+    x3 = x64.cpu().numpy()
+    # no guess at all: the same arithmetic
+    i4 = s.solve(b, None)
     s.close()
     assert i1.iterations > 0 and i2.iterations == 0 and i2.converged == 1
+    assert i3.iterations == i1.iterations and np.array_equal(x3, x1)
+    assert i4.iterations == i1.iterations and i4.rel_residual == i1.rel_residual
+
+
+def test_iteration_cap_stops_the_solve(cuda):
+    """The device stops a pass at max_iters; the host queue never runs the
+    solve past it and reports it unconverged."""
+    isl = W.workload(1)
+    for cap in (1, 3, 7):
+        info, x, _ = _solve(isl, cuda, tol=1e-6, max_iters=cap)
+        assert info.iterations <= cap and info.converged == 0, (cap, info.iterations)
+        assert np.isfinite(x).all()
