@@ -25,13 +25,13 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "time-to-1e-6 residual at 128^3 + batched islands/s vs plain, % of HBM peak"
 
 # Algorithmic bytes per ACTIVE voxel of each profiled finest-level kernel (each
-# array element counted once per launch, neighbour reuse free):
-#   0 ks_spmv   read z, p_in, kappa, anchor mass 16; write p_out, q 8        = 24
-#   1 ks_down   read r_in, q, kappa, diag, anchor mass, p 24, x (f64) 8;
-#               write x (f64) 8, r_out 4, coarse rhs 0.5                     = 44.5
-#   3 ks_up     read r, kappa, diag, anchor mass 16, coarse correction 0.5;
-#               write z 4                                                    = 20.5
-ALG_BYTES = {0: 24.0, 1: 44.5, 3: 20.5}
+# array element counted once per launch, neighbour reuse free; kappa is the
+# anchor-coded array, so no anchor-mass array is read):
+#   0 ks_spmv   read z, p_in, kappa 12; write p_out, q 8                     = 20
+#   1 ks_down   read r_in, q, kappa, diag, p 20, x (f64) 8;
+#               write x (f64) 8, r_out 4, coarse rhs 0.5                     = 40.5
+#   3 ks_up     read r, kappa, diag 12, coarse correction 0.5; write z 4     = 16.5
+ALG_BYTES = {0: 20.0, 1: 40.5, 3: 16.5}
 KERNEL_NAMES = {0: "ks_spmv", 1: "ks_down(update)", 3: "ks_up(finest)"}
 # HBM bytes per launch of these kernels on this workload, from the committed
 # FETCH_SIZE / WRITE_SIZE counter summary (copy-kernel calibrated).

@@ -11,9 +11,9 @@ namespace pfsf {
 // One island at one multigrid level, as the batched kernels see it.
 struct BatchLevel {
   pfsf_dims d;        // this level
-  const float* kap;   // kappa on the free set, -1 elsewhere
+  const float* kap;   // kappa on the free set, -1 elsewhere (finest level: anchor-coded, MgLevel::ks)
   const float* dg;    // diagonal
-  const float* am;    // anchor mass
+  const float* am;    // anchor mass (null on the finest level)
   const uint32_t* msk;  // coupling mask (levels >= 1), null on the finest level
   float scale, omega;
   pfsf_dims c;        // next coarser level

@@ -412,9 +412,11 @@ hipError_t BatchSolver::build_tables() {
       const MgLevel& M = S.level(l);
       BatchLevel D{};
       D.d = M.d;
-      D.kap = (const float*)M.kap;
+      // the finest level is read by the plane-streaming bodies only: the
+      // anchor-coded kappa, no anchor mass
+      D.kap = l == 0 ? M.ks : (const float*)M.kap;
       D.dg = M.dg;
-      D.am = M.am;
+      D.am = l == 0 ? nullptr : M.am;
       D.msk = M.msk;
       D.scale = M.scale;
       D.omega = M.omega;

@@ -61,6 +61,11 @@ struct MgLevel {
   void* kap = nullptr;    // float or bf16 kappa on the free set, -1 elsewhere
   float* dg = nullptr;    // diagonal (0 outside the free set)
   float* am = nullptr;    // anchor mass (couplings to Dirichlet voxels)
+  // finest level of a hierarchy (>= 2 levels), read by the plane-streaming
+  // kernels in place of kap and am: kappa on the free set, -kappa (at most
+  // -FLT_MIN) on anchors, -inf elsewhere, so the stencil forms the couplings
+  // to anchors itself
+  float* ks = nullptr;
   uint8_t* fr = nullptr;  // free mask
   uint32_t* msk = nullptr;  // coupling mask (levels > 0; bit nbit(dx,dy,dz))
   float* r = nullptr;     // restricted residual (levels > 0)

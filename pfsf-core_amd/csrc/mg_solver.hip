@@ -33,6 +33,7 @@
 // SH/mg_prolong.comp.glsl:39-80, NP/src/solver/dispatcher_vcycle.cpp:342-392
 // (fixed 16-step schedule, no tolerance); PCG kernels SH/pcg_*.comp.glsl,
 // NP/src/solver/dispatcher_pcg.cpp:254-297 (FP32 dots, N <= 512^2).
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -392,6 +393,23 @@ __global__ void __launch_bounds__(BS)
   if (!f) stk(kap, i, -1.0f);
 }
 
+// Anchor-coded kappa of the finest level (MgLevel::ks) from the final free
+// set: free kappa as is, anchors -max(kappa, FLT_MIN) (an anchor of kappa 0
+// still couples through its face weight), everything else -inf.
+__global__ void __launch_bounds__(BS)
+    k_stream_kappa(int64_t n, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                   const uint8_t* __restrict__ fr, const float* __restrict__ kap0,
+                   float* __restrict__ ks) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  float v = -INFINITY;
+  if (fr[i])
+    v = kap0[i];
+  else if (type[i] == 2)
+    v = -fmaxf(kapf(kappa[i]), FLT_MIN);
+  ks[i] = v;
+}
+
 // Tiles that hold at least one free voxel.
 __global__ void __launch_bounds__(BS)
     k_tile_flags(pfsf_dims d, const uint8_t* __restrict__ fr, int ntx, int nty, int tz,
@@ -571,6 +589,15 @@ static Lv<KT> view(const MgLevel& l) {
   return v;
 }
 
+// The finest level as the plane-streaming kernels read it: the anchor-coded
+// kappa, no anchor mass.
+static Lv<float> sview(const MgLevel& l) {
+  Lv<float> v = view<float>(l);
+  v.kap = l.ks;
+  v.am = nullptr;
+  return v;
+}
+
 static TL tl(const MgLevel& l) { return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids}; }
 
 static void fill0(float* p, int64_t n, hipStream_t s) {
@@ -668,6 +695,7 @@ hipError_t MgSolver::allocate(bool standalone) {
     L.kap = alloc<uint8_t>(L.n * (int64_t)ks, err);
     L.dg = alloc<float>(L.n, err);
     L.am = alloc<float>(L.n, err);
+    if (l == 0 && nlev_ > 1) L.ks = alloc<float>(L.n, err);
     L.fr = alloc<uint8_t>(L.n, err);
     if (l > 0) {
       L.r = alloc<float>(L.n, err);
@@ -843,6 +871,9 @@ hipError_t MgSolver::build() {
       k_sync_free<float><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (float*)L.kap);
     }
   }
+  if (F.ks)
+    k_stream_kappa<<<blocks_for(n0_, BS), BS, 0, s>>>(n0_, type_, kappa_, F.fr,
+                                                      (const float*)F.kap, F.ks);
   chk(hipGetLastError(), err);
   for (int l = 0; l < nlev_ && err == hipSuccess; l++) chk(build_tiles(levels_[l]), err);
   // Smoother weights and the Chebyshev interval use the Gershgorin bound
@@ -1013,10 +1044,10 @@ hipError_t MgSolver::precondition(int c) {
   MgLevel& B = levels_[1];
   const unsigned G = F.tiles.nlaunch;
   PFSF_TZF_DISPATCH(F, (ks_down<TZV, false><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr, nullptr,
+                          sview(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr, nullptr,
                           B.r, sc_, nullptr, 0u, nullptr)));
   (void)vcycle_below(1, false);
-  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(view<float>(F), tl(F), B.d, r_[c], B.e,
+  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(sview(F), tl(F), B.d, r_[c], B.e,
                                                           z_, sc_, prz_)));
   return hipGetLastError();
 }
@@ -1039,13 +1070,13 @@ hipError_t MgSolver::iteration(int c) {
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
-  PFSF_TZF_DISPATCH(F, (ks_spmv<TZV><<<G, BS, 0, s>>>(view<float>(F), tl(F), z_, p_[c],
+  PFSF_TZF_DISPATCH(F, (ks_spmv<TZV><<<G, BS, 0, s>>>(sview(F), tl(F), z_, p_[c],
                                                       p_[1 - c], q_, sc_, prz_, G, c, ppq_)));
   PFSF_TZF_DISPATCH(F, (ks_down<TZV, true><<<G, BS, 0, s>>>(
-                          view<float>(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
+                          sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
                           sc_, ppq_, G, prr_)));
   (void)vcycle_below(1, true);
-  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(view<float>(F), tl(F), B.d, r_[1 - c],
+  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(sview(F), tl(F), B.d, r_[1 - c],
                                                           B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
@@ -1283,9 +1314,16 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   auto once = [&](hipEvent_t a, hipEvent_t b) {
     switch (which) {
       case 0:
-        PFSF_TZF_DISPATCH(F, ext(ks_spmv<TZV>, G, a, b, view<float>(F), tl(F), (const float*)z_,
-                                (const float*)p_[0], p_[1], q_, sc_, (const double*)prz_, G, 0,
-                                ppq_));
+        if (nlev_ == 1) {  // the tiled SpMV of the Jacobi-PCG iteration
+          const unsigned nb = blocks_for(n0_, BS);
+          PFSF_TZ_DISPATCH(F, ext(k_spmv_dir<TZV, float>, G, a, b, view<float>(F), tl(F),
+                                  (const float*)z_, (const float*)p_[0], p_[1], q_, sc_,
+                                  (const double*)(prr_ + nb), nb, 0, ppq_));
+        } else {
+          PFSF_TZF_DISPATCH(F, ext(ks_spmv<TZV>, G, a, b, sview(F), tl(F), (const float*)z_,
+                                  (const float*)p_[0], p_[1], q_, sc_, (const double*)prz_, G, 0,
+                                  ppq_));
+        }
         break;
       case 1:
         if (nlev_ == 1) {
@@ -1293,7 +1331,7 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
           ext(k_update_jacobi, nb, a, b, n0_, (const float*)F.dg, x_, (const float*)r_[0], r_[1],
               (const float*)p_[1], (const float*)q_, z_, (const PcgScalars*)sc_, prr_);
         } else {
-          PFSF_TZF_DISPATCH(F, ext(ks_down<TZV, true>, G, a, b, view<float>(F), tl(F),
+          PFSF_TZF_DISPATCH(F, ext(ks_down<TZV, true>, G, a, b, sview(F), tl(F),
                                   levels_[1].d, (const float*)r_[0], r_[1], (const float*)q_,
                                   (const float*)p_[1], x_, levels_[1].r, sc_,
                                   (const double*)ppq_, G, prr_));
@@ -1318,7 +1356,7 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
         break;
       }
       default:
-        PFSF_TZF_DISPATCH(F, ext(ks_up<TZV, true>, G, a, b, view<float>(F), tl(F), levels_[1].d,
+        PFSF_TZF_DISPATCH(F, ext(ks_up<TZV, true>, G, a, b, sview(F), tl(F), levels_[1].d,
                                 (const float*)r_[1], (const float*)levels_[1].e, z_,
                                 (const PcgScalars*)sc_, prz_));
     }

@@ -12,9 +12,12 @@
 // Each z-halo plane is loaded once per column rather than once per
 // 32 x 8 x TZ tile.
 //
-// The arithmetic is that of spmv_body / down_body / up_body (same operand
-// order in the stencil), so the vector outputs equal the tiled kernels' bit
-// for bit; only the per-work-group partial sums cover other cell sets.
+// kappa comes in the anchor-coded form of the finest level (free kappa as
+// is, -kappa on anchors, -inf elsewhere): an anchor is staged as a coupled
+// cell of value 0, so the stencil forms the couplings to anchors itself and
+// no anchor-mass array is read.  The operator is that of spmv_body /
+// down_body / up_body (same couplings, same operand order, the anchor terms
+// summed inside the stencil instead of added as a precomputed mass).
 #pragma once
 
 #include <type_traits>
@@ -104,8 +107,9 @@ __device__ __forceinline__ void read_win(const PlaneRing& R, int slot, int c, Wi
 }
 
 // (A v) at the centre of plane window W0 with neighbour planes Wm (z-1) and
-// Wp (z+1): the operand order of stencil_apply, so the FP32 result is the same.
-__device__ __forceinline__ float stencil_win(const Win& Wm, const Win& W0, const Win& Wp, float mi,
+// Wp (z+1), in the operand order of stencil_apply; anchors are coupled cells
+// of value 0 in the window, so their couplings are part of the sums.
+__device__ __forceinline__ float stencil_win(const Win& Wm, const Win& W0, const Win& Wp,
                                              float scale) {
   const float vc = W0.v[4], si = W0.s[4];
   const float ki = si * si;
@@ -143,7 +147,7 @@ __device__ __forceinline__ float stencil_win(const Win& Wm, const Win& W0, const
   corner(Wp, 2);
   corner(Wp, 6);
   corner(Wp, 8);
-  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac));
 }
 
 // Flat indices of a plane's two cells (cell 0 stands in for a cell outside
@@ -162,9 +166,14 @@ __device__ __forceinline__ PlaneIdx plane_idx(const PlaneGeom& g, int z, pfsf_di
   return p;
 }
 
+// Stages one cell from its anchor-coded kappa k: free cells (k >= 0) keep
+// their value, anchors (finite k < 0) and the rest hold 0; the coupling
+// coding is sqrt|k| with the sign bit clear for free cells and anchors and
+// -0.0 for cells that couple to nothing (k = -inf), so is_free_s() reads
+// "coupled" here.
 __device__ __forceinline__ void ring_put(PlaneRing& R, int slot, int off, float k, float v) {
   R.v[slot][off] = k >= 0.f ? v : 0.f;
-  R.s[slot][off] = free_s(k);
+  R.s[slot][off] = k != -INFINITY ? __builtin_amdgcn_sqrtf(fabsf(k)) : -0.0f;
 }
 
 // Restriction over the z-pair of planes (zo even, zo + 1) held by one thread
@@ -180,7 +189,7 @@ __device__ __forceinline__ void restrict_pair(float a, int tx, int ty, int gx, i
 // ── p_out = z + beta p_in; q = A p_out; p.q (see spmv_body) ─────────────
 struct SpRaw {
   PlaneIdx ix;
-  float k, z, p, am, hk, hz, hp;
+  float k, z, p, hk, hz, hp;
 };
 
 template <int TZ>
@@ -190,8 +199,7 @@ __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, c
   R.k = ld(L.kap, R.ix.i);
   R.z = ld(z, R.ix.i);
   R.p = ld(pin, R.ix.i);
-  R.am = ld(L.am, R.ix.i);
-  float hk = -1.f, hz = 0.f, hp = 0.f;  // locals keep the struct in registers
+  float hk = -INFINITY, hz = 0.f, hp = 0.f;  // locals keep the struct in registers
   if (g.hown) {
     hk = ld(L.kap, R.ix.h);
     hz = ld(z, R.ix.h);
@@ -231,7 +239,7 @@ __device__ __forceinline__ void spmv_stream_body(
       constexpr int s = decltype(S_)::value;
       sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
     });
-    float pv_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
+    float pv_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
     Win W[3];
@@ -240,17 +248,16 @@ __device__ __forceinline__ void spmv_stream_body(
       const SpRaw cur = buf[s % kPD];
       if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, z, pin);
       const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -1.f;
+      const float ki = cur.ix.iv ? cur.k : -INFINITY;
       const float pv = ki >= 0.f ? cur.z + beta * cur.p : 0.f;
       ring_put(R, sl, g.li, ki, pv);
       if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
         ring_put(R, sl, g.lh, kh, cur.hz + beta * cur.hp);
       }
       if constexpr (s >= 1 && s <= TZ) {
         if (cur.ix.iv) st(pout, cur.ix.i, pv);
         pv_own[s & 1] = pv;
-        am_own[s & 1] = cur.am;
         fr_own[s & 1] = ki >= 0.f;
         ix_own[s & 1] = cur.ix;
       }
@@ -258,7 +265,7 @@ __device__ __forceinline__ void spmv_stream_body(
       read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
+        float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
         qi = fr_own[o] ? qi : 0.f;
         acc += (double)pv_own[o] * qi;
         if (ix_own[o].iv) st(q, ix_own[o].i, qi);
@@ -285,7 +292,7 @@ __global__ void __launch_bounds__(BS)
 // rhs P^T (r_out - A x).
 struct DnRaw {
   PlaneIdx ix;
-  float k, r, q, dg, am, p, hk, hr, hq, hdg;
+  float k, r, q, dg, p, hk, hr, hq, hdg;
   double x;
 };
 
@@ -298,10 +305,9 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, c
   R.r = ld(rin, R.ix.i);
   R.q = UPDATE ? ld(q, R.ix.i) : 0.f;
   R.dg = ld(L.dg, R.ix.i);
-  R.am = ld(L.am, R.ix.i);
   R.p = UPDATE ? ld(p, R.ix.i) : 0.f;
   R.x = UPDATE ? ld(x, R.ix.i) : 0.0;
-  float hk = -1.f, hr = 0.f, hq = 0.f, hdg = 1.f;  // locals keep the struct in registers
+  float hk = -INFINITY, hr = 0.f, hq = 0.f, hdg = 1.f;  // locals keep the struct in registers
   if (g.hown) {
     hk = ld(L.kap, R.ix.h);
     hr = ld(rin, R.ix.h);
@@ -348,7 +354,7 @@ __device__ __forceinline__ void down_stream_body(
       constexpr int s = decltype(S_)::value;
       dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
     });
-    float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f};
+    float r_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     float pair = 0.f;
     Win W[3];
@@ -358,11 +364,11 @@ __device__ __forceinline__ void down_stream_body(
       if constexpr (s + kPD < TZ + 2)
         dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, L, rin, q, p, x);
       const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -1.f;
+      const float ki = cur.ix.iv ? cur.k : -INFINITY;
       const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
       ring_put(R, sl, g.li, ki, om * ri * rcp(cur.dg));
       if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
         const float rh = UPDATE ? cur.hr - alpha * cur.hq : cur.hr;
         ring_put(R, sl, g.lh, kh, om * rh * rcp(cur.hdg));
       }
@@ -378,14 +384,13 @@ __device__ __forceinline__ void down_stream_body(
           acc += fr ? (double)ri * ri : 0.0;
         }
         r_own[s & 1] = ri;
-        am_own[s & 1] = cur.am;
         fr_own[s & 1] = fr;
       }
       __syncthreads();
       read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
+        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
         const float res = fr_own[o] ? r_own[o] - ax : 0.f;
         const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
         if constexpr ((s & 1) == 0) {
@@ -416,7 +421,7 @@ __global__ void __launch_bounds__(BS)
 // (r - A x~); with FINEST the r.z partials.
 struct UpRaw {
   PlaneIdx ix;
-  float k, r, dg, am, ec, hk, hr, hdg, hec;
+  float k, r, dg, ec, hk, hr, hdg, hec;
 };
 
 template <int TZ>
@@ -427,9 +432,8 @@ __device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, c
   R.k = ld(L.kap, R.ix.i);
   R.r = ld(r, R.ix.i);
   R.dg = ld(L.dg, R.ix.i);
-  R.am = ld(L.am, R.ix.i);
   R.ec = ld(ec, R.ix.iv ? flat32(g.gx >> 1, g.gy >> 1, zc, c) : 0);
-  float hk = -1.f, hr = 0.f, hdg = 1.f, hec = 0.f;  // locals keep the struct in registers
+  float hk = -INFINITY, hr = 0.f, hdg = 1.f, hec = 0.f;  // locals keep the struct in registers
   if (g.hown) {
     hk = ld(L.kap, R.ix.h);
     hr = ld(r, R.ix.h);
@@ -460,7 +464,7 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, in
       constexpr int s = decltype(S_)::value;
       up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
     });
-    float r_own[2] = {0.f, 0.f}, am_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
+    float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
     Win W[3];
@@ -469,15 +473,14 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, in
       const UpRaw cur = buf[s % kPD];
       if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, c, r, ec);
       const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -1.f;
+      const float ki = cur.ix.iv ? cur.k : -INFINITY;
       ring_put(R, sl, g.li, ki, om * cur.r * rcp(cur.dg) + cur.ec);
       if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -1.f;
+        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
         ring_put(R, sl, g.lh, kh, om * cur.hr * rcp(cur.hdg) + cur.hec);
       }
       if constexpr (s >= 1 && s <= TZ) {
         r_own[s & 1] = cur.r;
-        am_own[s & 1] = cur.am;
         dg_own[s & 1] = cur.dg;
         fr_own[s & 1] = ki >= 0.f;
         ix_own[s & 1] = cur.ix;
@@ -486,7 +489,7 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, in
       read_win(R, sl, g.li, W[s % 3]);
       if constexpr (s >= 2) {
         const int o = (s - 1) & 1;
-        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], am_own[o], L.scale);
+        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
         const float xi = W[(s - 1) % 3].v[4];
         float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
         zi = fr_own[o] ? zi : 0.f;
