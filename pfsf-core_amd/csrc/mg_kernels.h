@@ -440,7 +440,8 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
                                           float* __restrict__ rc, PcgScalars* sc,
                                           const double* __restrict__ pq_part, unsigned pq_n,
                                           bool publish, double* partials, unsigned pslot) {
-  if (sc->done) return;
+  // the done flag is tested once the operand loads are in flight
+  const int done = sc->done;
   __shared__ Tile<TZ> T;
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
@@ -478,6 +479,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
                    },
                    kk, raw);
   }
+  if (done) return;
   double alpha64 = 0.0;
   if (UPDATE) {
     const double pq = pq0;
@@ -536,7 +538,8 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
                                         pfsf_dims c, const float* __restrict__ r,
                                         const float* __restrict__ ec, float* __restrict__ out,
                                         const PcgScalars* sc, double* partials, unsigned pslot) {
-  if (sc->done) return;
+  // the done flag is tested once the own-cell loads are in flight
+  const int done = sc->done;
   __shared__ Tile<TZ> T;
   __shared__ double sh[1][BS / 64];
   double acc[1] = {0.0};
@@ -561,6 +564,7 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       dgr[tz] = ld(L.dg, i);
       mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
     }
+    if (done) return;
     stage<TZ>(T, L, x0, y0, z0, [&](int j, int gx_, int gy_, int gz_) {
       return om * ld(r, j) * rcp(ld(L.dg, j)) + ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c));
     });
@@ -577,6 +581,8 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       if (FINEST) acc[0] += fr ? (double)ri * zi : 0.0;
       if (inb[tz]) st(out, i0 + tz * sz, zi);
     }
+  } else if (done) {
+    return;
   }
   if (FINEST) write_partial_at(acc[0], partials, sh, pslot);
 }

@@ -199,15 +199,9 @@ __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, c
   R.k = ld(L.kap, R.ix.i);
   R.z = ld(z, R.ix.i);
   R.p = ld(pin, R.ix.i);
-  float hk = -INFINITY, hz = 0.f, hp = 0.f;  // locals keep the struct in registers
-  if (g.hown) {
-    hk = ld(L.kap, R.ix.h);
-    hz = ld(z, R.ix.h);
-    hp = ld(pin, R.ix.h);
-  }
-  R.hk = hk;
-  R.hz = hz;
-  R.hp = hp;
+  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hz = ld(z, R.ix.h);
+  R.hp = ld(pin, R.ix.h);
 }
 
 // Column (x0, y0, z0) of one system; `publish` (one work-group per system)
@@ -218,9 +212,18 @@ __device__ __forceinline__ void spmv_stream_body(
     const float* __restrict__ pin, float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
     const double* __restrict__ rz_part, unsigned rz_n, int slot, bool publish, double* partials,
     unsigned pslot) {
-  if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
+  // the first planes' loads do not depend on beta: they are in flight while
+  // the done flag and the r.z partials are read
+  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  SpRaw buf[kPD];
+  if (have)
+    static_for<kPD>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
+    });
+  if (sc->done) return;
   const bool first = !sc->have_dir;
   const double rz_old = sc->rzs[slot ^ 1];
   const double rz = block_total(rz_part, rz_n);
@@ -233,12 +236,6 @@ __device__ __forceinline__ void spmv_stream_body(
   double acc = 0.0;
   if (have) {
     const float beta = (float)beta64;
-    const PlaneGeom g = plane_geom(x0, y0, L.d);
-    SpRaw buf[kPD];
-    static_for<kPD>([&](auto S_) {
-      constexpr int s = decltype(S_)::value;
-      sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
-    });
     float pv_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
@@ -307,17 +304,10 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, c
   R.dg = ld(L.dg, R.ix.i);
   R.p = UPDATE ? ld(p, R.ix.i) : 0.f;
   R.x = UPDATE ? ld(x, R.ix.i) : 0.0;
-  float hk = -INFINITY, hr = 0.f, hq = 0.f, hdg = 1.f;  // locals keep the struct in registers
-  if (g.hown) {
-    hk = ld(L.kap, R.ix.h);
-    hr = ld(rin, R.ix.h);
-    hq = UPDATE ? ld(q, R.ix.h) : 0.f;
-    hdg = ld(L.dg, R.ix.h);
-  }
-  R.hk = hk;
-  R.hr = hr;
-  R.hq = hq;
-  R.hdg = hdg;
+  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hr = ld(rin, R.ix.h);
+  R.hq = UPDATE ? ld(q, R.ix.h) : 0.f;
+  R.hdg = ld(L.dg, R.ix.h);
 }
 
 template <int TZ, bool UPDATE>
@@ -327,9 +317,18 @@ __device__ __forceinline__ void down_stream_body(
     const float* __restrict__ p, double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
     const double* __restrict__ pq_part, unsigned pq_n, bool publish, double* partials,
     unsigned pslot) {
-  if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
+  // the first planes' loads do not depend on alpha: they are in flight while
+  // the done flag and the p.q partials are read
+  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  DnRaw buf[kPD];
+  if (have)
+    static_for<kPD>([&](auto S_) {
+      constexpr int s = decltype(S_)::value;
+      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
+    });
+  if (sc->done) return;
   double alpha64 = 0.0;
   if (UPDATE) {
     const double rz = sc->rz;
@@ -347,13 +346,7 @@ __device__ __forceinline__ void down_stream_body(
   double acc = 0.0;
   if (have) {
     const float alpha = (float)alpha64, om = L.omega;
-    const PlaneGeom g = plane_geom(x0, y0, L.d);
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    DnRaw buf[kPD];
-    static_for<kPD>([&](auto S_) {
-      constexpr int s = decltype(S_)::value;
-      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
-    });
     float r_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     float pair = 0.f;
@@ -433,17 +426,10 @@ __device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, c
   R.r = ld(r, R.ix.i);
   R.dg = ld(L.dg, R.ix.i);
   R.ec = ld(ec, R.ix.iv ? flat32(g.gx >> 1, g.gy >> 1, zc, c) : 0);
-  float hk = -INFINITY, hr = 0.f, hdg = 1.f, hec = 0.f;  // locals keep the struct in registers
-  if (g.hown) {
-    hk = ld(L.kap, R.ix.h);
-    hr = ld(r, R.ix.h);
-    hdg = ld(L.dg, R.ix.h);
-    hec = ld(ec, R.ix.hv ? flat32(g.hx >> 1, g.hy >> 1, zc, c) : 0);
-  }
-  R.hk = hk;
-  R.hr = hr;
-  R.hdg = hdg;
-  R.hec = hec;
+  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hr = ld(r, R.ix.h);
+  R.hdg = ld(L.dg, R.ix.h);
+  R.hec = ld(ec, R.ix.hv ? flat32(g.hx >> 1, g.hy >> 1, zc, c) : 0);
 }
 
 template <int TZ, bool FINEST>
@@ -452,18 +438,19 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, in
                                                const float* __restrict__ ec,
                                                float* __restrict__ out, const PcgScalars* sc,
                                                double* partials, unsigned pslot) {
-  if (sc->done) return;
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
-  double acc = 0.0;
-  if (have) {
-    const float om = L.omega;
-    const PlaneGeom g = plane_geom(x0, y0, L.d);
-    UpRaw buf[kPD];
+  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  UpRaw buf[kPD];
+  if (have)  // in flight while the done flag is read
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
     });
+  if (sc->done) return;
+  double acc = 0.0;
+  if (have) {
+    const float om = L.omega;
     float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
