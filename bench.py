@@ -50,6 +50,11 @@ def measured_traffic():
 # 1e-6 (counted once with oracle.pcg(workload(1), tol=1e-6) on the full solve:
 # 1032 iterations, true relative residual 9.72e-7).
 CPU_PCG_ITERS_TO_1E6 = 1032
+# Whole-solve algorithmic bytes per active voxel: a fused PCG iteration
+# (59 B) plus a 2-level V-cycle application (67 B) per iteration, and the
+# initial residual (17 B).
+SOLVE_ITER_BYTES = 126
+SOLVE_INIT_BYTES = 17
 
 
 def log(msg):
@@ -261,6 +266,48 @@ def label_bench(torch, D, W, isl, steps=10):
     return out
 
 
+def rbgs_bench(torch, D, W, sweeps=200, steps=5):
+    """The plain method on the dense 32^3 block (all voxels occupied, z=0
+    anchored): 200 eight-colour RBGS sweeps from phi = 0, then the failure
+    scan, on device arrays; the same run of the C oracle (one host thread) is
+    timed beside it and both outputs are compared bit for bit."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    isl = W.workload(0)
+    dev = torch.device("cuda:0")
+    vt, kp, b, rc, rt = (torch.from_numpy(a).to(dev)
+                         for a in (isl.vtype, isl.kappa, isl.source, isl.rcomp, isl.rtens))
+    phi = torch.zeros(isl.vtype.shape, dtype=torch.float32, device=dev)
+    code = torch.zeros_like(vt)
+    ts = []
+    for _ in range(steps + 1):
+        phi.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        D.rbgs(vt, kp, b, phi, sweeps)
+        D.failure_scan(vt, kp, phi, rc, rt, code)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gpu_ms = float(np.median(ts[1:])) * 1e3
+    t0 = time.perf_counter()
+    ref_phi = O.rbgs(isl, sweeps=sweeps)
+    ref_code = O.failure_scan(isl, ref_phi)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    n = isl.n_active
+    out = {"grid": [isl.lx, isl.ly, isl.lz], "active_voxels": n, "sweeps": sweeps,
+           "gpu_ms": round(gpu_ms, 3),
+           "gpu_voxel_updates_per_s": round(n * sweeps / (gpu_ms * 1e-3), 1),
+           "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1, "cpu_kind": "port",
+           "cpu_voxel_updates_per_s": round(n * sweeps / (cpu_ms * 1e-3), 1),
+           "phi_bit_exact": bool(np.array_equal(phi.cpu().numpy(), ref_phi)),
+           "codes_bit_exact": bool(np.array_equal(code.cpu().numpy(), ref_code)),
+           "note": "synchronous C-ABI calls timed on the host (8 colour launches per sweep)"}
+    log(f"rbgs 32^3: {out}")
+    return out
+
+
 def host_tick_bench(isl, steps=3):
     """The same 128^3 island through the host-buffer C ABI, as the Java runtime
     drives it: pfsf_register_island_buffers + pfsf_tick_dbb per tick (PCIe
@@ -308,6 +355,7 @@ def main():
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint record")
     ap.add_argument("--no-labels", action="store_true", help="skip the labelling record")
+    ap.add_argument("--no-rbgs", action="store_true", help="skip the dense 32^3 RBGS record")
     ap.add_argument("--no-host-tick", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) tick record")
     ap.add_argument("--batch-only", action="store_true",
@@ -378,6 +426,10 @@ def main():
     traffic = measured_traffic().get(KERNEL_NAMES[dom])
     # whole-iteration algorithmic rate
     it_bytes = sum(ALG_BYTES.values()) * n_act
+    # whole solve: per active voxel, a fused PCG iteration (SpMV + dot 13,
+    # update + z + dot 29, direction 17 = 59 B) plus one 2-level V-cycle
+    # (67 B) per iteration, and 17 B for the initial residual
+    solve_bytes = (SOLVE_ITER_BYTES * last.iterations + SOLVE_INIT_BYTES) * n_act
     hist = np.bincount(code.cpu().numpy().ravel(), minlength=5).tolist()
 
     out = {
@@ -400,6 +452,11 @@ def main():
         "pcg_iteration_ms": round(iter_ms, 5),
         "iteration_alg_GBps": round(it_bytes / (iter_ms * 1e-3) / 1e9, 1),
         "voxel_updates_per_s": round(n_act * last.iterations / (solve_ms * 1e-3), 1),
+        "solve_alg_bandwidth": {
+            "bytes": round(solve_bytes), "GBps": round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1),
+            "frac": round(solve_bytes / (solve_ms * 1e-3) / HBM_PEAK, 4),
+            "per_active_voxel": f"{SOLVE_ITER_BYTES} B x {last.iterations} iterations + "
+                                f"{SOLVE_INIT_BYTES} B"},
         "kernel_ms": {KERNEL_NAMES[k]: round(v, 5) for k, v in prof.items()},
         "failure_histogram": hist,
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom],
@@ -425,6 +482,8 @@ def main():
         out["adjoint_96"] = adjoint_bench(torch, W)
     if not args.no_labels:
         out["labels"] = label_bench(torch, D, W, isl)
+    if not args.no_rbgs:
+        out["rbgs_32"] = rbgs_bench(torch, D, W)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)

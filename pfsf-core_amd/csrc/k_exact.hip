@@ -14,6 +14,9 @@
 //   normalisation         NP/src/compute/normalize.cpp:29-63
 #pragma clang fp contract(off)
 
+#include <mutex>
+#include <vector>
+
 #include "pfsf_internal.h"
 
 namespace pfsf {
@@ -54,24 +57,37 @@ __device__ __forceinline__ int64_t flat(int x, int y, int z, pfsf_dims d) {
   return (int64_t)x + (int64_t)d.lx * ((int64_t)y + (int64_t)d.ly * z);
 }
 
-// The jacobiStep per-voxel update on a neighbourhood read from `phi`.
+// The jacobiStep per-voxel update on a neighbourhood read from `phi`.  All 26
+// neighbours are loaded first (a neighbour outside the grid reads the voxel
+// itself and is dropped), so a thread waits for memory once; a dropped or air
+// neighbour leaves both sums as they are (a select, the same bits as skipping
+// it), and the FP64 operations run in offset order as in the restatement.
 __device__ __forceinline__ float jacobi_voxel(int64_t i, XYZ p, pfsf_dims d,
                                               const uint8_t* __restrict__ type,
                                               const float* __restrict__ kappa,
                                               const float* __restrict__ src, const float* phi,
                                               float omega) {
   constexpr int O[26][3] = PFSF_OFFSETS;
+  float kn[26], pn[26];
+  bool in[26];
+  uint8_t tn[26];
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    in[k] = nx >= 0 && nx < d.lx && ny >= 0 && ny < d.ly && nz >= 0 && nz < d.lz;
+    int64_t j = in[k] ? flat(nx, ny, nz, d) : i;
+    tn[k] = type[j];
+    kn[k] = kappa[j];
+    pn[k] = phi[j];
+  }
   double diag = 0.0, off = 0.0;
   float ki = kapf(kappa[i]);
 #pragma unroll
   for (int k = 0; k < 26; k++) {
-    int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
-    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
-    int64_t j = flat(nx, ny, nz, d);
-    if (type[j] == 0) continue;
-    double a = couple_exact(k, ki, kapf(kappa[j]));
-    diag += a;
-    off += a * (double)phi[j];
+    const bool use = in[k] && tn[k] != 0;
+    double a = couple_exact(k, ki, kapf(kn[k]));
+    diag = use ? diag + a : diag;
+    off = use ? off + a * (double)pn[k] : off;
   }
   double prev = (double)phi[i];
   double b = (double)src[i];
@@ -453,15 +469,102 @@ hipError_t launch_jacobi_exact(pfsf_dims d, const uint8_t* type, const float* ka
   return hipGetLastError();
 }
 
+// One sweep: the eight colour passes in order.
+static void rbgs_sweep(pfsf_dims d, const uint8_t* type, const float* kappa, const float* src,
+                       float* phi, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+    int64_t hn = (int64_t)((d.lx - cx + 1) >> 1) * ((d.ly - cy + 1) >> 1) * ((d.lz - cz + 1) >> 1);
+    if (hn <= 0) continue;
+    k_rbgs_color<<<blocks_for(hn, BS), BS, 0, s>>>(d, type, kappa, src, phi, c);
+  }
+}
+
+// Runs of kRbgsGraphSweeps sweeps are replayed from a HIP graph holding
+// exactly those colour launches, captured once per device and argument set on
+// a private stream: one graph launch replaces 64 host launches, which bound
+// the sweep rate of small grids.  The kernels, their arguments and their
+// order are those of the direct launches, so the iterates are the same bits.
+// The last kRbgsGraphCache argument sets stay instantiated.
+constexpr int kRbgsGraphSweeps = 8;
+constexpr size_t kRbgsGraphCache = 8;
+
+struct RbgsGraph {
+  int dev;
+  pfsf_dims d;
+  const uint8_t* type;
+  const float *kappa, *src;
+  float* phi;
+  hipGraphExec_t exec;
+};
+
+static hipError_t rbgs_graph(pfsf_dims d, const uint8_t* type, const float* kappa,
+                             const float* src, float* phi, hipGraphExec_t* out) {
+  static std::mutex mu;
+  static std::vector<RbgsGraph> cache;  // least recently used first
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  for (size_t k = 0; k < cache.size(); k++) {
+    const RbgsGraph& e = cache[k];
+    if (e.dev == dev && e.d.lx == d.lx && e.d.ly == d.ly && e.d.lz == d.lz && e.type == type &&
+        e.kappa == kappa && e.src == src && e.phi == phi) {
+      RbgsGraph hit = e;
+      cache.erase(cache.begin() + k);
+      cache.push_back(hit);
+      *out = hit.exec;
+      return hipSuccess;
+    }
+  }
+  hipStream_t cs = nullptr;
+  err = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (err != hipSuccess) return err;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  err = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+  if (err == hipSuccess) {
+    for (int k = 0; k < kRbgsGraphSweeps; k++) rbgs_sweep(d, type, kappa, src, phi, cs);
+    hipError_t lerr = hipGetLastError();
+    err = hipStreamEndCapture(cs, &g);
+    if (err == hipSuccess) err = lerr;
+  }
+  if (err == hipSuccess) err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  if (err != hipSuccess) return err;
+  if (cache.size() >= kRbgsGraphCache) {
+    // the evicted graph may still be queued on some stream
+    (void)hipDeviceSynchronize();
+    (void)hipGraphExecDestroy(cache.front().exec);
+    cache.erase(cache.begin());
+  }
+  cache.push_back(RbgsGraph{dev, d, type, kappa, src, phi, exec});
+  *out = exec;
+  return hipSuccess;
+}
+
 hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
                              const float* src, float* phi, int sweeps, hipStream_t s) {
-  for (int sw = 0; sw < sweeps; sw++)
-    for (int c = 0; c < 8; c++) {
-      int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-      int64_t hn = (int64_t)((d.lx - cx + 1) >> 1) * ((d.ly - cy + 1) >> 1) * ((d.lz - cz + 1) >> 1);
-      if (hn <= 0) continue;
-      k_rbgs_color<<<blocks_for(hn, BS), BS, 0, s>>>(d, type, kappa, src, phi, c);
+  int sw = 0;
+  bool replay = false;  // a stream under capture takes the direct launches
+  if (sweeps >= kRbgsGraphSweeps) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) == hipSuccess)
+      replay = cap == hipStreamCaptureStatusNone;
+    else
+      (void)hipGetLastError();
+  }
+  if (replay) {
+    hipGraphExec_t exec = nullptr;
+    hipError_t err = rbgs_graph(d, type, kappa, src, phi, &exec);
+    if (err != hipSuccess) return err;
+    for (; sw + kRbgsGraphSweeps <= sweeps; sw += kRbgsGraphSweeps) {
+      err = hipGraphLaunch(exec, s);
+      if (err != hipSuccess) return err;
     }
+  }
+  for (; sw < sweeps; sw++) rbgs_sweep(d, type, kappa, src, phi, s);
   return hipGetLastError();
 }
 

@@ -51,6 +51,30 @@ def test_rbgs_bit_exact(cuda, name, mk):
     assert np.array_equal(phi.cpu().numpy().view(np.uint32), want.view(np.uint32)), name
 
 
+def test_rbgs_graph_replays_bit_exact(cuda):
+    """Runs of 8 sweeps replay a captured graph: 13 sweeps (one replay + 5
+    direct) then 16 more on the same buffers equal 29 oracle sweeps; ten
+    buffer sets of one grid (more than the graph cache keeps) each equal
+    their oracle run."""
+    isl = W.box_union(1100, 13, 0.5, 0.5, 40, (2, 7), True)
+    d = _isl_dev(isl, cuda)
+    phi = torch.zeros_like(d["kappa"])
+    D.rbgs(d["vtype"], d["kappa"], d["source"], phi, 13)
+    D.rbgs(d["vtype"], d["kappa"], d["source"], phi, 16)
+    want = O.rbgs(isl, sweeps=29)
+    assert np.array_equal(phi.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    small = W.dense_block(7, 8)
+    s = _isl_dev(small, cuda)
+    rng = np.random.default_rng(5)
+    starts = [(rng.random(small.vtype.shape) * 2).astype(np.float32) for _ in range(10)]
+    phis = [torch.from_numpy(p0).to(cuda) for p0 in starts]
+    for p in phis:
+        D.rbgs(s["vtype"], s["kappa"], s["source"], p, 8)
+    for p, p0 in zip(phis, starts):
+        want = O.rbgs(small, phi=p0, sweeps=8)
+        assert np.array_equal(p.cpu().numpy().view(np.uint32), want.view(np.uint32))
+
+
 def test_dense32_rbgs_200_sweeps_and_scan_bit_exact(cuda):
     """The dense 32^3 workload: 200 RBGS sweeps then the failure scan."""
     isl = W.workload(0)
