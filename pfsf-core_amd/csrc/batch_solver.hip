@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(BS)
     P.done = 3;
     return;
   }
-  const double stop_rel = fmax(0.5 * tol / rel, 2e-6);
+  const double stop_rel = fmax(0.5 * tol / rel, PFSF_PASS_FLOOR);
   P.done = 0;
   P.iters = 0;
   P.have_dir = 0;

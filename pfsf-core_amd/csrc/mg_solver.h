@@ -29,6 +29,13 @@ constexpr int kTX = 32, kTY = 8;
 // 12 need 39 at ~0.9 us more per step and iteration.
 constexpr int kCoarseSweeps = 6;
 
+// Smallest relative drop of the recursive residual a solve pass aims for
+// before the true FP64 residual is checked (a pass stops at
+// max(0.5 tol / rel, this)).
+#ifndef PFSF_PASS_FLOOR
+#define PFSF_PASS_FLOOR 2e-6
+#endif
+
 // Tile depths of a batchable hierarchy (finest level: 8-plane streaming
 // columns; coarse levels).
 constexpr int kBatchTZFine = 8, kBatchTZCoarse = 2;

@@ -1216,7 +1216,7 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   int passes = 0, total = 0;
   const IrScalars* irh = (const IrScalars*)(sc_host_ + 1);
   for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
-    double stop_rel = std::max(0.5 * tol / rel, 2e-6);
+    double stop_rel = std::max(0.5 * tol / rel, PFSF_PASS_FLOOR);
     mirror_[0] = mirror_[1] = 0;  // the stream is idle here (true_residual synchronised)
     k_pcg_begin<<<1, 1, 0, s>>>(sc_, ir_, stop_rel, max_it - total, 1);
     chk(precondition(0), err);

@@ -1,7 +1,7 @@
 // Plane-streaming kernels of the finest level (single-system and batched solver).
 //
 // A work-group owns a 32 x 8 column of the finest grid over TZ planes and
-// walks it plane by plane.  The staged value and the free_s coding of kappa
+// walks it plane by plane.  The staged value and the coupling coding of kappa
 // of each plane (with its one-cell halo ring, 34 x 10 cells) go into one of
 // two LDS plane slots; every thread then copies its 3 x 3 neighbourhood of
 // that plane into a register window holding the last three planes, so each
@@ -51,7 +51,7 @@ static_assert(kHaloCells <= BS, "one halo cell per thread at most");
 
 struct PlaneRing {
   float v[kRing][HXY];  // staged value (0 off the free set)
-  float s[kRing][HXY];  // free_s coding of kappa
+  float s[kRing][HXY];  // coupling coding of kappa (see ring_put)
 };
 
 // Thread geometry of a plane: the interior cell (tx, ty) and, for threads

@@ -3,6 +3,8 @@
 Bar: the FP64 residual ||b - A x|| / ||b|| (recomputed by the oracle from the
 returned x) <= 1e-6, and phi within 1e-5 relative L2 of the FP64 direct solve
 where the direct solve is affordable."""
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -57,6 +59,29 @@ def test_orphans_and_isolated_voxels_are_excluded(cuda):
     assert info.converged == 1
     assert np.all(phi[3:6, 6, 6] == 0)  # detached 3-voxel column: not solved
     assert phi[1, 1, 1] > 0 and phi[2, 1, 1] > phi[1, 1, 1]
+
+
+@pytest.mark.parametrize("levels", [2, 0])
+def test_zero_kappa_anchors_and_voxels(cuda, levels):
+    """Anchors of kappa 0 still couple through the face weight (kappa_i + 0) / 2
+    (the streaming kernels code them as -FLT_MIN, not as air), and free
+    voxels of kappa 0 couple through faces only: the solve matches the FP64
+    direct solve of that system (which differs from the unmodified island's
+    by ~150 % relative L2)."""
+    isl = W.small_box_union(2, L=16)
+    k = isl.kappa.copy()
+    k[isl.vtype == 2] = 0.0
+    rng = np.random.default_rng(7)
+    solid = np.argwhere(isl.vtype == 1)
+    pick = solid[rng.choice(len(solid), size=len(solid) // 10, replace=False)]
+    k[tuple(pick.T)] = 0.0
+    isl = dataclasses.replace(isl, kappa=k)
+    info, x, phi = _solve(isl, cuda, mg_levels=levels, tol=1e-8, max_iters=4000)
+    ref = O.direct_solve(isl)
+    assert info.converged == 1, (info.iterations, info.rel_residual)
+    assert O.residual(isl, x) <= 1e-8 * 1.01
+    err = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    assert err <= TOL_PHI, err
 
 
 @pytest.mark.parametrize("prec", [capi.PREC_FP32, capi.PREC_BF16_COARSE])
