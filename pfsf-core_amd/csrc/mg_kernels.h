@@ -712,21 +712,24 @@ __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
 }
 
-// The same coupling with the square root of the FP32 product formed from the
-// hardware reciprocal square root and one FP64 Newton step (relative error
-// about 2e-14 instead of a correctly rounded FP64 square root, at a fraction
-// of its cost): the convergence check needs the residual to far better than
-// its 1e-6 target, not to the last bit.
+// The same coupling with the square root of the FP32 product p formed from
+// the hardware FP32 square root s (relative error below 2^-23) plus one
+// correction e * (0.5 / s), e = p - s^2 (one FP32 FMA, within an FP32
+// rounding of the exact difference, itself about 2^-23 p) and 0.5 / s from
+// the FP32 reciprocal square root: the correction is small, so its FP32
+// error stays near 2^-45 of the root, and the sum is formed in FP64.  The
+// result is within about 3e-14 of the correctly rounded square root at one
+// FP64 add instead of an FP64 square root: the convergence check needs the
+// residual to far better than its 1e-6 target, not to the last bit.
 __device__ __forceinline__ double couple_f64_fast(int cls, float ki, float kj) {
   if (cls == 1) {
     float s = ki + kj;
     return 0.5 * (double)s;
   }
   const float p = ki * kj;
-  const double d = (double)p;
-  double r = (double)__builtin_amdgcn_rsqf(p);
-  r = r * (1.5 - 0.5 * d * r * r);
-  const double sq = p > 0.f ? d * r : 0.0;
+  const float s0 = __builtin_amdgcn_sqrtf(p);
+  const float corr = fmaf(-s0, s0, p) * (0.5f * __builtin_amdgcn_rsqf(p));
+  const double sq = p > 0.f ? (double)s0 + (double)corr : 0.0;
   return sq * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
 }
 
