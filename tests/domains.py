@@ -1,7 +1,12 @@
-"""Test geometries restated from the reference's Java test builders."""
+"""Test geometries restated from the reference's Java test builders, and the
+reference's geometry fixtures (tests/golden/pfsf-fixtures) as islands."""
+import base64
+import json
+import pathlib
+
 import numpy as np
 
-from pfsf_amd.workloads import ANCHOR, SOLID, Island
+from pfsf_amd.workloads import ANCHOR, GRAVITY, SOLID, Island
 
 
 def make(vtype, kappa=1.0, source=None, rcomp=None, rtens=None, name="t"):
@@ -45,3 +50,44 @@ def cantilever(N, pad=0, load=1.0):
     src = np.zeros(t.shape, np.float32)
     src[1:, pad, pad] = load
     return make(t, np.where(t != 0, 1.0, 0.0), src, name=f"cantilever{N}")
+
+
+FIXTURE_DIR = pathlib.Path(__file__).resolve().parent / "golden" / "pfsf-fixtures"
+
+
+def fixture_names():
+    return sorted(p.stem for p in FIXTURE_DIR.glob("*.json"))
+
+
+def reference_fixture(name):
+    """One of the reference's geometry fixtures (schema v1 of
+    Block Reality/api/src/test/resources/pfsf-fixtures/README.md): dims, the
+    anchor list, per-voxel material ids (base64 little-endian int32, flat
+    index (z*ly + y)*lx + x, 0 = air) and the material registry.  Mapped with
+    the workloads' units: kappa = Young's modulus in GPa, b = density * g *
+    1 m^3 * 1e-6 (MN) on solid voxels, rcomp / rtens in MPa; a voxel is an
+    anchor when listed in `anchors` or its material is an anchor material.
+    The fixtures hold no expected outputs: they are inputs only."""
+    d = json.loads((FIXTURE_DIR / f"{name}.json").read_text())
+    lx, ly, lz = d["dims"]["lx"], d["dims"]["ly"], d["dims"]["lz"]
+    ids = np.frombuffer(base64.b64decode(d["materials"]["voxels"]), dtype="<i4")
+    assert ids.size == lx * ly * lz, name
+    ids = ids.reshape(lz, ly, lx)
+    reg = {m["id"]: m for m in d["materials"]["registry"]}
+    vtype = np.zeros(ids.shape, np.uint8)
+    kappa = np.zeros(ids.shape, np.float32)
+    rho = np.zeros(ids.shape, np.float64)
+    rc = np.zeros(ids.shape, np.float32)
+    rt = np.zeros(ids.shape, np.float32)
+    for mid, m in reg.items():
+        sel = ids == mid
+        vtype[sel] = ANCHOR if m.get("is_anchor", False) else SOLID
+        kappa[sel] = np.float32(m["youngs_gpa"])
+        rho[sel] = m["density"]
+        rc[sel] = np.float32(m["rcomp"])
+        rt[sel] = np.float32(m["rtens"])
+    for x, y, z in d["anchors"]:
+        vtype[z, y, x] = ANCHOR
+    src = np.where(vtype == SOLID, rho * GRAVITY * 1e-6, 0.0).astype(np.float32)
+    isl = make(vtype, kappa, src, rc, rt, name=name)
+    return isl, int(d.get("ticks", 0))
