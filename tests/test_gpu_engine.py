@@ -168,3 +168,37 @@ def test_island_components_match_oracle_and_follow_host_types(cuda):
         wrec2, wnum2, _ = O.label_summarise(t2, O.label_sv(t2, 26)[0], 64)
         assert num2 == wnum2 and np.array_equal(rec2, wrec2)
         assert e.lib.pfsf_island_components(e.h, 99, 64, None, None, None) != capi.OK
+
+
+def test_reference_fixtures_in_one_tick(cuda):
+    """The reference's 20 geometry fixtures as 20 islands of one tick (the
+    batched solve, including an island of anchors only): every island's phi
+    matches its direct solve, its history the oracle's psi, and the failure
+    records (strengths lowered so that voxels fail) the oracle's scan."""
+    isls = [domains.reference_fixture(n)[0] for n in domains.fixture_names()]
+    origins = [(16 * k, -8, 3 * k) for k in range(len(isls))]
+    for isl in isls:
+        isl.rcomp[...] = np.where(isl.vtype != 0, np.float32(0.01), np.float32(0))
+        isl.rtens[...] = np.where(isl.vtype != 0, np.float32(0.003), np.float32(0))
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        for k, (isl, o) in enumerate(zip(isls, origins)):
+            assert e.add_island(k + 1, isl.lx, isl.ly, isl.lz, origin=o) == capi.OK
+            assert e.register(k + 1, isl, kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb(list(range(1, len(isls) + 1)), capacity=1 << 14)
+        assert r == capi.OK
+        want = []
+        for k, (isl, o) in enumerate(zip(isls, origins)):
+            phi = e.phi(k + 1).reshape(isl.vtype.shape)
+            ref = O.direct_solve(isl)
+            nref = np.linalg.norm(ref)
+            if nref == 0.0:
+                assert np.all(phi == 0), isl.name
+            else:
+                assert np.linalg.norm(phi - ref) / nref <= 1e-5, isl.name
+            r, H = e.read_history(k + 1, isl.n)
+            assert np.array_equal(H, O.psi(isl, phi).ravel()), isl.name
+            want.append(_expected_events(isl, phi, o))
+        want = np.concatenate(want)
+        assert len(want) > 0
+        assert np.array_equal(ev, want)
