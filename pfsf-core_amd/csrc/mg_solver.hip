@@ -88,11 +88,10 @@ __device__ __forceinline__ float offsum(const Lv<KT>& L, int x, int y, int z, fl
 
 // One-work-group reductions that remain separate kernels: the Jacobi
 // (single-level) iteration, a coarsest level too large for LDS, and the
-// FP64 norms of the solve set-up and the refinement passes.
+// FP64 residual norm of the refinement passes.
 enum FinMode {
   FIN_ALPHA = 0,  // p.q -> alpha = rz / pq
   FIN_UPDATE,     // r.r -> rr, iteration count, convergence
-  FIN_BNORM,      // b.b -> bb
   FIN_IR,         // r.r -> rr, rel
 };
 
@@ -118,9 +117,6 @@ __global__ void __launch_bounds__(BS)
       sc->iters += 1;
       if (v <= sc->stop2 || sc->iters >= sc->max_iters) sc->done = 1;
       if (mirror) publish_progress(mirror, sc->iters, sc->done, v);
-      break;
-    case FIN_BNORM:
-      ir->bb = v;
       break;
     case FIN_IR:
       ir->rr = v;
@@ -455,64 +451,55 @@ __global__ void __launch_bounds__(BS)
                              blockIdx.x, xzero);
 }
 
-// Iterate outside the free set is zero (the caller's guess may not be).
-// The FP64 iterate from the caller's guess, 0 off the free set; gflag[block]
-// records whether the block loaded a nonzero value (no atomics: one word per
-// block, combined by k_guess_flag).
+// Solve set-up over all cells, one pass: the FP64 iterate takes the caller's
+// guess on the free set (0 elsewhere, or everywhere without a guess), each
+// block writes a ||b||^2 partial over the free set and a flag telling whether
+// it saw a nonzero guess (combined by k_begin_fin).
 __global__ void __launch_bounds__(BS)
-    k_load_guess(int64_t n, const double* __restrict__ xin, const uint8_t* __restrict__ fr,
-                 double* __restrict__ x, int* __restrict__ gflag) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double v = 0.0;
-  if (i < n) {
-    v = fr[i] ? xin[i] : 0.0;
-    x[i] = v;
-  }
-  const int any = __syncthreads_or(v != 0.0);
-  if (threadIdx.x == 0) gflag[blockIdx.x] = any;
-}
-
-// *nz = seq (the number of this solve) when any block of k_load_guess saw a
-// nonzero initial guess.
-__global__ void __launch_bounds__(BS)
-    k_guess_flag(const int* __restrict__ gflag, unsigned nblk, int* nz, int seq) {
-  int any = 0;
-  for (unsigned b = threadIdx.x; b < nblk; b += BS) any |= gflag[b];
-  if (__syncthreads_or(any) && threadIdx.x == 0) *nz = seq;
-}
-
-__global__ void __launch_bounds__(BS)
-    k_mask64(int64_t n, const uint8_t* __restrict__ fr, double* __restrict__ x) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n && !fr[i]) x[i] = 0.0;
-}
-
-// ||b||^2 over the free set.
-__global__ void __launch_bounds__(BS)
-    k_bnorm(int64_t n, const uint8_t* __restrict__ fr, const float* __restrict__ b,
+    k_begin(int64_t n, const double* __restrict__ xin, const uint8_t* __restrict__ fr,
+            const float* __restrict__ b, double* __restrict__ x, int* __restrict__ gflag,
             double* partials) {
   __shared__ double sh[1][BS / 64];
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  double v = 0.0;
   double acc[1] = {0.0};
-  if (i < n && fr[i]) acc[0] = (double)b[i] * (double)b[i];
+  if (i < n) {
+    const bool f = fr[i] != 0;
+    v = (xin && f) ? xin[i] : 0.0;
+    x[i] = v;
+    if (f) acc[0] = (double)b[i] * (double)b[i];
+  }
+  const int any = __syncthreads_or(v != 0.0);
+  if (threadIdx.x == 0) gflag[blockIdx.x] = any;
   write_partials<1>(acc, partials, sh);
 }
 
+// ||b||^2 from k_begin's partials, and *nz = seq (the number of this solve)
+// when any of its blocks saw a nonzero initial guess.
 __global__ void __launch_bounds__(BS)
-    k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+    k_begin_fin(const double* __restrict__ partials, const int* __restrict__ gflag,
+                unsigned nblk, IrScalars* ir, int* nz, int seq) {
+  const double bb = block_total(partials, nblk);
+  int any = 0;
+  for (unsigned k = threadIdx.x; k < nblk; k += BS) any |= gflag[k];
+  if (__syncthreads_or(any) && threadIdx.x == 0) *nz = seq;
+  if (threadIdx.x == 0) ir->bb = bb;
+}
+
+// Solution copies: the FP64 iterate to the caller's FP64 and/or FP32 arrays.
+__global__ void __launch_bounds__(BS)
+    k_end(int64_t n, const double* __restrict__ x, double* __restrict__ x64,
+          float* __restrict__ phi32) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) y[i] = (float)x[i];
+  if (i >= n) return;
+  const double v = x[i];
+  if (x64) x64[i] = v;
+  if (phi32) phi32[i] = (float)v;
 }
 
 __global__ void __launch_bounds__(BS) k_zero64(int64_t n, double* __restrict__ v) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i < n) v[i] = 0.0;
-}
-
-__global__ void __launch_bounds__(BS)
-    k_copy64(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) b[i] = a[i];
 }
 
 // (Re)start the PCG recursion from the current true residual: no search
@@ -1203,14 +1190,8 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   chk(hipEventRecord(ev0_, s), err);
   solve_seq_ = solve_seq_ % 0x7ffffffe + 1;  // never 0, the flag's initial value
   const int seq = solve_seq_;
-  if (x64) {
-    k_load_guess<<<nb0, BS, 0, s>>>(n0_, x64, F.fr, x_, guess_flags_);
-    k_guess_flag<<<1, BS, 0, s>>>(guess_flags_, nb0, nz_flag_, seq);
-  } else {
-    k_zero64<<<nb0, BS, 0, s>>>(n0_, x_);
-  }
-  k_bnorm<<<nb0, BS, 0, s>>>(n0_, F.fr, b, partials_);
-  k_finalize<<<1, BS, 0, s>>>(FIN_BNORM, partials_, nb0, sc_, ir_, nullptr);
+  k_begin<<<nb0, BS, 0, s>>>(n0_, x64, F.fr, b, x_, guess_flags_, partials_);
+  k_begin_fin<<<1, BS, 0, s>>>(partials_, guess_flags_, nb0, ir_, nz_flag_, seq);
   double rel = 0.0;
   chk(true_residual(b, &rel, 0, seq), err);
   int passes = 0, total = 0;
@@ -1277,8 +1258,7 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
     rel = nrel;
     if (stalled) break;
   }
-  if (x64) k_copy64<<<nb0, BS, 0, s>>>(n0_, x_, x64);
-  if (phi32) k_to_f32<<<nb0, BS, 0, s>>>(n0_, x_, phi32);
+  if (x64 || phi32) k_end<<<nb0, BS, 0, s>>>(n0_, x_, x64, phi32);
   chk(hipEventRecord(ev1_, s), err);
   chk(hipEventSynchronize(ev1_), err);
   float ms = 0.f;
