@@ -179,14 +179,21 @@ def mixed_precision_bench(torch, D, W, steps=3):
             x.zero_()
             info = s.solve(b, x, ph)
             ms.append(info.solve_ms)
+        # the finest-level kernels at this size (launch events, as for 128^3)
+        kern = {k: s.profile(k, reps=20) for k in KERNEL_NAMES}
         s.close()
         code = torch.empty_like(vt)
         D.failure_scan(vt, kp, ph, rc, rt, code)
         err = float(torch.linalg.norm((ph - pr).double()) / torch.linalg.norm(pr.double()))
+        dn = kern[1]
         out[name] = {"solve_ms": round(float(np.median(ms)), 3), "iterations": info.iterations,
                      "passes": info.refinements, "levels": info.levels,
                      "rel_residual": info.rel_residual, "phi_rel_l2_vs_reference": err,
-                     "failure_codes_differing": int((code != code_ref).sum())}
+                     "failure_codes_differing": int((code != code_ref).sum()),
+                     "kernel_ms": {KERNEL_NAMES[k]: round(v, 5) for k, v in kern.items()},
+                     "ks_down_alg_GBps": round(ALG_BYTES[1] * isl.n_active / (dn * 1e-3) / 1e9, 1),
+                     "ks_down_frac": round(ALG_BYTES[1] * isl.n_active / (dn * 1e-3) / HBM_PEAK,
+                                           4)}
         log(f"256^3 {name}: {out[name]}")
     return out
 

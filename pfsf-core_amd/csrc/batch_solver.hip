@@ -95,9 +95,9 @@ __global__ void __launch_bounds__(BS)
   if (t.level < 0) return;
   const BatchLevel& D = lv[t.level];
   const BatchIsland& I = isl[D.island];
-  spmv_stream_body<TZ>(lv_of(D), true, t.x0, t.y0, t.z0, I.z, c ? I.p1 : I.p0, c ? I.p0 : I.p1,
-                       I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c, t.slot == I.tb,
-                       pq_part, (unsigned)t.slot);
+  spmv_stream_body<TZ>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, I.z, c ? I.p1 : I.p0,
+                       c ? I.p0 : I.p1, I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c,
+                       t.slot == I.tb, pq_part, (unsigned)t.slot);
 }
 
 template <int TZ, bool UPDATE>
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_stream_body<TZ, UPDATE>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
+  down_stream_body<TZ, UPDATE>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin, rout, I.q,
                                c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
                                (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
 }
@@ -124,8 +124,8 @@ __global__ void __launch_bounds__(BS)
   if (t.level < 0) return;
   const BatchLevel& D = lv[t.level];
   const BatchIsland& I = isl[D.island];
-  up_stream_body<TZ, true>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, ridx ? I.r1 : I.r0, D.ec, I.z,
-                           sc + D.island, rz_part, (unsigned)t.slot);
+  up_stream_body<TZ, true>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, D.c, ridx ? I.r1 : I.r0, D.ec,
+                           I.z, sc + D.island, rz_part, (unsigned)t.slot);
 }
 
 // Every island's coarsest level, one work-group each; with `check` the

@@ -94,6 +94,7 @@ constexpr uint32_t kAllCouplings = 0x07FFFFFFu;
 struct TL {
   int ntx, nty;
   const int* ids;
+  int n;  // entries in ids (the launch list, a multiple of 8)
 };
 
 // ── LDS tile machinery ─────────────────────────────────────────────────

@@ -1,6 +1,7 @@
 // Multigrid-preconditioned PCG solver object (host side of mg_solver.hip).
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "pfsf_internal.h"
@@ -35,6 +36,22 @@ constexpr int kCoarseSweeps = 6;
 #ifndef PFSF_PASS_FLOOR
 #define PFSF_PASS_FLOOR 2e-6
 #endif
+
+// Launch shapes of the finest level's plane-streaming kernels in a single
+// solve.  Each consumer of a partial sum reads one partial per work-group of
+// its producer, so with one work-group per column the partials read per
+// launch grow with the square of the column count (8,192 columns at 256^3:
+// 0.5 GB of L2 reads per kernel).  Lists of up to kStreamOneShot columns
+// (the 128^3 island: 952) take one column per work-group, as many as 256 CUs
+// hold at four 256-thread work-groups each; longer lists are walked in
+// strides by kStreamGroups work-groups (a multiple of 8: three per CU, what
+// the column-loop kernels' register use allows resident at once).
+#ifndef PFSF_STREAM_GROUPS
+#define PFSF_STREAM_GROUPS 768
+#endif
+constexpr int kStreamOneShot = 1024;
+constexpr int kStreamGroups = PFSF_STREAM_GROUPS;
+static_assert(kStreamGroups % 8 == 0, "XCD chunks need a multiple of 8 work-groups");
 
 // Tile depths of a batchable hierarchy (finest level: 8-plane streaming
 // columns; coarse levels).
@@ -126,6 +143,12 @@ class MgSolver {
   hipError_t build_tiles(MgLevel& L);
   hipError_t precondition(int c);       // z = M r[c], r.z
   hipError_t true_residual(const float* b, double* rel, int c, int seq = 0);  // b - A x into r[c]
+  // the finest streaming kernels walk their columns in strides
+  bool fine_loop() const { return levels_[0].tiles.nlaunch > kStreamOneShot; }
+  // their launch size (and partial-sum count)
+  unsigned fine_groups() const {
+    return fine_loop() ? (unsigned)kStreamGroups : (unsigned)levels_[0].tiles.nlaunch;
+  }
   hipError_t iteration(int c);         // one PCG iteration reading r[c], p[c]
   hipError_t vcycle_below(int from, bool check);  // levels from..L-1 given level `from` residual
   // n = 1, 2 or 4 iterations starting at parity c (graph replay once captured)

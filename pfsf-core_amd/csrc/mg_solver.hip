@@ -596,7 +596,9 @@ static Lv<float> sview(const MgLevel& l) {
   return v;
 }
 
-static TL tl(const MgLevel& l) { return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids}; }
+static TL tl(const MgLevel& l) {
+  return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids, l.tiles.nlaunch};
+}
 
 static void fill0(float* p, int64_t n, hipStream_t s) {
   k_fill<<<blocks_for(n, BS), BS, 0, s>>>(n, p, 0.f);
@@ -941,6 +943,18 @@ hipError_t MgSolver::update(const uint8_t* type, const float* kappa) {
   } else                           \
     PFSF_TZ_DISPATCH(L, CALL)
 
+// The finest level's streaming kernels with their column walk chosen at run
+// time (LOOPV: a long column list walked in strides, else one column per
+// work-group).
+#define PFSF_STREAM_DISPATCH(L, LOOP, CALL) \
+  if (LOOP) {                               \
+    constexpr bool LOOPV = true;            \
+    PFSF_TZF_DISPATCH(L, CALL);             \
+  } else {                                  \
+    constexpr bool LOOPV = false;           \
+    PFSF_TZF_DISPATCH(L, CALL);             \
+  }
+
 template <typename KT>
 static void launch_down_t(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
                           hipStream_t s) {
@@ -991,7 +1005,7 @@ static void launch_up(const MgLevel& A, const MgLevel& B, const float* r, float*
 // it fits LDS, else as a finalize kernel.
 hipError_t MgSolver::vcycle_below(int from, bool check) {
   hipStream_t s = stream_;
-  const unsigned G = levels_[0].tiles.nlaunch;
+  const unsigned G = fine_groups();  // r.r partials of the finest update
   MgLevel& C = levels_[nlev_ - 1];
   const bool lds = C.n <= kMaxLdsCoarse && padded_cells(C.d) <= kMaxLdsPadded;
   int st = std::min(coarse_sweeps_, 63);
@@ -1040,13 +1054,14 @@ hipError_t MgSolver::precondition(int c) {
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
-  const unsigned G = F.tiles.nlaunch;
-  PFSF_TZF_DISPATCH(F, (ks_down<TZV, false><<<G, BS, 0, s>>>(
-                          sview(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr, nullptr,
-                          B.r, sc_, nullptr, 0u, nullptr)));
+  const unsigned G = fine_groups();
+  const bool loop = fine_loop();
+  PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, false, LOOPV><<<G, BS, 0, s>>>(
+                                    sview(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr,
+                                    nullptr, B.r, sc_, nullptr, 0u, nullptr)));
   (void)vcycle_below(1, false);
-  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(sview(F), tl(F), B.d, r_[c], B.e,
-                                                          z_, sc_, prz_)));
+  PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<G, BS, 0, s>>>(
+                                    sview(F), tl(F), B.d, r_[c], B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
 
@@ -1068,14 +1083,17 @@ hipError_t MgSolver::iteration(int c) {
     return hipGetLastError();
   }
   MgLevel& B = levels_[1];
-  PFSF_TZF_DISPATCH(F, (ks_spmv<TZV><<<G, BS, 0, s>>>(sview(F), tl(F), z_, p_[c],
-                                                      p_[1 - c], q_, sc_, prz_, G, c, ppq_)));
-  PFSF_TZF_DISPATCH(F, (ks_down<TZV, true><<<G, BS, 0, s>>>(
-                          sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_, B.r,
-                          sc_, ppq_, G, prr_)));
+  const unsigned GS = fine_groups();
+  const bool loop = fine_loop();
+  PFSF_STREAM_DISPATCH(F, loop, (ks_spmv<TZV, LOOPV><<<GS, BS, 0, s>>>(
+                                    sview(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prz_, GS, c,
+                                    ppq_)));
+  PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
+                                    sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_,
+                                    B.r, sc_, ppq_, GS, prr_)));
   (void)vcycle_below(1, true);
-  PFSF_TZF_DISPATCH(F, (ks_up<TZV, true><<<G, BS, 0, s>>>(sview(F), tl(F), B.d, r_[1 - c],
-                                                          B.e, z_, sc_, prz_)));
+  PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
+                                    sview(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, prz_)));
   return hipGetLastError();
 }
 
@@ -1287,7 +1305,9 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
-  const unsigned G = F.tiles.nlaunch;
+  const unsigned G = F.tiles.nlaunch;  // tiled launches
+  const unsigned GS = nlev_ > 1 ? fine_groups() : G;  // streaming launches
+  const bool loop = nlev_ > 1 && fine_loop();
   if (which < 0 || which > 4 || reps <= 0) return hipErrorInvalidValue;
   if ((which == 2 || which == 3) && nlev_ < 2) return hipErrorInvalidValue;
   if (which == 2 && nlev_ < 3) return hipErrorInvalidValue;
@@ -1315,9 +1335,9 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
                                   (const float*)z_, (const float*)p_[0], p_[1], q_, sc_,
                                   (const double*)(prr_ + nb), nb, 0, ppq_));
         } else {
-          PFSF_TZF_DISPATCH(F, ext(ks_spmv<TZV>, G, a, b, sview(F), tl(F), (const float*)z_,
-                                  (const float*)p_[0], p_[1], q_, sc_, (const double*)prz_, G, 0,
-                                  ppq_));
+          PFSF_STREAM_DISPATCH(F, loop, ext(ks_spmv<TZV, LOOPV>, GS, a, b, sview(F), tl(F),
+                                            (const float*)z_, (const float*)p_[0], p_[1], q_,
+                                            sc_, (const double*)prz_, GS, 0, ppq_));
         }
         break;
       case 1:
@@ -1326,10 +1346,10 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
           ext(k_update_jacobi, nb, a, b, n0_, (const float*)F.dg, x_, (const float*)r_[0], r_[1],
               (const float*)p_[1], (const float*)q_, z_, (const PcgScalars*)sc_, prr_);
         } else {
-          PFSF_TZF_DISPATCH(F, ext(ks_down<TZV, true>, G, a, b, sview(F), tl(F),
-                                  levels_[1].d, (const float*)r_[0], r_[1], (const float*)q_,
-                                  (const float*)p_[1], x_, levels_[1].r, sc_,
-                                  (const double*)ppq_, G, prr_));
+          PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV>, GS, a, b, sview(F),
+                                            tl(F), levels_[1].d, (const float*)r_[0], r_[1],
+                                            (const float*)q_, (const float*)p_[1], x_,
+                                            levels_[1].r, sc_, (const double*)ppq_, GS, prr_));
         }
         break;
       case 2: {
@@ -1351,9 +1371,10 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
         break;
       }
       default:
-        PFSF_TZF_DISPATCH(F, ext(ks_up<TZV, true>, G, a, b, sview(F), tl(F), levels_[1].d,
-                                (const float*)r_[1], (const float*)levels_[1].e, z_,
-                                (const PcgScalars*)sc_, prz_));
+        PFSF_STREAM_DISPATCH(F, loop, ext(ks_up<TZV, true, LOOPV>, GS, a, b, sview(F), tl(F),
+                                          levels_[1].d, (const float*)r_[1],
+                                          (const float*)levels_[1].e, z_,
+                                          (const PcgScalars*)sc_, prz_));
     }
   };
   once(nullptr, nullptr);  // warm

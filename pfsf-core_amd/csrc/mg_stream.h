@@ -186,6 +186,56 @@ __device__ __forceinline__ void restrict_pair(float a, int tx, int ty, int gx, i
   if (!(tx & 1) && !(ty & 1) && X < c.lx && Y < c.ly && Z < c.lz) st(rc, flat32(X, Y, Z, c), a);
 }
 
+// Columns of one work-group.  get<TZ>(k, x0, y0, z0) gives its k-th column
+// (k = 0, 1, ...): 1 = a column at (x0, y0, z0), 0 = an empty slot of the
+// list (nothing to stream), -1 = no columns left.  A work-group carries its
+// PCG scalars, LDS ring and partial sum across its columns.
+//
+// StrideColumns: list slots blockIdx.x + k * gridDim.x of a single system's
+// tile list, so a grid of at most a few work-groups per CU walks a large
+// grid's columns and the consumers of its partial sums read that many
+// partials, not one per column.  gridDim.x is a multiple of 8 and slot p of
+// the list belongs to XCD chunk p mod 8, so all columns of a work-group come
+// from the chunk of the XCD it runs on.
+struct StrideColumns {
+  TL M;
+  template <int TZ>
+  __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
+    const int i = (int)blockIdx.x + k * (int)gridDim.x;
+    if (i >= M.n) return -1;
+    const int t = M.ids[i];
+    if (t < 0) return 0;
+    const int r = t / M.ntx;
+    x0 = (t % M.ntx) * TX;
+    y0 = (r % M.nty) * TY;
+    z0 = (r / M.nty) * TZ;
+    return 1;
+  }
+};
+
+// SlotColumn: the column of list slot blockIdx.x, one per work-group.
+struct SlotColumn {
+  TL M;
+  template <int TZ>
+  __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
+    if (k > 0) return -1;
+    return tile_origin<TZ>(M, x0, y0, z0) ? 1 : 0;
+  }
+};
+
+// OneColumn: the column of a batch tile record.
+struct OneColumn {
+  int x0, y0, z0;
+  template <int TZ>
+  __device__ __forceinline__ int get(int k, int& a, int& b, int& c) const {
+    if (k > 0) return -1;
+    a = x0;
+    b = y0;
+    c = z0;
+    return 1;
+  }
+};
+
 // ── p_out = z + beta p_in; q = A p_out; p.q (see spmv_body) ─────────────
 struct SpRaw {
   PlaneIdx ix;
@@ -204,21 +254,25 @@ __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, c
   R.hp = ld(pin, R.ix.h);
 }
 
-// Column (x0, y0, z0) of one system; `publish` (one work-group per system)
-// stores the PCG scalars, the p.q partial goes to partials[pslot].
-template <int TZ>
+// The columns of `cols` of one system; `publish` (one work-group per system)
+// stores the PCG scalars, the p.q partial over the work-group's columns goes
+// to partials[pslot].
+template <int TZ, typename Cols>
 __device__ __forceinline__ void spmv_stream_body(
-    const Lv<float>& L, bool have, int x0, int y0, int z0, const float* __restrict__ z,
+    const Lv<float>& L, const Cols& cols, const float* __restrict__ z,
     const float* __restrict__ pin, float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
     const double* __restrict__ rz_part, unsigned rz_n, int slot, bool publish, double* partials,
     unsigned pslot) {
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
+  static_assert((TZ + 2) % kRing == 0, "ring slots continue across columns");
   // the first planes' loads do not depend on beta: they are in flight while
   // the done flag and the r.z partials are read
-  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  int x0 = 0, y0 = 0, z0 = 0;
+  int col = cols.template get<TZ>(0, x0, y0, z0);
+  PlaneGeom g = plane_geom(x0, y0, L.d);
   SpRaw buf[kPD];
-  if (have)
+  if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
@@ -234,8 +288,9 @@ __device__ __forceinline__ void spmv_stream_body(
     sc->beta = beta64;
   }
   double acc = 0.0;
-  if (have) {
-    const float beta = (float)beta64;
+  const float beta = (float)beta64;
+  for (int k = 1; col >= 0; k++) {
+    if (col > 0) {
     float pv_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
@@ -268,19 +323,32 @@ __device__ __forceinline__ void spmv_stream_body(
         if (ix_own[o].iv) st(q, ix_own[o].i, qi);
       }
     });
+    }
+    col = cols.template get<TZ>(k, x0, y0, z0);
+    if (col > 0) {
+      g = plane_geom(x0, y0, L.d);
+      static_for<kPD>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
+      });
+    }
   }
   write_partial_at(acc, partials, sh, pslot);
 }
 
-template <int TZ>
+// LOOP: the work-groups walk the list in strides (StrideColumns); else one
+// column per work-group (SlotColumn).
+template <int TZ, bool LOOP>
 __global__ void __launch_bounds__(BS)
     ks_spmv(Lv<float> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
             float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
             const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  spmv_stream_body<TZ>(L, have, x0, y0, z0, z, pin, pout, q, sc, rz_part, rz_n, slot,
-                       blockIdx.x == 0, partials, blockIdx.x);
+  if constexpr (LOOP)
+    spmv_stream_body<TZ>(L, StrideColumns{M}, z, pin, pout, q, sc, rz_part, rz_n, slot,
+                         blockIdx.x == 0, partials, blockIdx.x);
+  else
+    spmv_stream_body<TZ>(L, SlotColumn{M}, z, pin, pout, q, sc, rz_part, rz_n, slot,
+                         blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── down-sweep of the finest level (see down_body) ─────────────────────
@@ -310,20 +378,23 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, c
   R.hdg = ld(L.dg, R.ix.h);
 }
 
-template <int TZ, bool UPDATE>
+template <int TZ, bool UPDATE, typename Cols>
 __device__ __forceinline__ void down_stream_body(
-    const Lv<float>& L, bool have, int x0, int y0, int z0, pfsf_dims c,
+    const Lv<float>& L, const Cols& cols, pfsf_dims c,
     const float* __restrict__ rin, float* __restrict__ rout, const float* __restrict__ q,
     const float* __restrict__ p, double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
     const double* __restrict__ pq_part, unsigned pq_n, bool publish, double* partials,
     unsigned pslot) {
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
+  static_assert((TZ + 2) % kRing == 0, "ring slots continue across columns");
   // the first planes' loads do not depend on alpha: they are in flight while
   // the done flag and the p.q partials are read
-  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  int x0 = 0, y0 = 0, z0 = 0;
+  int col = cols.template get<TZ>(0, x0, y0, z0);
+  PlaneGeom g = plane_geom(x0, y0, L.d);
   DnRaw buf[kPD];
-  if (have)
+  if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
@@ -344,9 +415,10 @@ __device__ __forceinline__ void down_stream_body(
     if (!ok) return;
   }
   double acc = 0.0;
-  if (have) {
-    const float alpha = (float)alpha64, om = L.omega;
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const float alpha = (float)alpha64, om = L.omega;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  for (int k = 1; col >= 0; k++) {
+    if (col > 0) {
     float r_own[2] = {0.f, 0.f};
     bool fr_own[2] = {false, false};
     float pair = 0.f;
@@ -393,20 +465,31 @@ __device__ __forceinline__ void down_stream_body(
         }
       }
     });
+    }
+    col = cols.template get<TZ>(k, x0, y0, z0);
+    if (col > 0) {
+      g = plane_geom(x0, y0, L.d);
+      static_for<kPD>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
+      });
+    }
   }
   if (UPDATE) write_partial_at(acc, partials, sh, pslot);
 }
 
-template <int TZ, bool UPDATE>
+template <int TZ, bool UPDATE, bool LOOP>
 __global__ void __launch_bounds__(BS)
     ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
             float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
             double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
             const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  down_stream_body<TZ, UPDATE>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n,
-                               blockIdx.x == 0, partials, blockIdx.x);
+  if constexpr (LOOP)
+    down_stream_body<TZ, UPDATE>(L, StrideColumns{M}, c, rin, rout, q, p, x, rc, sc, pq_part,
+                                 pq_n, blockIdx.x == 0, partials, blockIdx.x);
+  else
+    down_stream_body<TZ, UPDATE>(L, SlotColumn{M}, c, rin, rout, q, p, x, rc, sc, pq_part,
+                                 pq_n, blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── up-sweep of the finest level (see up_body) ─────────────────────────
@@ -432,25 +515,29 @@ __device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, c
   R.hec = ld(ec, R.ix.hv ? flat32(g.hx >> 1, g.hy >> 1, zc, c) : 0);
 }
 
-template <int TZ, bool FINEST>
-__device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, int x0, int y0,
-                                               int z0, pfsf_dims c, const float* __restrict__ r,
+template <int TZ, bool FINEST, typename Cols>
+__device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& cols,
+                                               pfsf_dims c, const float* __restrict__ r,
                                                const float* __restrict__ ec,
                                                float* __restrict__ out, const PcgScalars* sc,
                                                double* partials, unsigned pslot) {
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
-  const PlaneGeom g = plane_geom(x0, y0, L.d);
+  static_assert((TZ + 2) % kRing == 0, "ring slots continue across columns");
+  int x0 = 0, y0 = 0, z0 = 0;
+  int col = cols.template get<TZ>(0, x0, y0, z0);
+  PlaneGeom g = plane_geom(x0, y0, L.d);
   UpRaw buf[kPD];
-  if (have)  // in flight while the done flag is read
+  if (col > 0)  // in flight while the done flag is read
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
       up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
     });
   if (sc->done) return;
   double acc = 0.0;
-  if (have) {
-    const float om = L.omega;
+  const float om = L.omega;
+  for (int k = 1; col >= 0; k++) {
+    if (col > 0) {
     float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
     bool fr_own[2] = {false, false};
     PlaneIdx ix_own[2];
@@ -484,18 +571,28 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, bool have, in
         if (ix_own[o].iv) st(out, ix_own[o].i, zi);
       }
     });
+    }
+    col = cols.template get<TZ>(k, x0, y0, z0);
+    if (col > 0) {
+      g = plane_geom(x0, y0, L.d);
+      static_for<kPD>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
+      });
+    }
   }
   if (FINEST) write_partial_at(acc, partials, sh, pslot);
 }
 
-template <int TZ, bool FINEST>
+template <int TZ, bool FINEST, bool LOOP>
 __global__ void __launch_bounds__(BS)
     ks_up(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ r,
           const float* __restrict__ ec, float* __restrict__ out, const PcgScalars* sc,
           double* partials) {
-  int x0 = 0, y0 = 0, z0 = 0;
-  const bool have = tile_origin<TZ>(M, x0, y0, z0);
-  up_stream_body<TZ, FINEST>(L, have, x0, y0, z0, c, r, ec, out, sc, partials, blockIdx.x);
+  if constexpr (LOOP)
+    up_stream_body<TZ, FINEST>(L, StrideColumns{M}, c, r, ec, out, sc, partials, blockIdx.x);
+  else
+    up_stream_body<TZ, FINEST>(L, SlotColumn{M}, c, r, ec, out, sc, partials, blockIdx.x);
 }
 
 }  // namespace
