@@ -451,25 +451,28 @@ __global__ void __launch_bounds__(BS)
                              blockIdx.x, xzero);
 }
 
-// Solve set-up over all cells, one pass: the FP64 iterate takes the caller's
-// guess on the free set (0 elsewhere, or everywhere without a guess), each
-// block writes a ||b||^2 partial over the free set and a flag telling whether
-// it saw a nonzero guess (combined by k_begin_fin).
+// Solve set-up over all cells, one pass (grid-stride, at most kBeginBlocks
+// work-groups): the FP64 iterate takes the caller's guess on the free set (0
+// elsewhere, or everywhere without a guess), each work-group writes a ||b||^2
+// partial over the free set and a flag telling whether it saw a nonzero
+// guess (combined by k_begin_fin).
+constexpr unsigned kBeginBlocks = 1024;
+
 __global__ void __launch_bounds__(BS)
     k_begin(int64_t n, const double* __restrict__ xin, const uint8_t* __restrict__ fr,
             const float* __restrict__ b, double* __restrict__ x, int* __restrict__ gflag,
             double* partials) {
   __shared__ double sh[1][BS / 64];
-  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  double v = 0.0;
+  bool nonzero = false;
   double acc[1] = {0.0};
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
     const bool f = fr[i] != 0;
-    v = (xin && f) ? xin[i] : 0.0;
+    const double v = (xin && f) ? xin[i] : 0.0;
     x[i] = v;
-    if (f) acc[0] = (double)b[i] * (double)b[i];
+    nonzero |= v != 0.0;
+    if (f) acc[0] += (double)b[i] * (double)b[i];
   }
-  const int any = __syncthreads_or(v != 0.0);
+  const int any = __syncthreads_or(nonzero);
   if (threadIdx.x == 0) gflag[blockIdx.x] = any;
   write_partials<1>(acc, partials, sh);
 }
@@ -1208,8 +1211,9 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   chk(hipEventRecord(ev0_, s), err);
   solve_seq_ = solve_seq_ % 0x7ffffffe + 1;  // never 0, the flag's initial value
   const int seq = solve_seq_;
-  k_begin<<<nb0, BS, 0, s>>>(n0_, x64, F.fr, b, x_, guess_flags_, partials_);
-  k_begin_fin<<<1, BS, 0, s>>>(partials_, guess_flags_, nb0, ir_, nz_flag_, seq);
+  const unsigned nbg = std::min(nb0, kBeginBlocks);
+  k_begin<<<nbg, BS, 0, s>>>(n0_, x64, F.fr, b, x_, guess_flags_, partials_);
+  k_begin_fin<<<1, BS, 0, s>>>(partials_, guess_flags_, nbg, ir_, nz_flag_, seq);
   double rel = 0.0;
   chk(true_residual(b, &rel, 0, seq), err);
   int passes = 0, total = 0;
