@@ -185,3 +185,29 @@ def test_anchored_mask_matches_bfs(cuda):
         r, _ = O.anchored(isl)
         want = (r.astype(bool) & (isl.vtype == 1)).astype(np.uint8)
         assert np.array_equal(reach.cpu().numpy(), want)
+
+
+def test_workload_128_scan_and_history_bit_exact(cuda):
+    """Full size: the 128^3 island solved on the GPU, then the failure scan and
+    H = max(H, psi_e) of that phi on the GPU and in the oracle agree bit for
+    bit (the solution carries both crushing and tension codes)."""
+    isl = W.workload(1)
+    d = _isl_dev(isl, cuda)
+    s = D.Solver(d["vtype"], d["kappa"], tol=1e-6, max_iters=2000)
+    phi = torch.empty_like(d["kappa"])
+    info = s.solve(d["source"], None, phi)
+    s.close()
+    assert info.converged == 1
+    p = phi.cpu().numpy()
+    code = torch.empty_like(d["vtype"])
+    D.failure_scan(d["vtype"], d["kappa"], phi, d["rcomp"], d["rtens"], code)
+    got = code.cpu().numpy()
+    want = O.failure_scan(isl, p)
+    hist = np.bincount(want.ravel(), minlength=5)
+    assert hist[2] > 0 and hist[4] > 0, hist
+    assert np.array_equal(got, want)
+    h0 = np.zeros(isl.vtype.shape, np.float32)
+    H = torch.from_numpy(h0).to(cuda)
+    D.history_update(d["vtype"], d["kappa"], phi, H)
+    assert np.array_equal(H.cpu().numpy().view(np.uint32),
+                          O.history_update(isl, p, h0).view(np.uint32))
