@@ -291,38 +291,38 @@ __device__ __forceinline__ void spmv_stream_body(
   const float beta = (float)beta64;
   for (int k = 1; col >= 0; k++) {
     if (col > 0) {
-    float pv_own[2] = {0.f, 0.f};
-    bool fr_own[2] = {false, false};
-    PlaneIdx ix_own[2];
-    Win W[3];
-    static_for<TZ + 2>([&](auto S_) {
-      constexpr int s = decltype(S_)::value;
-      const SpRaw cur = buf[s % kPD];
-      if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, z, pin);
-      const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -INFINITY;
-      const float pv = ki >= 0.f ? cur.z + beta * cur.p : 0.f;
-      ring_put(R, sl, g.li, ki, pv);
-      if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
-        ring_put(R, sl, g.lh, kh, cur.hz + beta * cur.hp);
-      }
-      if constexpr (s >= 1 && s <= TZ) {
-        if (cur.ix.iv) st(pout, cur.ix.i, pv);
-        pv_own[s & 1] = pv;
-        fr_own[s & 1] = ki >= 0.f;
-        ix_own[s & 1] = cur.ix;
-      }
-      __syncthreads();
-      read_win(R, sl, g.li, W[s % 3]);
-      if constexpr (s >= 2) {
-        const int o = (s - 1) & 1;
-        float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
-        qi = fr_own[o] ? qi : 0.f;
-        acc += (double)pv_own[o] * qi;
-        if (ix_own[o].iv) st(q, ix_own[o].i, qi);
-      }
-    });
+      float pv_own[2] = {0.f, 0.f};
+      bool fr_own[2] = {false, false};
+      PlaneIdx ix_own[2];
+      Win W[3];
+      static_for<TZ + 2>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        const SpRaw cur = buf[s % kPD];
+        if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, z, pin);
+        const int sl = s % kRing;
+        const float ki = cur.ix.iv ? cur.k : -INFINITY;
+        const float pv = ki >= 0.f ? cur.z + beta * cur.p : 0.f;
+        ring_put(R, sl, g.li, ki, pv);
+        if (g.hown) {
+          const float kh = cur.ix.hv ? cur.hk : -INFINITY;
+          ring_put(R, sl, g.lh, kh, cur.hz + beta * cur.hp);
+        }
+        if constexpr (s >= 1 && s <= TZ) {
+          if (cur.ix.iv) st(pout, cur.ix.i, pv);
+          pv_own[s & 1] = pv;
+          fr_own[s & 1] = ki >= 0.f;
+          ix_own[s & 1] = cur.ix;
+        }
+        __syncthreads();
+        read_win(R, sl, g.li, W[s % 3]);
+        if constexpr (s >= 2) {
+          const int o = (s - 1) & 1;
+          float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
+          qi = fr_own[o] ? qi : 0.f;
+          acc += (double)pv_own[o] * qi;
+          if (ix_own[o].iv) st(q, ix_own[o].i, qi);
+        }
+      });
     }
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
@@ -419,52 +419,52 @@ __device__ __forceinline__ void down_stream_body(
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   for (int k = 1; col >= 0; k++) {
     if (col > 0) {
-    float r_own[2] = {0.f, 0.f};
-    bool fr_own[2] = {false, false};
-    float pair = 0.f;
-    Win W[3];
-    static_for<TZ + 2>([&](auto S_) {
-      constexpr int s = decltype(S_)::value;
-      const DnRaw cur = buf[s % kPD];
-      if constexpr (s + kPD < TZ + 2)
-        dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, L, rin, q, p, x);
-      const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -INFINITY;
-      const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
-      ring_put(R, sl, g.li, ki, om * ri * rcp(cur.dg));
-      if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
-        const float rh = UPDATE ? cur.hr - alpha * cur.hq : cur.hr;
-        ring_put(R, sl, g.lh, kh, om * rh * rcp(cur.hdg));
-      }
-      if constexpr (s >= 1 && s <= TZ) {
-        const bool fr = ki >= 0.f;
-        if (UPDATE) {
-          // p and q are zero outside the free set, so the update leaves x
-          // there unchanged and carries r there over
-          if (cur.ix.iv) {
-            st(x, cur.ix.i, cur.x + alpha64 * (double)cur.p);
-            st(rout, cur.ix.i, ri);
+      float r_own[2] = {0.f, 0.f};
+      bool fr_own[2] = {false, false};
+      float pair = 0.f;
+      Win W[3];
+      static_for<TZ + 2>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        const DnRaw cur = buf[s % kPD];
+        if constexpr (s + kPD < TZ + 2)
+          dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, L, rin, q, p, x);
+        const int sl = s % kRing;
+        const float ki = cur.ix.iv ? cur.k : -INFINITY;
+        const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
+        ring_put(R, sl, g.li, ki, om * ri * rcp(cur.dg));
+        if (g.hown) {
+          const float kh = cur.ix.hv ? cur.hk : -INFINITY;
+          const float rh = UPDATE ? cur.hr - alpha * cur.hq : cur.hr;
+          ring_put(R, sl, g.lh, kh, om * rh * rcp(cur.hdg));
+        }
+        if constexpr (s >= 1 && s <= TZ) {
+          const bool fr = ki >= 0.f;
+          if (UPDATE) {
+            // p and q are zero outside the free set, so the update leaves x
+            // there unchanged and carries r there over
+            if (cur.ix.iv) {
+              st(x, cur.ix.i, cur.x + alpha64 * (double)cur.p);
+              st(rout, cur.ix.i, ri);
+            }
+            acc += fr ? (double)ri * ri : 0.0;
           }
-          acc += fr ? (double)ri * ri : 0.0;
+          r_own[s & 1] = ri;
+          fr_own[s & 1] = fr;
         }
-        r_own[s & 1] = ri;
-        fr_own[s & 1] = fr;
-      }
-      __syncthreads();
-      read_win(R, sl, g.li, W[s % 3]);
-      if constexpr (s >= 2) {
-        const int o = (s - 1) & 1;
-        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
-        const float res = fr_own[o] ? r_own[o] - ax : 0.f;
-        const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
-        if constexpr ((s & 1) == 0) {
-          pair = res;
-        } else {
-          restrict_pair(pair + res, tx, ty, g.gx, g.gy, zo - 1, c, rc);
+        __syncthreads();
+        read_win(R, sl, g.li, W[s % 3]);
+        if constexpr (s >= 2) {
+          const int o = (s - 1) & 1;
+          const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
+          const float res = fr_own[o] ? r_own[o] - ax : 0.f;
+          const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
+          if constexpr ((s & 1) == 0) {
+            pair = res;
+          } else {
+            restrict_pair(pair + res, tx, ty, g.gx, g.gy, zo - 1, c, rc);
+          }
         }
-      }
-    });
+      });
     }
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
@@ -538,39 +538,39 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
   const float om = L.omega;
   for (int k = 1; col >= 0; k++) {
     if (col > 0) {
-    float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
-    bool fr_own[2] = {false, false};
-    PlaneIdx ix_own[2];
-    Win W[3];
-    static_for<TZ + 2>([&](auto S_) {
-      constexpr int s = decltype(S_)::value;
-      const UpRaw cur = buf[s % kPD];
-      if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, c, r, ec);
-      const int sl = s % kRing;
-      const float ki = cur.ix.iv ? cur.k : -INFINITY;
-      ring_put(R, sl, g.li, ki, om * cur.r * rcp(cur.dg) + cur.ec);
-      if (g.hown) {
-        const float kh = cur.ix.hv ? cur.hk : -INFINITY;
-        ring_put(R, sl, g.lh, kh, om * cur.hr * rcp(cur.hdg) + cur.hec);
-      }
-      if constexpr (s >= 1 && s <= TZ) {
-        r_own[s & 1] = cur.r;
-        dg_own[s & 1] = cur.dg;
-        fr_own[s & 1] = ki >= 0.f;
-        ix_own[s & 1] = cur.ix;
-      }
-      __syncthreads();
-      read_win(R, sl, g.li, W[s % 3]);
-      if constexpr (s >= 2) {
-        const int o = (s - 1) & 1;
-        const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
-        const float xi = W[(s - 1) % 3].v[4];
-        float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
-        zi = fr_own[o] ? zi : 0.f;
-        if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
-        if (ix_own[o].iv) st(out, ix_own[o].i, zi);
-      }
-    });
+      float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
+      bool fr_own[2] = {false, false};
+      PlaneIdx ix_own[2];
+      Win W[3];
+      static_for<TZ + 2>([&](auto S_) {
+        constexpr int s = decltype(S_)::value;
+        const UpRaw cur = buf[s % kPD];
+        if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, c, r, ec);
+        const int sl = s % kRing;
+        const float ki = cur.ix.iv ? cur.k : -INFINITY;
+        ring_put(R, sl, g.li, ki, om * cur.r * rcp(cur.dg) + cur.ec);
+        if (g.hown) {
+          const float kh = cur.ix.hv ? cur.hk : -INFINITY;
+          ring_put(R, sl, g.lh, kh, om * cur.hr * rcp(cur.hdg) + cur.hec);
+        }
+        if constexpr (s >= 1 && s <= TZ) {
+          r_own[s & 1] = cur.r;
+          dg_own[s & 1] = cur.dg;
+          fr_own[s & 1] = ki >= 0.f;
+          ix_own[s & 1] = cur.ix;
+        }
+        __syncthreads();
+        read_win(R, sl, g.li, W[s % 3]);
+        if constexpr (s >= 2) {
+          const int o = (s - 1) & 1;
+          const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
+          const float xi = W[(s - 1) % 3].v[4];
+          float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
+          zi = fr_own[o] ? zi : 0.f;
+          if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
+          if (ix_own[o].iv) st(out, ix_own[o].i, zi);
+        }
+      });
     }
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
