@@ -114,6 +114,22 @@ def test_workload_128_to_1e6(cuda):
     assert info.iterations <= 44, info.iterations
 
 
+def test_strided_column_walk_on_a_ragged_grid(cuda):
+    """150^3 (~1,750 finest columns, over the one-column-per-work-group
+    limit of 1,024): the finest kernels walk the column list in strides,
+    with columns cut by the grid in x, y and z and an empty slot in the
+    XCD-grouped list.  The 1e-6 solve meets the FP64 residual recomputed by
+    the oracle and agrees with a 1e-10 solve of the same system."""
+    isl = W.box_union(1150, 150, 0.6, 0.5, n_boxes=100000, edge=(2, 30))
+    info, x, phi = _solve(isl, cuda, tol=1e-6, max_iters=2000)
+    assert info.converged == 1, (info.iterations, info.rel_residual)
+    assert O.residual(isl, x) <= 1e-6 * 1.01
+    ref_info, xr, pr = _solve(isl, cuda, tol=1e-10, max_iters=4000)
+    assert ref_info.converged == 1 and O.residual(isl, xr) <= 1e-10 * 1.01
+    err = np.linalg.norm(phi - pr) / np.linalg.norm(pr)
+    assert err <= TOL_PHI, err
+
+
 def test_workload_256_mixed_precision(cuda):
     """256^3, kappa lognormal sigma 1.5, ~70 % occupied: FP32 and BF16-coarse
     solves reach 1e-6 (FP64 residual recomputed on the CPU); phi agrees with a
