@@ -61,19 +61,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(isl, sample_iters=40):
-    """The oracle's FP64 Jacobi-PCG (C, one thread) on a bounded sample."""
+def host_threads():
+    """Host threads for the all-core CPU legs: OMP_NUM_THREADS when set (the
+    GPU host's share), else the CPU count, at most 16."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return n if n > 0 else min(16, os.cpu_count() or 1)
+
+
+def cpu_baseline(isl, sample_iters=40, threads=1):
+    """The oracle's FP64 Jacobi-PCG (C; its per-voxel operator loop on
+    `threads` threads, the dot products serial) on a bounded sample."""
     from oracle import oracle as O
 
+    used = O.set_threads(threads)
     t = time.perf_counter()
     _, it, _ = O.pcg(isl, tol=0.0, maxit=sample_iters)
     dt = time.perf_counter() - t
+    O.set_threads(1)
     per_iter = dt / max(it, 1)
     iters = CPU_PCG_ITERS_TO_1E6
-    return {"value": per_iter * iters * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+    return {"value": per_iter * iters * 1e3, "unit": "ms", "cores": used, "kind": "port",
             "sample": f"{it} FP64 Jacobi-PCG iterations of the C oracle on the 128^3 island "
-                      f"({dt:.1f} s, {per_iter * 1e3:.1f} ms/iter), extrapolated to the "
-                      f"{iters} iterations that solver needs for 1e-6"}
+                      f"({dt:.1f} s, {per_iter * 1e3:.1f} ms/iter, {used} thread(s)), "
+                      f"extrapolated to the {iters} iterations that solver needs for 1e-6"}
 
 
 def batch_bench(torch, D, W, steps=3, plain_stride=16):
@@ -302,11 +312,19 @@ def rbgs_bench(torch, D, W, sweeps=200, steps=5):
     ref_phi = O.rbgs(isl, sweeps=sweeps)
     ref_code = O.failure_scan(isl, ref_phi)
     cpu_ms = (time.perf_counter() - t0) * 1e3
+    nthr = O.set_threads(host_threads())  # the same run on all host threads (same bits)
+    t0 = time.perf_counter()
+    all_phi = O.rbgs(isl, sweeps=sweeps)
+    all_code = O.failure_scan(isl, all_phi)
+    cpu_all_ms = (time.perf_counter() - t0) * 1e3
+    O.set_threads(1)
+    assert np.array_equal(all_phi, ref_phi) and np.array_equal(all_code, ref_code)
     n = isl.n_active
     out = {"grid": [isl.lx, isl.ly, isl.lz], "active_voxels": n, "sweeps": sweeps,
            "gpu_ms": round(gpu_ms, 3),
            "gpu_voxel_updates_per_s": round(n * sweeps / (gpu_ms * 1e-3), 1),
            "cpu_ms": round(cpu_ms, 1), "cpu_cores": 1, "cpu_kind": "port",
+           "cpu_all_cores_ms": round(cpu_all_ms, 1), "cpu_all_cores": nthr,
            "cpu_voxel_updates_per_s": round(n * sweeps / (cpu_ms * 1e-3), 1),
            "phi_bit_exact": bool(np.array_equal(phi.cpu().numpy(), ref_phi)),
            "codes_bit_exact": bool(np.array_equal(code.cpu().numpy(), ref_code)),
@@ -478,6 +496,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(isl)
+        out["cpu_baseline_all_cores"] = cpu_baseline(isl, threads=host_threads())
     solver.close()
     if not args.no_host_tick:
         out["host_buffer_tick_128"] = host_tick_bench(isl)

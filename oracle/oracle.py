@@ -81,8 +81,16 @@ def lib():
         L.or_label_bfs.restype = C.c_int64
         L.or_label_summarise.argtypes = [I, I, I, P, P, I, P, C.POINTER(I)]
         L.or_label_summarise.restype = I
+        L.or_set_threads.argtypes = [I]
+        L.or_set_threads.restype = I
+        L.or_set_threads(1)  # single-threaded unless a caller asks for more
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-voxel loops (results do not depend on it)."""
+    return int(lib().or_set_threads(int(n)))
 
 
 def _p(a: np.ndarray):

@@ -25,6 +25,7 @@
  * in Java.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -97,6 +98,15 @@ void or_jacobi_step(int lx, int ly, int lz, const uint8_t* type, const float* ka
       }
 }
 
+/* Threads of the parallel per-voxel loops (RBGS colour passes, diagonal,
+ * operator, failure scan).  Every reduction stays serial and each voxel's
+ * value is formed by one thread in the same order, so results are the same
+ * bits for any thread count.  Returns the count now in effect. */
+int or_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
 /* In-place 8-colour Gauss-Seidel with the jacobiStep per-voxel update.
  * Colour = (x&1) | (y&1)<<1 | (z&1)<<2, colours 0..7 ascending
  * (SH/rbgs_smooth.comp.glsl:86-87, NP/src/solver/jacobi_solver.cpp:163-175).
@@ -104,7 +114,8 @@ void or_jacobi_step(int lx, int ly, int lz, const uint8_t* type, const float* ka
 void or_rbgs(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
              const float* source, float* phi, int sweeps) {
   for (int s = 0; s < sweeps; s++)
-    for (int c = 0; c < 8; c++)
+    for (int c = 0; c < 8; c++) {
+#pragma omp parallel for schedule(static)
       for (int z = (c >> 2) & 1; z < lz; z += 2)
         for (int y = (c >> 1) & 1; y < ly; y += 2)
           for (int x = c & 1; x < lx; x += 2) {
@@ -129,11 +140,13 @@ void or_rbgs(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
             double pj = (diag > 1e-15) ? (b + off) / diag : (double)phi[i];
             phi[i] = (float)((pj - (double)phi[i]) + (double)phi[i]);
           }
+    }
 }
 
 /* Diagonal D_i = sum_j a_ij over non-air neighbours (anchors included); 0 on
  * non-solid rows.  Same accumulation order as jacobiStep's diagA. */
 void or_diag(int lx, int ly, int lz, const uint8_t* type, const float* kappa, double* d) {
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < lz; z++)
     for (int y = 0; y < ly; y++)
       for (int x = 0; x < lx; x++) {
@@ -159,6 +172,7 @@ void or_diag(int lx, int ly, int lz, const uint8_t* type, const float* kappa, do
  * neighbours (Dirichlet), air skipped.  y = 0 on non-solid rows. */
 void or_apply(int lx, int ly, int lz, const uint8_t* type, const float* kappa, const double* x,
               double* y) {
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < lz; z++)
     for (int yy = 0; yy < ly; yy++)
       for (int xx = 0; xx < lx; xx++) {
@@ -269,6 +283,7 @@ int or_pcg(int lx, int ly, int lz, const uint8_t* type, const float* kappa, cons
 void or_failure_scan(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
                      const float* phi, const float* rcomp, const float* rtens, float phi_orphan,
                      uint8_t* code) {
+#pragma omp parallel for schedule(static)
   for (int z = 0; z < lz; z++)
     for (int y = 0; y < ly; y++)
       for (int x = 0; x < lx; x++) {

@@ -322,3 +322,21 @@ def test_load_path_gradient_matches_finite_differences():
         fd = (vals[0] - vals[1]) / (ks[0] - ks[1])
         assert abs(fd - g.ravel()[m]) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g).max()), (m, fd,
                                                                                    g.ravel()[m])
+
+
+def test_oracle_results_do_not_depend_on_thread_count():
+    """The parallel per-voxel loops (operator, diagonal, RBGS colour passes,
+    failure scan) give the same bits on 1 and 4 threads; PCG keeps its
+    serial reductions, so its iterates match too."""
+    isl = W.small_box_union(3, L=20)
+    rng = np.random.default_rng(8)
+    x = rng.random(isl.vtype.shape)
+    phi = (rng.random(isl.vtype.shape) * 2).astype(np.float32)
+    runs = []
+    for t in (1, 4):
+        assert O.set_threads(t) == t
+        runs.append((O.apply(isl, x), O.diag(isl), O.rbgs(isl, sweeps=20),
+                     O.failure_scan(isl, phi), O.pcg(isl, tol=0.0, maxit=20)[0]))
+    O.set_threads(1)
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
