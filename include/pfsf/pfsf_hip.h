@@ -91,6 +91,24 @@ PFSF_API pfsf_result pfsf_hip_normalize_soa6(float* source, float* rcomp, float*
                                              float* cond, int64_t n, float* sigma_max_host,
                                              void* stream);
 
+/* Island arrays built on the device from a per-voxel material index grid;
+ * replaces the Java data prep PFSFDataBuilder.buildFromLookups
+ * (PFSFDataBuilder.java:77-133) with the conductivity rule
+ * PFSFConductivity.sigma (PFSFConductivity.java:36-97, force arm 0, no wind
+ * bias).  material_index[i] in [0, n_materials) selects materials[] (device
+ * array), anything else is air.  indestructible (n_materials flags), anchor
+ * (per voxel), curing (per voxel hydration, clamped to [0, 1]) and fill (per
+ * voxel fill ratio) may be NULL: none, none, 1 and 1.  Writes type (0 air,
+ * 1 solid, 2 anchor), source = density * fill * 9.81 * 1 m^3 (FP64 rounded
+ * to FP32), the SoA-6 conductivity (6n; dir * n + i; -x, +x, -y, +y, -z, +z;
+ * y vertical) scaled by sqrt(curing), rcomp, rtens and max_phi (+inf); air
+ * voxels get zeros.  Fluid-pressure source terms are not added. */
+PFSF_API pfsf_result pfsf_hip_build_island_data(
+    pfsf_dims d, const int32_t* material_index, const pfsf_material* materials,
+    int32_t n_materials, const uint8_t* indestructible, const uint8_t* anchor,
+    const float* curing, const float* fill, uint8_t* type, float* source, float* conductivity,
+    float* rcomp, float* rtens, float* max_phi, void* stream);
+
 /* Per-voxel kappa from SoA-6 face conductivities: kappa_i = max_d cond[d*n+i]. */
 PFSF_API pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa,
                                               void* stream);

@@ -81,11 +81,31 @@ def lib():
         L.or_label_bfs.restype = C.c_int64
         L.or_label_summarise.argtypes = [I, I, I, P, P, I, P, C.POINTER(I)]
         L.or_label_summarise.restype = I
+        L.or_build_island_data.argtypes = [I, I, I, P, P, I, P, P, P, P, P, P, P, P, P]
         L.or_set_threads.argtypes = [I]
         L.or_set_threads.restype = I
         L.or_set_threads(1)  # single-threaded unless a caller asks for more
         _lib = L
     return _lib
+
+
+def build_island_data(mat_idx, table, anchor=None, curing=None, fill=None):
+    """PFSFDataBuilder data prep over a material index grid (lz, ly, lx)
+    int32; table rows {density, rcomp, rtens, indestructible}.  Returns
+    type, source, cond (6, lz, ly, lx), rcomp, rtens, max_phi."""
+    mi = np.ascontiguousarray(mat_idx, np.int32)
+    tb = np.ascontiguousarray(table, np.float32).reshape(-1, 4)
+    lz, ly, lx = mi.shape
+    opt = [None if a is None else np.ascontiguousarray(a, dt)
+           for a, dt in ((anchor, np.uint8), (curing, np.float32), (fill, np.float32))]
+    out = dict(type=np.empty(mi.shape, np.uint8), source=np.empty(mi.shape, np.float32),
+               cond=np.empty((6,) + mi.shape, np.float32), rcomp=np.empty(mi.shape, np.float32),
+               rtens=np.empty(mi.shape, np.float32), max_phi=np.empty(mi.shape, np.float32))
+    lib().or_build_island_data(lx, ly, lz, _p(mi), _p(tb), tb.shape[0],
+                               *[None if a is None else _p(a) for a in opt],
+                               *[_p(out[k]) for k in ("type", "source", "cond", "rcomp", "rtens",
+                                                      "max_phi")])
+    return out
 
 
 def set_threads(n: int) -> int:

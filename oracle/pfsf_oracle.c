@@ -107,6 +107,75 @@ int or_set_threads(int n) {
   return omp_get_max_threads();
 }
 
+/* Island data prep, PFSFDataBuilder.buildFromLookups (PFSFDataBuilder.java:
+ * 77-133) over a material index grid (index outside [0, nmat) = air), with
+ * the conductivity rule PFSFConductivity.sigma (PFSFConductivity.java:36-97)
+ * at force arm 0 and no wind bias.  mat holds nmat rows {density, rcomp,
+ * rtens, indestructible (0/1)}; anchor, curing and fill may be NULL.  Java's
+ * double/float steps are kept: Math.min/max on doubles (NaN if either is
+ * NaN), the base and the tension ratio rounded to float, sigma * sqrt(h)
+ * in float, the source in double rounded to float. */
+static double j_min(double a, double b) { return (a != a || b != b) ? NAN : (a < b ? a : b); }
+static double j_max(double a, double b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+static float j_minf(float a, float b) { return (a != a || b != b) ? NAN : (a < b ? a : b); }
+static float j_maxf(float a, float b) { return (a != a || b != b) ? NAN : (a > b ? a : b); }
+
+static float or_sigma(const float* mi, const float* mj, int vertical) {
+  double rci = mi[3] != 0.0f ? 1e6 : (double)mi[1];
+  double rcj = mj[3] != 0.0f ? 1e6 : (double)mj[1];
+  float base = (float)j_min(rci, rcj);
+  if (base <= 0) return 0.0f;
+  if (vertical) return base;
+  double rti = mi[3] != 0.0f ? 1e6 : (double)mi[2];
+  double rtj = mj[3] != 0.0f ? 1e6 : (double)mj[2];
+  double avg = (rti + rtj) / 2.0;
+  float ratio = (float)j_min(1.0, avg / j_max(base, 1.0));
+  float sigmaH = base * ratio;
+  float decay = 1.0f;
+  float result = sigmaH * decay;
+  if (isnan(result) || isinf(result)) return 0.0f;
+  return result;
+}
+
+void or_build_island_data(int lx, int ly, int lz, const int32_t* mat_idx, const float* mat,
+                          int nmat, const uint8_t* anchor, const float* curing, const float* fill,
+                          uint8_t* type, float* source, float* cond, float* rcomp, float* rtens,
+                          float* max_phi) {
+  static const int DV[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  int64_t n = (int64_t)lx * ly * lz;
+  memset(type, 0, (size_t)n);
+  memset(source, 0, sizeof(float) * (size_t)n);
+  memset(cond, 0, sizeof(float) * (size_t)(6 * n));
+  memset(rcomp, 0, sizeof(float) * (size_t)n);
+  memset(rtens, 0, sizeof(float) * (size_t)n);
+  memset(max_phi, 0, sizeof(float) * (size_t)n);
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        int m = mat_idx[i];
+        if (m < 0 || m >= nmat) continue;
+        const float* mi = mat + 4 * m;
+        float fillRatio = fill ? fill[i] : 1.0f;
+        float hDeg = curing ? j_maxf(0.0f, j_minf(1.0f, curing[i])) : 1.0f;
+        float sigmaScale = (float)sqrt((double)hDeg);
+        source[i] = (float)((double)mi[0] * fillRatio * 9.81 * 1.0);
+        type[i] = (anchor && anchor[i]) ? T_ANCHOR : T_SOLID;
+        max_phi[i] = INFINITY;
+        rcomp[i] = mi[1];
+        rtens[i] = mi[2];
+        for (int dir = 0; dir < 6; dir++) {
+          int nx = x + DV[dir][0], ny = y + DV[dir][1], nz = z + DV[dir][2];
+          float sigma = 0.0f;
+          if (nx >= 0 && nx < lx && ny >= 0 && ny < ly && nz >= 0 && nz < lz) {
+            int mj = mat_idx[IDX(nx, ny, nz, lx, ly)];
+            if (mj >= 0 && mj < nmat) sigma = or_sigma(mi, mat + 4 * mj, dir == 2 || dir == 3);
+          }
+          cond[dir * n + i] = sigma * sigmaScale;
+        }
+      }
+}
+
 /* In-place 8-colour Gauss-Seidel with the jacobiStep per-voxel update.
  * Colour = (x&1) | (y&1)<<1 | (z&1)<<2, colours 0..7 ascending
  * (SH/rbgs_smooth.comp.glsl:86-87, NP/src/solver/jacobi_solver.cpp:163-175).

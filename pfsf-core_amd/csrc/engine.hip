@@ -814,6 +814,21 @@ pfsf_result pfsf_hip_normalize_soa6(float* source, float* rcomp, float* rtens, f
   RET(e);
 }
 
+pfsf_result pfsf_hip_build_island_data(pfsf_dims d, const int32_t* material_index,
+                                      const pfsf_material* materials, int32_t n_materials,
+                                      const uint8_t* indestructible, const uint8_t* anchor,
+                                      const float* curing, const float* fill, uint8_t* type,
+                                      float* source, float* conductivity, float* rcomp,
+                                      float* rtens, float* max_phi, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!material_index || !type || !source || !conductivity || !rcomp || !rtens || !max_phi ||
+      n_materials < 0 || (n_materials > 0 && !materials))
+    return PFSF_ERROR_INVALID_ARG;
+  RET(launch_build_island_data(d, material_index, materials, n_materials, indestructible, anchor,
+                               curing, fill, type, source, conductivity, rcomp, rtens, max_phi,
+                               (hipStream_t)stream));
+}
+
 pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa, void* stream) {
   if (!cond || !kappa || n <= 0) return PFSF_ERROR_INVALID_ARG;
   RET(launch_kappa_from_soa6(cond, n, kappa, (hipStream_t)stream));

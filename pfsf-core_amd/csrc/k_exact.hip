@@ -381,6 +381,95 @@ __global__ void __launch_bounds__(BS)
   if (ct) ct[ci] = (nk >= ns && nk >= na) ? 2 : (ns >= na ? 1 : 0);
 }
 
+// ── island data prep (PFSFDataBuilder.java:77-133) ─────────────────────
+// Java's Math.min / Math.max on doubles: NaN if either operand is NaN.
+__device__ __forceinline__ double jmin(double a, double b) {
+  return (a != a || b != b) ? (double)NAN : (a < b ? a : b);
+}
+__device__ __forceinline__ double jmax(double a, double b) {
+  return (a != a || b != b) ? (double)NAN : (a > b ? a : b);
+}
+__device__ __forceinline__ float jminf(float a, float b) {
+  return (a != a || b != b) ? NAN : (a < b ? a : b);
+}
+__device__ __forceinline__ float jmaxf(float a, float b) {
+  return (a != a || b != b) ? NAN : (a > b ? a : b);
+}
+
+// Face conductivity between the materials of two member voxels
+// (PFSFConductivity.sigma, PFSFConductivity.java:36-97, force arm 0, no wind
+// bias): the weaker compressive strength for a vertical (y) face; for a
+// horizontal face that base times min(1, mean tensile strength /
+// max(base, 1)), 0 when not finite.  An indestructible material counts 1e6
+// for both strengths.  Strengths are compared in FP64, the base and the ratio
+// rounded to FP32 and multiplied in FP32, as in the Java.
+__device__ __forceinline__ float face_sigma(const pfsf_material& mi, bool indi,
+                                            const pfsf_material& mj, bool indj, bool vertical) {
+  const double rci = indi ? 1e6 : (double)mi.rcomp;
+  const double rcj = indj ? 1e6 : (double)mj.rcomp;
+  const float base = (float)jmin(rci, rcj);
+  if (base <= 0.0f) return 0.0f;
+  if (vertical) return base;
+  const double rti = indi ? 1e6 : (double)mi.rtens;
+  const double rtj = indj ? 1e6 : (double)mj.rtens;
+  const double avg = (rti + rtj) / 2.0;
+  const float ratio = (float)jmin(1.0, avg / jmax((double)base, 1.0));
+  const float r = base * ratio;
+  return (isnan(r) || isinf(r)) ? 0.0f : r;
+}
+
+// One voxel of the island arrays from a material index grid (index outside
+// [0, nmat) = air): type (anchor from the anchor flags), self-weight source
+// density * fill * g * 1 m^3 (FP64, rounded to FP32), rcomp, rtens, max_phi
+// (+inf) and the six face conductivities sigma * sqrt(curing clamped to
+// [0, 1]) in SoA order dir * n + i, directions -x, +x, -y, +y, -z, +z
+// (PFSFConductivity.dirToIndex, y vertical).  Air voxels get zeros.
+__global__ void __launch_bounds__(BS)
+    k_build_island(pfsf_dims d, const int32_t* __restrict__ mat_idx,
+                   const pfsf_material* __restrict__ mats, int32_t nmat,
+                   const uint8_t* __restrict__ indestructible, const uint8_t* __restrict__ anchor,
+                   const float* __restrict__ curing, const float* __restrict__ fill,
+                   uint8_t* __restrict__ type, float* __restrict__ source,
+                   float* __restrict__ cond, float* __restrict__ rcomp,
+                   float* __restrict__ rtens, float* __restrict__ max_phi) {
+  const int64_t n = nvox(d);
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int32_t m = mat_idx[i];
+  if (m < 0 || m >= nmat) {
+    type[i] = 0;
+    source[i] = 0.0f;
+    rcomp[i] = 0.0f;
+    rtens[i] = 0.0f;
+    max_phi[i] = 0.0f;
+    for (int dir = 0; dir < 6; dir++) cond[dir * n + i] = 0.0f;
+    return;
+  }
+  const pfsf_material mi = mats[m];
+  const bool indi = indestructible && indestructible[m];
+  const float hdeg = curing ? jmaxf(0.0f, jminf(1.0f, curing[i])) : 1.0f;
+  const float scale = (float)sqrt((double)hdeg);
+  const float fr = fill ? fill[i] : 1.0f;
+  source[i] = (float)((double)mi.density * (double)fr * 9.81 * 1.0);
+  type[i] = (anchor && anchor[i]) ? 2 : 1;
+  max_phi[i] = INFINITY;
+  rcomp[i] = mi.rcomp;
+  rtens[i] = mi.rtens;
+  const XYZ p = unflat(i, d);
+  constexpr int DV[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+  for (int dir = 0; dir < 6; dir++) {
+    const int nx = p.x + DV[dir][0], ny = p.y + DV[dir][1], nz = p.z + DV[dir][2];
+    float sg = 0.0f;
+    if (nx >= 0 && nx < d.lx && ny >= 0 && ny < d.ly && nz >= 0 && nz < d.lz) {
+      const int32_t mj = mat_idx[flat(nx, ny, nz, d)];
+      if (mj >= 0 && mj < nmat)
+        sg = face_sigma(mi, indi, mats[mj], indestructible && indestructible[mj], dir == 2 || dir == 3);
+    }
+    cond[dir * n + i] = sg * scale;
+  }
+}
+
 __global__ void __launch_bounds__(BS)
     k_kappa_soa6(const float* __restrict__ cond, int64_t n, float* __restrict__ kappa) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
@@ -609,6 +698,18 @@ hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* 
                              float* coarse, uint8_t* ctype, hipStream_t s) {
   pfsf_dims c{(fine.lx + 1) / 2, (fine.ly + 1) / 2, (fine.lz + 1) / 2};
   k_downsample<<<blocks_for(nvox(c), BS), BS, 0, s>>>(fine, field, type, coarse, ctype);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_island_data(pfsf_dims d, const int32_t* mat_idx,
+                                   const pfsf_material* mats, int32_t nmat,
+                                   const uint8_t* indestructible, const uint8_t* anchor,
+                                   const float* curing, const float* fill, uint8_t* type,
+                                   float* source, float* cond, float* rcomp, float* rtens,
+                                   float* max_phi, hipStream_t s) {
+  k_build_island<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, mat_idx, mats, nmat, indestructible,
+                                                        anchor, curing, fill, type, source, cond,
+                                                        rcomp, rtens, max_phi);
   return hipGetLastError();
 }
 

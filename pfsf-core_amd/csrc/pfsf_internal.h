@@ -46,6 +46,12 @@ int64_t compact_scratch_elems(pfsf_dims d);
 hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* type,
                              float* coarse, uint8_t* ctype, hipStream_t s);
 hipError_t launch_kappa_from_soa6(const float* cond, int64_t n, float* kappa, hipStream_t s);
+hipError_t launch_build_island_data(pfsf_dims d, const int32_t* mat_idx,
+                                   const pfsf_material* mats, int32_t nmat,
+                                   const uint8_t* indestructible, const uint8_t* anchor,
+                                   const float* curing, const float* fill, uint8_t* type,
+                                   float* source, float* cond, float* rcomp, float* rtens,
+                                   float* max_phi, hipStream_t s);
 hipError_t launch_max_soa6(const float* cond, int64_t n6, unsigned* max_bits, hipStream_t s);
 hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
                          const unsigned* max_bits, hipStream_t s);

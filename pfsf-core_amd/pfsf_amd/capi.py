@@ -11,6 +11,8 @@ import ctypes as C
 import os
 import pathlib
 
+import numpy as np
+
 PKG_ROOT = pathlib.Path(__file__).resolve().parent.parent
 # PFSF_LIBRARY names another build of libpfsf (e.g. a tuning variant).
 LIB_PATH = pathlib.Path(os.environ.get("PFSF_LIBRARY", PKG_ROOT / "lib" / "libpfsf.so"))
@@ -88,6 +90,11 @@ class SolveInfo(C.Structure):
 P, I32, I64, F32, F64, U32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint32
 R = C.c_int  # pfsf_result
 
+# pfsf_material (pfsf_types.h): six floats and a bool, 4-byte aligned (28 bytes)
+MATERIAL_DTYPE = np.dtype([("density", "<f4"), ("rcomp", "<f4"), ("rtens", "<f4"),
+                           ("youngs_gpa", "<f4"), ("poisson", "<f4"), ("gc", "<f4"),
+                           ("is_anchor", "u1")], align=True)
+
 class BatchIsland(C.Structure):
     _fields_ = [("dims", Dims), ("type", C.c_void_p), ("kappa", C.c_void_p)]
 
@@ -162,6 +169,7 @@ SIGNATURES = {
     "pfsf_hip_downsample_2to1": (R, [Dims, P, P, P, P, P]),
     "pfsf_hip_normalize_soa6": (R, [P, P, P, P, I64, C.POINTER(F32), P]),
     "pfsf_hip_kappa_from_soa6": (R, [P, I64, P, P]),
+    "pfsf_hip_build_island_data": (R, [Dims, P, P, I32, P, P, P, P, P, P, P, P, P, P, P]),
     "pfsf_solver_create": (R, [C.POINTER(SolverDesc), C.POINTER(P)]),
     "pfsf_solver_solve": (R, [P, P, P, P, C.POINTER(SolveInfo)]),
     "pfsf_solver_update": (R, [P, P, P]),

@@ -93,6 +93,36 @@ def kappa_from_soa6(cond, kappa):
         _p(cond), kappa.numel(), _p(kappa), _stream()))
 
 
+def build_island_data(material_index, materials, indestructible=None, anchor=None, curing=None,
+                      fill=None):
+    """Island arrays from a (lz, ly, lx) int32 material index grid on the
+    device (pfsf_hip_build_island_data).  materials: a capi.MATERIAL_DTYPE
+    array (host); indestructible: per-material uint8 flags (host) or None;
+    anchor (uint8), curing and fill (float32): device tensors or None.
+    Returns type, source, cond (6, lz, ly, lx), rcomp, rtens, max_phi."""
+    import numpy as np
+    import torch
+
+    dev = material_index.device
+    mats = np.ascontiguousarray(materials, dtype=capi.MATERIAL_DTYPE)
+    mt = torch.from_numpy(mats.view(np.uint8).copy()).to(dev)
+    ind = None if indestructible is None else torch.from_numpy(
+        np.ascontiguousarray(indestructible, np.uint8)).to(dev)
+    sh = material_index.shape
+    out = dict(type=torch.empty(sh, dtype=torch.uint8, device=dev),
+               source=torch.empty(sh, dtype=torch.float32, device=dev),
+               cond=torch.empty((6,) + tuple(sh), dtype=torch.float32, device=dev),
+               rcomp=torch.empty(sh, dtype=torch.float32, device=dev),
+               rtens=torch.empty(sh, dtype=torch.float32, device=dev),
+               max_phi=torch.empty(sh, dtype=torch.float32, device=dev))
+    opt = [None if a is None else _p(a) for a in (ind, anchor, curing, fill)]
+    capi.check("pfsf_hip_build_island_data", _lib().pfsf_hip_build_island_data(
+        _dims(material_index), _p(material_index), _p(mt), len(mats), *opt,
+        *[_p(out[k]) for k in ("type", "source", "cond", "rcomp", "rtens", "max_phi")],
+        _stream()))
+    return out
+
+
 def anchored_mask(vtype, reach):
     capi.check("pfsf_hip_anchored_mask", _lib().pfsf_hip_anchored_mask(
         _dims(vtype), _p(vtype), _p(reach), _stream()))

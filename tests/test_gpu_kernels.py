@@ -211,3 +211,62 @@ def test_workload_128_scan_and_history_bit_exact(cuda):
     D.history_update(d["vtype"], d["kappa"], phi, H)
     assert np.array_equal(H.cpu().numpy().view(np.uint32),
                           O.history_update(isl, p, h0).view(np.uint32))
+
+
+def _material_table(rows):
+    """capi.MATERIAL_DTYPE table and the oracle's {density, rcomp, rtens,
+    indestructible} rows from (density, rcomp, rtens, indestructible)."""
+    from pfsf_amd import capi
+
+    t = np.zeros(len(rows), dtype=capi.MATERIAL_DTYPE)
+    for k, (dens, rc, rt, _) in enumerate(rows):
+        t[k]["density"], t[k]["rcomp"], t[k]["rtens"] = dens, rc, rt
+    ind = np.array([r[3] for r in rows], np.uint8)
+    return t, ind, np.array(rows, np.float32)
+
+
+def _check_build(cuda, mi, rows, anchor=None, curing=None, fill=None):
+    table, ind, orows = _material_table(rows)
+    dev = lambda a: None if a is None else torch.from_numpy(a).to(cuda)  # noqa: E731
+    got = D.build_island_data(dev(mi), table, ind, dev(anchor), dev(curing), dev(fill))
+    want = O.build_island_data(mi, orows, anchor, curing, fill)
+    for k in ("type", "source", "cond", "rcomp", "rtens", "max_phi"):
+        g = got[k].cpu().numpy()
+        w = want[k]
+        assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), k
+
+
+def test_island_data_prep_bit_exact_on_fixtures(cuda):
+    """The reference fixtures' material grids and registries through the
+    device data prep and the oracle: every array bit for bit, with random
+    curing and fill fields."""
+    import base64
+    import json
+
+    rng = np.random.default_rng(21)
+    for name in domains.fixture_names():
+        d = json.loads((domains.FIXTURE_DIR / f"{name}.json").read_text())
+        lx, ly, lz = d["dims"]["lx"], d["dims"]["ly"], d["dims"]["lz"]
+        ids = np.frombuffer(base64.b64decode(d["materials"]["voxels"]), "<i4").reshape(lz, ly, lx)
+        reg = d["materials"]["registry"]
+        slot = {m["id"]: k for k, m in enumerate(reg)}
+        mi = np.vectorize(lambda v: slot.get(int(v), -1))(ids).astype(np.int32)
+        rows = [(m["density"], m["rcomp"], m["rtens"], 0) for m in reg]
+        anchor = np.zeros(mi.shape, np.uint8)
+        for x, y, z in d["anchors"]:
+            anchor[z, y, x] = 1
+        curing = rng.uniform(-0.2, 1.2, mi.shape).astype(np.float32)
+        fill = rng.uniform(0.3, 1.0, mi.shape).astype(np.float32)
+        _check_build(cuda, mi, rows, anchor, curing, fill)
+        _check_build(cuda, mi, rows)
+
+
+def test_island_data_prep_bit_exact_on_random_grids(cuda):
+    rng = np.random.default_rng(22)
+    rows = [(2400, 30, 3, 0), (7850, 250, 400, 0), (600, 8, 0.5, 0), (2600, 5, 1, 1),
+            (1500, 0, 0, 0), (1800, 12.5, 2.25, 0)]
+    for shape in [(1, 1, 1), (3, 5, 7), (17, 9, 33), (40, 40, 40)]:
+        mi = rng.integers(-1, len(rows) + 1, shape).astype(np.int32)  # -1 and len(rows) are air
+        anchor = (rng.random(shape) < 0.1).astype(np.uint8)
+        curing = rng.uniform(0.0, 1.0, shape).astype(np.float32)
+        _check_build(cuda, mi, rows, anchor, curing, None)

@@ -340,3 +340,41 @@ def test_oracle_results_do_not_depend_on_thread_count():
     O.set_threads(1)
     for a, b in zip(*runs):
         assert np.array_equal(a, b)
+
+
+def test_island_data_prep_known_answers():
+    """PFSFDataBuilder + PFSFConductivity.sigma on a hand case (values derived
+    by hand from the rule, y vertical): concrete (rcomp 30, rtens 3), steel
+    (250, 400), an indestructible material (strength 1e6), air."""
+    C, S, I = 0, 1, 2
+    mi = np.array([[[C, C], [C, S], [I, -1]],
+                   [[I, I], [C, C], [-1, -1]]], np.int32)  # (lz=2, ly=3, lx=2)
+    table = [[2400, 30, 3, 0], [7850, 250, 400, 0], [2600, 5, 1, 1]]
+    curing = np.ones(mi.shape, np.float32)
+    curing[1, 1, 0] = 0.25  # sqrt(0.25) = 0.5 scales that voxel's faces
+    curing[1, 1, 1] = 7.0   # clamped to 1
+    fill = np.ones(mi.shape, np.float32)
+    fill[1, 1, 1] = 0.5
+    anchor = np.zeros(mi.shape, np.uint8)
+    anchor[0, 0, 0] = 1
+    o = O.build_island_data(mi, table, anchor, curing, fill)
+    cond = o["cond"]
+    NX, PX, NY, PY, NZ, PZ = range(6)
+    assert cond[PX, 0, 0, 0] == 3.0     # concrete-concrete horizontal: 30 * 3/30
+    assert cond[PY, 0, 0, 0] == 30.0    # concrete-concrete vertical
+    assert cond[PX, 0, 1, 0] == 30.0    # concrete-steel horizontal: ratio capped at 1
+    assert cond[NY, 0, 1, 1] == 30.0    # steel-concrete vertical: the weaker
+    assert cond[PY, 0, 1, 0] == 30.0    # concrete-indestructible vertical
+    assert cond[PX, 0, 2, 0] == 0.0     # face to air
+    assert cond[NX, 0, 0, 0] == 0.0     # face outside the grid
+    assert cond[PX, 1, 0, 0] == 1e6     # indestructible pair horizontal
+    assert cond[PZ, 0, 0, 0] == 30.0    # concrete-indestructible across z (horizontal): ratio 1
+    assert cond[PX, 1, 1, 0] == 1.5     # curing 0.25: 3.0 * 0.5
+    assert cond[NX, 1, 1, 1] == 3.0     # curing 7 clamps to 1
+    assert o["source"][0, 0, 0] == np.float32(2400 * 9.81)
+    assert o["source"][0, 1, 1] == np.float32(7850 * 9.81)
+    assert o["source"][1, 1, 1] == np.float32(2400 * 0.5 * 9.81)
+    assert o["type"][0, 0, 0] == 2 and o["type"][0, 1, 1] == 1 and o["type"][0, 2, 1] == 0
+    assert o["rcomp"][0, 1, 1] == 250 and o["rtens"][0, 2, 0] == 1
+    assert np.isinf(o["max_phi"][0, 0, 0]) and o["max_phi"][0, 2, 1] == 0
+    assert np.all(cond[:, 0, 2, 1] == 0) and o["source"][0, 2, 1] == 0
