@@ -333,6 +333,50 @@ def rbgs_bench(torch, D, W, sweeps=200, steps=5):
     return out
 
 
+def data_prep_bench(torch, D, isl, steps=10):
+    """Island data prep on the device (pfsf_hip_build_island_data) for the
+    128^3 island's voxels with 16 seeded materials (one indestructible) and
+    random curing, against the C oracle's restatement of the Java loop on the
+    same inputs (one host thread), all outputs compared bit for bit."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from pfsf_amd import capi
+
+    rng = np.random.default_rng(0x9F5F0000 + 6)
+    nm = 16
+    rows = np.stack([rng.uniform(1500, 7800, nm), rng.uniform(5, 60, nm),
+                     rng.uniform(0.5, 6, nm), (np.arange(nm) == 0).astype(float)], 1)
+    table = np.zeros(nm, dtype=capi.MATERIAL_DTYPE)
+    table["density"], table["rcomp"], table["rtens"] = rows[:, 0], rows[:, 1], rows[:, 2]
+    ind = rows[:, 3].astype(np.uint8)
+    mi = np.where(isl.vtype != 0, rng.integers(0, nm, isl.vtype.shape), -1).astype(np.int32)
+    anchor = (isl.vtype == 2).astype(np.uint8)
+    curing = rng.uniform(0.5, 1.0, isl.vtype.shape).astype(np.float32)
+    dev = torch.device("cuda:0")
+    gm, ga, gc = (torch.from_numpy(a).to(dev) for a in (mi, anchor, curing))
+    D.build_island_data(gm, table, ind, ga, gc)  # warm-up
+    ts = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        got = D.build_island_data(gm, table, ind, ga, gc)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    gpu_ms = float(np.median(ts)) * 1e3
+    t0 = time.perf_counter()
+    want = O.build_island_data(mi, rows.astype(np.float32), anchor, curing)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    exact = all(np.array_equal(got[k].cpu().numpy().view(np.uint8), want[k].view(np.uint8))
+                for k in ("type", "source", "cond", "rcomp", "rtens", "max_phi"))
+    out = {"grid": [isl.lx, isl.ly, isl.lz], "materials": nm, "gpu_ms": round(gpu_ms, 3),
+           "gpu_voxels_per_s": round(isl.n / (gpu_ms * 1e-3), 1), "cpu_ms": round(cpu_ms, 1),
+           "cpu_cores": 1, "cpu_kind": "port", "bit_exact": bool(exact),
+           "note": "synchronous C-ABI call timed on the host, material table upload included"}
+    log(f"data prep 128^3: {out}")
+    return out
+
+
 def host_tick_bench(isl, steps=3):
     """The same 128^3 island through the host-buffer C ABI, as the Java runtime
     drives it: pfsf_register_island_buffers + pfsf_tick_dbb per tick (PCIe
@@ -381,6 +425,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 96^3 adjoint record")
     ap.add_argument("--no-labels", action="store_true", help="skip the labelling record")
     ap.add_argument("--no-rbgs", action="store_true", help="skip the dense 32^3 RBGS record")
+    ap.add_argument("--no-prep", action="store_true", help="skip the island data-prep record")
     ap.add_argument("--no-host-tick", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) tick record")
     ap.add_argument("--batch-only", action="store_true",
@@ -510,6 +555,8 @@ def main():
         out["labels"] = label_bench(torch, D, W, isl)
     if not args.no_rbgs:
         out["rbgs_32"] = rbgs_bench(torch, D, W)
+    if not args.no_prep:
+        out["data_prep_128"] = data_prep_bench(torch, D, isl)
     if args.calibrate_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
