@@ -435,8 +435,8 @@ def main():
                          "hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(islands shard by bin-packing; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (islands would "
+              "shard across GPUs by bin-packing them by voxel count).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

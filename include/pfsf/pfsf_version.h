@@ -21,7 +21,7 @@ PFSF_API bool pfsf_has_feature(const char* name);
 /* pfsf_version.h:40.  Static "<compiler> <flags> <arch>" string. */
 PFSF_API const char* pfsf_build_info(void);
 
-/* pfsf_version.h:57.  Contract semver of pfsf_v1.abi.json; this build
+/* pfsf_version.h:57.  Contract semver of pfsf_v1.abi.json; this library
  * reports "1.6.0" (1.5.0 plus the additive pfsf_hip.h surface). */
 PFSF_API const char* pfsf_abi_contract_version(void);
 
