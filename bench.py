@@ -380,8 +380,8 @@ def data_prep_bench(torch, D, isl, steps=10):
 def host_tick_bench(isl, steps=3):
     """The same 128^3 island through the host-buffer C ABI, as the Java runtime
     drives it: pfsf_register_island_buffers + pfsf_tick_dbb per tick (PCIe
-    upload of every registered array, hierarchy rebuild, solve, history, scan,
-    failure compaction, phi readback).  Cold: phi zeroed before each tick;
+    upload of every registered array, hierarchy rebuild when type or kappa
+    changed, solve, history, scan, failure compaction, phi readback).  Cold: phi zeroed before each tick;
     warm: the previous tick's phi is the start (an unchanged island)."""
     import numpy as np
 

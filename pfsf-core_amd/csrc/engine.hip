@@ -76,6 +76,11 @@ struct Island {
   uint32_t* lab_rec = nullptr;    // kMaxLabelComponents records of 12 words
   uint32_t* lab_scratch = nullptr;
   std::unique_ptr<MgSolver> solver;
+  // type and kappa the solver's hierarchy was built from, and a device flag:
+  // a re-upload of identical arrays keeps the hierarchy
+  uint8_t* built_type = nullptr;
+  float* built_kappa = nullptr;
+  int32_t* topo_diff = nullptr;
   std::vector<void*> owned;
 };
 
@@ -199,11 +204,66 @@ bool upload(pfsf_engine_t* E, Island& is) {
   return hipGetLastError() == hipSuccess;
 }
 
+// Sets *diff when type or kappa (compared as bits) differs from the copy the
+// hierarchy was built from.
+__global__ void k_topo_diff(int64_t n, const uint8_t* __restrict__ type,
+                            const uint32_t* __restrict__ kappa,
+                            const uint8_t* __restrict__ built_type,
+                            const uint32_t* __restrict__ built_kappa, int32_t* diff) {
+  bool d = false;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256)
+    d |= (type[i] != built_type[i]) | (kappa[i] != built_kappa[i]);
+  if (__syncthreads_or(d) && threadIdx.x == 0) *diff = 1;
+}
+
+// True when the island's device type and kappa equal, bit for bit, the
+// arrays its solver's hierarchy was built from (one pass over 10 B per voxel
+// and a stream synchronisation, against a full hierarchy rebuild).
+bool topo_unchanged(pfsf_engine_t* E, Island& is, bool* same) {
+  hipStream_t s = E->stream;
+  int32_t diff = 1;
+  const unsigned blocks = (unsigned)std::min<int64_t>((is.n + 255) / 256, 2048);
+  if (hipMemsetAsync(is.topo_diff, 0, sizeof(int32_t), s) != hipSuccess) return false;
+  k_topo_diff<<<blocks, 256, 0, s>>>(is.n, is.type, (const uint32_t*)is.kappa, is.built_type,
+                                     (const uint32_t*)is.built_kappa, is.topo_diff);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&diff, is.topo_diff, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return false;
+  *same = diff == 0;
+  return true;
+}
+
+// Records the type and kappa a hierarchy was just built from.
+bool keep_built_topo(pfsf_engine_t* E, Island& is) {
+  hipStream_t s = E->stream;
+  if (!is.built_type &&
+      !(dev_alloc(is, &is.built_type, is.n) && dev_alloc(is, &is.built_kappa, is.n) &&
+        dev_alloc(is, &is.topo_diff, 1)))
+    return false;
+  return hipMemcpyAsync(is.built_type, is.type, is.n, hipMemcpyDeviceToDevice, s) ==
+             hipSuccess &&
+         hipMemcpyAsync(is.built_kappa, is.kappa, 4 * is.n, hipMemcpyDeviceToDevice, s) ==
+             hipSuccess;
+}
+
 // The island's solver, created or rebuilt when its voxels changed.  With
 // multigrid on, solvers are batchable so that several dirty islands can be
-// solved in one batched solve.
+// solved in one batched solve.  A re-upload that leaves type and kappa
+// unchanged (the host-buffer path uploads every array on every tick) keeps
+// the hierarchy: it is a function of those two arrays only.
 pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
   if (is.solver && !is.topo_dirty) return PFSF_OK;
+  if (is.solver && is.built_type) {
+    bool same = false;
+    if (!topo_unchanged(E, is, &same)) return PFSF_ERROR_VULKAN;
+    if (same) {
+      is.topo_dirty = false;
+      return PFSF_OK;
+    }
+  }
   if (!is.solver) {
     pfsf_solver_desc d{};
     d.dims = is.dims;
@@ -220,6 +280,7 @@ pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
   } else if (is.solver->update(is.type, is.kappa) != hipSuccess) {
     return PFSF_ERROR_VULKAN;
   }
+  if (!keep_built_topo(E, is)) return PFSF_ERROR_OUT_OF_VRAM;
   is.topo_dirty = false;
   return PFSF_OK;
 }

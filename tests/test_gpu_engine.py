@@ -1,6 +1,8 @@
 """The drop-in tick path (pfsf.h) on the GPU, driven the way the Java runtime
 drives it (NativePFSFRuntime.java:267-383): add island, register host
 buffers (phi doubles as the stress readback), tick_dbb, decode failures."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -116,6 +118,54 @@ def test_budget_zero_leaves_island_dirty(cuda):
         r, ev = e.tick_dbb([5])
         assert r == capi.OK and len(ev) == 0
         assert np.all(e.phi(5) == 0)  # nothing dispatched, nothing read back
+
+
+def _fresh_tick(isl, phi0):
+    """One tick of a new engine on isl, warm-started from phi0."""
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(1, isl, phi=phi0.copy(), kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb([1], capacity=1 << 14)
+        assert r == capi.OK
+        return e.phi(1).copy(), ev
+
+
+def test_reupload_keeps_or_rebuilds_the_hierarchy(cuda):
+    """Each tick re-uploads every registered host array.  With type and kappa
+    unchanged the island keeps its multigrid hierarchy; a changed kappa or
+    type rebuilds it.  Either way the tick equals, bit for bit, the tick of a
+    new engine (which builds its hierarchy) from the same warm start, and each
+    re-solve iterates (the source changes every tick)."""
+    isl = W.small_box_union(3, L=20)
+    isl.rcomp[...] = np.float32(0.5)
+    isl.rtens[...] = np.float32(0.25)
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(1, isl, kappa=isl.kappa) == capi.OK
+        assert e.tick_dbb([1], capacity=1 << 14)[0] == capi.OK
+        keep = e._keep[1]
+        rng = np.random.default_rng(5)
+        solid = np.flatnonzero(keep["vt"] == 1)
+        for step in ("same", "kappa", "type", "same"):
+            phi0 = e.phi(1).copy()
+            keep["src"] *= np.float32(1.25)
+            if step == "kappa":
+                keep["kappa"][rng.choice(solid, 40, replace=False)] *= np.float32(3.0)
+            elif step == "type":
+                keep["vt"][rng.choice(solid, 5, replace=False)] = 0
+            cur = dataclasses.replace(
+                isl, vtype=keep["vt"].reshape(isl.vtype.shape).copy(),
+                kappa=keep["kappa"].reshape(isl.vtype.shape).copy(),
+                source=keep["src"].reshape(isl.vtype.shape).copy())
+            e.lib.pfsf_mark_full_rebuild(e.h, 1)
+            r, ev = e.tick_dbb([1], capacity=1 << 14)
+            assert r == capi.OK
+            want_phi, want_ev = _fresh_tick(cur, phi0)
+            assert not np.array_equal(want_phi, phi0), step
+            assert np.array_equal(e.phi(1), want_phi), step
+            assert np.array_equal(ev, want_ev), step
 
 
 def test_multi_island_tick_is_batched_and_matches_oracle(cuda):
