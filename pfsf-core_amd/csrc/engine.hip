@@ -138,8 +138,9 @@ bool dev_alloc(Island& is, T** out, int64_t n) {
 }
 
 int64_t island_bytes(int64_t n) {
-  // type, kappa, cond(6), source, rcomp, rtens, max_phi, phi, x64, H, code + solver (~2.3x fine)
-  return n * (1 + 4 + 24 + 4 + 4 + 4 + 4 + 4 + 8 + 4 + 1) + n * 60;
+  // type, kappa, cond(6), source, rcomp, rtens, max_phi, phi, x64, H, code, the type and kappa
+  // the hierarchy was built from + solver (~2.3x fine)
+  return n * (1 + 4 + 24 + 4 + 4 + 4 + 4 + 4 + 8 + 4 + 1 + 5) + n * 60;
 }
 
 bool allocate(pfsf_engine_t* E, Island& is) {
@@ -217,22 +218,31 @@ __global__ void k_topo_diff(int64_t n, const uint8_t* __restrict__ type,
   if (__syncthreads_or(d) && threadIdx.x == 0) *diff = 1;
 }
 
-// True when the island's device type and kappa equal, bit for bit, the
-// arrays its solver's hierarchy was built from (one pass over 10 B per voxel
-// and a stream synchronisation, against a full hierarchy rebuild).
-bool topo_unchanged(pfsf_engine_t* E, Island& is, bool* same) {
+// Clears topo_dirty of each island whose device type and kappa equal, bit for
+// bit, the arrays its solver's hierarchy was built from: one pass over 10 B
+// per voxel per island, all queued, then one stream synchronisation (against
+// a full hierarchy rebuild per island).
+bool settle_topo(pfsf_engine_t* E, const std::vector<Island*>& todo) {
   hipStream_t s = E->stream;
-  int32_t diff = 1;
-  const unsigned blocks = (unsigned)std::min<int64_t>((is.n + 255) / 256, 2048);
-  if (hipMemsetAsync(is.topo_diff, 0, sizeof(int32_t), s) != hipSuccess) return false;
-  k_topo_diff<<<blocks, 256, 0, s>>>(is.n, is.type, (const uint32_t*)is.kappa, is.built_type,
-                                     (const uint32_t*)is.built_kappa, is.topo_diff);
-  if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(&diff, is.topo_diff, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-          hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return false;
-  *same = diff == 0;
+  std::vector<Island*> chk;
+  for (Island* is : todo)
+    if (is->solver && is->topo_dirty && is->built_type) chk.push_back(is);
+  if (chk.empty()) return true;
+  std::vector<int32_t> diff(chk.size(), 1);
+  for (size_t k = 0; k < chk.size(); k++) {
+    Island& is = *chk[k];
+    const unsigned blocks = (unsigned)std::min<int64_t>((is.n + 255) / 256, 2048);
+    if (hipMemsetAsync(is.topo_diff, 0, sizeof(int32_t), s) != hipSuccess) return false;
+    k_topo_diff<<<blocks, 256, 0, s>>>(is.n, is.type, (const uint32_t*)is.kappa, is.built_type,
+                                       (const uint32_t*)is.built_kappa, is.topo_diff);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&diff[k], is.topo_diff, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+      return false;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) return false;
+  for (size_t k = 0; k < chk.size(); k++)
+    if (diff[k] == 0) chk[k]->topo_dirty = false;
   return true;
 }
 
@@ -251,19 +261,9 @@ bool keep_built_topo(pfsf_engine_t* E, Island& is) {
 
 // The island's solver, created or rebuilt when its voxels changed.  With
 // multigrid on, solvers are batchable so that several dirty islands can be
-// solved in one batched solve.  A re-upload that leaves type and kappa
-// unchanged (the host-buffer path uploads every array on every tick) keeps
-// the hierarchy: it is a function of those two arrays only.
+// solved in one batched solve.
 pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
   if (is.solver && !is.topo_dirty) return PFSF_OK;
-  if (is.solver && is.built_type) {
-    bool same = false;
-    if (!topo_unchanged(E, is, &same)) return PFSF_ERROR_VULKAN;
-    if (same) {
-      is.topo_dirty = false;
-      return PFSF_OK;
-    }
-  }
   if (!is.solver) {
     pfsf_solver_desc d{};
     d.dims = is.dims;
@@ -414,6 +414,10 @@ pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count,
     if (is->host.registered && !upload(E, *is)) return PFSF_ERROR_VULKAN;
     todo.push_back(is);
   }
+  // a re-upload that left type and kappa unchanged (the host-buffer path
+  // uploads every array on every tick) keeps the island's hierarchy, a
+  // function of those two arrays only
+  if (E->pcg && !settle_topo(E, todo)) return PFSF_ERROR_VULKAN;
   if (E->pcg && E->cfg.enable_multigrid && todo.size() >= 2) {
     pfsf_result r = solve_batch(E, todo);
     if (r != PFSF_OK) return r;

@@ -120,51 +120,61 @@ def test_budget_zero_leaves_island_dirty(cuda):
         assert np.all(e.phi(5) == 0)  # nothing dispatched, nothing read back
 
 
-def _fresh_tick(isl, phi0):
-    """One tick of a new engine on isl, warm-started from phi0."""
+def _fresh_tick(isls, phis):
+    """One tick of a new engine on the islands, warm-started from phis."""
     with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
         assert e.init() == capi.OK
-        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
-        assert e.register(1, isl, phi=phi0.copy(), kappa=isl.kappa) == capi.OK
-        r, ev = e.tick_dbb([1], capacity=1 << 14)
+        for k, (isl, phi0) in enumerate(zip(isls, phis)):
+            assert e.add_island(k + 1, isl.lx, isl.ly, isl.lz, origin=(40 * k, 0, 0)) == capi.OK
+            assert e.register(k + 1, isl, phi=phi0.copy(), kappa=isl.kappa) == capi.OK
+        r, ev = e.tick_dbb(list(range(1, len(isls) + 1)), capacity=1 << 14)
         assert r == capi.OK
-        return e.phi(1).copy(), ev
+        return [e.phi(k + 1).copy() for k in range(len(isls))], ev
 
 
-def test_reupload_keeps_or_rebuilds_the_hierarchy(cuda):
+@pytest.mark.parametrize("n_islands", [1, 2])
+def test_reupload_keeps_or_rebuilds_the_hierarchy(cuda, n_islands):
     """Each tick re-uploads every registered host array.  With type and kappa
-    unchanged the island keeps its multigrid hierarchy; a changed kappa or
-    type rebuilds it.  Either way the tick equals, bit for bit, the tick of a
-    new engine (which builds its hierarchy) from the same warm start, and each
-    re-solve iterates (the source changes every tick)."""
-    isl = W.small_box_union(3, L=20)
-    isl.rcomp[...] = np.float32(0.5)
-    isl.rtens[...] = np.float32(0.25)
+    unchanged an island keeps its multigrid hierarchy; a changed kappa or type
+    rebuilds it.  Either way the tick equals, bit for bit, the tick of a new
+    engine (which builds every hierarchy) from the same warm starts, and each
+    re-solve iterates (the sources change every tick).  With two islands the
+    tick takes the batched solve and only the first island changes."""
+    isls = [W.small_box_union(3, L=20), W.small_box_union(8, L=14)][:n_islands]
+    ids = list(range(1, n_islands + 1))
+    for isl in isls:
+        isl.rcomp[...] = np.float32(0.5)
+        isl.rtens[...] = np.float32(0.25)
     with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
         assert e.init() == capi.OK
-        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
-        assert e.register(1, isl, kappa=isl.kappa) == capi.OK
-        assert e.tick_dbb([1], capacity=1 << 14)[0] == capi.OK
+        for k, isl in enumerate(isls):
+            assert e.add_island(k + 1, isl.lx, isl.ly, isl.lz, origin=(40 * k, 0, 0)) == capi.OK
+            assert e.register(k + 1, isl, kappa=isl.kappa) == capi.OK
+        assert e.tick_dbb(ids, capacity=1 << 14)[0] == capi.OK
         keep = e._keep[1]
         rng = np.random.default_rng(5)
         solid = np.flatnonzero(keep["vt"] == 1)
         for step in ("same", "kappa", "type", "same"):
-            phi0 = e.phi(1).copy()
-            keep["src"] *= np.float32(1.25)
+            phis = [e.phi(i).copy() for i in ids]
+            for i in ids:
+                e._keep[i]["src"] *= np.float32(1.25)
             if step == "kappa":
                 keep["kappa"][rng.choice(solid, 40, replace=False)] *= np.float32(3.0)
             elif step == "type":
                 keep["vt"][rng.choice(solid, 5, replace=False)] = 0
-            cur = dataclasses.replace(
-                isl, vtype=keep["vt"].reshape(isl.vtype.shape).copy(),
-                kappa=keep["kappa"].reshape(isl.vtype.shape).copy(),
-                source=keep["src"].reshape(isl.vtype.shape).copy())
-            e.lib.pfsf_mark_full_rebuild(e.h, 1)
-            r, ev = e.tick_dbb([1], capacity=1 << 14)
+            cur = [dataclasses.replace(
+                isl, vtype=e._keep[i]["vt"].reshape(isl.vtype.shape).copy(),
+                kappa=e._keep[i]["kappa"].reshape(isl.vtype.shape).copy(),
+                source=e._keep[i]["src"].reshape(isl.vtype.shape).copy())
+                for i, isl in zip(ids, isls)]
+            for i in ids:
+                e.lib.pfsf_mark_full_rebuild(e.h, i)
+            r, ev = e.tick_dbb(ids, capacity=1 << 14)
             assert r == capi.OK
-            want_phi, want_ev = _fresh_tick(cur, phi0)
-            assert not np.array_equal(want_phi, phi0), step
-            assert np.array_equal(e.phi(1), want_phi), step
+            want_phis, want_ev = _fresh_tick(cur, phis)
+            for i, want, phi0 in zip(ids, want_phis, phis):
+                assert not np.array_equal(want, phi0), (step, i)
+                assert np.array_equal(e.phi(i), want), (step, i)
             assert np.array_equal(ev, want_ev), step
 
 
