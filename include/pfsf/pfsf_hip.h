@@ -109,9 +109,66 @@ PFSF_API pfsf_result pfsf_hip_build_island_data(
     const float* curing, const float* fill, uint8_t* type, float* source, float* conductivity,
     float* rcomp, float* rtens, float* max_phi, void* stream);
 
-/* Per-voxel kappa from SoA-6 face conductivities: kappa_i = max_d cond[d*n+i]. */
+/* Per-voxel kappa from SoA-6 face conductivities: kappa_i = max_d cond[d*n+i]
+ * over the finite faces (0 when none is positive and finite). */
 PFSF_API pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa,
                                               void* stream);
+
+/* `steps` Chebyshev-accelerated Jacobi steps (SH/jacobi_smooth.comp.glsl:
+ * 215-226, schedule NP/src/compute/chebyshev.cpp:30-57): step k is
+ * pfsf_hip_jacobi_step with omega = pfsf_chebyshev_omega(k, rho_spec) from
+ * the current phi; phi (in/out) and tmp (N floats, scratch) alternate and the
+ * result ends in phi.  Bit-identical to the same steps of the CPU
+ * restatement. */
+PFSF_API pfsf_result pfsf_hip_chebyshev_jacobi(pfsf_dims d, const uint8_t* type,
+                                               const float* kappa, const float* source,
+                                               float* phi, float* tmp, int32_t steps,
+                                               float rho_spec, void* stream);
+
+/* ── SoA-6 operator ──────────────────────────────────────────────────────
+ * The operator the drop-in tick uses when an island registers only its six
+ * face conductivities (cond, 6N, SoA d * N + i, d = -x, +x, -y, +y, -z, +z;
+ * PFSFDataBuilder.java:124-132).  The reference's shader form
+ * (SH/pcg_matvec.comp.glsl:42-128) builds each weight from voxel i's faces
+ * alone and is not symmetric; here a pair takes the mean of its endpoints'
+ * half weights (symmetric; equal to the shader's weight when both agree):
+ *   face   a = 0.5 (c_i[d] + c_j[d'])
+ *   edge   a = 0.5 * 0.5 (sqrt(c_i[da] c_i[db]) + sqrt(c_j[da'] c_j[db']))
+ *   corner a = 0.1666667 * 0.5 (cbrt(c_i[dx] c_i[dy] c_i[dz]) + cbrt(...))
+ * with ' the opposite direction and c = cond if 0 < cond <= FLT_MAX, else 0;
+ * FP64, products left to right, the cube root from correctly rounded basic
+ * operations (bit-identical on any IEEE FP64 implementation).  Air and
+ * out-of-grid neighbours have no coupling; anchors are Dirichlet 0.
+ *
+ * The coupling table cw (device, 13N doubles) holds the weights to the 13
+ * forward neighbours (even stencil offsets k, slot k / 2). */
+PFSF_API pfsf_result pfsf_hip_soa6_couplings(pfsf_dims d, const uint8_t* type, const float* cond,
+                                             double* cw, void* stream);
+
+/* y = A x in FP64 with the SoA-6 couplings, 0 on non-solid rows. */
+PFSF_API pfsf_result pfsf_hip_soa6_apply_f64(pfsf_dims d, const uint8_t* type, const double* cw,
+                                             const double* x, double* y, void* stream);
+
+/* pfsf_hip_failure_scan on the SoA-6 couplings (same precedence, FP64 fluxes
+ * in stencil order). */
+PFSF_API pfsf_result pfsf_hip_soa6_failure_scan(pfsf_dims d, const uint8_t* type,
+                                                const double* cw, const float* phi,
+                                                const float* rcomp, const float* rtens,
+                                                float phi_orphan, uint8_t* code, void* stream);
+
+/* History update with the anisotropic energy density
+ * psi = 0.5 (kx gx^2 + ky gy^2 + kz gz^2), k_a = 0.5 (c[-a] + c[+a]) (FP32,
+ * this order; gradients as pfsf_hip_history_update). */
+PFSF_API pfsf_result pfsf_hip_soa6_history_update(pfsf_dims d, const uint8_t* type,
+                                                  const float* cond, const float* phi, float* H,
+                                                  float* psi, void* stream);
+
+/* `sweeps` 8-colour Gauss-Seidel sweeps (pfsf_hip_rbgs) with the SoA-6
+ * couplings: phi_i = (b_i + sum a_ij phi_j) / sum a_ij in FP64, stencil
+ * order, kept when sum a_ij <= 1e-15. */
+PFSF_API pfsf_result pfsf_hip_soa6_rbgs(pfsf_dims d, const uint8_t* type, const double* cw,
+                                        const float* source, float* phi, int32_t sweeps,
+                                        void* stream);
 
 /* ── Solver object: PCG to a relative-residual tolerance ─────────────── */
 
@@ -292,9 +349,13 @@ PFSF_API pfsf_result pfsf_hip_label_components(pfsf_dims d, const uint8_t* type,
                                                void* stream);
 
 /* ── Engine extension: exact per-voxel kappa ────────────────────────────
- * By default a registered island's kappa is the per-voxel maximum of its
- * SoA-6 conductivities.  This host buffer (float32 x N, valid for the
- * island's lifetime) overrides it with an exact per-voxel kappa. */
+ * By default a registered island is solved with the SoA-6 operator of its
+ * face conductivities (above).  This host buffer (float32 x N, valid for the
+ * island's lifetime) switches the island to the per-voxel kappa operator of
+ * the Java CPU reference; the island's SoA-6 faces are then not read by the
+ * solve, the history update or the failure scan.  A non-finite entry of
+ * kappa or of the source counts as 0, as does a non-finite registered phi
+ * (the warm start). */
 PFSF_API pfsf_result pfsf_register_island_kappa(pfsf_engine engine, int32_t island_id,
                                                 void* kappa_addr, int64_t bytes);
 
@@ -312,6 +373,21 @@ PFSF_API void pfsf_set_solve_tolerance(pfsf_engine engine, double tol, int32_t m
 PFSF_API pfsf_result pfsf_island_components(pfsf_engine engine, int32_t island_id,
                                             int32_t max_components, uint32_t* header,
                                             uint32_t* records, uint32_t* labels_out);
+
+/* Per-island state after a tick.  A solve that stops short of the tolerance
+ * (iteration cap, breakdown) leaves the island dirty, so the next tick
+ * continues from its phi; its failure records are still reported. */
+typedef struct {
+  pfsf_solve_info last_solve; /* the island's last solve (zeros before the first); the
+                                 fixed RBGS schedule (PCG off) reports converged = 1 */
+  int32_t solver_builds;      /* multigrid hierarchies built for the island so far */
+  int32_t operator_mode;      /* 0: registered per-voxel kappa; 1: SoA-6 faces */
+  int32_t dirty;              /* 1 while the island waits for a (further) solve */
+  int32_t reserved;
+} pfsf_island_status;
+
+PFSF_API pfsf_result pfsf_get_island_status(pfsf_engine engine, int32_t island_id,
+                                            pfsf_island_status* out);
 
 /* Device history field of an island (N floats), for inspection. */
 PFSF_API pfsf_result pfsf_read_history(pfsf_engine engine, int32_t island_id, float* out,

@@ -15,7 +15,9 @@ extern "C" {
 PFSF_API uint32_t pfsf_abi_version(void);
 
 /* pfsf_version.h:34.  Known names here: "hip.gfx950", "solve.mg_pcg",
- * "solve.batched", "solve.mixed_bf16", "solve.adjoint", "scan.compact". */
+ * "solve.batched", "solve.mixed_bf16", "solve.adjoint", "scan.compact",
+ * "labels.union_find", "operator.soa6", "smooth.chebyshev",
+ * "compute.normalize", "compute.downsample". */
 PFSF_API bool pfsf_has_feature(const char* name);
 
 /* pfsf_version.h:40.  Static "<compiler> <flags> <arch>" string. */

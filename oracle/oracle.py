@@ -82,6 +82,15 @@ def lib():
         L.or_label_summarise.argtypes = [I, I, I, P, P, I, P, C.POINTER(I)]
         L.or_label_summarise.restype = I
         L.or_build_island_data.argtypes = [I, I, I, P, P, I, P, P, P, P, P, P, P, P, P]
+        L.or_cbrt.argtypes = [C.c_double]
+        L.or_cbrt.restype = C.c_double
+        L.or_soa6_couplings.argtypes = [I, I, I, P, P, P]
+        L.or_soa6_apply.argtypes = [I, I, I, P, P, P, P]
+        L.or_soa6_diag.argtypes = [I, I, I, P, P, P]
+        L.or_soa6_failure_scan.argtypes = [I, I, I, P, P, P, P, P, C.c_float, P]
+        L.or_soa6_psi.argtypes = [I, I, I, P, P, P, P]
+        L.or_soa6_history_update.argtypes = [I, I, I, P, P, P, P]
+        L.or_soa6_rbgs.argtypes = [I, I, I, P, P, P, P, I]
         L.or_set_threads.argtypes = [I]
         L.or_set_threads.restype = I
         L.or_set_threads(1)  # single-threaded unless a caller asks for more
@@ -221,6 +230,176 @@ def downsample_2to1(fine, ftype, lx, ly, lz):
     return c, ct
 
 
+# ------------------------------------------------------- SoA-6 operator
+
+def cbrt(v: float) -> float:
+    return float(lib().or_cbrt(float(v)))
+
+
+def _cond(isl, cond):
+    c = np.ascontiguousarray(cond, np.float32).reshape((6,) + isl.vtype.shape)
+    return c
+
+
+def soa6_couplings(isl, cond):
+    """Coupling table (13, lz, ly, lx) FP64 of the SoA-6 operator."""
+    cw = np.empty((13,) + isl.vtype.shape, np.float64)
+    lib().or_soa6_couplings(*_dims(isl), _p(isl.vtype), _p(_cond(isl, cond)), _p(cw))
+    return cw
+
+
+def soa6_apply(isl, cw, x):
+    y = np.empty(isl.vtype.shape, np.float64)
+    lib().or_soa6_apply(*_dims(isl), _p(isl.vtype), _p(np.ascontiguousarray(cw)),
+                        _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+def soa6_diag(isl, cw):
+    d = np.empty(isl.vtype.shape, np.float64)
+    lib().or_soa6_diag(*_dims(isl), _p(isl.vtype), _p(np.ascontiguousarray(cw)), _p(d))
+    return d
+
+
+def soa6_residual(isl, cw, x, b=None):
+    """||b - A x|| / ||b|| over solid voxels (FP64)."""
+    b = (isl.source if b is None else np.asarray(b)).astype(np.float64)
+    r = np.where(isl.vtype == 1, b - soa6_apply(isl, cw, x), 0.0)
+    bb = float(np.sum(np.where(isl.vtype == 1, b, 0.0) ** 2))
+    return float(np.sqrt(np.sum(r * r) / (bb if bb > 0 else 1.0)))
+
+
+def soa6_failure_scan(isl, cw, phi, phi_orphan=1e6):
+    code = np.empty(isl.vtype.shape, np.uint8)
+    lib().or_soa6_failure_scan(*_dims(isl), _p(isl.vtype), _p(np.ascontiguousarray(cw)),
+                               _p(np.ascontiguousarray(phi, np.float32)), _p(isl.rcomp),
+                               _p(isl.rtens), phi_orphan, _p(code))
+    return code
+
+
+def soa6_psi(isl, cond, phi):
+    out = np.empty(isl.vtype.shape, np.float32)
+    lib().or_soa6_psi(*_dims(isl), _p(isl.vtype), _p(_cond(isl, cond)),
+                      _p(np.ascontiguousarray(phi, np.float32)), _p(out))
+    return out
+
+
+def soa6_history_update(isl, cond, phi, H):
+    H = np.array(H, np.float32)
+    lib().or_soa6_history_update(*_dims(isl), _p(isl.vtype), _p(_cond(isl, cond)),
+                                 _p(np.ascontiguousarray(phi, np.float32)), _p(H))
+    return H
+
+
+def soa6_rbgs(isl, cw, phi=None, sweeps=1):
+    phi = np.zeros(isl.vtype.shape, np.float32) if phi is None else np.array(phi, np.float32)
+    lib().or_soa6_rbgs(*_dims(isl), _p(isl.vtype), _p(np.ascontiguousarray(cw)),
+                       _p(isl.source), _p(phi), sweeps)
+    return phi
+
+
+def soa6_pair_weights(isl, cond):
+    """Independent numpy formulation of the SoA-6 couplings (np.sqrt,
+    np.cbrt): list over the 26 offsets of (mask, i_flat, j_flat, a)."""
+    c = _cond(isl, cond).astype(np.float64)
+    c = np.where((c > 0) & (c <= np.finfo(np.float32).max), c, 0.0)
+    lz, ly, lx = isl.vtype.shape
+    t = isl.vtype
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    out = []
+    for k, o in enumerate(OFFSETS):
+        nx, ny, nz = x + o[0], y + o[1], z + o[2]
+        ok = (nx >= 0) & (nx < lx) & (ny >= 0) & (ny < ly) & (nz >= 0) & (nz < lz) & (t != 0)
+        jx, jy, jz = np.clip(nx, 0, lx - 1), np.clip(ny, 0, ly - 1), np.clip(nz, 0, lz - 1)
+        ok &= t[jz, jy, jx] != 0
+        hi = np.ones(t.shape)
+        hj = np.ones(t.shape)
+        nd = 0
+        for a in range(3):
+            if o[a] == 0:
+                continue
+            nd += 1
+            di = 2 * a + (1 if o[a] > 0 else 0)
+            dj = 2 * a + (0 if o[a] > 0 else 1)
+            hi = hi * c[di]
+            hj = hj * c[dj][jz, jy, jx]
+        if nd == 2:
+            hi, hj, w = np.sqrt(hi), np.sqrt(hj), float(EDGE_P)
+        elif nd == 3:
+            hi, hj, w = np.cbrt(hi), np.cbrt(hj), float(CORNER_P)
+        else:
+            w = 1.0
+        a = w * 0.5 * (hi + hj)
+        out.append((ok, (x + lx * (y + ly * z)).ravel(), (jx + lx * (jy + ly * jz)).ravel(),
+                    np.where(ok, a, 0.0)))
+    return out
+
+
+def assemble_soa6(isl, cond):
+    """Sparse FP64 matrix of the SoA-6 operator over the free set (solid
+    voxels 26-connected to an anchor with a non-zero diagonal).  Returns
+    (A, free_flat_indices)."""
+    import scipy.sparse as sp
+
+    t = isl.vtype.ravel()
+    reach, _ = anchored(isl)
+    pw = soa6_pair_weights(isl, cond)
+    dg = np.zeros(isl.n)
+    for ok, ii, jj, a in pw:
+        m = ok.ravel() & (t[ii] == 1)
+        np.add.at(dg, ii[m], a.ravel()[m])
+    free = (t == 1) & (reach.ravel() != 0) & (dg > 0)
+    fidx = np.flatnonzero(free)
+    ids = -np.ones(isl.n, np.int64)
+    ids[fidx] = np.arange(fidx.size)
+    rows, cols, vals = [np.arange(fidx.size)], [np.arange(fidx.size)], [dg[fidx]]
+    for ok, ii, jj, a in pw:
+        m = ok.ravel() & free[ii] & free[jj]
+        rows.append(ids[ii[m]])
+        cols.append(ids[jj[m]])
+        vals.append(-a.ravel()[m])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(fidx.size, fidx.size))
+    return A, fidx
+
+
+def direct_solve_soa6(isl, cond, b=None):
+    """FP64 sparse direct solve of the SoA-6 system on the free set."""
+    import scipy.sparse.linalg as spla
+
+    A, fidx = assemble_soa6(isl, cond)
+    b = isl.source if b is None else b
+    phi = np.zeros(isl.n, np.float64)
+    if fidx.size:
+        phi[fidx] = spla.spsolve(A.tocsc(), np.asarray(b, np.float64).ravel()[fidx])
+    return phi.reshape(isl.vtype.shape)
+
+
+def sparse_scatter(records, arrays):
+    """SH/sparse_scatter.comp.glsl:21-67: each 48-byte record {int32 flat,
+    f32 source, u32 type, f32 max_phi, f32 rcomp, f32 rtens, f32 cond[6]}
+    overwrites voxel `flat` of the island arrays (dict of flat numpy arrays:
+    source, type, max_phi, rcomp, rtens, cond (6n) in SoA order d * n + i).
+    A record whose index lies outside [0, n) is skipped (the shader does not
+    check; this library defines the skip).  Records must name distinct
+    voxels (the device applies them concurrently)."""
+    rec = np.ascontiguousarray(records).view(np.uint32).reshape(-1, 12)
+    n = arrays["source"].size
+    out = {k: np.array(v) for k, v in arrays.items()}
+    for r in rec:
+        i = int(np.int32(r[0]))
+        if i < 0 or i >= n:
+            continue
+        out["source"][i] = r[1:2].view(np.float32)[0]
+        out["type"][i] = np.uint8(r[2] & 0xFF)
+        out["max_phi"][i] = r[3:4].view(np.float32)[0]
+        out["rcomp"][i] = r[4:5].view(np.float32)[0]
+        out["rtens"][i] = r[5:6].view(np.float32)[0]
+        for d in range(6):
+            out["cond"][d * n + i] = r[6 + d:7 + d].view(np.float32)[0]
+    return out
+
+
 # ------------------------------------------------ reference CPU primitives
 
 def ref_lib():
@@ -233,6 +412,10 @@ def ref_lib():
     L.pfsf_chebyshev_omega.argtypes = [C.c_int32, C.c_float]
     L.pfsf_chebyshev_omega.restype = C.c_float
     L.pfsf_downsample_2to1.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, P, P]
+    L.pfsf_precompute_omega_table.argtypes = [C.c_float, P, C.c_int32]
+    L.pfsf_precompute_omega_table.restype = C.c_int32
+    L.pfsf_estimate_spectral_radius.argtypes = [C.c_int32, C.c_float]
+    L.pfsf_estimate_spectral_radius.restype = C.c_float
     return L
 
 

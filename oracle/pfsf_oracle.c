@@ -830,3 +830,252 @@ int or_label_summarise(int lx, int ly, int lz, const uint8_t* type, const uint32
   free(slot);
   return num;
 }
+
+/* ── SoA-6 operator (face conductivities per voxel) ─────────────────────
+ * The drop-in tick receives six face conductivities per voxel, SoA order
+ * cond[d * n + i], d = -x, +x, -y, +y, -z, +z (PFSFDataBuilder.java:124-132,
+ * PFSFConductivity.java:37-91).  The reference GPU operator
+ * (SH/pcg_matvec.comp.glsl:42-128, SH/rbgs_smooth.comp.glsl:116-163) builds
+ * edge and corner weights from voxel i's own faces only, so it is not
+ * symmetric.  This operator takes the mean of the two endpoints' half
+ * weights, which makes it symmetric and equal to the shader's weight when
+ * both endpoints agree:
+ *   face   a = 0.5 * (c_i[d] + c_j[opp d])
+ *   edge   a = EDGE_P   * 0.5 * (sqrt(c_i[da] c_i[db]) + sqrt(c_j[da'] c_j[db']))
+ *   corner a = CORNER_P * 0.5 * (cbrt(c_i[dx] c_i[dy] c_i[dz]) + cbrt(c_j[..']))
+ * with d the face directions of the offset's components (' = opposite sign),
+ * c = cond where 0 < cond <= FLT_MAX and 0 otherwise, all in FP64 with the
+ * products formed left to right.  Pairs with an air or out-of-grid endpoint
+ * have no coupling.  The 13 couplings of a voxel to its forward neighbours
+ * (the even offsets of the stencil order: dz > 0, or dz = 0 and dy > 0, or
+ * dz = dy = 0 and dx > 0) are tabulated as cw[(k / 2) * n + i]; an odd
+ * (backward) offset k of voxel i reads the entry of its opposite offset at
+ * the neighbour. */
+static const int OPP[26] = {1,  0,  3,  2,  5,  4,  9,  8,  7,  6,  13, 12, 11,
+                            10, 17, 16, 15, 14, 25, 24, 23, 22, 21, 20, 19, 18};
+
+/* Cube root from basic IEEE operations only (each correctly rounded), so
+ * that any conforming FP64 implementation gives the same bits: v = m 2^(3q)
+ * with m in [0.5, 4), a chord of cbrt over that interval as the first guess
+ * (error below 11 %), then six Newton steps y <- (2y + m / y^2) / 3, which
+ * converge quadratically to within an ulp of cbrt(m). */
+double or_cbrt(double v) {
+  if (!(v > 0.0)) return 0.0;
+  if (v > 1.7976931348623157e308) return v;
+  int e;
+  double m = frexp(v, &e);
+  int r = e % 3;
+  if (r < 0) r += 3;
+  int q = (e - r) / 3;
+  m = ldexp(m, r);
+  double y = 0.79370052598409979 + 0.22677165 * (m - 0.5);
+  for (int it = 0; it < 6; it++) y = (2.0 * y + m / (y * y)) / 3.0;
+  return ldexp(y, q);
+}
+
+static inline double soa6_c(const float* cond, int64_t n, int d, int64_t i) {
+  float c = cond[d * n + i];
+  return (c > 0.0f && c <= 3.40282347e38f) ? (double)c : 0.0;
+}
+
+/* Face direction index of one offset component: axis a (0 x, 1 y, 2 z),
+ * sign s != 0. */
+static inline int soa6_dir(int a, int s) { return 2 * a + (s > 0 ? 1 : 0); }
+
+/* Half weight of voxel i toward offset (dx,dy,dz) (sign flipped with
+ * flip = -1: the neighbour's half toward i). */
+static double soa6_half(const float* cond, int64_t n, int64_t i, const int* o, int flip) {
+  int dirs[3], nd = 0;
+  for (int a = 0; a < 3; a++)
+    if (o[a] != 0) dirs[nd++] = soa6_dir(a, o[a] * flip);
+  if (nd == 1) return soa6_c(cond, n, dirs[0], i);
+  if (nd == 2) return sqrt(soa6_c(cond, n, dirs[0], i) * soa6_c(cond, n, dirs[1], i));
+  return or_cbrt(soa6_c(cond, n, dirs[0], i) * soa6_c(cond, n, dirs[1], i) *
+                 soa6_c(cond, n, dirs[2], i));
+}
+
+void or_soa6_couplings(int lx, int ly, int lz, const uint8_t* type, const float* cond,
+                       double* cw) {
+  int64_t n = (int64_t)lx * ly * lz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        for (int k = 0; k < 26; k += 2) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          double a = 0.0;
+          if (type[i] != T_AIR && nx >= 0 && nx < lx && ny >= 0 && ny < ly && nz >= 0 &&
+              nz < lz) {
+            int64_t j = IDX(nx, ny, nz, lx, ly);
+            if (type[j] != T_AIR) {
+              double h = 0.5 * (soa6_half(cond, n, i, OFFS[k], 1) +
+                                soa6_half(cond, n, j, OFFS[k], -1));
+              int nd = (OFFS[k][0] != 0) + (OFFS[k][1] != 0) + (OFFS[k][2] != 0);
+              a = nd == 1 ? h : (nd == 2 ? (double)EDGE_P * h : (double)CORNER_P * h);
+            }
+          }
+          cw[(k / 2) * n + i] = a;
+        }
+      }
+}
+
+/* Coupling of voxel i to its in-grid neighbour j at offset k. */
+static inline double soa6_at(const double* cw, int64_t n, int k, int64_t i, int64_t j) {
+  return (k & 1) ? cw[(OPP[k] / 2) * n + j] : cw[(k / 2) * n + i];
+}
+
+void or_soa6_apply(int lx, int ly, int lz, const uint8_t* type, const double* cw,
+                   const double* x, double* y) {
+  int64_t n = (int64_t)lx * ly * lz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < lz; z++)
+    for (int yy = 0; yy < ly; yy++)
+      for (int xx = 0; xx < lx; xx++) {
+        int64_t i = IDX(xx, yy, z, lx, ly);
+        if (!is_solid(type[i])) {
+          y[i] = 0.0;
+          continue;
+        }
+        double acc = 0.0, xi = x[i];
+        for (int k = 0; k < 26; k++) {
+          int nx = xx + OFFS[k][0], ny = yy + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          uint8_t tj = type[j];
+          if (tj == T_AIR) continue;
+          double xj = is_solid(tj) ? x[j] : 0.0;
+          acc += soa6_at(cw, n, k, i, j) * (xi - xj);
+        }
+        y[i] = acc;
+      }
+}
+
+void or_soa6_diag(int lx, int ly, int lz, const uint8_t* type, const double* cw, double* d) {
+  int64_t n = (int64_t)lx * ly * lz;
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        double acc = 0.0;
+        if (is_solid(type[i]))
+          for (int k = 0; k < 26; k++) {
+            int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+            if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+            int64_t j = IDX(nx, ny, nz, lx, ly);
+            if (type[j] == T_AIR) continue;
+            acc += soa6_at(cw, n, k, i, j);
+          }
+        d[i] = acc;
+      }
+}
+
+/* Failure scan of or_failure_scan with the SoA-6 couplings (same precedence,
+ * FP64 fluxes in stencil order). */
+void or_soa6_failure_scan(int lx, int ly, int lz, const uint8_t* type, const double* cw,
+                          const float* phi, const float* rcomp, const float* rtens,
+                          float phi_orphan, uint8_t* code) {
+  int64_t n = (int64_t)lx * ly * lz;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        code[i] = 0;
+        if (!is_solid(type[i])) continue;
+        double pi = (double)phi[i];
+        double diag = 0.0, fin = 0.0, fout = 0.0;
+        for (int k = 0; k < 26; k++) {
+          int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          double a = soa6_at(cw, n, k, i, j);
+          diag += a;
+          double dp = (double)phi[j] - pi;
+          if (dp > 0.0)
+            fin += a * dp;
+          else if (dp < 0.0)
+            fout += a * (-dp);
+        }
+        if (diag == 0.0 || phi[i] > phi_orphan) {
+          code[i] = 3;
+          continue;
+        }
+        if (rcomp[i] > 0.0f && fin > (double)rcomp[i]) {
+          code[i] = 2;
+          continue;
+        }
+        if (rtens[i] > 0.0f && fout > (double)rtens[i]) code[i] = 4;
+      }
+}
+
+/* Elastic energy density with a per-axis conductivity, the anisotropic form
+ * of 0.5 * kappa * |grad phi|^2:
+ *   psi = 0.5 * (kx gx^2 + ky gy^2 + kz gz^2),  k_a = 0.5 * (c[-a] + c[+a]),
+ * gradients as in or_psi; FP32 in this order. */
+void or_soa6_psi(int lx, int ly, int lz, const uint8_t* type, const float* cond,
+                 const float* phi, float* psi) {
+  int64_t n = (int64_t)lx * ly * lz;
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        if (!is_solid(type[i])) {
+          psi[i] = 0.0f;
+          continue;
+        }
+        int64_t sy = lx, sz = (int64_t)lx * ly;
+        float gx = axis_grad(type, phi, i, x > 0, x + 1 < lx, i - 1, i + 1);
+        float gy = axis_grad(type, phi, i, y > 0, y + 1 < ly, i - sy, i + sy);
+        float gz = axis_grad(type, phi, i, z > 0, z + 1 < lz, i - sz, i + sz);
+        float k[3];
+        for (int a = 0; a < 3; a++)
+          k[a] = 0.5f * ((float)soa6_c(cond, n, 2 * a, i) + (float)soa6_c(cond, n, 2 * a + 1, i));
+        float e = k[0] * (gx * gx);
+        e = e + k[1] * (gy * gy);
+        e = e + k[2] * (gz * gz);
+        psi[i] = 0.5f * e;
+      }
+}
+
+void or_soa6_history_update(int lx, int ly, int lz, const uint8_t* type, const float* cond,
+                            const float* phi, float* H) {
+  int64_t n = (int64_t)lx * ly * lz;
+  float* psi = (float*)malloc(sizeof(float) * (size_t)n);
+  or_soa6_psi(lx, ly, lz, type, cond, phi, psi);
+  for (int64_t i = 0; i < n; i++)
+    if (is_solid(type[i]) && psi[i] > H[i]) H[i] = psi[i];
+  free(psi);
+}
+
+/* 8-colour Gauss-Seidel of or_rbgs with the SoA-6 couplings. */
+void or_soa6_rbgs(int lx, int ly, int lz, const uint8_t* type, const double* cw,
+                  const float* source, float* phi, int sweeps) {
+  int64_t n = (int64_t)lx * ly * lz;
+  for (int s = 0; s < sweeps; s++)
+    for (int c = 0; c < 8; c++) {
+#pragma omp parallel for schedule(static)
+      for (int z = (c >> 2) & 1; z < lz; z += 2)
+        for (int y = (c >> 1) & 1; y < ly; y += 2)
+          for (int x = c & 1; x < lx; x += 2) {
+            int64_t i = IDX(x, y, z, lx, ly);
+            if (!is_solid(type[i])) {
+              phi[i] = 0.0f;
+              continue;
+            }
+            double diag = 0.0, off = 0.0;
+            for (int k = 0; k < 26; k++) {
+              int nx = x + OFFS[k][0], ny = y + OFFS[k][1], nz = z + OFFS[k][2];
+              if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+              int64_t j = IDX(nx, ny, nz, lx, ly);
+              if (type[j] == T_AIR) continue;
+              double a = soa6_at(cw, n, k, i, j);
+              diag += a;
+              off += a * (double)phi[j];
+            }
+            double pj = (diag > 1e-15) ? ((double)source[i] + off) / diag : (double)phi[i];
+            phi[i] = (float)pj;
+          }
+    }
+}

@@ -21,6 +21,7 @@
 #include "mg_solver.h"
 #include "pfsf/pfsf.h"
 #include "pfsf_internal.h"
+#include "soa6.h"
 
 using namespace pfsf;
 
@@ -51,12 +52,14 @@ struct Island {
   pfsf_dims dims{0, 0, 0};
   int64_t n = 0;
   bool dirty = true;
-  bool topo_dirty = true;  // kappa / type changed since the hierarchy was built
+  bool topo_dirty = true;  // type or kappa changed since the hierarchy was built
   bool have_phi = false;
+  // operator mode: SoA-6 faces (no per-voxel kappa registered) or kappa
+  bool soa6 = false;
   HostBufs host;
   // device state
   uint8_t* type = nullptr;
-  float* kappa = nullptr;
+  float* kappa = nullptr;  // registered kappa, or the SoA-6 proxy (largest face)
   float* cond = nullptr;
   float* source = nullptr;
   float* rcomp = nullptr;
@@ -66,8 +69,6 @@ struct Island {
   double* x64 = nullptr;
   float* H = nullptr;
   uint8_t* code = nullptr;
-  int32_t* events = nullptr;  // 1 + 4*capacity int32
-  int32_t event_cap = 0;
   int32_t* scratch = nullptr;
   int64_t scratch_elems = 0;
   uint32_t* sparse_dev = nullptr;
@@ -75,12 +76,17 @@ struct Island {
   uint32_t* labels = nullptr;     // component labels (N), allocated on first use
   uint32_t* lab_rec = nullptr;    // kMaxLabelComponents records of 12 words
   uint32_t* lab_scratch = nullptr;
+  double* cw = nullptr;           // SoA-6 coupling table (13 N), allocated on first use
   std::unique_ptr<MgSolver> solver;
-  // type and kappa the solver's hierarchy was built from, and a device flag:
-  // a re-upload of identical arrays keeps the hierarchy
+  std::unique_ptr<Soa6Solver> s6;
+  int64_t extra_bytes = 0;        // SoA-6 table and solver, counted in vram_used
+  int32_t builds = 0;             // hierarchies built for this island
+  pfsf_solve_info last{};         // its last solve
+  // type and kappa the solver's hierarchy was built from: a re-upload of
+  // identical arrays keeps the hierarchy
   uint8_t* built_type = nullptr;
   float* built_kappa = nullptr;
-  int32_t* topo_diff = nullptr;
+  bool built_valid = false;
   std::vector<void*> owned;
 };
 
@@ -106,6 +112,14 @@ struct pfsf_engine_t {
   mutable std::mutex stats_mtx;
   float last_tick_ms = 0.f;
   int64_t vram_used = 0;
+  // per tick: topology flags of the re-uploaded islands and the failure
+  // tuples of all reported islands, each read back in one copy
+  int32_t* flags_dev = nullptr;
+  int32_t* flags_host = nullptr;  // pinned
+  size_t flags_cap = 0;
+  int32_t* ev_dev = nullptr;
+  int32_t* ev_host = nullptr;  // pinned
+  int64_t ev_cap = 0;          // tuples
 };
 
 namespace {
@@ -122,10 +136,21 @@ pfsf_config default_config() {
 
 void free_island(Island& is) {
   is.solver.reset();
+  is.s6.reset();
   for (void* p : is.owned) (void)hipFree(p);
   is.owned.clear();
   if (is.sparse_host) (void)hipHostFree(is.sparse_host);
   is.sparse_host = nullptr;
+}
+
+void free_tick_buffers(pfsf_engine_t* E) {
+  if (E->flags_dev) (void)hipFree(E->flags_dev);
+  if (E->flags_host) (void)hipHostFree(E->flags_host);
+  if (E->ev_dev) (void)hipFree(E->ev_dev);
+  if (E->ev_host) (void)hipHostFree(E->ev_host);
+  E->flags_dev = E->flags_host = E->ev_dev = E->ev_host = nullptr;
+  E->flags_cap = 0;
+  E->ev_cap = 0;
 }
 
 template <typename T>
@@ -149,13 +174,11 @@ bool allocate(pfsf_engine_t* E, Island& is) {
             dev_alloc(is, &is.cond, 6 * n) && dev_alloc(is, &is.source, n) &&
             dev_alloc(is, &is.rcomp, n) && dev_alloc(is, &is.rtens, n) &&
             dev_alloc(is, &is.max_phi, n) && dev_alloc(is, &is.phi, n) &&
-            dev_alloc(is, &is.x64, n) && dev_alloc(is, &is.H, n) && dev_alloc(is, &is.code, n);
+            dev_alloc(is, &is.x64, n) && dev_alloc(is, &is.H, n) && dev_alloc(is, &is.code, n) &&
+            dev_alloc(is, &is.built_type, n) && dev_alloc(is, &is.built_kappa, n);
   if (!ok) return false;
   is.scratch_elems = compact_scratch_elems(is.dims);
-  is.event_cap = 4096;
-  ok = dev_alloc(is, &is.scratch, is.scratch_elems) &&
-       dev_alloc(is, &is.events, 1 + 4 * (int64_t)is.event_cap);
-  if (!ok) return false;
+  if (!dev_alloc(is, &is.scratch, is.scratch_elems)) return false;
   hipStream_t s = E->stream;
   ok = hipMemsetAsync(is.type, 0, n, s) == hipSuccess &&
        hipMemsetAsync(is.kappa, 0, 4 * n, s) == hipSuccess &&
@@ -176,9 +199,46 @@ Island* find(pfsf_engine_t* E, int32_t id) {
   return it == E->islands.end() ? nullptr : it->second.get();
 }
 
+// Warm start: the registered phi widened to FP64 (a non-finite entry starts
+// at 0).
 __global__ void k_widen(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
   int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) b[i] = (double)a[i];
+  if (i < n) {
+    const float v = a[i];
+    b[i] = isfinite(v) ? (double)v : 0.0;
+  }
+}
+
+// A non-finite entry of an uploaded source or kappa counts as 0 (no load, no
+// coupling), so one bad value cannot spread NaN through a solve.
+__global__ void k_finite_or_zero(int64_t n, float* __restrict__ a) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && !isfinite(a[i])) a[i] = 0.0f;
+}
+
+// The operator data of the uploaded arrays: the registered kappa (finite),
+// or from the SoA-6 faces the coupling table and the isotropic proxy kappa
+// of the preconditioner.
+bool derive(pfsf_engine_t* E, Island& is) {
+  hipStream_t s = E->stream;
+  const int64_t n = is.n;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  k_finite_or_zero<<<nb, 256, 0, s>>>(n, is.source);
+  if (is.host.kappa) {
+    is.soa6 = false;
+    k_finite_or_zero<<<nb, 256, 0, s>>>(n, is.kappa);
+  } else {
+    is.soa6 = true;
+    if (!is.cw) {
+      if (!dev_alloc(is, &is.cw, kSoa6Slots * n)) return false;
+      is.extra_bytes += 8 * kSoa6Slots * n;
+      E->vram_used += 8 * kSoa6Slots * n;
+    }
+    if (launch_kappa_from_soa6(is.cond, n, is.kappa, s) != hipSuccess ||
+        launch_soa6_couplings(is.dims, is.type, is.cond, is.cw, s) != hipSuccess)
+      return false;
+  }
+  return hipGetLastError() == hipSuccess;
 }
 
 // Host DBB -> device (NP/src/core/island_buffer.cpp:427-598).
@@ -194,11 +254,9 @@ bool upload(pfsf_engine_t* E, Island& is) {
             hipMemcpyAsync(is.max_phi, h.max_phi, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(is.phi, h.phi, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess;
   if (!ok) return false;
-  if (h.kappa)
-    ok = hipMemcpyAsync(is.kappa, h.kappa, 4 * n, hipMemcpyHostToDevice, s) == hipSuccess;
-  else
-    ok = launch_kappa_from_soa6(is.cond, n, is.kappa, s) == hipSuccess;
-  if (!ok) return false;
+  if (h.kappa && hipMemcpyAsync(is.kappa, h.kappa, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+    return false;
+  if (!derive(E, is)) return false;
   // the registered phi is the warm start of the solve
   k_widen<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, is.phi, is.x64);
   is.topo_dirty = true;
@@ -218,51 +276,90 @@ __global__ void k_topo_diff(int64_t n, const uint8_t* __restrict__ type,
   if (__syncthreads_or(d) && threadIdx.x == 0) *diff = 1;
 }
 
+bool has_solver(const Island& is) { return is.soa6 ? (bool)is.s6 : (bool)is.solver; }
+
 // Clears topo_dirty of each island whose device type and kappa equal, bit for
-// bit, the arrays its solver's hierarchy was built from: one pass over 10 B
-// per voxel per island, all queued, then one stream synchronisation (against
-// a full hierarchy rebuild per island).
+// bit, the arrays its hierarchy was built from: one pass over 5 B per voxel
+// per island, all flags in one device array, read back with one copy and one
+// stream synchronisation (against a full hierarchy rebuild per island).
 bool settle_topo(pfsf_engine_t* E, const std::vector<Island*>& todo) {
   hipStream_t s = E->stream;
   std::vector<Island*> chk;
   for (Island* is : todo)
-    if (is->solver && is->topo_dirty && is->built_type) chk.push_back(is);
+    if (has_solver(*is) && is->topo_dirty && is->built_valid) chk.push_back(is);
   if (chk.empty()) return true;
-  std::vector<int32_t> diff(chk.size(), 1);
+  if (chk.size() > E->flags_cap) {
+    if (E->flags_dev) (void)hipFree(E->flags_dev);
+    if (E->flags_host) (void)hipHostFree(E->flags_host);
+    E->flags_dev = E->flags_host = nullptr;
+    E->flags_cap = 0;
+    const size_t cap = std::max<size_t>(64, 2 * chk.size());
+    if (hipMalloc((void**)&E->flags_dev, cap * sizeof(int32_t)) != hipSuccess) return false;
+    if (hipHostMalloc((void**)&E->flags_host, cap * sizeof(int32_t)) != hipSuccess) return false;
+    E->flags_cap = cap;
+  }
+  if (hipMemsetAsync(E->flags_dev, 0, chk.size() * sizeof(int32_t), s) != hipSuccess)
+    return false;
   for (size_t k = 0; k < chk.size(); k++) {
     Island& is = *chk[k];
     const unsigned blocks = (unsigned)std::min<int64_t>((is.n + 255) / 256, 2048);
-    if (hipMemsetAsync(is.topo_diff, 0, sizeof(int32_t), s) != hipSuccess) return false;
     k_topo_diff<<<blocks, 256, 0, s>>>(is.n, is.type, (const uint32_t*)is.kappa, is.built_type,
-                                       (const uint32_t*)is.built_kappa, is.topo_diff);
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&diff[k], is.topo_diff, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-      return false;
+                                       (const uint32_t*)is.built_kappa, E->flags_dev + k);
   }
-  if (hipStreamSynchronize(s) != hipSuccess) return false;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(E->flags_host, E->flags_dev, chk.size() * sizeof(int32_t),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return false;
   for (size_t k = 0; k < chk.size(); k++)
-    if (diff[k] == 0) chk[k]->topo_dirty = false;
+    if (E->flags_host[k] == 0) chk[k]->topo_dirty = false;
   return true;
 }
 
 // Records the type and kappa a hierarchy was just built from.
 bool keep_built_topo(pfsf_engine_t* E, Island& is) {
   hipStream_t s = E->stream;
-  if (!is.built_type &&
-      !(dev_alloc(is, &is.built_type, is.n) && dev_alloc(is, &is.built_kappa, is.n) &&
-        dev_alloc(is, &is.topo_diff, 1)))
-    return false;
-  return hipMemcpyAsync(is.built_type, is.type, is.n, hipMemcpyDeviceToDevice, s) ==
-             hipSuccess &&
-         hipMemcpyAsync(is.built_kappa, is.kappa, 4 * is.n, hipMemcpyDeviceToDevice, s) ==
-             hipSuccess;
+  is.built_valid =
+      hipMemcpyAsync(is.built_type, is.type, is.n, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(is.built_kappa, is.kappa, 4 * is.n, hipMemcpyDeviceToDevice, s) ==
+          hipSuccess;
+  return is.built_valid;
+}
+
+pfsf_result from_hip(hipError_t e) {
+  if (e == hipSuccess) return PFSF_OK;
+  (void)hipGetLastError();
+  return e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : PFSF_ERROR_VULKAN;
 }
 
 // The island's solver, created or rebuilt when its voxels changed.  With
-// multigrid on, solvers are batchable so that several dirty islands can be
-// solved in one batched solve.
+// multigrid on, kappa-mode solvers are batchable so that several dirty
+// islands can be solved in one batched solve.  A SoA-6 island's solver
+// re-reads the coupling table on every solve (the faces may change while the
+// type and the proxy kappa, which fix the hierarchy, do not).
 pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
+  if (is.soa6) {
+    is.solver.reset();
+    if (!is.s6) {
+      auto sv = std::make_unique<Soa6Solver>();
+      hipError_t e = sv->create(is.dims, is.type, is.cond, is.cw, is.kappa,
+                                E->cfg.enable_multigrid != 0, E->tol, E->max_iters, E->stream);
+      if (e != hipSuccess) return from_hip(e);
+      const int64_t b = sv->device_bytes();
+      is.extra_bytes += b;
+      E->vram_used += b;
+      is.s6 = std::move(sv);
+      is.builds++;
+    } else if (is.topo_dirty) {
+      hipError_t e = is.s6->rebuild();
+      if (e != hipSuccess) return from_hip(e);
+      is.builds++;
+    }
+    if (is.topo_dirty && !keep_built_topo(E, is)) return PFSF_ERROR_VULKAN;
+    is.topo_dirty = false;
+    return from_hip(is.s6->refresh());
+  }
+  is.s6.reset();
   if (is.solver && !is.topo_dirty) return PFSF_OK;
   if (!is.solver) {
     pfsf_solver_desc d{};
@@ -275,12 +372,14 @@ pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
     d.max_iters = E->max_iters;
     d.stream = E->stream;
     auto sv = std::make_unique<MgSolver>();
-    if (sv->create(d, E->cfg.enable_multigrid != 0) != hipSuccess) return PFSF_ERROR_OUT_OF_VRAM;
+    hipError_t e = sv->create(d, E->cfg.enable_multigrid != 0);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? PFSF_ERROR_OUT_OF_VRAM : from_hip(e);
     is.solver = std::move(sv);
   } else if (is.solver->update(is.type, is.kappa) != hipSuccess) {
     return PFSF_ERROR_VULKAN;
   }
-  if (!keep_built_topo(E, is)) return PFSF_ERROR_OUT_OF_VRAM;
+  is.builds++;
+  if (!keep_built_topo(E, is)) return PFSF_ERROR_VULKAN;
   is.topo_dirty = false;
   return PFSF_OK;
 }
@@ -288,33 +387,46 @@ pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
 // History and failure scan of a solved island.
 pfsf_result post_solve(pfsf_engine_t* E, Island& is) {
   hipStream_t s = E->stream;
-  if (E->cfg.enable_phase_field &&
-      launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
-  if (launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens, kPhiOrphan,
-                          is.code, s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
+  hipError_t e = hipSuccess;
+  if (E->cfg.enable_phase_field)
+    e = is.soa6 ? launch_soa6_history(is.dims, is.type, is.cond, is.phi, is.H, nullptr, s)
+                : launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s);
+  if (e != hipSuccess) return PFSF_ERROR_VULKAN;
+  e = is.soa6 ? launch_soa6_failure_scan(is.dims, is.type, is.cw, is.phi, is.rcomp, is.rtens,
+                                         kPhiOrphan, is.code, s)
+              : launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens,
+                                    kPhiOrphan, is.code, s);
+  if (e != hipSuccess) return PFSF_ERROR_VULKAN;
   is.have_phi = true;
   return PFSF_OK;
 }
 
 pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
   hipStream_t s = E->stream;
+  pfsf_solve_info info{};
   if (E->pcg) {
     pfsf_result r = ensure_solver(E, is);
     if (r != PFSF_OK) return r;
-    pfsf_solve_info info{};
-    if (is.solver->solve(is.source, is.x64, is.phi, &info) != hipSuccess) return PFSF_ERROR_VULKAN;
+    hipError_t e = is.soa6 ? is.s6->solve(is.source, is.x64, is.phi, &info)
+                           : is.solver->solve(is.source, is.x64, is.phi, &info);
+    if (e != hipSuccess) return PFSF_ERROR_VULKAN;
   } else {
-    if (launch_rbgs_exact(is.dims, is.type, is.kappa, is.source, is.phi, kRbgsSweeps, s) !=
-        hipSuccess)
-      return PFSF_ERROR_VULKAN;
+    // the reference's fixed smoothing schedule: no tolerance applies, so the
+    // island is done for this tick (reported as converged, 0 iterations)
+    hipError_t e = is.soa6 ? launch_soa6_rbgs(is.dims, is.type, is.cw, is.source, is.phi,
+                                              kRbgsSweeps, s)
+                           : launch_rbgs_exact(is.dims, is.type, is.kappa, is.source, is.phi,
+                                               kRbgsSweeps, s);
+    if (e != hipSuccess) return PFSF_ERROR_VULKAN;
     k_widen<<<(unsigned)((is.n + 255) / 256), 256, 0, s>>>(is.n, is.phi, is.x64);
+    info.converged = 1;
   }
+  is.last = info;
   return post_solve(E, is);
 }
 
-// Several dirty islands in one batched solve (same numerics per island).
+// Several kappa-mode dirty islands in one batched solve (same numerics per
+// island).
 pfsf_result solve_batch(pfsf_engine_t* E, const std::vector<Island*>& todo) {
   std::vector<MgSolver*> sv;
   std::vector<const float*> b;
@@ -331,77 +443,81 @@ pfsf_result solve_batch(pfsf_engine_t* E, const std::vector<Island*>& todo) {
   BatchSolver batch;
   if (batch.attach(sv, E->tol, E->max_iters, E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
   pfsf_batch_info info{};
-  if (batch.solve(b.data(), x.data(), phi.data(), &info, nullptr) != hipSuccess)
+  std::vector<pfsf_solve_info> per(todo.size());
+  if (batch.solve(b.data(), x.data(), phi.data(), &info, per.data()) != hipSuccess)
     return PFSF_ERROR_VULKAN;
-  for (Island* is : todo) {
-    pfsf_result r = post_solve(E, *is);
+  for (size_t k = 0; k < todo.size(); k++) {
+    todo[k]->last = per[k];
+    pfsf_result r = post_solve(E, *todo[k]);
     if (r != PFSF_OK) return r;
   }
   return PFSF_OK;
 }
 
-// Appends this island's failures to a {count, tuples...} buffer (host).
-pfsf_result drain_failures(pfsf_engine_t* E, Island& is, int32_t* header, int32_t* tuples,
-                           int32_t cap) {
+// Failure tuples of every solved island, appended on the device in tick
+// order into one {count, tuples} buffer, and the stress readbacks: all queued,
+// one copy of the tuples, one stream synchronisation.  out: host {count,
+// tuples} (count at out[0]) with room for `cap` tuples, or NULL.
+pfsf_result report(pfsf_engine_t* E, const std::vector<Island*>& todo, int32_t* out, int64_t cap) {
   hipStream_t s = E->stream;
-  int32_t count = *header;
-  int32_t room = cap - count;
-  if (room <= 0) return PFSF_OK;
-  int32_t want = room < is.event_cap ? room : is.event_cap;
-  if (launch_compact(is.dims, is.code, is.desc.origin, is.events, want, is.scratch,
-                     is.scratch_elems, s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
-  int32_t got = 0;
-  if (hipMemcpyAsync(&got, is.events, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
-  if (got > 0 &&
-      (hipMemcpyAsync(tuples + 4 * (int64_t)count, is.events + 1, 16 * (size_t)got,
-                      hipMemcpyDeviceToHost, s) != hipSuccess ||
-       hipStreamSynchronize(s) != hipSuccess))
-    return PFSF_ERROR_VULKAN;
-  *header = count + got;
-  return PFSF_OK;
-}
-
-// Failure records and stress readback of a solved island.
-pfsf_result report(pfsf_engine_t* E, Island& is, pfsf_tick_result* result, void* fail_addr,
-                   int64_t fail_bytes) {
-  if (fail_addr && fail_bytes >= 20) {
-    int32_t cap = (int32_t)((fail_bytes - 4) / 16);
-    int32_t* hdr = (int32_t*)fail_addr;
-    pfsf_result r = drain_failures(E, is, hdr, hdr + 1, cap);
-    if (r != PFSF_OK) return r;
+  if (out && cap > 0) {
+    int64_t vox = 0;
+    for (Island* is : todo) vox += is->n;
+    cap = std::min<int64_t>(cap, vox);  // never more failures than voxels
   }
-  if (result && result->events && result->count < result->capacity) {
-    int32_t hdr = result->count;
-    pfsf_result r = drain_failures(E, is, &hdr, (int32_t*)result->events, result->capacity);
-    if (r != PFSF_OK) return r;
-    result->count = hdr;
+  const bool events = out && cap > 0;
+  if (events && cap > E->ev_cap) {
+    if (E->ev_dev) (void)hipFree(E->ev_dev);
+    if (E->ev_host) (void)hipHostFree(E->ev_host);
+    E->ev_dev = E->ev_host = nullptr;
+    E->ev_cap = 0;
+    const int64_t c = std::max<int64_t>(cap, 1024);
+    if (hipMalloc((void**)&E->ev_dev, sizeof(int32_t) * (1 + 4 * (size_t)c)) != hipSuccess ||
+        hipHostMalloc((void**)&E->ev_host, sizeof(int32_t) * (1 + 4 * (size_t)c)) != hipSuccess)
+      return PFSF_ERROR_OUT_OF_VRAM;
+    E->ev_cap = c;
   }
-  if (is.host.stress && is.host.stress_bytes > 0) {
-    int64_t cnt = is.host.stress_bytes / 4;
-    if (cnt > is.n) cnt = is.n;
-    if (hipMemcpyAsync(is.host.stress, is.phi, 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(E->stream) != hipSuccess)
-      return PFSF_ERROR_VULKAN;
+  if (events) {
+    if (hipMemsetAsync(E->ev_dev, 0, sizeof(int32_t), s) != hipSuccess) return PFSF_ERROR_VULKAN;
+    for (Island* is : todo)
+      if (launch_compact(is->dims, is->code, is->desc.origin, E->ev_dev, (int32_t)cap,
+                         is->scratch, is->scratch_elems, s, true) != hipSuccess)
+        return PFSF_ERROR_VULKAN;
   }
-  is.dirty = false;
+  for (Island* is : todo)
+    if (is->host.stress && is->host.stress_bytes > 0) {
+      const int64_t cnt = std::min<int64_t>(is->host.stress_bytes / 4, is->n);
+      if (hipMemcpyAsync(is->host.stress, is->phi, 4 * cnt, hipMemcpyDeviceToHost, s) !=
+          hipSuccess)
+        return PFSF_ERROR_VULKAN;
+    }
+  if (events && hipMemcpyAsync(E->ev_host, E->ev_dev, sizeof(int32_t) * (1 + 4 * (size_t)cap),
+                               hipMemcpyDeviceToHost, s) != hipSuccess)
+    return PFSF_ERROR_VULKAN;
+  if (hipStreamSynchronize(s) != hipSuccess) return PFSF_ERROR_VULKAN;
+  if (events) {
+    const int32_t got = std::min<int32_t>(E->ev_host[0], (int32_t)cap);
+    out[0] = got;
+    if (got > 0) memcpy(out + 1, E->ev_host + 1, 16 * (size_t)got);
+  }
+  // a solve that stopped short of the tolerance leaves the island dirty: the
+  // next tick continues from its phi
+  for (Island* is : todo) is->dirty = !is->last.converged;
   return PFSF_OK;
 }
 
 // The reference walks the dirty islands one at a time, uploading and solving
 // each within the tick budget (NP/src/pfsf_engine.cpp:424-600).  Here the
 // islands that fit the budget are uploaded first; with PCG and multigrid on,
-// two or more of them are then solved in one batched solve, else one by one;
-// each then gets its history update, failure scan and reports.
-pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count,
-                      pfsf_tick_result* result, void* fail_addr, int64_t fail_bytes) {
+// two or more kappa-mode islands are then solved in one batched solve, the
+// rest one by one; each then gets its history update and failure scan, and
+// the reports of all islands come back together.
+pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count, int32_t* out,
+                      int64_t cap) {
   if (!E->available) return PFSF_ERROR_NOT_INIT;
   if (count > 0 && !ids) return PFSF_ERROR_INVALID_ARG;
   auto t0 = std::chrono::steady_clock::now();
-  if (result) result->count = 0;
+  if (out) out[0] = 0;
   std::vector<Island*> todo;
   for (int32_t k = 0; k < count; k++) {
     Island* is = find(E, ids[k]);
@@ -418,20 +534,20 @@ pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count,
   // uploads every array on every tick) keeps the island's hierarchy, a
   // function of those two arrays only
   if (E->pcg && !settle_topo(E, todo)) return PFSF_ERROR_VULKAN;
-  if (E->pcg && E->cfg.enable_multigrid && todo.size() >= 2) {
-    pfsf_result r = solve_batch(E, todo);
+  std::vector<Island*> batch, single;
+  for (Island* is : todo) (is->soa6 ? single : batch).push_back(is);
+  if (E->pcg && E->cfg.enable_multigrid && batch.size() >= 2) {
+    pfsf_result r = solve_batch(E, batch);
     if (r != PFSF_OK) return r;
   } else {
-    for (Island* is : todo) {
-      pfsf_result r = solve_island(E, *is);
-      if (r != PFSF_OK) return r;
-    }
+    single.insert(single.begin(), batch.begin(), batch.end());
   }
-  for (Island* is : todo) {
-    pfsf_result r = report(E, *is, result, fail_addr, fail_bytes);
+  for (Island* is : single) {
+    pfsf_result r = solve_island(E, *is);
     if (r != PFSF_OK) return r;
   }
-  if (hipStreamSynchronize(E->stream) != hipSuccess) return PFSF_ERROR_VULKAN;
+  pfsf_result r = report(E, todo, out, cap);
+  if (r != PFSF_OK) return r;
   std::lock_guard<std::mutex> lk(E->stats_mtx);
   E->last_tick_ms =
       std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -475,6 +591,7 @@ void pfsf_shutdown(pfsf_engine E) {
   (void)hipStreamSynchronize(E->stream);
   for (auto& kv : E->islands) free_island(*kv.second);
   E->islands.clear();
+  free_tick_buffers(E);
   (void)hipStreamDestroy(E->stream);
   E->stream = nullptr;
   E->available = false;
@@ -527,7 +644,16 @@ void pfsf_set_solve_tolerance(pfsf_engine E, double tol, int32_t max_iters) {
   if (!E) return;
   if (tol > 0) E->tol = tol;
   if (max_iters > 0) E->max_iters = max_iters;
-  for (auto& kv : E->islands) kv.second->solver.reset();  // rebuilt with the new settings
+  for (auto& kv : E->islands) {  // rebuilt with the new settings
+    Island& is = *kv.second;
+    if (is.s6) {
+      E->vram_used -= is.s6->device_bytes();
+      is.extra_bytes -= is.s6->device_bytes();
+    }
+    is.solver.reset();
+    is.s6.reset();
+    is.topo_dirty = true;
+  }
 }
 
 // ═════════════════════════════ islands ═══════════════════════════════════
@@ -546,7 +672,7 @@ pfsf_result pfsf_add_island(pfsf_engine E, const pfsf_island_desc* desc) {
       old->desc = *desc;
       return PFSF_OK;
     }
-    E->vram_used -= island_bytes(old->n);
+    E->vram_used -= island_bytes(old->n) + old->extra_bytes;
     free_island(*old);
     E->islands.erase(desc->island_id);
   }
@@ -569,7 +695,7 @@ void pfsf_remove_island(pfsf_engine E, int32_t id) {
   Island* is = find(E, id);
   if (!is) return;
   (void)hipStreamSynchronize(E->stream);
-  E->vram_used -= island_bytes(is->n);
+  E->vram_used -= island_bytes(is->n) + is->extra_bytes;
   free_island(*is);
   E->islands.erase(id);
 }
@@ -661,14 +787,26 @@ pfsf_result pfsf_register_stress_readback(pfsf_engine E, int32_t id, void* addr,
 pfsf_result pfsf_tick(pfsf_engine E, const int32_t* ids, int32_t count, int64_t,
                       pfsf_tick_result* result) {
   CHECK_ENGINE(E);
-  return tick_impl(E, ids, count, result, nullptr, 0);
+  if (result) result->count = 0;
+  if (result && result->events && result->capacity > 0) {
+    // pfsf_failure_event is {x, y, z, type}: the tuple layout after a count
+    std::vector<int32_t> buf(1 + 4 * (size_t)result->capacity);
+    pfsf_result r = tick_impl(E, ids, count, buf.data(), result->capacity);
+    if (r == PFSF_OK) {
+      result->count = buf[0];
+      memcpy(result->events, buf.data() + 1, 16 * (size_t)buf[0]);
+    }
+    return r;
+  }
+  return tick_impl(E, ids, count, nullptr, 0);
 }
 
 pfsf_result pfsf_tick_dbb(pfsf_engine E, const int32_t* ids, int32_t count, int64_t,
                           void* fail_addr, int64_t fail_bytes) {
   CHECK_ENGINE(E);
   if (fail_addr && fail_bytes >= 4) *(int32_t*)fail_addr = 0;
-  return tick_impl(E, ids, count, nullptr, fail_addr, fail_bytes);
+  const int64_t cap = (fail_addr && fail_bytes >= 20) ? (fail_bytes - 4) / 16 : 0;
+  return tick_impl(E, ids, count, cap > 0 ? (int32_t*)fail_addr : nullptr, cap);
 }
 
 pfsf_result pfsf_read_stress(pfsf_engine E, int32_t id, float* out, int32_t cap,
@@ -684,6 +822,19 @@ pfsf_result pfsf_read_stress(pfsf_engine E, int32_t id, float* out, int32_t cap,
                   hipStreamSynchronize(E->stream) != hipSuccess))
     return PFSF_ERROR_VULKAN;
   *out_count = (int32_t)cnt;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_get_island_status(pfsf_engine E, int32_t id, pfsf_island_status* out) {
+  CHECK_ENGINE(E);
+  if (!out) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  *out = pfsf_island_status{};
+  out->last_solve = is->last;
+  out->solver_builds = is->builds;
+  out->operator_mode = is->soa6 ? 1 : 0;
+  out->dirty = is->dirty ? 1 : 0;
   return PFSF_OK;
 }
 
@@ -770,8 +921,7 @@ pfsf_result pfsf_notify_sparse_updates(pfsf_engine E, int32_t id, int32_t count)
       launch_sparse_scatter(is->sparse_dev, count, is->n, is->source, is->cond, is->type,
                             is->max_phi, is->rcomp, is->rtens, s) != hipSuccess)
     return PFSF_ERROR_VULKAN;
-  if (!is->host.kappa && launch_kappa_from_soa6(is->cond, is->n, is->kappa, s) != hipSuccess)
-    return PFSF_ERROR_VULKAN;
+  if (!derive(E, *is)) return PFSF_ERROR_VULKAN;
   if (hipStreamSynchronize(s) != hipSuccess) return PFSF_ERROR_VULKAN;
   is->dirty = true;
   is->topo_dirty = true;
@@ -784,8 +934,10 @@ const char* pfsf_version(void) { return "0.1.0-hip"; }
 uint32_t pfsf_abi_version(void) { return (1u << 16) | (6u << 8) | 0u; }
 bool pfsf_has_feature(const char* name) {
   if (!name) return false;
-  static const char* kFeat[] = {"hip.gfx950", "solve.mg_pcg", "solve.mixed_bf16", "scan.compact",
-                                "labels.union_find"};
+  static const char* kFeat[] = {"hip.gfx950",        "solve.mg_pcg",      "solve.batched",
+                                "solve.mixed_bf16",  "solve.adjoint",     "scan.compact",
+                                "labels.union_find", "operator.soa6",     "smooth.chebyshev",
+                                "compute.normalize", "compute.downsample"};
   for (const char* f : kFeat)
     if (strcmp(f, name) == 0) return true;
   return false;
@@ -892,6 +1044,53 @@ pfsf_result pfsf_hip_build_island_data(pfsf_dims d, const int32_t* material_inde
   RET(launch_build_island_data(d, material_index, materials, n_materials, indestructible, anchor,
                                curing, fill, type, source, conductivity, rcomp, rtens, max_phi,
                                (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_chebyshev_jacobi(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                      const float* source, float* phi, float* tmp, int32_t steps,
+                                      float rho_spec, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !source || !phi || !tmp || phi == tmp || steps < 0)
+    return PFSF_ERROR_INVALID_ARG;
+  RET(launch_chebyshev_jacobi(d, type, kappa, source, phi, tmp, steps, rho_spec,
+                              (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_soa6_couplings(pfsf_dims d, const uint8_t* type, const float* cond,
+                                    double* cw, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cond || !cw) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_soa6_couplings(d, type, cond, cw, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_soa6_apply_f64(pfsf_dims d, const uint8_t* type, const double* cw,
+                                    const double* x, double* y, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cw || !x || !y || x == y) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_soa6_apply_f64(d, type, cw, x, y, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_soa6_failure_scan(pfsf_dims d, const uint8_t* type, const double* cw,
+                                       const float* phi, const float* rcomp, const float* rtens,
+                                       float phi_orphan, uint8_t* code, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cw || !phi || !rcomp || !rtens || !code) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_soa6_failure_scan(d, type, cw, phi, rcomp, rtens, phi_orphan, code,
+                               (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_soa6_history_update(pfsf_dims d, const uint8_t* type, const float* cond,
+                                         const float* phi, float* H, float* psi, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cond || !phi || (!H && !psi)) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_soa6_history(d, type, cond, phi, H, psi, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_soa6_rbgs(pfsf_dims d, const uint8_t* type, const double* cw,
+                               const float* source, float* phi, int32_t sweeps, void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cw || !source || !phi || sweeps < 0) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_soa6_rbgs(d, type, cw, source, phi, sweeps, (hipStream_t)stream));
 }
 
 pfsf_result pfsf_hip_kappa_from_soa6(const float* cond, int64_t n, float* kappa, void* stream) {

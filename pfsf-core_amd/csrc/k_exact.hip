@@ -17,6 +17,7 @@
 #include <mutex>
 #include <vector>
 
+#include "pfsf/pfsf_diagnostics.h"
 #include "pfsf_internal.h"
 
 namespace pfsf {
@@ -299,8 +300,10 @@ __global__ void __launch_bounds__(CB)
   if (threadIdx.x == 0) counts[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(CB) k_compact_scan(int nb, int* __restrict__ counts) {
-  // Exclusive scan of nb block counts in place; counts[nb] = total.
+__global__ void __launch_bounds__(CB)
+    k_compact_scan(int nb, int* __restrict__ counts, const int32_t* __restrict__ append_hdr) {
+  // Exclusive scan of nb block counts in place; counts[nb] = total,
+  // counts[nb + 1] = the tuples already in the output (append mode) or 0.
   __shared__ int sh[CB];
   int per = (nb + CB - 1) / CB;
   int b0 = threadIdx.x * per;
@@ -315,7 +318,10 @@ __global__ void __launch_bounds__(CB) k_compact_scan(int nb, int* __restrict__ c
       counts[b0 + e] = ex;
       ex += v;
     }
-  if (threadIdx.x == 0) counts[nb] = tot;
+  if (threadIdx.x == 0) {
+    counts[nb] = tot;
+    counts[nb + 1] = append_hdr ? append_hdr[0] : 0;
+  }
 }
 
 __global__ void __launch_bounds__(CB)
@@ -328,7 +334,8 @@ __global__ void __launch_bounds__(CB)
   for (int e = 0; e < CPT; e++)
     if (base + e < n && code[base + e] != 0) c++;
   int tot;
-  int rank = offs[blockIdx.x] + block_excl_scan(c, sh, &tot);
+  const int prior = offs[nb + 1];
+  int rank = prior + offs[blockIdx.x] + block_excl_scan(c, sh, &tot);
   for (int e = 0; e < CPT; e++) {
     int64_t i = base + e;
     if (i < n && code[i] != 0) {
@@ -344,7 +351,7 @@ __global__ void __launch_bounds__(CB)
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int total = offs[nb];
+    int total = prior + offs[nb];
     out[0] = total < capacity ? total : capacity;
   }
 }
@@ -474,10 +481,11 @@ __global__ void __launch_bounds__(BS)
     k_kappa_soa6(const float* __restrict__ cond, int64_t n, float* __restrict__ kappa) {
   int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  float m = cond[i];
-  for (int dd = 1; dd < 6; dd++) {
-    float v = cond[dd * n + i];
-    m = v > m ? v : m;
+  // non-finite faces count as 0, so the maximum is always finite and >= 0
+  float m = 0.0f;
+  for (int dd = 0; dd < 6; dd++) {
+    const float v = cond[dd * n + i];
+    m = (v > m && v <= 3.40282347e38f) ? v : m;
   }
   kappa[i] = m;
 }
@@ -556,6 +564,28 @@ hipError_t launch_jacobi_exact(pfsf_dims d, const uint8_t* type, const float* ka
                                hipStream_t s) {
   k_jacobi_exact<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, src, prev, out, omega);
   return hipGetLastError();
+}
+
+// Chebyshev-accelerated Jacobi (SH/jacobi_smooth.comp.glsl:215-226 with the
+// schedule of NP/src/compute/chebyshev.cpp:30-57): step k is the jacobiStep
+// with omega_k = pfsf_chebyshev_omega(k, rho), alternating between phi and
+// tmp; the result ends in phi.
+hipError_t launch_chebyshev_jacobi(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                   const float* src, float* phi, float* tmp, int steps,
+                                   float rho, hipStream_t s) {
+  float* cur = phi;
+  float* nxt = tmp;
+  for (int k = 0; k < steps; k++) {
+    const float w = pfsf_chebyshev_omega(k, rho);
+    k_jacobi_exact<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, src, cur, nxt, w);
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && cur != phi)
+    e = hipMemcpyAsync(phi, cur, sizeof(float) * (size_t)nvox(d), hipMemcpyDeviceToDevice, s);
+  return e;
 }
 
 // One sweep: the eight colour passes in order.
@@ -681,15 +711,15 @@ hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa
   return hipGetLastError();
 }
 
-int64_t compact_scratch_elems(pfsf_dims d) { return (int64_t)blocks_for(nvox(d), CSEG) + 1; }
+int64_t compact_scratch_elems(pfsf_dims d) { return (int64_t)blocks_for(nvox(d), CSEG) + 2; }
 
 hipError_t launch_compact(pfsf_dims d, const uint8_t* code, pfsf_pos origin, int32_t* out,
                           int32_t capacity, int32_t* scratch, int64_t scratch_elems,
-                          hipStream_t s) {
+                          hipStream_t s, bool append) {
   int nb = (int)blocks_for(nvox(d), CSEG);
-  if (scratch_elems < nb + 1) return hipErrorInvalidValue;
+  if (scratch_elems < nb + 2) return hipErrorInvalidValue;
   k_compact_count<<<nb, CB, 0, s>>>(nvox(d), code, scratch);
-  k_compact_scan<<<1, CB, 0, s>>>(nb, scratch);
+  k_compact_scan<<<1, CB, 0, s>>>(nb, scratch, append ? out : nullptr);
   k_compact_write<<<nb, CB, 0, s>>>(d, code, scratch, nb, origin, out, capacity);
   return hipGetLastError();
 }

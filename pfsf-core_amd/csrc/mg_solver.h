@@ -132,6 +132,14 @@ class MgSolver {
   const float* kappa() const { return kappa_; }
   const uint8_t* free_mask() const { return levels_.empty() ? nullptr : levels_[0].fr; }
   hipStream_t stream() const { return stream_; }
+  // Preconditioner-only use by an outer Krylov solve with another operator:
+  // precond_begin() clears the stop flag, then each precond_apply() writes
+  // z = M r into precond_out() from the residual in precond_in() (FP32, zero
+  // outside free_mask()), on stream().
+  float* precond_in() const { return r_[0]; }
+  const float* precond_out() const { return z_; }
+  hipError_t precond_begin();
+  hipError_t precond_apply() { return precondition(0); }
 
  private:
   void configure(const pfsf_solver_desc& desc, int min_levels, int tz_fine, int tz_coarse);

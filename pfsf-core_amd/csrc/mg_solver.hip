@@ -836,6 +836,19 @@ hipError_t MgSolver::build() {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   drop_graphs();
+  // type and kappa may still be in flight on the caller's stream (an upload
+  // queued just before): the build waits for it
+  hipStream_t caller = (hipStream_t)desc_.stream;
+  if (caller != s) {
+    hipEvent_t ev = nullptr;
+    chk(hipEventCreateWithFlags(&ev, hipEventDisableTiming), err);
+    if (ev) {
+      chk(hipEventRecord(ev, caller), err);
+      chk(hipStreamWaitEvent(s, ev, 0), err);
+      (void)hipEventDestroy(ev);
+    }
+    if (err != hipSuccess) return err;
+  }
   // q_ is free until the first solve: it carries the component parents and
   // is cleared again afterwards (every work vector starts at zero)
   chk(anchored_mask(d0_, type_, reach_, (uint32_t*)q_, flag_, s), err);
@@ -1065,6 +1078,11 @@ hipError_t MgSolver::precondition(int c) {
   (void)vcycle_below(1, false);
   PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<G, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[c], B.e, z_, sc_, prz_)));
+  return hipGetLastError();
+}
+
+hipError_t MgSolver::precond_begin() {
+  k_profile_begin<<<1, 1, 0, stream_>>>(sc_);
   return hipGetLastError();
 }
 

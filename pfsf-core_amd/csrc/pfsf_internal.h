@@ -30,6 +30,9 @@ inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) /
 hipError_t launch_jacobi_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
                                const float* src, const float* prev, float* out, float omega,
                                hipStream_t s);
+hipError_t launch_chebyshev_jacobi(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                   const float* src, float* phi, float* tmp, int steps,
+                                   float rho, hipStream_t s);
 hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
                              const float* src, float* phi, int sweeps, hipStream_t s);
 hipError_t launch_failure_scan(pfsf_dims d, const uint8_t* type, const float* kappa,
@@ -39,9 +42,13 @@ hipError_t launch_history(pfsf_dims d, const uint8_t* type, const float* kappa,
                           const float* phi, float* H, float* psi, hipStream_t s);
 hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
                             const double* x, double* y, hipStream_t s);
+// Failure tuples of `code` into out = {count, capacity x {x,y,z,code}} in
+// ascending flat order.  append: the tuples follow the out[0] already there
+// (several islands into one buffer, in launch order); out[0] becomes the
+// clamped running count.
 hipError_t launch_compact(pfsf_dims d, const uint8_t* code, pfsf_pos origin, int32_t* out,
                           int32_t capacity, int32_t* scratch, int64_t scratch_elems,
-                          hipStream_t s);
+                          hipStream_t s, bool append = false);
 int64_t compact_scratch_elems(pfsf_dims d);
 hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* type,
                              float* coarse, uint8_t* ctype, hipStream_t s);

@@ -95,6 +95,11 @@ MATERIAL_DTYPE = np.dtype([("density", "<f4"), ("rcomp", "<f4"), ("rtens", "<f4"
                            ("youngs_gpa", "<f4"), ("poisson", "<f4"), ("gc", "<f4"),
                            ("is_anchor", "u1")], align=True)
 
+class IslandStatus(C.Structure):
+    _fields_ = [("last_solve", SolveInfo), ("solver_builds", C.c_int32),
+                ("operator_mode", C.c_int32), ("dirty", C.c_int32), ("reserved", C.c_int32)]
+
+
 class BatchIsland(C.Structure):
     _fields_ = [("dims", Dims), ("type", C.c_void_p), ("kappa", C.c_void_p)]
 
@@ -188,6 +193,19 @@ SIGNATURES = {
     "pfsf_register_island_kappa": (R, [P, I32, P, I64]),
     "pfsf_set_solve_tolerance": (None, [P, F64, I32]),
     "pfsf_read_history": (R, [P, I32, P, I32, C.POINTER(I32)]),
+    "pfsf_get_island_status": (R, [P, I32, C.POINTER(IslandStatus)]),
+    "pfsf_hip_chebyshev_jacobi": (R, [Dims, P, P, P, P, P, I32, F32, P]),
+    "pfsf_hip_soa6_couplings": (R, [Dims, P, P, P, P]),
+    "pfsf_hip_soa6_apply_f64": (R, [Dims, P, P, P, P, P]),
+    "pfsf_hip_soa6_failure_scan": (R, [Dims, P, P, P, P, P, F32, P, P]),
+    "pfsf_hip_soa6_history_update": (R, [Dims, P, P, P, P, P, P]),
+    "pfsf_hip_soa6_rbgs": (R, [Dims, P, P, P, P, I32, P]),
+    # pfsf_compute.h / pfsf_diagnostics.h (host primitives)
+    "pfsf_normalize_soa6": (None, [P, P, P, P, P, I32, C.POINTER(F32)]),
+    "pfsf_downsample_2to1": (None, [P, P, I32, I32, I32, P, P]),
+    "pfsf_chebyshev_omega": (F32, [I32, F32]),
+    "pfsf_precompute_omega_table": (I32, [F32, P, I32]),
+    "pfsf_estimate_spectral_radius": (F32, [I32, F32]),
 }
 
 _lib = None

@@ -233,3 +233,38 @@ class BatchSolver:
             self.close()
         except Exception:
             pass
+
+
+# ── Chebyshev-Jacobi and the SoA-6 operator ─────────────────────────────
+
+def chebyshev_jacobi(vtype, kappa, source, phi, tmp, steps, rho):
+    capi.check("pfsf_hip_chebyshev_jacobi", _lib().pfsf_hip_chebyshev_jacobi(
+        _dims(vtype), _p(vtype), _p(kappa), _p(source), _p(phi), _p(tmp), steps, rho, _stream()))
+
+
+def soa6_couplings(vtype, cond, cw):
+    """cond: (6, lz, ly, lx) float32; cw: (13, lz, ly, lx) float64."""
+    capi.check("pfsf_hip_soa6_couplings", _lib().pfsf_hip_soa6_couplings(
+        _dims(vtype), _p(vtype), _p(cond), _p(cw), _stream()))
+
+
+def soa6_apply_f64(vtype, cw, x, y):
+    capi.check("pfsf_hip_soa6_apply_f64", _lib().pfsf_hip_soa6_apply_f64(
+        _dims(vtype), _p(vtype), _p(cw), _p(x), _p(y), _stream()))
+
+
+def soa6_failure_scan(vtype, cw, phi, rcomp, rtens, code, phi_orphan=1e6):
+    capi.check("pfsf_hip_soa6_failure_scan", _lib().pfsf_hip_soa6_failure_scan(
+        _dims(vtype), _p(vtype), _p(cw), _p(phi), _p(rcomp), _p(rtens), phi_orphan, _p(code),
+        _stream()))
+
+
+def soa6_history_update(vtype, cond, phi, H=None, psi=None):
+    capi.check("pfsf_hip_soa6_history_update", _lib().pfsf_hip_soa6_history_update(
+        _dims(vtype), _p(vtype), _p(cond), _p(phi), _p(H) if H is not None else None,
+        _p(psi) if psi is not None else None, _stream()))
+
+
+def soa6_rbgs(vtype, cw, source, phi, sweeps):
+    capi.check("pfsf_hip_soa6_rbgs", _lib().pfsf_hip_soa6_rbgs(
+        _dims(vtype), _p(vtype), _p(cw), _p(source), _p(phi), sweeps, _stream()))

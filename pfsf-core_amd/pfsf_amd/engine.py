@@ -49,7 +49,9 @@ class Engine:
 
     def register(self, island_id, isl, phi=None, cond=None, max_phi=None, kappa=None,
                  stress_is_phi=True) -> int:
-        """Register host buffers of a workloads.Island (SoA-6 cond defaults to kappa x6)."""
+        """Register host buffers of a workloads.Island.  Without `kappa` the
+        island is solved with the SoA-6 operator of `cond` (default: kappa on
+        all six faces); with `kappa` the per-voxel kappa operator is used."""
         n = isl.n
         phi = np.zeros(n, np.float32) if phi is None else np.ascontiguousarray(phi, np.float32)
         if cond is None:
@@ -125,3 +127,12 @@ class Engine:
         s = capi.Stats()
         r = self.lib.pfsf_get_stats(self.h, C.byref(s))
         return r, s
+
+    def status(self, island_id):
+        """pfsf_get_island_status: (result, IslandStatus)."""
+        st = capi.IslandStatus()
+        r = self.lib.pfsf_get_island_status(self.h, island_id, C.byref(st))
+        return r, st
+
+    def set_tolerance(self, tol, max_iters=0):
+        self.lib.pfsf_set_solve_tolerance(self.h, tol, max_iters)

@@ -91,3 +91,39 @@ def reference_fixture(name):
     src = np.where(vtype == SOLID, rho * GRAVITY * 1e-6, 0.0).astype(np.float32)
     isl = make(vtype, kappa, src, rc, rt, name=name)
     return isl, int(d.get("ticks", 0))
+
+
+def fixture_material_grid(name):
+    """A fixture as the Java data builder sees it: the material index grid
+    (lz, ly, lx) int32 into the fixture's registry (-1 = air), the registry
+    rows (density, rcomp, rtens, indestructible) and the anchor flags (uint8)
+    from the fixture's anchor list."""
+    d = json.loads((FIXTURE_DIR / f"{name}.json").read_text())
+    lx, ly, lz = d["dims"]["lx"], d["dims"]["ly"], d["dims"]["lz"]
+    ids = np.frombuffer(base64.b64decode(d["materials"]["voxels"]), "<i4").reshape(lz, ly, lx)
+    reg = d["materials"]["registry"]
+    slot = {m["id"]: k for k, m in enumerate(reg)}
+    mi = np.vectorize(lambda v: slot.get(int(v), -1))(ids).astype(np.int32)
+    rows = [(m["density"], m["rcomp"], m["rtens"], 0) for m in reg]
+    anchor = np.zeros(mi.shape, np.uint8)
+    for x, y, z in d["anchors"]:
+        anchor[z, y, x] = 1
+    return mi, rows, anchor
+
+
+def soa6_island(name, normalize=True):
+    """A fixture through the Java data-prep rule (the oracle's restatement of
+    PFSFDataBuilder / PFSFConductivity, no curing) and, as the Java caller
+    does before registering, the sigma-max normalisation: an Island (kappa =
+    largest face, for reference only) and its SoA-6 faces (6, lz, ly, lx)."""
+    from oracle import oracle as O
+
+    mi, rows, anchor = fixture_material_grid(name)
+    a = O.build_island_data(mi, rows, anchor)
+    src, rc, rt, cond = a["source"], a["rcomp"], a["rtens"], a["cond"]
+    if normalize:
+        src, rc, rt, cond, _ = O.normalize_soa6(src, rc, rt, cond)
+    src = np.where(a["type"] == SOLID, src, np.float32(0)).astype(np.float32)
+    kap = cond.max(axis=0).astype(np.float32)
+    isl = make(a["type"], kap, src, rc, rt, name=name)
+    return isl, np.ascontiguousarray(cond, np.float32)

@@ -66,19 +66,25 @@ def test_tick_legacy_result_and_capacity(cuda):
         assert np.array_equal(ev, _expected_events(isl, phi, (0, 0, 0))[:5])
 
 
-def test_soa6_conductivity_maps_to_per_voxel_max(cuda):
+def test_faces_only_registration_solves_the_soa6_operator(cuda):
+    """Without a registered kappa the island's faces define the operator
+    (pfsf_hip_soa6_*): isotropic faces with some halved still solve the
+    SoA-6 system, not the per-voxel maximum."""
     isl = W.small_box_union(1, L=10)
     rng = np.random.default_rng(0)
     cond = np.tile(isl.kappa.ravel(), 6).astype(np.float32)
-    cond[rng.integers(0, cond.size, 50)] *= 0.5  # per-voxel max is still kappa
+    cond[rng.integers(0, cond.size, 50)] *= 0.5
     with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
         assert e.init() == capi.OK
         assert e.add_island(3, isl.lx, isl.ly, isl.lz) == capi.OK
         assert e.register(3, isl, cond=cond) == capi.OK
         assert e.tick_dbb([3])[0] == capi.OK
-        ref = O.direct_solve(isl)
+        c6 = cond.reshape((6,) + isl.vtype.shape)
+        ref = O.direct_solve_soa6(isl, c6)
         phi = e.phi(3).reshape(isl.vtype.shape)
         assert np.linalg.norm(phi - ref) / np.linalg.norm(ref) <= 1e-5
+        r, st = e.status(3)
+        assert r == capi.OK and st.operator_mode == 1 and st.last_solve.converged == 1
 
 
 def test_orphan_cluster_reports_no_support_only_for_isolated(cuda):
@@ -262,3 +268,149 @@ def test_reference_fixtures_in_one_tick(cuda):
         want = np.concatenate(want)
         assert len(want) > 0
         assert np.array_equal(ev, want)
+
+
+def _status(e, i):
+    r, st = e.status(i)
+    assert r == capi.OK
+    return st
+
+
+@pytest.mark.parametrize("n_islands", [1, 2])
+def test_reupload_rebuild_counts(cuda, n_islands):
+    """Re-uploads with unchanged type and kappa keep the hierarchy (no build);
+    a changed kappa or type rebuilds it once, for the single and the batched
+    tick alike."""
+    isls = [W.small_box_union(3, L=14), W.small_box_union(8, L=12)][:n_islands]
+    ids = list(range(1, n_islands + 1))
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        for k, isl in enumerate(isls):
+            assert e.add_island(k + 1, isl.lx, isl.ly, isl.lz) == capi.OK
+            assert e.register(k + 1, isl, kappa=isl.kappa) == capi.OK
+        assert e.tick_dbb(ids)[0] == capi.OK
+        assert [_status(e, i).solver_builds for i in ids] == [1] * n_islands
+        keep = e._keep[1]
+        solid = np.flatnonzero(keep["vt"] == 1)
+        want = [1] * n_islands
+        for step in ("same", "kappa", "same", "type", "same"):
+            if step == "kappa":
+                keep["kappa"][solid[:7]] *= np.float32(2.0)
+            elif step == "type":
+                keep["vt"][solid[-3:]] = 0
+            for i in ids:
+                e.lib.pfsf_mark_full_rebuild(e.h, i)
+            assert e.tick_dbb(ids)[0] == capi.OK
+            if step != "same":
+                want[0] += 1
+            assert [_status(e, i).solver_builds for i in ids] == want, step
+            assert all(_status(e, i).dirty == 0 for i in ids)
+
+
+def test_new_tolerance_then_changed_kappa_matches_a_fresh_engine(cuda):
+    """pfsf_set_solve_tolerance drops every solver; the next tick with a
+    changed kappa builds new ones from the uploaded arrays (the build waits
+    for the upload), so it equals a new engine's tick bit for bit."""
+    isls = [W.small_box_union(4, L=16), W.small_box_union(6, L=13)]
+    ids = [1, 2]
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        for k, isl in enumerate(isls):
+            assert e.add_island(k + 1, isl.lx, isl.ly, isl.lz, origin=(40 * k, 0, 0)) == capi.OK
+            assert e.register(k + 1, isl, kappa=isl.kappa) == capi.OK
+        assert e.tick_dbb(ids)[0] == capi.OK
+        e.set_tolerance(1e-7, 800)
+        phis = [e.phi(i).copy() for i in ids]
+        rng = np.random.default_rng(1)
+        for i in ids:
+            k = e._keep[i]["kappa"]
+            k[rng.choice(np.flatnonzero(k > 0), 30, replace=False)] *= np.float32(4.0)
+            e.lib.pfsf_mark_full_rebuild(e.h, i)
+        r, ev = e.tick_dbb(ids, capacity=1 << 14)
+        assert r == capi.OK
+        cur = [dataclasses.replace(isl, kappa=e._keep[i]["kappa"].reshape(isl.vtype.shape).copy())
+               for i, isl in zip(ids, isls)]
+        with Engine(max_island_size=10**7, tick_budget_ms=10_000) as f:
+            assert f.init() == capi.OK
+            f.set_tolerance(1e-7, 800)
+            for k, (isl, phi0) in enumerate(zip(cur, phis)):
+                assert f.add_island(k + 1, isl.lx, isl.ly, isl.lz, origin=(40 * k, 0, 0)) == 0
+                assert f.register(k + 1, isl, phi=phi0.copy(), kappa=isl.kappa) == capi.OK
+            r2, ev2 = f.tick_dbb(ids, capacity=1 << 14)
+            assert r2 == capi.OK
+            for i in ids:
+                assert np.array_equal(e.phi(i), f.phi(i)), i
+            assert np.array_equal(ev, ev2)
+        for i in ids:
+            assert _status(e, i).last_solve.rel_residual <= 1e-7
+
+
+@pytest.mark.parametrize("soa6", [False, True])
+def test_non_finite_inputs_count_as_zero(cuda, soa6):
+    """NaN / Inf in kappa (or a face), in the source and in the warm start:
+    the tick ends, and equals the tick of the same island with those entries
+    set to 0, bit for bit (phi, history and failure records)."""
+    isl = W.small_box_union(7, L=12)
+    isl.rcomp[...] = np.where(isl.vtype != 0, np.float32(0.3), np.float32(0))
+    isl.rtens[...] = np.where(isl.vtype != 0, np.float32(0.1), np.float32(0))
+    rng = np.random.default_rng(2)
+    solid = np.flatnonzero(isl.vtype.ravel() == 1)
+    bad = rng.choice(solid, 12, replace=False)
+    cond = np.tile(isl.kappa.ravel(), 6).astype(np.float32)
+    cond *= rng.uniform(0.5, 1.5, cond.size).astype(np.float32)
+    phi0 = np.zeros(isl.n, np.float32)
+
+    def run(kap, src, c, p0):
+        with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+            assert e.init() == capi.OK
+            assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
+            i2 = dataclasses.replace(isl, source=src.reshape(isl.vtype.shape))
+            assert e.register(1, i2, phi=p0.copy(), cond=c,
+                              kappa=None if soa6 else kap) == capi.OK
+            r, ev = e.tick_dbb([1], capacity=1 << 14)
+            assert r == capi.OK
+            r, H = e.read_history(1, isl.n)
+            return e.phi(1).copy(), ev, H, _status(e, 1)
+
+    kap, src, c, p0 = (isl.kappa.ravel().copy(), isl.source.ravel().copy(), cond.copy(),
+                       phi0.copy())
+    kap[bad[:4]] = np.nan
+    kap[bad[4:6]] = np.inf
+    c[bad[:3]] = np.nan
+    c[5 * isl.n + bad[3:6]] = np.inf
+    src[bad[6:9]] = np.nan
+    src[bad[9:]] = -np.inf
+    p0[bad] = np.nan
+    got = run(kap, src, c, p0)
+    kz, sz, cz = kap.copy(), src.copy(), c.copy()
+    kz[~np.isfinite(kz)] = 0
+    sz[~np.isfinite(sz)] = 0
+    cz[~np.isfinite(cz)] = 0
+    want = run(kz, sz, cz, np.zeros(isl.n, np.float32))
+    assert np.all(np.isfinite(got[0]))
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    assert np.array_equal(got[2], want[2])
+    assert got[3].last_solve.converged == 1
+
+
+@pytest.mark.parametrize("soa6", [False, True])
+def test_unconverged_solve_leaves_the_island_dirty(cuda, soa6):
+    """A solve capped before the tolerance reports converged = 0 and keeps the
+    island dirty; following ticks continue from its phi until it converges."""
+    isl = W.small_box_union(2, L=20)
+    with Engine(max_island_size=10**7, tick_budget_ms=10_000) as e:
+        assert e.init() == capi.OK
+        e.set_tolerance(1e-6, 3)
+        assert e.add_island(1, isl.lx, isl.ly, isl.lz) == capi.OK
+        assert e.register(1, isl, kappa=None if soa6 else isl.kappa) == capi.OK
+        rels = []
+        for _ in range(40):
+            assert e.tick_dbb([1])[0] == capi.OK
+            st = _status(e, 1)
+            rels.append(st.last_solve.rel_residual)
+            if not st.dirty:
+                break
+        assert len(rels) > 2 and rels[0] > 1e-6
+        assert st.dirty == 0 and st.last_solve.converged == 1 and rels[-1] <= 1e-6
+        # a clean island is not solved again
+        assert e.tick_dbb([1])[0] == capi.OK and _status(e, 1).last_solve.rel_residual == rels[-1]
