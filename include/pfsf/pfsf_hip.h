@@ -76,6 +76,18 @@ PFSF_API pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const 
 PFSF_API pfsf_result pfsf_hip_compact_failures(pfsf_dims d, const uint8_t* code, pfsf_pos origin,
                                                int32_t* out, int32_t capacity, void* stream);
 
+/* No counterpart in the reference: pfsf_hip_compact_failures for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same output, bit for
+ * bit; it calls no allocator.  Its count/scan scratch is a library-owned
+ * static device array of 1 MiB on each GPU (held until the process ends), so
+ * grids of more than 268,433,408 voxels return PFSF_ERROR_INVALID_ARG.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams) race, and the caller must order them. */
+PFSF_API pfsf_result pfsf_hip_compact_failures_noalloc(pfsf_dims d, const uint8_t* code,
+                                                       pfsf_pos origin, int32_t* out,
+                                                       int32_t capacity, void* stream);
+
 /* 2:1 downsample of a per-voxel field and its types (NP/src/compute/
  * downsample.cpp:39-98): mean over in-bounds children; type anchor if
  * anchors >= solids and >= airs, else solid if solids >= airs, else air. */

@@ -724,6 +724,33 @@ hipError_t launch_compact(pfsf_dims d, const uint8_t* code, pfsf_pos origin, int
   return hipGetLastError();
 }
 
+// ── Compaction without allocation ───────────────────────────────────────
+// The count/scan scratch of pfsf_hip_compact_failures_noalloc: a library-owned
+// static device array (one copy per GPU, held until the process ends).  Host
+// code never passes `scratch` itself (a host-side stand-in): it passes the
+// device address hipGetSymbolAddress returns, looked up once per device on the
+// first call with that device current.
+constexpr int64_t kNoAllocScratchInts = int64_t(1) << 18;  // 1 MiB
+static __device__ __align__(256) int32_t scratch[kNoAllocScratchInts];
+
+static hipError_t noalloc_scratch(int32_t** out) {
+  static std::mutex mu;
+  static void* per_dev[64] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!per_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    per_dev[dev] = p;
+  }
+  *out = (int32_t*)per_dev[dev];
+  return hipSuccess;
+}
+
 hipError_t launch_downsample(pfsf_dims fine, const float* field, const uint8_t* type,
                              float* coarse, uint8_t* ctype, hipStream_t s) {
   pfsf_dims c{(fine.lx + 1) / 2, (fine.ly + 1) / 2, (fine.lz + 1) / 2};
@@ -764,3 +791,19 @@ hipError_t launch_scale3(float* a, float* b, float* c, float* cond, int64_t n,
 }
 
 }  // namespace pfsf
+
+extern "C" pfsf_result pfsf_hip_compact_failures_noalloc(pfsf_dims d, const uint8_t* code,
+                                                        pfsf_pos origin, int32_t* out,
+                                                        int32_t capacity, void* stream) {
+  using namespace pfsf;
+  if (d.lx <= 0 || d.ly <= 0 || d.lz <= 0 || !code || !out || capacity < 0)
+    return PFSF_ERROR_INVALID_ARG;
+  const int64_t ne = compact_scratch_elems(d);
+  if (ne > kNoAllocScratchInts) return PFSF_ERROR_INVALID_ARG;  // grid beyond the scratch size
+  int32_t* ws = nullptr;
+  hipError_t e = noalloc_scratch(&ws);
+  if (e == hipSuccess) e = launch_compact(d, code, origin, out, capacity, ws, ne, (hipStream_t)stream);
+  if (e == hipSuccess) return PFSF_OK;
+  (void)hipGetLastError();
+  return PFSF_ERROR_VULKAN;
+}

@@ -171,6 +171,7 @@ SIGNATURES = {
     "pfsf_hip_history_update": (R, [Dims, P, P, P, P, P, P]),
     "pfsf_hip_apply_f64": (R, [Dims, P, P, P, P, P]),
     "pfsf_hip_compact_failures": (R, [Dims, P, Pos, P, I32, P]),
+    "pfsf_hip_compact_failures_noalloc": (R, [Dims, P, Pos, P, I32, P]),
     "pfsf_hip_downsample_2to1": (R, [Dims, P, P, P, P, P]),
     "pfsf_hip_normalize_soa6": (R, [P, P, P, P, I64, C.POINTER(F32), P]),
     "pfsf_hip_kappa_from_soa6": (R, [P, I64, P, P]),
