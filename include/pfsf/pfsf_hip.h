@@ -64,6 +64,30 @@ PFSF_API pfsf_result pfsf_hip_history_update(pfsf_dims d, const uint8_t* type,
                                              const float* kappa, const float* phi, float* H,
                                              float* psi, void* stream);
 
+/* Phase-field damage step (SH/phase_field_evolve.comp.glsl:50-168, the
+ * hybrid AT2 form of Ambati et al. 2015).  For each solid voxel whose code is
+ * 0 (a voxel with a failure code from the previous scan is frozen), over its
+ * in-grid non-air face neighbours (order -x, +x, -y, +y, -z, +z):
+ * lap_phi += phi_j - phi_i and sumSigma += cond[d * N + i] for finite phi_j,
+ * lap_d += d_j - d_i for finite d_j; with H > 0,
+ *   h = hydration clamped to [0.01, 1] (1 when hydration is NULL),
+ *   Gc = gc * h * sqrt(h),
+ *   H_eff = spectral_split ? max(H, max(lap_phi, 0) / max(sumSigma, 1e-12)) : H,
+ *   d' = clamp((2 H_eff + Gc l0 lap_d) / max(2 H_eff + Gc / l0, 1e-8), d, 1),
+ *   d' = max(clamp(d (1 - relax) + d' relax, 0, 1), d) (d if not finite: the
+ *   mix can round below d by an ulp, and damage never decreases);
+ * FP32 in this order.  Reads damage_in, writes damage_out (must differ: every
+ * voxel reads its neighbours' old damage), and sets code = 1 (CANTILEVER)
+ * where d' > 0.95.  The engine uses l0 = 2, gc = 100, relax = 0.3 and the
+ * spectral split (NP/src/solver/dispatcher.cpp:240-248). */
+PFSF_API pfsf_result pfsf_hip_phase_field_evolve(pfsf_dims d, const uint8_t* type,
+                                                 const float* cond, const float* phi,
+                                                 const float* H, const float* hydration,
+                                                 uint8_t* code, const float* damage_in,
+                                                 float* damage_out, float l0, float gc,
+                                                 float relax, int32_t spectral_split,
+                                                 void* stream);
+
 /* y = A x in FP64 (exact coefficients), 0 on non-solid rows. */
 PFSF_API pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
                                         const double* x, double* y, void* stream);
@@ -400,6 +424,11 @@ typedef struct {
 
 PFSF_API pfsf_result pfsf_get_island_status(pfsf_engine engine, int32_t island_id,
                                             pfsf_island_status* out);
+
+/* Device damage field of an island (N floats, the phase-field d after the
+ * last tick; 0 before any), for inspection. */
+PFSF_API pfsf_result pfsf_read_damage(pfsf_engine engine, int32_t island_id, float* out,
+                                      int32_t capacity, int32_t* out_count);
 
 /* Device history field of an island (N floats), for inspection. */
 PFSF_API pfsf_result pfsf_read_history(pfsf_engine engine, int32_t island_id, float* out,

@@ -91,6 +91,8 @@ def lib():
         L.or_soa6_psi.argtypes = [I, I, I, P, P, P, P]
         L.or_soa6_history_update.argtypes = [I, I, I, P, P, P, P]
         L.or_soa6_rbgs.argtypes = [I, I, I, P, P, P, P, I]
+        L.or_phase_field.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, C.c_float, C.c_float,
+                                     C.c_float, I]
         L.or_set_threads.argtypes = [I]
         L.or_set_threads.restype = I
         L.or_set_threads(1)  # single-threaded unless a caller asks for more
@@ -373,6 +375,27 @@ def direct_solve_soa6(isl, cond, b=None):
     if fidx.size:
         phi[fidx] = spla.spsolve(A.tocsc(), np.asarray(b, np.float64).ravel()[fidx])
     return phi.reshape(isl.vtype.shape)
+
+
+PF_L0, PF_GC, PF_RELAX = 2.0, 100.0, 0.3  # dispatcher.cpp:240-248 values
+
+
+def phase_field(isl, cond, phi, H, d, code, hydration=None, l0=PF_L0, gc=PF_GC,
+                relax=PF_RELAX, split=True):
+    """One phase-field damage step (SH/phase_field_evolve.comp.glsl): returns
+    (d_new, fractured u8).  cond (6, lz, ly, lx); code = the previous failure
+    codes (voxels with a code are skipped)."""
+    sh = isl.vtype.shape
+    d_out = np.empty(sh, np.float32)
+    frac = np.empty(sh, np.uint8)
+    hy = None if hydration is None else np.ascontiguousarray(hydration, np.float32)
+    lib().or_phase_field(*_dims(isl), _p(isl.vtype), _p(_cond(isl, cond)),
+                         _p(np.ascontiguousarray(phi, np.float32)),
+                         _p(np.ascontiguousarray(H, np.float32)), None if hy is None else _p(hy),
+                         _p(np.ascontiguousarray(code, np.uint8)),
+                         _p(np.ascontiguousarray(d, np.float32)), _p(d_out), _p(frac), l0, gc,
+                         relax, 1 if split else 0)
+    return d_out, frac
 
 
 def sparse_scatter(records, arrays):

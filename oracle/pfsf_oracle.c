@@ -1079,3 +1079,76 @@ void or_soa6_rbgs(int lx, int ly, int lz, const uint8_t* type, const double* cw,
           }
     }
 }
+
+/* ── Phase-field damage evolution ──────────────────────────────────────
+ * SH/phase_field_evolve.comp.glsl:50-168 (Ambati 2015 hybrid AT2 form), with
+ * the engine's parameters (NP/src/solver/dispatcher.cpp:240-248: l0 =
+ * max(PHASE_FIELD_L0, 2), G_c = G_C_CONCRETE, relax = PHASE_FIELD_RELAX,
+ * spectral split on).  Per solid voxel whose previous failure code is 0, over
+ * the in-grid non-air face neighbours (order -x, +x, -y, +y, -z, +z):
+ *   lap_phi += phi_j - phi_i and sumSigma += cond[d][i] for finite phi_j,
+ *   lap_d   += d_j - d_i for finite d_j;
+ * H <= 0 (or NaN) leaves d; else
+ *   h = hydration clamped to [0.01, 1] (1 without a field), Gc = gc h sqrt(h)
+ *   H_eff = split ? max(H, max(lap_phi, 0) / max(sumSigma, 1e-12)) : H
+ *   d' = clamp((2 H_eff + Gc l0 lap_d) / max(2 H_eff + Gc / l0, 1e-8), d_i, 1)
+ *   d' = max(clamp(d_i (1 - relax) + d' relax, 0, 1), d_i), d_i if not finite
+ *   (the shader's mix can round below d_i by an ulp; damage is irreversible),
+ * FP32 in this order.  The shader updates d in place while neighbours read
+ * it; here every voxel reads d_in and writes d_out (a Jacobi step), so the
+ * result does not depend on thread order.  frac[i] = 1 where d' > 0.95
+ * (FAIL_CANTILEVER), else 0. */
+void or_phase_field(int lx, int ly, int lz, const uint8_t* type, const float* cond,
+                    const float* phi, const float* H, const float* hydration, const uint8_t* code,
+                    const float* d_in, float* d_out, uint8_t* frac, float l0, float gc,
+                    float relax, int split) {
+  static const int DV[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  int64_t n = (int64_t)lx * ly * lz;
+  for (int z = 0; z < lz; z++)
+    for (int y = 0; y < ly; y++)
+      for (int x = 0; x < lx; x++) {
+        int64_t i = IDX(x, y, z, lx, ly);
+        float di = d_in[i];
+        d_out[i] = di;
+        frac[i] = 0;
+        if (!is_solid(type[i]) || code[i] != 0) continue;
+        float pi = phi[i], ssum = 0.0f, lp = 0.0f, ld = 0.0f;
+        for (int d = 0; d < 6; d++) {
+          int nx = x + DV[d][0], ny = y + DV[d][1], nz = z + DV[d][2];
+          if (nx < 0 || nx >= lx || ny < 0 || ny >= ly || nz < 0 || nz >= lz) continue;
+          int64_t j = IDX(nx, ny, nz, lx, ly);
+          if (type[j] == T_AIR) continue;
+          float pj = phi[j], dj = d_in[j];
+          if (isfinite(pj)) {
+            lp += pj - pi;
+            ssum += cond[d * n + i];
+          }
+          if (isfinite(dj)) ld += dj - di;
+        }
+        float hv = H[i];
+        if (!(hv > 0.0f)) continue;
+        float hy = hydration ? hydration[i] : 1.0f;
+        float hd = hy > 0.01f ? (hy < 1.0f ? hy : 1.0f) : 0.01f;
+        float gce = gc * (hd * (float)sqrt((double)hd));
+        float he = hv;
+        if (split) {
+          float sd = ssum > 1e-12f ? ssum : 1e-12f;
+          float fo = lp > 0.0f ? lp : 0.0f;
+          float pp = fo / sd;
+          he = pp > hv ? pp : hv;
+        }
+        float num = 2.0f * he + gce * l0 * ld;
+        float den = 2.0f * he + gce / l0;
+        den = den > 1e-8f ? den : 1e-8f;
+        float q = num / den;
+        q = q > di ? q : di;
+        q = q < 1.0f ? q : 1.0f;
+        float dn = di * (1.0f - relax) + q * relax;
+        dn = dn > 0.0f ? dn : 0.0f;
+        dn = dn < 1.0f ? dn : 1.0f;
+        dn = dn > di ? dn : di; /* the mix rounds; damage never decreases */
+        if (!isfinite(dn)) dn = di;
+        d_out[i] = dn;
+        frac[i] = dn > 0.95f ? 1 : 0;
+      }
+}

@@ -31,6 +31,10 @@ constexpr int kMaxSparse = 512;     // MAX_SPARSE_UPDATES_PER_TICK
 constexpr int kSparseRecord = 48;   // bytes per record
 constexpr float kPhiOrphan = 1e6f;  // PFSFConstants.PHI_ORPHAN_THRESHOLD
 constexpr int kRbgsSweeps = 16;     // STEPS_MAJOR (NP/src/core/constants.h:37)
+// phase-field parameters of the reference engine (NP/src/solver/dispatcher.cpp:
+// 240-248, NP/src/core/constants.h:81-86): l0 = max(PHASE_FIELD_L0, 2),
+// G_C_CONCRETE, PHASE_FIELD_RELAX, spectral split on
+constexpr float kPfL0 = 2.0f, kPfGc = 100.0f, kPfRelax = 0.3f;
 
 struct HostBufs {
   bool registered = false;
@@ -45,6 +49,7 @@ struct HostBufs {
   void* stress = nullptr;
   int64_t stress_bytes = 0;
   bool lookups = false;
+  void* curing = nullptr;  // hydration per voxel (registered lookups), or none
 };
 
 struct Island {
@@ -68,6 +73,8 @@ struct Island {
   float* phi = nullptr;
   double* x64 = nullptr;
   float* H = nullptr;
+  float* dmg[2] = {nullptr, nullptr};  // phase-field damage (current, next)
+  float* hyd = nullptr;                // hydration (allocated with registered lookups)
   uint8_t* code = nullptr;
   int32_t* scratch = nullptr;
   int64_t scratch_elems = 0;
@@ -163,9 +170,9 @@ bool dev_alloc(Island& is, T** out, int64_t n) {
 }
 
 int64_t island_bytes(int64_t n) {
-  // type, kappa, cond(6), source, rcomp, rtens, max_phi, phi, x64, H, code, the type and kappa
-  // the hierarchy was built from + solver (~2.3x fine)
-  return n * (1 + 4 + 24 + 4 + 4 + 4 + 4 + 4 + 8 + 4 + 1 + 5) + n * 60;
+  // type, kappa, cond(6), source, rcomp, rtens, max_phi, phi, x64, H, damage(2), code, the
+  // type and kappa the hierarchy was built from + solver (~2.3x fine)
+  return n * (1 + 4 + 24 + 4 + 4 + 4 + 4 + 4 + 8 + 4 + 8 + 1 + 5) + n * 60;
 }
 
 bool allocate(pfsf_engine_t* E, Island& is) {
@@ -175,6 +182,7 @@ bool allocate(pfsf_engine_t* E, Island& is) {
             dev_alloc(is, &is.rcomp, n) && dev_alloc(is, &is.rtens, n) &&
             dev_alloc(is, &is.max_phi, n) && dev_alloc(is, &is.phi, n) &&
             dev_alloc(is, &is.x64, n) && dev_alloc(is, &is.H, n) && dev_alloc(is, &is.code, n) &&
+            dev_alloc(is, &is.dmg[0], n) && dev_alloc(is, &is.dmg[1], n) &&
             dev_alloc(is, &is.built_type, n) && dev_alloc(is, &is.built_kappa, n);
   if (!ok) return false;
   is.scratch_elems = compact_scratch_elems(is.dims);
@@ -190,6 +198,7 @@ bool allocate(pfsf_engine_t* E, Island& is) {
        hipMemsetAsync(is.phi, 0, 4 * n, s) == hipSuccess &&
        hipMemsetAsync(is.x64, 0, 8 * n, s) == hipSuccess &&
        hipMemsetAsync(is.H, 0, 4 * n, s) == hipSuccess &&
+       hipMemsetAsync(is.dmg[0], 0, 4 * n, s) == hipSuccess &&
        hipMemsetAsync(is.code, 0, n, s) == hipSuccess;
   return ok && hipStreamSynchronize(s) == hipSuccess;
 }
@@ -256,6 +265,11 @@ bool upload(pfsf_engine_t* E, Island& is) {
   if (!ok) return false;
   if (h.kappa && hipMemcpyAsync(is.kappa, h.kappa, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
     return false;
+  if (h.curing) {
+    if (!is.hyd && !dev_alloc(is, &is.hyd, n)) return false;
+    if (hipMemcpyAsync(is.hyd, h.curing, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess)
+      return false;
+  }
   if (!derive(E, is)) return false;
   // the registered phi is the warm start of the solve
   k_widen<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, is.phi, is.x64);
@@ -384,18 +398,28 @@ pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
   return PFSF_OK;
 }
 
-// History and failure scan of a solved island.
+// History, phase field and failure scan of a solved island, in the
+// reference's order (NP/src/pfsf_engine.cpp:484-486): H = max(H, psi_e); the
+// damage step (voxels with last tick's failure codes frozen); the scan; then
+// CANTILEVER for voxels the scan leaves at 0 whose damage passed 0.95.
 pfsf_result post_solve(pfsf_engine_t* E, Island& is) {
   hipStream_t s = E->stream;
   hipError_t e = hipSuccess;
-  if (E->cfg.enable_phase_field)
+  const bool pf = E->cfg.enable_phase_field;
+  if (pf) {
     e = is.soa6 ? launch_soa6_history(is.dims, is.type, is.cond, is.phi, is.H, nullptr, s)
                 : launch_history(is.dims, is.type, is.kappa, is.phi, is.H, nullptr, s);
-  if (e != hipSuccess) return PFSF_ERROR_VULKAN;
+    if (e == hipSuccess)
+      e = launch_phase_field(is.dims, is.type, is.cond, is.phi, is.H, is.hyd, is.code, is.dmg[0],
+                             is.dmg[1], kPfL0, kPfGc, kPfRelax, 1, s);
+    if (e != hipSuccess) return PFSF_ERROR_VULKAN;
+    std::swap(is.dmg[0], is.dmg[1]);
+  }
   e = is.soa6 ? launch_soa6_failure_scan(is.dims, is.type, is.cw, is.phi, is.rcomp, is.rtens,
                                          kPhiOrphan, is.code, s)
               : launch_failure_scan(is.dims, is.type, is.kappa, is.phi, is.rcomp, is.rtens,
                                     kPhiOrphan, is.code, s);
+  if (e == hipSuccess && pf) e = launch_mark_fractured(is.n, is.type, is.dmg[0], is.code, s);
   if (e != hipSuccess) return PFSF_ERROR_VULKAN;
   is.have_phi = true;
   return PFSF_OK;
@@ -769,6 +793,7 @@ pfsf_result pfsf_register_island_lookups(pfsf_engine E, int32_t id,
       l->fluid_pressure_bytes < 4 * n || l->curing_bytes < 4 * n)
     return PFSF_ERROR_INVALID_ARG;
   is->host.lookups = true;
+  is->host.curing = l->curing_addr;  // hydration of the phase field
   return PFSF_OK;
 }
 
@@ -835,6 +860,22 @@ pfsf_result pfsf_get_island_status(pfsf_engine E, int32_t id, pfsf_island_status
   out->solver_builds = is->builds;
   out->operator_mode = is->soa6 ? 1 : 0;
   out->dirty = is->dirty ? 1 : 0;
+  return PFSF_OK;
+}
+
+pfsf_result pfsf_read_damage(pfsf_engine E, int32_t id, float* out, int32_t cap,
+                             int32_t* out_count) {
+  CHECK_ENGINE(E);
+  if (!out || !out_count) return PFSF_ERROR_INVALID_ARG;
+  Island* is = find(E, id);
+  if (!is) return PFSF_ERROR_INVALID_ARG;
+  int64_t cnt = cap < is->n ? cap : is->n;
+  if (cnt < 0) cnt = 0;
+  if (cnt > 0 && (hipMemcpyAsync(out, is->dmg[0], 4 * cnt, hipMemcpyDeviceToHost, E->stream) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(E->stream) != hipSuccess))
+    return PFSF_ERROR_VULKAN;
+  *out_count = (int32_t)cnt;
   return PFSF_OK;
 }
 
@@ -980,6 +1021,19 @@ pfsf_result pfsf_hip_history_update(pfsf_dims d, const uint8_t* type, const floa
   DIMS_OR_FAIL(d);
   if (!type || !kappa || !phi || (!H && !psi)) return PFSF_ERROR_INVALID_ARG;
   RET(launch_history(d, type, kappa, phi, H, psi, (hipStream_t)stream));
+}
+
+pfsf_result pfsf_hip_phase_field_evolve(pfsf_dims d, const uint8_t* type, const float* cond,
+                                        const float* phi, const float* H, const float* hydration,
+                                        uint8_t* code, const float* damage_in, float* damage_out,
+                                        float l0, float gc, float relax, int32_t spectral_split,
+                                        void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !cond || !phi || !H || !code || !damage_in || !damage_out ||
+      damage_in == damage_out)
+    return PFSF_ERROR_INVALID_ARG;
+  RET(launch_phase_field(d, type, cond, phi, H, hydration, code, damage_in, damage_out, l0, gc,
+                         relax, spectral_split, (hipStream_t)stream));
 }
 
 pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,

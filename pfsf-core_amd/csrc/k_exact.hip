@@ -548,7 +548,97 @@ __global__ void __launch_bounds__(64)
   for (int dd = 0; dd < 6; dd++) cond[dd * n + i] = __uint_as_float(r[6 + dd]);
 }
 
+// Phase-field damage step (SH/phase_field_evolve.comp.glsl:50-168, hybrid
+// AT2 form of Ambati et al. 2015): the FP32 expression order of the CPU
+// restatement, reading d_in and writing d_out (a Jacobi step; the shader's
+// in-place update makes neighbours race).  Voxels with a failure code from
+// the previous scan are frozen; code becomes 1 (CANTILEVER) where d > 0.95.
+__global__ void __launch_bounds__(BS)
+    k_phase_field(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ cond,
+                  const float* __restrict__ phi, const float* __restrict__ H,
+                  const float* __restrict__ hydration, uint8_t* __restrict__ code,
+                  const float* __restrict__ d_in, float* __restrict__ d_out, float l0, float gc,
+                  float relax, int split) {
+  constexpr int DV[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  const int64_t n = nvox(d);
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const float di = d_in[i];
+  float dn = di;
+  const uint8_t c0 = code[i];
+  if (is_solid(type[i]) && c0 == 0) {
+    const XYZ p = unflat(i, d);
+    const float pi = phi[i];
+    float ssum = 0.0f, lp = 0.0f, ld = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int nx = p.x + DV[k][0], ny = p.y + DV[k][1], nz = p.z + DV[k][2];
+      if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+      const int64_t j = flat(nx, ny, nz, d);
+      if (type[j] == 0) continue;
+      const float pj = phi[j], dj = d_in[j];
+      if (isfinite(pj)) {
+        lp += pj - pi;
+        ssum += cond[k * n + i];
+      }
+      if (isfinite(dj)) ld += dj - di;
+    }
+    const float hv = H[i];
+    if (hv > 0.0f) {
+      const float hy = hydration ? hydration[i] : 1.0f;
+      const float hd = hy > 0.01f ? (hy < 1.0f ? hy : 1.0f) : 0.01f;
+      const float gce = gc * (hd * (float)sqrt((double)hd));
+      float he = hv;
+      if (split) {
+        const float sd = ssum > 1e-12f ? ssum : 1e-12f;
+        const float fo = lp > 0.0f ? lp : 0.0f;
+        const float pp = fo / sd;
+        he = pp > hv ? pp : hv;
+      }
+      const float num = 2.0f * he + gce * l0 * ld;
+      float den = 2.0f * he + gce / l0;
+      den = den > 1e-8f ? den : 1e-8f;
+      float q = num / den;
+      q = q > di ? q : di;
+      q = q < 1.0f ? q : 1.0f;
+      float v = di * (1.0f - relax) + q * relax;
+      v = v > 0.0f ? v : 0.0f;
+      v = v < 1.0f ? v : 1.0f;
+      v = v > di ? v : di;  // the mix rounds; damage never decreases
+      dn = isfinite(v) ? v : di;
+      if (dn > 0.95f) code[i] = 1;
+    }
+  }
+  d_out[i] = dn;
+}
+
+// After the failure scan: a voxel the scan leaves at 0 whose damage exceeds
+// the fracture threshold keeps CANTILEVER (the scan's codes take precedence).
+__global__ void __launch_bounds__(BS)
+    k_mark_fractured(int64_t n, const uint8_t* __restrict__ type, const float* __restrict__ dmg,
+                     uint8_t* __restrict__ code) {
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n && is_solid(type[i]) && code[i] == 0 && dmg[i] > 0.95f) code[i] = 1;
+}
+
 }  // namespace
+
+hipError_t launch_phase_field(pfsf_dims d, const uint8_t* type, const float* cond,
+                              const float* phi, const float* H, const float* hydration,
+                              uint8_t* code, const float* d_in, float* d_out, float l0, float gc,
+                              float relax, int split, hipStream_t s) {
+  if (nvox(d) == 0) return hipSuccess;
+  k_phase_field<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, cond, phi, H, hydration, code,
+                                                       d_in, d_out, l0, gc, relax, split);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_fractured(int64_t n, const uint8_t* type, const float* dmg, uint8_t* code,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_mark_fractured<<<blocks_for(n, BS), BS, 0, s>>>(n, type, dmg, code);
+  return hipGetLastError();
+}
 
 hipError_t launch_sparse_scatter(const uint32_t* rec, int count, int64_t n, float* source,
                                  float* cond, uint8_t* type, float* max_phi, float* rcomp,

@@ -40,6 +40,14 @@ hipError_t launch_failure_scan(pfsf_dims d, const uint8_t* type, const float* ka
                                float phi_orphan, uint8_t* code, hipStream_t s);
 hipError_t launch_history(pfsf_dims d, const uint8_t* type, const float* kappa,
                           const float* phi, float* H, float* psi, hipStream_t s);
+// Phase-field damage step (d_in -> d_out; code: previous codes in, 1 where
+// the voxel fractures) and the CANTILEVER merge after a failure scan.
+hipError_t launch_phase_field(pfsf_dims d, const uint8_t* type, const float* cond,
+                              const float* phi, const float* H, const float* hydration,
+                              uint8_t* code, const float* d_in, float* d_out, float l0, float gc,
+                              float relax, int split, hipStream_t s);
+hipError_t launch_mark_fractured(int64_t n, const uint8_t* type, const float* dmg, uint8_t* code,
+                                 hipStream_t s);
 hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
                             const double* x, double* y, hipStream_t s);
 // Failure tuples of `code` into out = {count, capacity x {x,y,z,code}} in

@@ -104,6 +104,12 @@ class Engine:
         r = self.lib.pfsf_read_stress(self.h, island_id, _addr(out), n, C.byref(cnt))
         return r, out[:cnt.value]
 
+    def read_damage(self, island_id, n):
+        out = np.zeros(n, np.float32)
+        cnt = C.c_int32(0)
+        r = self.lib.pfsf_read_damage(self.h, island_id, _addr(out), n, C.byref(cnt))
+        return r, out[:cnt.value]
+
     def read_history(self, island_id, n):
         out = np.zeros(n, np.float32)
         cnt = C.c_int32(0)

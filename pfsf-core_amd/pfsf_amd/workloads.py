@@ -137,18 +137,19 @@ def workload(index: int) -> Island:
     """The single-island workloads, by index.
 
     0: dense 32^3, z=0 anchored, kappa lognormal sigma 0.5 (RBGS 200 sweeps).
-    1: 128^3 ~60 % box union on an anchored slab, sigma 0.5 (PCG to 1e-6).
+    1: 128^3 60 % box union on an anchored slab, sigma 0.5 (PCG to 1e-6);
+       boxes are placed until the union holds 60 % of the grid.
     3: 256^3 ~70 % box union, sigma 1.5 (mixed precision).
     4: 96^3 box union (config-1 recipe) for the phase-field adjoint.
     """
     if index == 0:
         return dense_block(0, 32, 0.5)
     if index == 1:
-        return box_union(1, 128, 0.60, 0.5)
+        return box_union(1, 128, 0.60, 0.5, n_boxes=100000)
     if index == 3:
         return box_union(3, 256, 0.70, 1.5, n_boxes=100000)
     if index == 4:
-        return box_union(4, 96, 0.60, 0.5)
+        return box_union(4, 96, 0.60, 0.5, n_boxes=100000)
     raise ValueError(f"no single-island workload {index}")
 
 

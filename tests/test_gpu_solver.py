@@ -4,6 +4,7 @@ Bar: the FP64 residual ||b - A x|| / ||b|| (recomputed by the oracle from the
 returned x) <= 1e-6, and phi within 1e-5 relative L2 of the FP64 direct solve
 where the direct solve is affordable."""
 import dataclasses
+import os
 
 import numpy as np
 import pytest
@@ -109,9 +110,19 @@ def test_workload_128_to_1e6(cuda):
     # phi is the FP32 rounding of x
     assert np.array_equal(phi, x.astype(np.float32))
     # iteration budget of the V-cycle with connectivity-masked coarse levels
-    # and the [lmax/100, lmax] coarsest Chebyshev interval (40 measured; 43
-    # with [lmax/30, lmax], 52 with the unmasked rediscretisation)
+    # and the [lmax/100, lmax] coarsest Chebyshev interval
     assert info.iterations <= 44, info.iterations
+    # phi against the oracle's FP64 Jacobi-PCG solution to 1e-10 (all host
+    # threads; the same iterates on any thread count)
+    O.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
+    try:
+        ref, it, rr = O.pcg(isl, tol=1e-10, maxit=20000)
+    finally:
+        O.set_threads(1)
+    assert rr <= 1e-9, rr
+    err = np.linalg.norm(phi - ref) / np.linalg.norm(ref)
+    print(f"128^3: phi rel-L2 vs FP64 oracle solution {err:.3e} ({it} oracle iterations)")
+    assert err <= 1e-5, err
 
 
 def test_strided_column_walk_on_a_ragged_grid(cuda):
