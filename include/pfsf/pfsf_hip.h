@@ -41,7 +41,11 @@ PFSF_API pfsf_result pfsf_hip_jacobi_step(pfsf_dims d, const uint8_t* type, cons
 /* `sweeps` in-place 8-colour Gauss-Seidel sweeps, colours 0..7 with colour
  * (x&1)|(y&1)<<1|(z&1)<<2 (SH/rbgs_smooth.comp.glsl:86-87,
  * NP/src/solver/jacobi_solver.cpp:163-175), each voxel updated with the
- * jacobiStep formula (omega 1).  Bit-identical to the CPU restatement. */
+ * jacobiStep formula (omega 1).  Bit-identical to the CPU restatement.  With
+ * at least 8 sweeps on a grid of up to 2^21 voxels (stream not under
+ * capture) the sweeps run in one persistent launch over precomputed
+ * couplings (a temporary of 220 B per voxel) and the call synchronises the
+ * stream; otherwise it is asynchronous. */
 PFSF_API pfsf_result pfsf_hip_rbgs(pfsf_dims d, const uint8_t* type, const float* kappa,
                                    const float* source, float* phi, int32_t sweeps, void* stream);
 
@@ -87,6 +91,18 @@ PFSF_API pfsf_result pfsf_hip_phase_field_evolve(pfsf_dims d, const uint8_t* typ
                                                  float* damage_out, float l0, float gc,
                                                  float relax, int32_t spectral_split,
                                                  void* stream);
+
+/* Macro-block residual map (SH/failure_scan.comp.glsl:183-195, blocks of
+ * 8^3 voxels): block_max[(x/8) + bx * ((y/8) + by * (z/8))], bx = ceil(lx/8),
+ * by = ceil(ly/8), receives the largest |b_i - (A phi)_i| over the block's
+ * solid voxels (the residual in FP64 with pfsf_hip_apply_f64's operator and
+ * phi widened from FP32, rounded to FP32), 0 for blocks without one.  The
+ * reference seeds its RBGS block skip from this map; a tolerance solve
+ * cannot skip blocks (a local change moves phi in nearly every block), so
+ * here it is a diagnostic of where a solution is still off. */
+PFSF_API pfsf_result pfsf_hip_macro_residual(pfsf_dims d, const uint8_t* type,
+                                             const float* kappa, const float* phi,
+                                             const float* b, float* block_max, void* stream);
 
 /* y = A x in FP64 (exact coefficients), 0 on non-solid rows. */
 PFSF_API pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,

@@ -93,6 +93,7 @@ def lib():
         L.or_soa6_rbgs.argtypes = [I, I, I, P, P, P, P, I]
         L.or_phase_field.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, C.c_float, C.c_float,
                                      C.c_float, I]
+        L.or_macro_residual.argtypes = [I, I, I, P, P, P, P, P]
         L.or_set_threads.argtypes = [I]
         L.or_set_threads.restype = I
         L.or_set_threads(1)  # single-threaded unless a caller asks for more
@@ -375,6 +376,16 @@ def direct_solve_soa6(isl, cond, b=None):
     if fidx.size:
         phi[fidx] = spla.spsolve(A.tocsc(), np.asarray(b, np.float64).ravel()[fidx])
     return phi.reshape(isl.vtype.shape)
+
+
+def macro_residual(isl, phi, b=None):
+    """8^3 block maxima of |b - A phi| over solid voxels (FP32), shape
+    (ceil(lz/8), ceil(ly/8), ceil(lx/8))."""
+    b = isl.source if b is None else np.ascontiguousarray(b, np.float32)
+    out = np.empty(((isl.lz + 7) // 8, (isl.ly + 7) // 8, (isl.lx + 7) // 8), np.float32)
+    lib().or_macro_residual(*_dims(isl), _p(isl.vtype), _p(isl.kappa),
+                            _p(np.ascontiguousarray(phi, np.float32)), _p(b), _p(out))
+    return out
 
 
 PF_L0, PF_GC, PF_RELAX = 2.0, 100.0, 0.3  # dispatcher.cpp:240-248 values

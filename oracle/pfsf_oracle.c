@@ -1152,3 +1152,30 @@ void or_phase_field(int lx, int ly, int lz, const uint8_t* type, const float* co
         frac[i] = dn > 0.95f ? 1 : 0;
       }
 }
+
+/* Macro-block residual map (SH/failure_scan.comp.glsl:183-195, blocks of
+ * MACRO_BLOCK_SIZE = 8): for each 8^3 block the largest |b_i - (A phi)_i|
+ * over its solid voxels, the residual in FP64 as or_apply forms it (phi
+ * widened from FP32), rounded to FP32; 0 for blocks without a solid voxel.
+ * The shader's |flux_in - flux_out + rho| is the same quantity. */
+void or_macro_residual(int lx, int ly, int lz, const uint8_t* type, const float* kappa,
+                       const float* phi, const float* b, float* blk) {
+  int64_t n = (int64_t)lx * ly * lz;
+  int bx = (lx + 7) / 8, by = (ly + 7) / 8, bz = (lz + 7) / 8;
+  double* x = (double*)malloc(sizeof(double) * (size_t)n);
+  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; i++) x[i] = (double)phi[i];
+  or_apply(lx, ly, lz, type, kappa, x, y);
+  memset(blk, 0, sizeof(float) * (size_t)bx * by * bz);
+  for (int z = 0; z < lz; z++)
+    for (int yy = 0; yy < ly; yy++)
+      for (int xx = 0; xx < lx; xx++) {
+        int64_t i = IDX(xx, yy, z, lx, ly);
+        if (!is_solid(type[i])) continue;
+        float r = (float)fabs((double)b[i] - y[i]);
+        int64_t k = (xx / 8) + (int64_t)bx * ((yy / 8) + (int64_t)by * (z / 8));
+        if (r > blk[k]) blk[k] = r;
+      }
+  free(x);
+  free(y);
+}

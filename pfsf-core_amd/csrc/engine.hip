@@ -1036,6 +1036,14 @@ pfsf_result pfsf_hip_phase_field_evolve(pfsf_dims d, const uint8_t* type, const 
                          relax, spectral_split, (hipStream_t)stream));
 }
 
+pfsf_result pfsf_hip_macro_residual(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                    const float* phi, const float* b, float* block_max,
+                                    void* stream) {
+  DIMS_OR_FAIL(d);
+  if (!type || !kappa || !phi || !b || !block_max) return PFSF_ERROR_INVALID_ARG;
+  RET(launch_macro_residual(d, type, kappa, phi, b, block_max, (hipStream_t)stream));
+}
+
 pfsf_result pfsf_hip_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
                                const double* x, double* y, void* stream) {
   DIMS_OR_FAIL(d);

@@ -14,6 +14,7 @@
 //   normalisation         NP/src/compute/normalize.cpp:29-63
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -132,6 +133,162 @@ __global__ void __launch_bounds__(BS)
     return;
   }
   phi[i] = jacobi_voxel(i, p, d, type, kappa, src, phi, 1.0f);
+}
+
+// ── RBGS in one persistent launch ───────────────────────────────────────
+// The couplings of every voxel (couple_exact, stencil order, 0 and a clear
+// mask bit where the neighbour is outside the grid or air) and its ordered
+// diagonal are formed once, so a sweep reads them instead of forming 20 FP64
+// square roots per voxel; the update is jacobi_voxel's, operation for
+// operation.
+__global__ void __launch_bounds__(BS)
+    k_rbgs_coef(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                double* __restrict__ coef, double* __restrict__ diag, uint32_t* __restrict__ mask) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  const int64_t n = nvox(d);
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const XYZ p = unflat(i, d);
+  const float ki = kapf(kappa[i]);
+  double dg = 0.0;
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    const int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    const bool in = nx >= 0 && nx < d.lx && ny >= 0 && ny < d.ly && nz >= 0 && nz < d.lz;
+    const int64_t j = in ? flat(nx, ny, nz, d) : i;
+    const bool use = in && type[j] != 0;
+    const double a = couple_exact(k, ki, kapf(kappa[j]));
+    dg = use ? dg + a : dg;
+    m |= use ? (1u << k) : 0u;
+    coef[k * n + i] = use ? a : 0.0;
+  }
+  diag[i] = dg;
+  mask[i] = m;
+}
+
+// Work-group g owns z-planes [g * ppw, (g + 1) * ppw).  Colours 0-3 lie on
+// even planes and read only odd-plane neighbours across planes (4-7 the
+// reverse), so a sweep is two phases: even planes (colours 0..3 in order,
+// a work-group barrier between colours), then odd planes.  Before phase p a
+// work-group waits until both slab neighbours have finished phase p - 1 (it
+// then reads their newest values and they no longer read the planes it
+// overwrites); after it, it publishes p + 1.  Every wait is bounded: a wait
+// that runs out raises *abort, and every work-group then stops.
+constexpr unsigned kRbgsSpinLimit = 1u << 25;
+
+__device__ __forceinline__ bool rbgs_wait(const unsigned* flag, unsigned want, unsigned* abort) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
+    if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (++spins > kRbgsSpinLimit) {
+      __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(BS)
+    k_rbgs_persistent(pfsf_dims d, const uint8_t* __restrict__ type,
+                      const double* __restrict__ coef, const double* __restrict__ diag,
+                      const uint32_t* __restrict__ mask, const float* __restrict__ src,
+                      float* phi, int sweeps, int ppw, unsigned* flags, unsigned* abort) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  __shared__ int go;
+  const int g = (int)blockIdx.x, G = (int)gridDim.x;
+  const int z0 = g * ppw, z1 = min(d.lz, z0 + ppw);
+  const int64_t n = nvox(d);
+  for (int s = 0; s < sweeps; s++) {
+    for (int half = 0; half < 2; half++) {
+      const unsigned p = 2u * (unsigned)s + (unsigned)half;
+      if (threadIdx.x == 0) {
+        bool ok = true;
+        if (g > 0) ok = rbgs_wait(flags + g - 1, p, abort);
+        if (ok && g + 1 < G) ok = rbgs_wait(flags + g + 1, p, abort);
+        go = ok;
+      }
+      __syncthreads();
+      if (!go) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      const int zs = z0 + ((z0 & 1) != half ? 1 : 0);  // first plane of this parity
+      const int nzp = zs < z1 ? (z1 - zs + 1) / 2 : 0;
+      for (int cc = 0; cc < 4; cc++) {
+        const int cx = cc & 1, cy = (cc >> 1) & 1;
+        const int hx = (d.lx - cx + 1) >> 1, hy = (d.ly - cy + 1) >> 1;
+        const int per = hx * hy;
+        for (int t = threadIdx.x; t < per * nzp; t += BS) {
+          const int pl = t / per, r = t - pl * per;
+          XYZ q;
+          q.x = 2 * (r % hx) + cx;
+          q.y = 2 * (r / hx) + cy;
+          q.z = zs + 2 * pl;
+          const int64_t i = flat(q.x, q.y, q.z, d);
+          if (!is_solid(type[i])) {
+            phi[i] = 0.0f;
+            continue;
+          }
+          const uint32_t m = mask[i];
+          double off = 0.0;
+#pragma unroll
+          for (int k = 0; k < 26; k++) {
+            const bool use = (m >> k) & 1u;
+            const int64_t j = use ? flat(q.x + O[k][0], q.y + O[k][1], q.z + O[k][2], d) : i;
+            const double a = coef[k * n + i];
+            const float pk = phi[j];
+            off = use ? off + a * (double)pk : off;
+          }
+          const double dg = diag[i];
+          const double prev = (double)phi[i];
+          const double b = (double)src[i];
+          const double pj = (dg > 1e-15) ? (b + off) / dg : prev;
+          phi[i] = (float)(1.0 * (pj - prev) + prev);
+        }
+        __syncthreads();
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(flags + g, p + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Macro-block residual map (SH/failure_scan.comp.glsl:183-195, 8^3 blocks):
+// the FP64 residual b - A phi of k_apply_f64's operator at each solid voxel
+// (phi widened from FP32), |r| rounded to FP32 and maximised per block as
+// IEEE bits (non-negative floats order as unsigned integers).
+__global__ void __launch_bounds__(BS)
+    k_macro_residual(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                     const float* __restrict__ phi, const float* __restrict__ b,
+                     unsigned* __restrict__ blk) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i >= nvox(d) || !is_solid(type[i])) return;
+  const XYZ p = unflat(i, d);
+  const float ki = kapf(kappa[i]);
+  const double xi = (double)phi[i];
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 26; k++) {
+    const int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
+    if (nx < 0 || nx >= d.lx || ny < 0 || ny >= d.ly || nz < 0 || nz >= d.lz) continue;
+    const int64_t j = flat(nx, ny, nz, d);
+    const uint8_t tj = type[j];
+    if (tj == 0) continue;
+    const double xj = is_solid(tj) ? (double)phi[j] : 0.0;
+    acc += couple_exact(k, ki, kapf(kappa[j])) * (xi - xj);
+  }
+  const float r = (float)fabs((double)b[i] - acc);
+  const int bx = (d.lx + 7) / 8, by = (d.ly + 7) / 8;
+  atomicMax(blk + (p.x / 8) + (int64_t)bx * ((p.y / 8) + (int64_t)by * (p.z / 8)),
+            __float_as_uint(r));
+}
+
+__global__ void k_zero_u32(unsigned* a, int n) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) a[i] = 0u;
 }
 
 // Failure scan over 32 x 8 x 4 tiles of the whole grid, one 256-thread
@@ -753,6 +910,56 @@ static hipError_t rbgs_graph(pfsf_dims d, const uint8_t* type, const float* kapp
   return hipSuccess;
 }
 
+// Persistent form: up to kRbgsPersistMax voxels (the coupling table takes
+// 220 B per voxel), at least kRbgsGraphSweeps sweeps, a stream not under
+// capture.  Work-groups own one or more whole z-planes (at most one per CU).
+constexpr int64_t kRbgsPersistMax = int64_t(1) << 21;
+
+static hipError_t rbgs_persistent(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                  const float* src, float* phi, int sweeps, hipStream_t s) {
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const int ppw = (d.lz + std::max(cus, 1) - 1) / std::max(cus, 1);
+  const int G = (d.lz + ppw - 1) / ppw;
+  const int64_t n = nvox(d);
+  const size_t bytes = sizeof(double) * 27 * (size_t)n + sizeof(uint32_t) * (size_t)n +
+                       sizeof(unsigned) * (size_t)(G + 1);
+  char* ws = nullptr;
+  e = hipMallocAsync((void**)&ws, bytes, s);
+  if (e != hipSuccess) return e;
+  double* coef = (double*)ws;
+  double* diag = coef + 26 * n;
+  uint32_t* mask = (uint32_t*)(diag + n);
+  unsigned* flags = (unsigned*)(mask + n);
+  unsigned* abort = flags + G;
+  k_rbgs_coef<<<blocks_for(n, BS), BS, 0, s>>>(d, type, kappa, coef, diag, mask);
+  k_zero_u32<<<(G + 1 + 255) / 256, 256, 0, s>>>(flags, G + 1);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    const uint8_t* t = type;
+    const double *cf = coef, *dg = diag;
+    const uint32_t* mk = mask;
+    const float* sr = src;
+    float* ph = phi;
+    int sw = sweeps, pw = ppw;
+    unsigned *fl = flags, *ab = abort;
+    void* args[] = {&d, &t, &cf, &dg, &mk, &sr, &ph, &sw, &pw, &fl, &ab};
+    // cooperative: the launch fails rather than start work-groups that could
+    // not all be resident (each waits for its neighbours)
+    e = hipLaunchCooperativeKernel((const void*)k_rbgs_persistent, dim3(G), dim3(BS), args, 0,
+                                   s);
+  }
+  unsigned ab_host = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&ab_host, abort, sizeof(unsigned),
+                                          hipMemcpyDeviceToHost, s);
+  (void)hipFreeAsync(ws, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && ab_host) e = hipErrorLaunchTimeOut;
+  return e;
+}
+
 hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
                              const float* src, float* phi, int sweeps, hipStream_t s) {
   int sw = 0;
@@ -764,6 +971,8 @@ hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kapp
     else
       (void)hipGetLastError();
   }
+  if (replay && nvox(d) <= kRbgsPersistMax)
+    return rbgs_persistent(d, type, kappa, src, phi, sweeps, s);
   if (replay) {
     hipGraphExec_t exec = nullptr;
     hipError_t err = rbgs_graph(d, type, kappa, src, phi, &exec);
@@ -774,6 +983,17 @@ hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kapp
     }
   }
   for (; sw < sweeps; sw++) rbgs_sweep(d, type, kappa, src, phi, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_macro_residual(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                const float* phi, const float* b, float* blk, hipStream_t s) {
+  const int64_t nb = (int64_t)((d.lx + 7) / 8) * ((d.ly + 7) / 8) * ((d.lz + 7) / 8);
+  if (nb == 0) return hipSuccess;
+  if (nb > 0x7fffffff) return hipErrorInvalidValue;
+  k_zero_u32<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>((unsigned*)blk, (int)nb);
+  k_macro_residual<<<blocks_for(nvox(d), BS), BS, 0, s>>>(d, type, kappa, phi, b,
+                                                          (unsigned*)blk);
   return hipGetLastError();
 }
 

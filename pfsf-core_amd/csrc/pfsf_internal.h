@@ -48,6 +48,9 @@ hipError_t launch_phase_field(pfsf_dims d, const uint8_t* type, const float* con
                               float relax, int split, hipStream_t s);
 hipError_t launch_mark_fractured(int64_t n, const uint8_t* type, const float* dmg, uint8_t* code,
                                  hipStream_t s);
+// 8^3 block maxima of |b - A phi| (float per block, overwritten).
+hipError_t launch_macro_residual(pfsf_dims d, const uint8_t* type, const float* kappa,
+                                const float* phi, const float* b, float* blk, hipStream_t s);
 hipError_t launch_apply_f64(pfsf_dims d, const uint8_t* type, const float* kappa,
                             const double* x, double* y, hipStream_t s);
 // Failure tuples of `code` into out = {count, capacity x {x,y,z,code}} in

@@ -196,6 +196,7 @@ SIGNATURES = {
     "pfsf_read_history": (R, [P, I32, P, I32, C.POINTER(I32)]),
     "pfsf_get_island_status": (R, [P, I32, C.POINTER(IslandStatus)]),
     "pfsf_read_damage": (R, [P, I32, P, I32, C.POINTER(I32)]),
+    "pfsf_hip_macro_residual": (R, [Dims, P, P, P, P, P, P]),
     "pfsf_hip_phase_field_evolve": (R, [Dims, P, P, P, P, P, P, P, P, F32, F32, F32, I32, P]),
     "pfsf_hip_chebyshev_jacobi": (R, [Dims, P, P, P, P, P, I32, F32, P]),
     "pfsf_hip_soa6_couplings": (R, [Dims, P, P, P, P]),

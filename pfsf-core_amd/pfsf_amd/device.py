@@ -268,3 +268,9 @@ def soa6_history_update(vtype, cond, phi, H=None, psi=None):
 def soa6_rbgs(vtype, cw, source, phi, sweeps):
     capi.check("pfsf_hip_soa6_rbgs", _lib().pfsf_hip_soa6_rbgs(
         _dims(vtype), _p(vtype), _p(cw), _p(source), _p(phi), sweeps, _stream()))
+
+
+def macro_residual(vtype, kappa, phi, b, out):
+    """8^3 block maxima of |b - A phi| into out (float32, ceil(l/8) per axis)."""
+    capi.check("pfsf_hip_macro_residual", _lib().pfsf_hip_macro_residual(
+        _dims(vtype), _p(vtype), _p(kappa), _p(phi), _p(b), _p(out), _stream()))

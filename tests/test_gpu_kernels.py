@@ -270,3 +270,18 @@ def test_island_data_prep_bit_exact_on_random_grids(cuda):
         anchor = (rng.random(shape) < 0.1).astype(np.uint8)
         curing = rng.uniform(0.0, 1.0, shape).astype(np.float32)
         _check_build(cuda, mi, rows, anchor, curing, None)
+
+
+@pytest.mark.parametrize("seed,L", [(0, 20), (1, 33)])
+def test_macro_residual_map_bit_exact(cuda, seed, L):
+    """pfsf_hip_macro_residual: 8^3 block maxima of |b - A phi| equal the
+    oracle's, on a perturbed solution and on ragged block edges."""
+    isl = W.small_box_union(seed, L=L)
+    rng = np.random.default_rng(seed)
+    phi = (O.direct_solve(isl) + rng.normal(0, 1e-3, isl.vtype.shape)).astype(np.float32)
+    d = _isl_dev(isl, cuda)
+    out = torch.full(((isl.lz + 7) // 8, (isl.ly + 7) // 8, (isl.lx + 7) // 8), -1.0,
+                     dtype=torch.float32, device=cuda)
+    D.macro_residual(d["vtype"], d["kappa"], torch.from_numpy(phi).to(cuda), d["source"], out)
+    want = O.macro_residual(isl, phi)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
