@@ -373,9 +373,11 @@ def rbgs_bench(torch, D, W, sweeps=200, steps=5):
            "cpu_voxel_updates_per_s": round(n * sweeps / (cpu_ms * 1e-3), 1),
            "phi_bit_exact": bool(np.array_equal(phi.cpu().numpy(), ref_phi)),
            "codes_bit_exact": bool(np.array_equal(code.cpu().numpy(), ref_code)),
-           "note": "synchronous C-ABI calls timed on the host: one persistent launch for the "
-                   "200 sweeps (work-groups own z-slabs and wait only for their neighbours, twice "
-                   "per sweep) plus the coupling set-up and the failure scan"}
+           "note": "synchronous C-ABI calls timed on the host: the 200 sweeps in one launch "
+                   "(work-group z keeps plane z and its couplings in LDS; neighbouring planes "
+                   "pass as tagged 8-byte words, two dependent hand-offs per sweep) plus the "
+                   "failure scan.  The bound is the chain of 400 hand-offs and 1600 colour "
+                   "passes, not HBM: the HBM fraction is given for the record only"}
     log(f"rbgs 32^3: {out}")
     return out
 

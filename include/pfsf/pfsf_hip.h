@@ -41,11 +41,13 @@ PFSF_API pfsf_result pfsf_hip_jacobi_step(pfsf_dims d, const uint8_t* type, cons
 /* `sweeps` in-place 8-colour Gauss-Seidel sweeps, colours 0..7 with colour
  * (x&1)|(y&1)<<1|(z&1)<<2 (SH/rbgs_smooth.comp.glsl:86-87,
  * NP/src/solver/jacobi_solver.cpp:163-175), each voxel updated with the
- * jacobiStep formula (omega 1).  Bit-identical to the CPU restatement.  With
- * at least 8 sweeps on a grid of up to 2^21 voxels (stream not under
- * capture) the sweeps run in one persistent launch over precomputed
- * couplings (a temporary of 220 B per voxel) and the call synchronises the
- * stream; otherwise it is asynchronous. */
+ * jacobiStep formula (omega 1).  Bit-identical to the CPU restatement.  On
+ * a stream not under capture all sweeps run in one launch with the
+ * couplings formed once, and the call synchronises the stream: planes of at
+ * most 32 x 32 with no more planes than CUs stay in LDS (a temporary of 8 B
+ * per voxel), other grids of up to 2^21 voxels read a coupling table (a
+ * temporary of 224 B per voxel).  Larger grids, and streams under capture,
+ * take eight colour launches per sweep, asynchronously. */
 PFSF_API pfsf_result pfsf_hip_rbgs(pfsf_dims d, const uint8_t* type, const float* kappa,
                                    const float* source, float* phi, int32_t sweeps, void* stream);
 

@@ -135,23 +135,227 @@ __global__ void __launch_bounds__(BS)
   phi[i] = jacobi_voxel(i, p, d, type, kappa, src, phi, 1.0f);
 }
 
-// ── RBGS in one persistent launch ───────────────────────────────────────
-// The couplings of every voxel (couple_exact, stencil order, 0 and a clear
-// mask bit where the neighbour is outside the grid or air) and its ordered
-// diagonal are formed once, so a sweep reads them instead of forming 20 FP64
-// square roots per voxel; the update is jacobi_voxel's, operation for
-// operation.
+// ── RBGS in one launch ──────────────────────────────────────────────────
+// Both single-launch forms let a work-group own whole z-planes for the run.
+// Colours 0-3 lie on even planes and read only odd planes across z (4-7 the
+// reverse), so a sweep is an even-plane phase then an odd-plane phase, and a
+// work-group only ever waits for its two slab neighbours.  Every wait is
+// bounded: one that runs out raises *abort, every work-group then stops and
+// the host reports a time-out.
+constexpr unsigned kRbgsSpinLimit = 1u << 25;
+
+// One more round of a bounded wait: false once *abort is raised (by this
+// wait running out or by another work-group).
+__device__ __forceinline__ bool rbgs_spin(unsigned* spins, unsigned* abort) {
+  if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+  if (++*spins > kRbgsSpinLimit) {
+    __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+  }
+  __builtin_amdgcn_s_sleep(1);
+  return true;
+}
+
+// Plane form, for planes of at most 32 x 32 voxels: work-group z owns plane
+// z.  Its LDS holds planes z-1, z, z+1 with a one-voxel border; the border,
+// missing planes and air read type 0, kappa 0 and phi 0.  The couplings are
+// formed once (couple_exact, so the same bits as forming them per sweep):
+// faces as their float sum kappa_i + kappa_j, edges in FP64 in LDS, corners
+// and the ordered diagonal in registers.  A coupling to a dropped neighbour
+// (outside the grid or air) is stored as 0 and that neighbour's phi reads 0,
+// so the sweep adds 0 * 0 = +0 where the restatement skips the term, which
+// leaves the sum unchanged (it starts at +0 and a round-to-nearest sum is
+// never -0 unless both addends are).  A sweep then reads only the 26
+// neighbour phi of each voxel.
+//
+// Planes pass between work-groups as 8-byte words {phi bits, tag} stored and
+// loaded with agent-scope atomics, so a word's data and its readiness arrive
+// together and no fence is needed.  Tag 1 is the starting plane and sweep s
+// publishes tag s + 2; at sweep s an even plane waits for its neighbours'
+// tag s + 1 and an odd plane for tag s + 2.  A writer never overwrites a word
+// still to be read: it publishes sweep s + 1 only after reading what its
+// reader published once it had read sweep s.
+constexpr int kPlaneSide = 32;
+constexpr int kPlaneVox = kPlaneSide * kPlaneSide;
+constexpr int kPlanePad = (kPlaneSide + 2) * (kPlaneSide + 2);
+constexpr int kPlaneWords = 2 * kPlaneVox / BS;  // neighbour words per thread
+static_assert(kPlaneVox / 4 <= BS, "one voxel of each colour per thread");
+
 __global__ void __launch_bounds__(BS)
-    k_rbgs_coef(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
-                double* __restrict__ coef, double* __restrict__ diag, uint32_t* __restrict__ mask) {
+    k_rbgs_planes(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                  const float* __restrict__ src, float* __restrict__ phi, int sweeps,
+                  uint64_t* words, unsigned* abort) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  __shared__ double s_edge[12 * kPlaneVox];
+  __shared__ float s_face[6 * kPlaneVox];
+  __shared__ float s_kap[3 * kPlanePad];
+  __shared__ float s_phi[3 * kPlanePad];
+  __shared__ float s_src[kPlaneVox];
+  __shared__ uint8_t s_typ[3 * kPlanePad];
+  __shared__ int go;
+  const int z = (int)blockIdx.x;
+  const int P = d.lx * d.ly, W = d.lx + 2, PP = W * (d.ly + 2);
+  for (int t = threadIdx.x; t < 3 * PP; t += BS) {
+    const int pl = t / PP, o = t - pl * PP, zz = z + pl - 1;
+    const int x = o % W - 1, y = o / W - 1;
+    const bool in = zz >= 0 && zz < d.lz && x >= 0 && x < d.lx && y >= 0 && y < d.ly;
+    const int64_t gi = in ? (int64_t)x + (int64_t)d.lx * y + (int64_t)P * zz : 0;
+    const uint8_t ty = in ? type[gi] : (uint8_t)0;
+    s_typ[pl * kPlanePad + o] = ty;
+    s_kap[pl * kPlanePad + o] = ty != 0 ? kapf(kappa[gi]) : 0.0f;
+    s_phi[pl * kPlanePad + o] = ty != 0 ? phi[gi] : 0.0f;
+    if (pl == 1 && in) s_src[x + d.lx * y] = src[gi];
+  }
+  if (threadIdx.x == 0) go = 1;
+  __syncthreads();
+  // thread t owns in-plane voxel t of each of the plane's four colours
+  double cor[4][8], dgv[4], bv[4];
+  bool has[4], solid[4];
+  int own[4], lds[4];
+#pragma unroll
+  for (int cc = 0; cc < 4; cc++) {
+    const int cx = cc & 1, cy = (cc >> 1) & 1;
+    const int hx = (d.lx - cx + 1) >> 1, hy = (d.ly - cy + 1) >> 1;
+    const int r = threadIdx.x;
+    has[cc] = r < hx * hy;
+    const int x = has[cc] ? 2 * (r % hx) + cx : 0, y = has[cc] ? 2 * (r / hx) + cy : 0;
+    const int o = x + d.lx * y, lo = kPlanePad + (x + 1) + W * (y + 1);
+    own[cc] = o;
+    lds[cc] = lo;
+    const float ki = s_kap[lo];
+    double dg = 0.0;
+#pragma unroll
+    for (int k = 0; k < 26; k++) {
+      const int li = lo + O[k][2] * kPlanePad + O[k][0] + W * O[k][1];
+      const bool use = s_typ[li] != 0;
+      const float kj = s_kap[li];
+      const double a = couple_exact(k, ki, kj);
+      dg = use ? dg + a : dg;
+      if (k < 6) {
+        if (has[cc]) s_face[k * kPlaneVox + o] = use ? ki + kj : 0.0f;
+      } else if (k < 18) {
+        if (has[cc]) s_edge[(k - 6) * kPlaneVox + o] = use ? a : 0.0;
+      } else {
+        cor[cc][k - 18] = use ? a : 0.0;
+      }
+    }
+    dgv[cc] = dg;
+    bv[cc] = has[cc] ? (double)s_src[o] : 0.0;
+    solid[cc] = is_solid(s_typ[lo]);
+  }
+  uint64_t* mine = words + (int64_t)P * z;
+  for (int o = threadIdx.x; o < P; o += BS) {
+    const float v = s_phi[kPlanePad + (o % d.lx + 1) + W * (o / d.lx + 1)];
+    __hip_atomic_store(mine + o, (uint64_t)__float_as_uint(v) | (uint64_t(1) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  for (int s = 0; s < sweeps; s++) {
+    const uint32_t want = (uint32_t)(s + 1 + (z & 1));
+    uint64_t w[kPlaneWords];
+    uint32_t pending = 0;
+#pragma unroll
+    for (int q = 0; q < kPlaneWords; q++) {
+      const int t = threadIdx.x + q * BS, h = t >= P, o = t - h * P, zz = z + (h ? 1 : -1);
+      const bool need = t < 2 * P && zz >= 0 && zz < d.lz;
+      w[q] = need ? __hip_atomic_load(words + (int64_t)P * zz + o, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT)
+                  : (uint64_t)want << 32;
+      pending |= (uint32_t)(w[q] >> 32) != want ? 1u << q : 0u;
+    }
+    unsigned spins = 0;
+    while (pending) {
+      if (!rbgs_spin(&spins, abort)) {
+        go = 0;
+        break;
+      }
+#pragma unroll
+      for (int q = 0; q < kPlaneWords; q++) {
+        if (!((pending >> q) & 1u)) continue;
+        const int t = threadIdx.x + q * BS, h = t >= P, o = t - h * P, zz = z + (h ? 1 : -1);
+        w[q] = __hip_atomic_load(words + (int64_t)P * zz + o, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(w[q] >> 32) == want) pending &= ~(1u << q);
+      }
+    }
+    __syncthreads();
+    if (!go) return;
+#pragma unroll
+    for (int q = 0; q < kPlaneWords; q++) {
+      const int t = threadIdx.x + q * BS, h = t >= P, o = t - h * P, zz = z + (h ? 1 : -1);
+      if (t < 2 * P && zz >= 0 && zz < d.lz) {
+        const int li = (h ? 2 : 0) * kPlanePad + (o % d.lx + 1) + W * (o / d.lx + 1);
+        if (s_typ[li] != 0) s_phi[li] = __uint_as_float((uint32_t)w[q]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) {
+      if (has[cc]) {
+        const int o = own[cc], lo = lds[cc];
+        double off = 0.0;
+#pragma unroll
+        for (int k = 0; k < 26; k++) {
+          const float pn = s_phi[lo + O[k][2] * kPlanePad + O[k][0] + W * O[k][1]];
+          double a;
+          if (k < 6)
+            a = 0.5 * (double)s_face[k * kPlaneVox + o];
+          else if (k < 18)
+            a = s_edge[(k - 6) * kPlaneVox + o];
+          else
+            a = cor[cc][k - 18];
+          off = off + a * (double)pn;
+        }
+        const double prev = (double)s_phi[lo];
+        const double pj = (dgv[cc] > 1e-15) ? (bv[cc] + off) / dgv[cc] : prev;
+        const float v = (float)(1.0 * (pj - prev) + prev);
+        s_phi[lo] = solid[cc] ? v : 0.0f;
+      }
+      __syncthreads();
+    }
+    const uint64_t tag = (uint64_t)(uint32_t)(s + 2) << 32;
+    for (int o = threadIdx.x; o < P; o += BS) {
+      const float v = s_phi[kPlanePad + (o % d.lx + 1) + W * (o / d.lx + 1)];
+      __hip_atomic_store(mine + o, (uint64_t)__float_as_uint(v) | tag, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      if (s + 1 == sweeps) phi[(int64_t)o + (int64_t)P * z] = v;
+    }
+  }
+}
+
+// Table form, for larger planes: the couplings live in a device table.
+// Voxel (x, y, z) of colour c = (x&1) | (y&1)<<1 | (z&1)<<2 sits at
+// colour_base(c) + (x>>1) + hx_c * ((y>>1) + hy_c * (z>>1)): a colour pass
+// over a plane reads its table rows contiguously.
+__host__ __device__ inline int64_t colour_base(pfsf_dims d, int c) {
+  int64_t b = 0;
+  for (int k = 0; k < c; k++) {
+    const int cx = k & 1, cy = (k >> 1) & 1, cz = (k >> 2) & 1;
+    b += (int64_t)((d.lx - cx + 1) >> 1) * ((d.ly - cy + 1) >> 1) * ((d.lz - cz + 1) >> 1);
+  }
+  return b;
+}
+
+// Per voxel, at its colour-major position: the 26 couplings (couple_exact in
+// stencil order, 0 where the neighbour is outside the grid or air), the
+// ordered diagonal, a mask (bit k: neighbour k used; bit 26: solid) and the
+// source.
+__global__ void __launch_bounds__(BS)
+    k_rbgs_table(pfsf_dims d, const uint8_t* __restrict__ type, const float* __restrict__ kappa,
+                 const float* __restrict__ src, double* __restrict__ coef,
+                 double* __restrict__ diag, uint32_t* __restrict__ mask,
+                 float* __restrict__ srcc) {
   constexpr int O[26][3] = PFSF_OFFSETS;
   const int64_t n = nvox(d);
   const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   const XYZ p = unflat(i, d);
+  const int c = (p.x & 1) | ((p.y & 1) << 1) | ((p.z & 1) << 2);
+  const int hx = (d.lx - (p.x & 1) + 1) >> 1, hy = (d.ly - (p.y & 1) + 1) >> 1;
+  const int64_t o = colour_base(d, c) + (p.x >> 1) + (int64_t)hx * ((p.y >> 1) + (int64_t)hy * (p.z >> 1));
   const float ki = kapf(kappa[i]);
   double dg = 0.0;
-  uint32_t m = 0;
+  uint32_t m = is_solid(type[i]) ? 1u << 26 : 0u;
 #pragma unroll
   for (int k = 0; k < 26; k++) {
     const int nx = p.x + O[k][0], ny = p.y + O[k][1], nz = p.z + O[k][2];
@@ -160,41 +364,24 @@ __global__ void __launch_bounds__(BS)
     const bool use = in && type[j] != 0;
     const double a = couple_exact(k, ki, kapf(kappa[j]));
     dg = use ? dg + a : dg;
-    m |= use ? (1u << k) : 0u;
-    coef[k * n + i] = use ? a : 0.0;
+    m |= use ? 1u << k : 0u;
+    coef[k * n + o] = use ? a : 0.0;
   }
-  diag[i] = dg;
-  mask[i] = m;
+  diag[o] = dg;
+  mask[o] = m;
+  srcc[o] = src[i];
 }
 
-// Work-group g owns z-planes [g * ppw, (g + 1) * ppw).  Colours 0-3 lie on
-// even planes and read only odd-plane neighbours across planes (4-7 the
-// reverse), so a sweep is two phases: even planes (colours 0..3 in order,
-// a work-group barrier between colours), then odd planes.  Before phase p a
-// work-group waits until both slab neighbours have finished phase p - 1 (it
-// then reads their newest values and they no longer read the planes it
-// overwrites); after it, it publishes p + 1.  Every wait is bounded: a wait
-// that runs out raises *abort, and every work-group then stops.
-constexpr unsigned kRbgsSpinLimit = 1u << 25;
-
-__device__ __forceinline__ bool rbgs_wait(const unsigned* flag, unsigned want, unsigned* abort) {
-  unsigned spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < want) {
-    if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    if (++spins > kRbgsSpinLimit) {
-      __hip_atomic_store(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  return true;
-}
-
+// Table form: work-group g owns planes [g * ppw, (g + 1) * ppw).  Before
+// phase p (p = 2 * sweep + parity) it waits until both slab neighbours have
+// published p (so it reads their newest planes and they no longer read the
+// planes it overwrites); after it, it publishes p + 1.  The neighbour phi
+// addresses come from the coordinates alone, so all loads of a voxel issue
+// at once.
 __global__ void __launch_bounds__(BS)
-    k_rbgs_persistent(pfsf_dims d, const uint8_t* __restrict__ type,
-                      const double* __restrict__ coef, const double* __restrict__ diag,
-                      const uint32_t* __restrict__ mask, const float* __restrict__ src,
-                      float* phi, int sweeps, int ppw, unsigned* flags, unsigned* abort) {
+    k_rbgs_slabs(pfsf_dims d, const double* __restrict__ coef, const double* __restrict__ diag,
+                 const uint32_t* __restrict__ mask, const float* __restrict__ srcc, float* phi,
+                 int sweeps, int ppw, unsigned* flags, unsigned* abort) {
   constexpr int O[26][3] = PFSF_OFFSETS;
   __shared__ int go;
   const int g = (int)blockIdx.x, G = (int)gridDim.x;
@@ -205,8 +392,13 @@ __global__ void __launch_bounds__(BS)
       const unsigned p = 2u * (unsigned)s + (unsigned)half;
       if (threadIdx.x == 0) {
         bool ok = true;
-        if (g > 0) ok = rbgs_wait(flags + g - 1, p, abort);
-        if (ok && g + 1 < G) ok = rbgs_wait(flags + g + 1, p, abort);
+        unsigned spins = 0;
+        while (ok && g > 0 &&
+               __hip_atomic_load(flags + g - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < p)
+          ok = rbgs_spin(&spins, abort);
+        while (ok && g + 1 < G &&
+               __hip_atomic_load(flags + g + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < p)
+          ok = rbgs_spin(&spins, abort);
         go = ok;
       }
       __syncthreads();
@@ -218,32 +410,31 @@ __global__ void __launch_bounds__(BS)
         const int cx = cc & 1, cy = (cc >> 1) & 1;
         const int hx = (d.lx - cx + 1) >> 1, hy = (d.ly - cy + 1) >> 1;
         const int per = hx * hy;
+        const int64_t cb = colour_base(d, cc + 4 * half);
         for (int t = threadIdx.x; t < per * nzp; t += BS) {
           const int pl = t / per, r = t - pl * per;
-          XYZ q;
-          q.x = 2 * (r % hx) + cx;
-          q.y = 2 * (r / hx) + cy;
-          q.z = zs + 2 * pl;
-          const int64_t i = flat(q.x, q.y, q.z, d);
-          if (!is_solid(type[i])) {
-            phi[i] = 0.0f;
-            continue;
-          }
-          const uint32_t m = mask[i];
-          double off = 0.0;
+          const int x = 2 * (r % hx) + cx, y = 2 * (r / hx) + cy, z = zs + 2 * pl;
+          const int64_t i = flat(x, y, z, d);
+          const int64_t o = cb + r + (int64_t)per * (z >> 1);
+          const uint32_t m = mask[o];
+          double a[26];
+          float pk[26];
 #pragma unroll
           for (int k = 0; k < 26; k++) {
-            const bool use = (m >> k) & 1u;
-            const int64_t j = use ? flat(q.x + O[k][0], q.y + O[k][1], q.z + O[k][2], d) : i;
-            const double a = coef[k * n + i];
-            const float pk = phi[j];
-            off = use ? off + a * (double)pk : off;
+            const int nx = x + O[k][0], ny = y + O[k][1], nz = z + O[k][2];
+            const bool in = nx >= 0 && nx < d.lx && ny >= 0 && ny < d.ly && nz >= 0 && nz < d.lz;
+            a[k] = coef[k * n + o];
+            pk[k] = phi[in ? flat(nx, ny, nz, d) : i];
           }
-          const double dg = diag[i];
+          const double dg = diag[o];
           const double prev = (double)phi[i];
-          const double b = (double)src[i];
+          const double b = (double)srcc[o];
+          double off = 0.0;
+#pragma unroll
+          for (int k = 0; k < 26; k++) off = (m >> k) & 1u ? off + a[k] * (double)pk[k] : off;
           const double pj = (dg > 1e-15) ? (b + off) / dg : prev;
-          phi[i] = (float)(1.0 * (pj - prev) + prev);
+          const float v = (float)(1.0 * (pj - prev) + prev);
+          phi[i] = (m >> 26) & 1u ? v : 0.0f;
         }
         __syncthreads();
       }
@@ -846,10 +1037,10 @@ static void rbgs_sweep(pfsf_dims d, const uint8_t* type, const float* kappa, con
   }
 }
 
-// Runs of kRbgsGraphSweeps sweeps are replayed from a HIP graph holding
-// exactly those colour launches, captured once per device and argument set on
-// a private stream: one graph launch replaces 64 host launches, which bound
-// the sweep rate of small grids.  The kernels, their arguments and their
+// Grids beyond both single-launch forms replay runs of kRbgsGraphSweeps
+// sweeps from a HIP graph holding exactly those colour launches, captured
+// once per device and argument set on a private stream: one graph launch
+// replaces 64 host launches.  The kernels, their arguments and their
 // order are those of the direct launches, so the iterates are the same bits.
 // The last kRbgsGraphCache argument sets stay instantiated.
 constexpr int kRbgsGraphSweeps = 8;
@@ -910,76 +1101,123 @@ static hipError_t rbgs_graph(pfsf_dims d, const uint8_t* type, const float* kapp
   return hipSuccess;
 }
 
-// Persistent form: up to kRbgsPersistMax voxels (the coupling table takes
-// 220 B per voxel), at least kRbgsGraphSweeps sweeps, a stream not under
-// capture.  Work-groups own one or more whole z-planes (at most one per CU).
-constexpr int64_t kRbgsPersistMax = int64_t(1) << 21;
+// Single-launch forms (stream not under capture).  The plane form takes
+// planes of at most kPlaneSide x kPlaneSide and no more planes than it can
+// keep resident (its LDS allows one work-group per CU), with an 8 B word per
+// voxel as workspace; the table form takes up to kRbgsTableMax voxels (its
+// table holds 224 B per voxel) in work-groups of one or more whole planes,
+// at most one per CU.  Both launch cooperatively, so the launch fails rather
+// than start work-groups that could not all be resident, and both
+// synchronise the stream to read the abort word.
+constexpr int64_t kRbgsTableMax = int64_t(1) << 21;
 
-static hipError_t rbgs_persistent(pfsf_dims d, const uint8_t* type, const float* kappa,
-                                  const float* src, float* phi, int sweeps, hipStream_t s) {
-  int dev = 0, cus = 0;
+static hipError_t device_cus(int* cus) {
+  int dev = 0;
   hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
-  const int ppw = (d.lz + std::max(cus, 1) - 1) / std::max(cus, 1);
-  const int G = (d.lz + ppw - 1) / ppw;
-  const int64_t n = nvox(d);
-  const size_t bytes = sizeof(double) * 27 * (size_t)n + sizeof(uint32_t) * (size_t)n +
-                       sizeof(unsigned) * (size_t)(G + 1);
-  char* ws = nullptr;
-  e = hipMallocAsync((void**)&ws, bytes, s);
-  if (e != hipSuccess) return e;
-  double* coef = (double*)ws;
-  double* diag = coef + 26 * n;
-  uint32_t* mask = (uint32_t*)(diag + n);
-  unsigned* flags = (unsigned*)(mask + n);
-  unsigned* abort = flags + G;
-  k_rbgs_coef<<<blocks_for(n, BS), BS, 0, s>>>(d, type, kappa, coef, diag, mask);
-  k_zero_u32<<<(G + 1 + 255) / 256, 256, 0, s>>>(flags, G + 1);
-  e = hipGetLastError();
-  if (e == hipSuccess) {
-    const uint8_t* t = type;
-    const double *cf = coef, *dg = diag;
-    const uint32_t* mk = mask;
-    const float* sr = src;
-    float* ph = phi;
-    int sw = sweeps, pw = ppw;
-    unsigned *fl = flags, *ab = abort;
-    void* args[] = {&d, &t, &cf, &dg, &mk, &sr, &ph, &sw, &pw, &fl, &ab};
-    // cooperative: the launch fails rather than start work-groups that could
-    // not all be resident (each waits for its neighbours)
-    e = hipLaunchCooperativeKernel((const void*)k_rbgs_persistent, dim3(G), dim3(BS), args, 0,
-                                   s);
-  }
+  if (e == hipSuccess) e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+  return e;
+}
+
+static hipError_t finish_single_launch(hipError_t e, char* ws, const unsigned* abort,
+                                       hipStream_t s) {
   unsigned ab_host = 0;
-  if (e == hipSuccess) e = hipMemcpyAsync(&ab_host, abort, sizeof(unsigned),
-                                          hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&ab_host, abort, sizeof(unsigned), hipMemcpyDeviceToHost, s);
   (void)hipFreeAsync(ws, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e == hipSuccess && ab_host) e = hipErrorLaunchTimeOut;
   return e;
 }
 
+static bool rbgs_planes_fit(pfsf_dims d, int cus) {
+  if (d.lx > kPlaneSide || d.ly > kPlaneSide) return false;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbgs_planes, BS, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return (int64_t)d.lz <= (int64_t)per_cu * cus;
+}
+
+static hipError_t rbgs_planes(pfsf_dims d, const uint8_t* type, const float* kappa,
+                              const float* src, float* phi, int sweeps, hipStream_t s) {
+  const int64_t n = nvox(d);
+  char* ws = nullptr;
+  hipError_t e = hipMallocAsync((void**)&ws, sizeof(uint64_t) * (size_t)n + 256, s);
+  if (e != hipSuccess) return e;
+  uint64_t* words = (uint64_t*)ws;
+  unsigned* abort = (unsigned*)(words + n);
+  k_zero_u32<<<blocks_for(2 * n + 1, 256), 256, 0, s>>>((unsigned*)ws, (int)(2 * n + 1));
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    const uint8_t* t = type;
+    const float *k = kappa, *b = src;
+    float* ph = phi;
+    int sw = sweeps;
+    uint64_t* wd = words;
+    unsigned* ab = abort;
+    void* args[] = {&d, &t, &k, &b, &ph, &sw, &wd, &ab};
+    e = hipLaunchCooperativeKernel((const void*)k_rbgs_planes, dim3(d.lz), dim3(BS), args, 0, s);
+  }
+  return finish_single_launch(e, ws, abort, s);
+}
+
+static hipError_t rbgs_table(pfsf_dims d, const uint8_t* type, const float* kappa,
+                             const float* src, float* phi, int sweeps, int cus, hipStream_t s) {
+  const int ppw = (d.lz + std::max(cus, 1) - 1) / std::max(cus, 1);
+  const int G = (d.lz + ppw - 1) / ppw;
+  const int64_t n = nvox(d);
+  const size_t bytes = sizeof(double) * 27 * (size_t)n + sizeof(uint32_t) * (size_t)n +
+                       sizeof(float) * (size_t)n + sizeof(unsigned) * (size_t)(G + 1);
+  char* ws = nullptr;
+  hipError_t e = hipMallocAsync((void**)&ws, bytes, s);
+  if (e != hipSuccess) return e;
+  double* coef = (double*)ws;
+  double* diag = coef + 26 * n;
+  uint32_t* mask = (uint32_t*)(diag + n);
+  float* srcc = (float*)(mask + n);
+  unsigned* flags = (unsigned*)(srcc + n);
+  unsigned* abort = flags + G;
+  k_rbgs_table<<<blocks_for(n, BS), BS, 0, s>>>(d, type, kappa, src, coef, diag, mask, srcc);
+  k_zero_u32<<<(G + 1 + 255) / 256, 256, 0, s>>>(flags, G + 1);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    const double *cf = coef, *dg = diag;
+    const uint32_t* mk = mask;
+    const float* sr = srcc;
+    float* ph = phi;
+    int sw = sweeps, pw = ppw;
+    unsigned *fl = flags, *ab = abort;
+    void* args[] = {&d, &cf, &dg, &mk, &sr, &ph, &sw, &pw, &fl, &ab};
+    e = hipLaunchCooperativeKernel((const void*)k_rbgs_slabs, dim3(G), dim3(BS), args, 0, s);
+  }
+  return finish_single_launch(e, ws, abort, s);
+}
+
 hipError_t launch_rbgs_exact(pfsf_dims d, const uint8_t* type, const float* kappa,
                              const float* src, float* phi, int sweeps, hipStream_t s) {
+  if (sweeps <= 0) return hipSuccess;
+  bool capturing = true;  // a stream under capture takes the direct launches
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) == hipSuccess)
+    capturing = cap != hipStreamCaptureStatusNone;
+  else
+    (void)hipGetLastError();
   int sw = 0;
-  bool replay = false;  // a stream under capture takes the direct launches
-  if (sweeps >= kRbgsGraphSweeps) {
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cap) == hipSuccess)
-      replay = cap == hipStreamCaptureStatusNone;
-    else
-      (void)hipGetLastError();
-  }
-  if (replay && nvox(d) <= kRbgsPersistMax)
-    return rbgs_persistent(d, type, kappa, src, phi, sweeps, s);
-  if (replay) {
-    hipGraphExec_t exec = nullptr;
-    hipError_t err = rbgs_graph(d, type, kappa, src, phi, &exec);
+  if (!capturing) {
+    int cus = 0;
+    hipError_t err = device_cus(&cus);
     if (err != hipSuccess) return err;
-    for (; sw + kRbgsGraphSweeps <= sweeps; sw += kRbgsGraphSweeps) {
-      err = hipGraphLaunch(exec, s);
+    if (rbgs_planes_fit(d, cus)) return rbgs_planes(d, type, kappa, src, phi, sweeps, s);
+    if (nvox(d) <= kRbgsTableMax) return rbgs_table(d, type, kappa, src, phi, sweeps, cus, s);
+    if (sweeps >= kRbgsGraphSweeps) {
+      hipGraphExec_t exec = nullptr;
+      err = rbgs_graph(d, type, kappa, src, phi, &exec);
       if (err != hipSuccess) return err;
+      for (; sw + kRbgsGraphSweeps <= sweeps; sw += kRbgsGraphSweeps) {
+        err = hipGraphLaunch(exec, s);
+        if (err != hipSuccess) return err;
+      }
     }
   }
   for (; sw < sweeps; sw++) rbgs_sweep(d, type, kappa, src, phi, s);

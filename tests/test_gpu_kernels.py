@@ -89,6 +89,68 @@ def test_dense32_rbgs_200_sweeps_and_scan_bit_exact(cuda):
     assert np.array_equal(code.cpu().numpy(), O.failure_scan(isl, want))
 
 
+def _random_island(shape, seed, air=0.2):
+    """Random (lz, ly, lx) grid: air, anchors on part of z = 0, kappa with
+    zeros (and non-zero values on air, which no coupling may use), a source,
+    and a random starting phi everywhere (NaN on some air voxels), which the
+    sweeps must skip or overwrite."""
+    rng = np.random.default_rng(seed)
+    vt = np.where(rng.random(shape) < air, W.AIR, W.SOLID).astype(np.uint8)
+    vt[0][rng.random(shape[1:]) < 0.5] = W.ANCHOR
+    zeros = np.zeros(shape, np.float32)
+    isl = W._finish(vt, zeros, rng.random(shape).astype(np.float32), zeros, zeros, "random")
+    kappa = rng.uniform(0.5, 40.0, shape).astype(np.float32)
+    kappa[rng.random(shape) < 0.03] = 0.0
+    isl.kappa = kappa
+    phi0 = (rng.random(shape) * 2 - 0.5).astype(np.float32)
+    phi0[(vt == W.AIR) & (rng.random(shape) < 0.3)] = np.nan
+    return isl, phi0
+
+
+# (lz, ly, lx) and sweeps, named for the form pfsf_hip_rbgs runs them in
+RBGS_FORMS = [("planes-odd", (9, 17, 31), 11),  # planes of at most 32 x 32
+              ("planes-flat", (1, 20, 20), 9),
+              ("planes-thin", (5, 32, 1), 9),
+              ("planes-deep", (40, 1, 32), 3),
+              ("table-wide", (10, 12, 40), 9),  # planes wider than 32
+              ("table-tall", (8, 33, 12), 9),
+              ("table-many", (300, 16, 16), 5),  # more planes than CUs
+              ("graph", (130, 128, 128), 9)]  # beyond 2^21 voxels: replays + direct
+
+
+@pytest.mark.parametrize("name,shape,sweeps", RBGS_FORMS)
+def test_rbgs_forms_bit_exact(cuda, name, shape, sweeps):
+    isl, phi0 = _random_island(shape, sum(shape))
+    d = _isl_dev(isl, cuda)
+    phi = torch.from_numpy(phi0).to(cuda)
+    D.rbgs(d["vtype"], d["kappa"], d["source"], phi, sweeps)
+    O.set_threads(8)
+    try:
+        want = O.rbgs(isl, phi=phi0, sweeps=sweeps)
+    finally:
+        O.set_threads(1)
+    assert np.array_equal(phi.cpu().numpy().view(np.uint32), want.view(np.uint32)), name
+
+
+def test_rbgs_under_stream_capture_bit_exact(cuda):
+    """On a stream under capture the sweeps are recorded as direct colour
+    launches; every replay runs them again on the same buffers."""
+    isl, phi0 = _random_island((6, 10, 14), 21)
+    d = _isl_dev(isl, cuda)
+    phi = torch.from_numpy(phi0).to(cuda)
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        D.rbgs(d["vtype"], d["kappa"], d["source"], phi, 3)
+    want = phi0
+    for rep in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        want = O.rbgs(isl, phi=want, sweeps=3)
+        assert np.array_equal(phi.cpu().numpy().view(np.uint32), want.view(np.uint32)), rep
+        d["source"].mul_(1.0)  # eager work between replays
+
+
 @pytest.mark.parametrize("name,mk", CASES)
 def test_failure_scan_bit_exact(cuda, name, mk):
     isl = mk()
