@@ -110,7 +110,7 @@ def cpu_baseline(isl, sample_iters=40):
     return all_cores, one_core
 
 
-def batch_bench(torch, D, W, steps=3, plain_stride=16):
+def batch_bench(torch, D, W, steps=3, plain_stride=16, coarse_sweeps=0):
     """Batched solve of the 4,096-island batch against plain per-island solves.
 
     Islands/s = 4096 / (batched solve time); the plain rate comes from solving
@@ -130,7 +130,7 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16):
     n_act = int(sum(i.n_active for i in isls))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bat = D.BatchSolver(list(zip(vt, kp)), tol=1e-6, max_iters=2000)
+    bat = D.BatchSolver(list(zip(vt, kp)), tol=1e-6, max_iters=2000, coarse_sweeps=coarse_sweeps)
     setup_s = time.perf_counter() - t0
     log(f"batch: hierarchies built in {setup_s:.1f} s")
     bat.solve(bs, None, phis)  # warm-up (graph capture)
@@ -164,7 +164,7 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16):
     sample = order[::plain_stride]
     t_plain, it_plain = 0.0, 0
     for k in sample:
-        s = D.Solver(vt[k], kp[k], tol=1e-6, max_iters=2000)
+        s = D.Solver(vt[k], kp[k], tol=1e-6, max_iters=2000, coarse_sweeps=coarse_sweeps)
         x = torch.zeros(isls[k].vtype.shape, dtype=torch.float64, device=dev)
         s.solve(bs[k], x)  # warm-up (graph capture)
         torch.cuda.synchronize()
@@ -195,7 +195,7 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16):
                                  "64..262144, 1-8 anchors each, cold start"}
 
 
-def mixed_precision_bench(torch, D, W, steps=3):
+def mixed_precision_bench(torch, D, W, steps=3, coarse_sweeps=0):
     """The 256^3 island (kappa lognormal sigma 1.5, ~70 % occupied): time to
     1e-6 with FP32 coarse levels and with BF16 coarse kappa; phi against a
     1e-11 solve of the same system, and the failure codes of both."""
@@ -218,7 +218,8 @@ def mixed_precision_bench(torch, D, W, steps=3):
     out = {"grid": [isl.lx, isl.ly, isl.lz], "active_voxels": isl.n_active, "tol": 1e-6,
            "reference": f"same system solved to {iref.rel_residual:.2e} (FP64 residual)"}
     for prec, name in ((0, "fp32"), (1, "bf16_coarse")):
-        s = D.Solver(vt, kp, precision=prec, tol=1e-6, max_iters=2000)
+        s = D.Solver(vt, kp, precision=prec, tol=1e-6, max_iters=2000,
+                     coarse_sweeps=coarse_sweeps)
         x = torch.zeros_like(xr)
         ph = torch.empty_like(pr)
         s.solve(b, x, ph)  # warm-up (graph capture)
@@ -512,7 +513,8 @@ def main():
     ap.add_argument("--levels", type=int, default=0, help="MG levels (0 = auto hierarchy)")
     ap.add_argument("--precision", type=int, default=0, help="0 FP32, 1 BF16 coarse kappa")
     ap.add_argument("--coarse-sweeps", type=int, default=0,
-                    help="Chebyshev steps on the coarsest level (0 = solver default)")
+                    help="Chebyshev steps on the coarsest level of the 128^3, 256^3 and "
+                         "batched solves (0 = solver default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-island batch")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 mixed-precision record")
@@ -540,7 +542,8 @@ def main():
     from pfsf_amd import workloads as W
 
     if args.batch_only:
-        print(json.dumps({"batched": batch_bench(torch, D, W, steps=args.steps)}))
+        print(json.dumps({"batched": batch_bench(torch, D, W, steps=args.steps,
+                                                 coarse_sweeps=args.coarse_sweeps)}))
         return
 
     isl = W.workload(1)
@@ -654,9 +657,10 @@ def main():
         out["host_buffer_tick_128"] = host_tick_bench(isl)
         out["host_buffer_tick_256_islands"] = multi_island_tick_bench(W)
     if not args.no_batch:
-        out["batched"] = batch_bench(torch, D, W)
+        out["batched"] = batch_bench(torch, D, W, coarse_sweeps=args.coarse_sweeps)
     if not args.no_256:
-        out["mixed_precision_256"] = mixed_precision_bench(torch, D, W)
+        out["mixed_precision_256"] = mixed_precision_bench(torch, D, W,
+                                                           coarse_sweeps=args.coarse_sweeps)
     if not args.no_adjoint:
         out["adjoint_96"] = adjoint_bench(torch, W)
     if not args.no_labels:
