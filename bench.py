@@ -128,22 +128,36 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16, coarse_sweeps=0):
     bs = [torch.from_numpy(i.source).to(dev) for i in isls]
     phis = [torch.empty(i.vtype.shape, dtype=torch.float32, device=dev) for i in isls]
     n_act = int(sum(i.n_active for i in isls))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bat = D.BatchSolver(list(zip(vt, kp)), tol=1e-6, max_iters=2000, coarse_sweeps=coarse_sweeps)
-    setup_s = time.perf_counter() - t0
-    log(f"batch: hierarchies built in {setup_s:.1f} s")
-    bat.solve(bs, None, phis)  # warm-up (graph capture)
-    times, infos = [], []
-    for _ in range(steps):
+    from pfsf_amd import capi
+
+    def run(shape, precision, label):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        info, per = bat.solve(bs, None, phis, per_island=True)
-        infos.append(info)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        log(f"batch: solve {times[-1] * 1e3:.1f} ms, max iterations {infos[-1].max_iterations}")
-    bat.close()
+        bat = D.BatchSolver(list(zip(vt, kp)), tol=1e-6, max_iters=2000,
+                            coarse_sweeps=coarse_sweeps, shape=shape, precision=precision)
+        setup = time.perf_counter() - t0
+        log(f"batch {label}: hierarchies built in {setup:.1f} s")
+        classes = bat.classes()
+        tuning = bat.tuning()
+        bat.solve(bs, None, phis)  # warm-up (graph capture)
+        ts, inf = [], []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            i, per = bat.solve(bs, None, phis, per_island=True)
+            inf.append(i)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            log(f"batch {label}: solve {ts[-1] * 1e3:.1f} ms, max iterations {i.max_iterations}")
+        bat.close()
+        return setup, (classes, tuning), ts, inf, per
+
+    # the headline batch: tile shapes picked per island-size class by the tuner
+    setup_s, (classes, tuning), times, infos, per = run(capi.BATCH_AUTOTUNE, 0, "tuned")
+    # beside it: every island on the default shape, and BF16 coarse kappa
+    # on the tuned shapes
+    _, _, t_def, i_def, _ = run(0, 0, "default shape")
+    s_bf, c_bf, t_bf, i_bf, _ = run(capi.BATCH_AUTOTUNE, 1, "bf16 coarse")
     t_batch = float(np.median(times))
     info = infos[-1]
     dev_ms = float(np.median([i.solve_ms for i in infos]))
@@ -188,6 +202,19 @@ def batch_bench(torch, D, W, steps=3, plain_stride=16, coarse_sweeps=0):
         "mean_iterations": round(info.total_iterations / info.islands, 1),
         "graph_sweeps": info.sweeps, "max_rel_residual": info.max_rel_residual,
         "device_ms": round(info.solve_ms, 2), "roofline": roof,
+        "shapes": {"tuned_setup_s": round(setup_s, 2), "classes": classes, "decision": tuning,
+                   "candidates": "fine column depth x coarse tile depth; ms = the class "
+                                 "sample's 16-iteration solve with that shape",
+                   "default_8x2_batch_ms": round(float(np.median(t_def)) * 1e3, 2),
+                   "default_8x2_max_iterations": i_def[-1].max_iterations},
+        "bf16_coarse": {"batch_ms": round(float(np.median(t_bf)) * 1e3, 2),
+                        "islands_per_s": round(len(isls) / float(np.median(t_bf)), 1),
+                        "speedup_vs_plain_fp32": round(t_plain_all / float(np.median(t_bf)), 2),
+                        "converged": i_bf[-1].converged,
+                        "max_iterations": i_bf[-1].max_iterations,
+                        "max_rel_residual": i_bf[-1].max_rel_residual,
+                        "setup_s": round(s_bf, 2),
+                        "tuned": [[c["tz_fine"], c["tz_coarse"]] for c in c_bf[0]]},
         "plain_sample": f"{len(sample)} islands (every {plain_stride}th by voxel count), "
                         f"{t_plain * 1e3:.1f} ms summed, {it_plain} PCG iterations, scaled "
                         f"x{plain_stride}",

@@ -304,7 +304,29 @@ typedef struct {
   double tol;                       /* per-island ||b - A x|| / ||b|| target */
   int32_t max_iters;                /* cap on PCG iterations per island */
   void* stream;
+  int32_t precision; /* PFSF_PREC_FP32 or PFSF_PREC_BF16_COARSE, for every island */
+  int32_t shape;     /* tile shape candidate 0..PFSF_BATCH_SHAPES-1 for every island (0: the
+                        default), or PFSF_BATCH_AUTOTUNE: timed per island-size class */
 } pfsf_batch_desc;
+
+/* Tile shapes of a batch, per island-size class (cells lx*ly*lz in
+ * [min_cells, max_cells)).  Candidates 0..5, in `ms` order: (finest column
+ * depth, coarse tile depth) = (8,2) (4,2) (16,2) (8,4) (4,4) (16,4); 0 is the
+ * default.  With PFSF_BATCH_AUTOTUNE each class present in the batch times
+ * every candidate on a sample of up to 32 of its islands (a sub-batch built
+ * with that shape, 16 PCG iterations per island after a warm-up solve) and
+ * keeps the fastest. */
+#define PFSF_BATCH_SHAPES 6
+#define PFSF_BATCH_CLASSES 4
+#define PFSF_BATCH_AUTOTUNE (-1)
+typedef struct {
+  int64_t min_cells, max_cells;
+  int32_t islands;    /* islands of the batch in this class */
+  int32_t sample;     /* islands timed per candidate (0: not tuned) */
+  int32_t tz_fine;    /* chosen finest-level column depth (planes) */
+  int32_t tz_coarse;  /* chosen coarse-level tile depth */
+  float ms[PFSF_BATCH_SHAPES]; /* tuning solve time per candidate (0: not timed) */
+} pfsf_batch_class;
 
 typedef struct {
   int32_t islands;
@@ -329,6 +351,28 @@ PFSF_API pfsf_result pfsf_batch_create(const pfsf_batch_desc* desc, pfsf_batch* 
 PFSF_API pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, double* const* x64,
                                       float* const* phi32, pfsf_batch_info* info,
                                       pfsf_solve_info* per_island);
+
+/* The batch's size classes and their tile shapes (PFSF_BATCH_CLASSES
+ * records; at most `cap` written).  Returns the number of classes, or -1 for
+ * a NULL batch. */
+PFSF_API int32_t pfsf_batch_classes(pfsf_batch s, pfsf_batch_class* out, int32_t cap);
+
+/* The tuner's whole-batch decision.  Islands of different tile depths cannot
+ * share a launch, so mixing shapes splits every stage of an iteration into
+ * one launch per depth; per-class timings alone do not predict that.  The
+ * tuner therefore times, on a stratified sample of the batch (islands evenly
+ * spaced in size order, the largest included), one shape for every island
+ * (the default and each class's fastest) and, when the classes disagree,
+ * the per-class mix; an option replaces the default, and the mix the best
+ * single shape, only when it is at least 3 % faster. */
+typedef struct {
+  int32_t sample;  /* islands of the stratified sample (0: not tuned) */
+  int32_t mixed;   /* 1: every class keeps its own fastest shape */
+  float mixed_ms;  /* sample time with the per-class shapes (0: not timed) */
+  float uniform_ms[PFSF_BATCH_SHAPES]; /* sample time with one shape for all (0: not timed) */
+} pfsf_batch_tuning;
+
+PFSF_API pfsf_result pfsf_batch_tuning_info(pfsf_batch s, pfsf_batch_tuning* out);
 
 PFSF_API void pfsf_batch_destroy(pfsf_batch s);
 

@@ -11,7 +11,8 @@ namespace pfsf {
 // One island at one multigrid level, as the batched kernels see it.
 struct BatchLevel {
   pfsf_dims d;        // this level
-  const float* kap;   // kappa on the free set, -1 elsewhere (finest level: anchor-coded, MgLevel::ks)
+  const void* kap;    // kappa on the free set, -1 elsewhere: float, or bf16 on the coarse levels of a
+                      // BF16-coarse batch (finest level: anchor-coded float, MgLevel::ks)
   const float* dg;    // diagonal
   const float* am;    // anchor mass (null on the finest level)
   const uint32_t* msk;  // coupling mask (levels >= 1), null on the finest level
@@ -55,6 +56,18 @@ struct BatchTile {
   int slot;   // partial-sum slot (finest level)
 };
 
+// A run of one launch list whose tiles share one depth (islands of
+// different size classes may have different tile depths): records
+// [off, off + count) of the level's list.
+struct TileGroup {
+  int tz, off, count;
+};
+
+// Tile-shape candidates (finest column depth, coarse tile depth), in the
+// order of pfsf_batch_class::ms; candidate 0 is the default.
+constexpr int kShapeFine[PFSF_BATCH_SHAPES] = {8, 4, 16, 8, 4, 16};
+constexpr int kShapeCoarse[PFSF_BATCH_SHAPES] = {2, 2, 2, 4, 4, 4};
+
 // Per-island outcome of a batched solve.
 struct BatchState {
   double bb;        // ||b||^2 over the free set
@@ -70,6 +83,9 @@ class BatchSolver {
  public:
   ~BatchSolver();
   hipError_t create(const pfsf_batch_desc& desc);
+  // The batch with island i built in tile shape kShape*[shape[i]] (create()
+  // picks the shapes from desc.shape, or by tuning).
+  hipError_t build(const pfsf_batch_desc& desc, const std::vector<int>& shape);
   // A batch over solvers owned by the caller (each built with two or more
   // levels and the batch tile depths, see MgSolver::create(desc, true)); the
   // solvers must outlive the batch, and their vectors are the batch's.
@@ -78,12 +94,17 @@ class BatchSolver {
   hipError_t solve(const float* const* b, double* const* x64, float* const* phi32,
                    pfsf_batch_info* info, pfsf_solve_info* per_island);
   int islands() const { return (int)sol_.size(); }
+  pfsf_batch_class shape_class(int k) const { return classes_[k]; }
+  pfsf_batch_tuning tuning() const { return tuning_; }
 
  private:
+  // times every candidate shape on a sample of each class (`cls`: class per island)
+  hipError_t tune(const pfsf_batch_desc& desc, const std::vector<int>& cls);
   hipError_t build_tables();
   hipError_t iteration(int c);
   hipError_t capture();
   hipError_t select_islands(const int* live);  // tile lists of the live islands only
+  std::vector<BatchTile> grouped(int l, const int* live, std::vector<TileGroup>& g) const;
   hipError_t vcycle_coarse(bool check);
   hipError_t precondition();
   void release();
@@ -101,6 +122,10 @@ class BatchSolver {
   BatchIsland* isl_d_ = nullptr;
   std::vector<BatchTile*> tiles_;  // per level: launch list (live islands)
   std::vector<int> ntiles_;        // per level: launch size
+  std::vector<std::vector<TileGroup>> groups_;  // per level: the list's runs of one depth
+  bool bf16_ = false;              // coarse kappa in BF16 (every island)
+  pfsf_batch_class classes_[PFSF_BATCH_CLASSES] = {};
+  pfsf_batch_tuning tuning_{};
   std::vector<std::vector<BatchTile>> all_tiles_;  // per level: every island's tiles, by island
   std::vector<std::vector<int>> tile_begin_;       // per level: island -> first index (n + 1)
   std::vector<int> listed_;        // islands in the current launch lists
@@ -109,8 +134,9 @@ class BatchSolver {
   int nchunks_ = 0;
   double* pcell_ = nullptr;        // per-chunk partial sums
   double* pres_ = nullptr;         // per-tile partial sums of the FP64 residual
-  BatchTile* all0_ = nullptr;      // every island's finest tiles, XCD order
+  BatchTile* all0_ = nullptr;      // every island's finest tiles, XCD order per depth
   int nall0_ = 0;
+  std::vector<TileGroup> groups0_;  // all0_'s runs of one depth
   int* flags_ = nullptr;           // device: per island 1 while its PCG pass runs
   std::vector<int> flags_host_;
   int nslots_ = 0;

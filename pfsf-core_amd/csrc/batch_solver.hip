@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "batch_solver.h"
@@ -26,8 +27,6 @@
 namespace pfsf {
 namespace {
 
-constexpr int kTZFine = kBatchTZFine;      // tile depth of every island's finest level
-constexpr int kTZCoarse = kBatchTZCoarse;  // tile depth of the coarse levels
 constexpr int kChunk = 1024;  // cells per work-group of the per-cell passes
 // Coarsest-level Chebyshev steps in a batch: a batched sweep costs ~5 ms of
 // fine-level work against ~50 us for the coarse solve, so the steps that
@@ -35,10 +34,11 @@ constexpr int kChunk = 1024;  // cells per work-group of the per-cell passes
 // at most on the 4,096-island batch; 6 steps: 37).
 constexpr int kBatchCoarseSweeps = 12;
 
-__device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
-  Lv<float> L;
+template <typename KT>
+__device__ __forceinline__ Lv<KT> lv_of(const BatchLevel& D) {
+  Lv<KT> L;
   L.d = D.d;
-  L.kap = D.kap;
+  L.kap = (const KT*)D.kap;
   L.dg = D.dg;
   L.am = D.am;
   L.msk = D.msk;
@@ -50,7 +50,7 @@ __device__ __forceinline__ Lv<float> lv_of(const BatchLevel& D) {
 // ── batched stencil launches (one work-group per tile record) ──────────
 // UPDATE (finest level, iteration c): r[c] -> r[1-c] with p[1-c], q.
 // Otherwise the down-sweep of the level's own right-hand side.
-template <int TZ, bool UPDATE, bool MASK = false>
+template <int TZ, typename KT, bool UPDATE, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     kb_down(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
             const BatchIsland* __restrict__ isl, PcgScalars* sc, const double* __restrict__ pq_part,
@@ -61,14 +61,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_body<TZ, float, UPDATE, MASK>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
+  down_body<TZ, KT, UPDATE, MASK>(lv_of<KT>(D), true, t.x0, t.y0, t.z0, D.c, rin, rout, I.q,
                                c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
                                (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
 }
 
 // FINEST: z = M r[ridx] and the r.z partial.  Otherwise the up-sweep of a
 // coarse level into its correction.
-template <int TZ, bool FINEST>
+template <int TZ, typename KT, bool FINEST>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     kb_up(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
           const BatchIsland* __restrict__ isl, const PcgScalars* sc, double* rz_part, int ridx) {
@@ -79,11 +79,11 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
   const float* r = FINEST ? (ridx ? I.r1 : I.r0) : D.r;
   float* out = FINEST ? I.z : D.e;
   // every level below the finest carries a coupling mask
-  up_body<TZ, float, FINEST, !FINEST>(lv_of(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
+  up_body<TZ, KT, FINEST, !FINEST>(lv_of<KT>(D), true, t.x0, t.y0, t.z0, D.c, r, D.ec, out,
                                       sc + D.island, rz_part, (unsigned)t.slot);
 }
 
-// Finest level: the plane-streaming bodies (one 32 x 8 x kTZFine column per
+// Finest level: the plane-streaming bodies (one 32 x 8 x TZ column per
 // tile record): the SpMV with the new direction, the update + down-sweep
 // and the up-sweep with r.z.
 template <int TZ>
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BS)
   if (t.level < 0) return;
   const BatchLevel& D = lv[t.level];
   const BatchIsland& I = isl[D.island];
-  spmv_stream_body<TZ>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, I.z, c ? I.p1 : I.p0,
+  spmv_stream_body<TZ>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, I.z, c ? I.p1 : I.p0,
                        c ? I.p0 : I.p1, I.q, sc + D.island, rz_part + I.tb, (unsigned)I.tn, c,
                        t.slot == I.tb, pq_part, (unsigned)t.slot);
 }
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_stream_body<TZ, UPDATE>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin, rout, I.q,
+  down_stream_body<TZ, UPDATE>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin, rout, I.q,
                                c ? I.p0 : I.p1, I.x, D.rc, sc + D.island, pq_part + I.tb,
                                (unsigned)I.tn, t.slot == I.tb, rr_part, (unsigned)t.slot);
 }
@@ -124,19 +124,20 @@ __global__ void __launch_bounds__(BS)
   if (t.level < 0) return;
   const BatchLevel& D = lv[t.level];
   const BatchIsland& I = isl[D.island];
-  up_stream_body<TZ, true>(lv_of(D), OneColumn{t.x0, t.y0, t.z0}, D.c, ridx ? I.r1 : I.r0, D.ec,
+  up_stream_body<TZ, true>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, ridx ? I.r1 : I.r0, D.ec,
                            I.z, sc + D.island, rz_part, (unsigned)t.slot);
 }
 
 // Every island's coarsest level, one work-group each; with `check` the
 // island's convergence test from its r.r partials runs first.
+template <typename KT>
 __global__ void __launch_bounds__(kMaxLdsCoarse)
     kb_cheb(const BatchLevel* __restrict__ lv, const BatchIsland* __restrict__ isl,
             const float* __restrict__ coef, int steps, PcgScalars* sc,
             const double* __restrict__ rr_part, int check) {
   const BatchIsland& I = isl[blockIdx.x];
   const BatchLevel& D = lv[I.coarsest];
-  cheb_lds_body<float>(lv_of(D), D.r, D.e, coef, steps, sc + blockIdx.x, rr_part + I.tb,
+  cheb_lds_body<KT>(lv_of<KT>(D), D.r, D.e, coef, steps, sc + blockIdx.x, rr_part + I.tb,
                        check ? (unsigned)I.tn : 0u);
 }
 
@@ -178,6 +179,7 @@ __global__ void __launch_bounds__(BS)
 // True FP64 residual of the unfinished islands into their r[0], one finest
 // tile per work-group (every island's tiles, in XCD order), with the ||r||^2
 // partial in the tile's slot.
+template <int TZ>
 __global__ void __launch_bounds__(BS)
     kb_residual_tiles(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
                       const BatchIsland* __restrict__ isl, const BatchState* __restrict__ st,
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   // before the first pass an island without an initial guess has x = 0
   const bool xzero = S.passes == 0 && !I.xio;
-  true_residual_tile<kTZFine>(I.d, true, t.x0, t.y0, t.z0, I.type, I.kappa, I.fr, I.b, I.x,
+  true_residual_tile<TZ>(I.d, true, t.x0, t.y0, t.z0, I.type, I.kappa, I.fr, I.b, I.x,
                               I.r0, part, (unsigned)t.slot, xzero);
 }
 
@@ -319,10 +321,172 @@ static std::vector<BatchTile> xcd_order(const std::vector<BatchTile>& v) {
   return o;
 }
 
+// Island-size classes of the shape tuner (cells lx*ly*lz): [bound k, bound k+1).
+constexpr int64_t kClassBound[PFSF_BATCH_CLASSES + 1] = {0, int64_t(1) << 12, int64_t(1) << 15,
+                                                         int64_t(1) << 18, INT64_MAX};
+constexpr int kTuneSample = 32;  // islands timed per class and candidate
+constexpr int kTuneIters = 16;   // PCG iterations per island of a tuning solve
+
 hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
+  hipError_t err = hipSuccess;
+  const int n = desc.count;
+  std::vector<int> cls(n, 0);
+  for (int k = 0; k < PFSF_BATCH_CLASSES; k++) {
+    pfsf_batch_class& C = classes_[k];
+    C = pfsf_batch_class{};
+    C.min_cells = kClassBound[k];
+    C.max_cells = kClassBound[k + 1];
+    const int c0 = desc.shape >= 0 ? desc.shape : 0;
+    C.tz_fine = kShapeFine[c0];
+    C.tz_coarse = kShapeCoarse[c0];
+  }
+  for (int i = 0; i < n; i++) {
+    const int64_t c = nvox(desc.islands[i].dims);
+    int k = 0;
+    while (k + 1 < PFSF_BATCH_CLASSES && c >= kClassBound[k + 1]) k++;
+    cls[i] = k;
+    classes_[k].islands++;
+  }
+  if (desc.shape == PFSF_BATCH_AUTOTUNE && chk(tune(desc, cls), err)) return err;
+  std::vector<int> shape(n, 0);
+  for (int i = 0; i < n; i++) {
+    const pfsf_batch_class& C = classes_[cls[i]];
+    for (int c = 0; c < PFSF_BATCH_SHAPES; c++)
+      if (kShapeFine[c] == C.tz_fine && kShapeCoarse[c] == C.tz_coarse) shape[i] = c;
+  }
+  return build(desc, shape);
+}
+
+// Solve time (ms) of the islands `pick` of desc built with island pick[j]'s
+// shape shape[j]: a right-hand side of ones, a tolerance no solve reaches and
+// kTuneIters PCG iterations per island, timed after a warm-up solve (graph
+// capture).
+static hipError_t time_shapes(const pfsf_batch_desc& desc, const std::vector<int>& pick,
+                              const std::vector<int>& shape, float* ms) {
+  hipError_t err = hipSuccess;
+  const int m = (int)pick.size();
+  std::vector<pfsf_batch_island> sub(m);
+  int64_t cells = 0;
+  for (int j = 0; j < m; j++) {
+    sub[j] = desc.islands[pick[j]];
+    cells += nvox(sub[j].dims);
+  }
+  float* ones = nullptr;
+  if (chk(hipMalloc((void**)&ones, sizeof(float) * (size_t)cells), err)) return err;
+  std::vector<const float*> bp(m);
+  int64_t off = 0;
+  for (int j = 0; j < m; j++) {
+    bp[j] = ones + off;
+    off += nvox(sub[j].dims);
+  }
+  chk(hipMemsetD32((hipDeviceptr_t)ones, 0x3f800000, (size_t)cells), err);  // 1.0f
+  pfsf_batch_desc td = desc;
+  td.count = m;
+  td.islands = sub.data();
+  td.shape = 0;
+  td.tol = 1e-300;
+  td.max_iters = kTuneIters;
+  {
+    BatchSolver t;
+    pfsf_batch_info bi{};
+    if (err == hipSuccess) chk(t.build(td, shape), err);
+    if (err == hipSuccess) chk(t.solve(bp.data(), nullptr, nullptr, &bi, nullptr), err);
+    if (err == hipSuccess) chk(t.solve(bp.data(), nullptr, nullptr, &bi, nullptr), err);
+    *ms = (float)bi.solve_ms;
+  }
+  (void)hipFree(ones);
+  return err;
+}
+
+// Up to `count` of `order`'s islands, evenly spaced, the last included.
+static std::vector<int> evenly(const std::vector<int>& order, int count) {
+  const int m = std::min<int>((int)order.size(), count);
+  std::vector<int> out(m);
+  for (int j = 0; j < m; j++)
+    out[j] = order[m == 1 ? (int)order.size() - 1
+                          : (int)((int64_t)j * ((int64_t)order.size() - 1) / (m - 1))];
+  return out;
+}
+
+// Per class present: every candidate shape on up to kTuneSample of its
+// islands (evenly spaced in size order).  Then the whole-batch decision on a
+// stratified sample of kTuneStrata islands: one shape for every island (the
+// default and each class's fastest) and, when the classes disagree, the
+// per-class mix.  An option replaces the default only when it is faster by
+// kTuneMargin, more than the run-to-run spread of these timings (1-2 %).
+// Measured on the 4,096-island batch: the small classes prefer 4-plane
+// columns in isolation (7-13 % on their samples), but on the stratified
+// sample the mix was 0.2 % and 4-plane columns everywhere 6.5 % slower than
+// 8x2, and on the whole batch the mix took 144.2 ms against 142.6 ms.
+constexpr int kTuneStrata = 256;
+constexpr float kTuneMargin = 0.03f;
+
+hipError_t BatchSolver::tune(const pfsf_batch_desc& desc, const std::vector<int>& cls) {
+  hipError_t err = hipSuccess;
+  const int n = desc.count;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return nvox(desc.islands[a].dims) < nvox(desc.islands[b].dims);
+  });
+  int best[PFSF_BATCH_CLASSES] = {};
+  for (int k = 0; k < PFSF_BATCH_CLASSES && err == hipSuccess; k++) {
+    std::vector<int> mem;
+    for (int i : order)
+      if (cls[i] == k) mem.push_back(i);
+    if (mem.empty()) continue;
+    const std::vector<int> pick = evenly(mem, kTuneSample);
+    for (int c = 0; c < PFSF_BATCH_SHAPES && err == hipSuccess; c++) {
+      chk(time_shapes(desc, pick, std::vector<int>(pick.size(), c), &classes_[k].ms[c]), err);
+      if (classes_[k].ms[c] < classes_[k].ms[best[k]]) best[k] = c;
+    }
+    classes_[k].sample = (int)pick.size();
+  }
+  if (err != hipSuccess) return err;
+  const std::vector<int> strata = evenly(order, kTuneStrata);
+  tuning_ = pfsf_batch_tuning{};
+  tuning_.sample = (int)strata.size();
+  bool agree = true;
+  int first = -1;
+  std::vector<bool> uniform(PFSF_BATCH_SHAPES, false);
+  uniform[0] = true;
+  for (int k = 0; k < PFSF_BATCH_CLASSES; k++)
+    if (classes_[k].sample > 0) {
+      uniform[best[k]] = true;
+      if (first < 0) first = best[k];
+      agree = agree && best[k] == first;
+    }
+  int choice = 0;
+  float choice_ms = INFINITY;
+  for (int c = 0; c < PFSF_BATCH_SHAPES && err == hipSuccess; c++) {
+    if (!uniform[c]) continue;
+    chk(time_shapes(desc, strata, std::vector<int>(strata.size(), c), &tuning_.uniform_ms[c]),
+        err);
+    // candidate 0 (the default) is timed first
+    if (c == 0 || tuning_.uniform_ms[c] < (1.0f - kTuneMargin) * choice_ms) {
+      choice_ms = tuning_.uniform_ms[c];
+      choice = c;
+    }
+  }
+  if (err == hipSuccess && !agree) {
+    std::vector<int> sh(strata.size());
+    for (size_t j = 0; j < strata.size(); j++) sh[j] = best[cls[strata[j]]];
+    chk(time_shapes(desc, strata, sh, &tuning_.mixed_ms), err);
+    tuning_.mixed = tuning_.mixed_ms < (1.0f - kTuneMargin) * choice_ms ? 1 : 0;
+  }
+  for (int k = 0; k < PFSF_BATCH_CLASSES; k++) {
+    const int c = tuning_.mixed && classes_[k].sample > 0 ? best[k] : choice;
+    classes_[k].tz_fine = kShapeFine[c];
+    classes_[k].tz_coarse = kShapeCoarse[c];
+  }
+  return err;
+}
+
+hipError_t BatchSolver::build(const pfsf_batch_desc& desc, const std::vector<int>& shape) {
   hipError_t err = hipSuccess;
   desc_ = desc;
   coarse_sweeps_ = desc.coarse_sweeps > 0 ? std::min(desc.coarse_sweeps, 63) : kBatchCoarseSweeps;
+  bf16_ = desc.precision == PFSF_PREC_BF16_COARSE;
   const int n = desc.count;
   if (chk(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), err)) return err;
   chk(hipEventCreate(&ev0_), err);
@@ -339,12 +503,12 @@ hipError_t BatchSolver::create(const pfsf_batch_desc& desc) {
     sd.dims = bi.dims;
     sd.type = bi.type;
     sd.kappa = bi.kappa;
-    sd.precision = PFSF_PREC_FP32;
+    sd.precision = desc.precision;
     sd.coarse_sweeps = coarse_sweeps_;
     sd.tol = desc.tol;
     sd.max_iters = desc.max_iters;
     isl_[i].reset(new MgSolver());
-    isl_[i]->configure_pooled(sd, kTZFine, kTZCoarse);
+    isl_[i]->configure_pooled(sd, kShapeFine[shape[i]], kShapeCoarse[shape[i]]);
     const MgLevel& C = isl_[i]->level(isl_[i]->levels() - 1);
     if (C.n > kMaxLdsCoarse || padded_cells(C.d) > kMaxLdsPadded) return hipErrorInvalidValue;
   }
@@ -371,12 +535,18 @@ hipError_t BatchSolver::attach(const std::vector<MgSolver*>& solvers, double tol
   desc_.stream = caller;
   if (solvers.empty()) return hipErrorInvalidValue;
   coarse_sweeps_ = std::min(solvers[0]->coarse_sweeps(), 63);
+  if (solvers[0]->levels() < 2) return hipErrorInvalidValue;
+  bf16_ = solvers[0]->level(1).bf16;
+  desc_.precision = bf16_ ? PFSF_PREC_BF16_COARSE : PFSF_PREC_FP32;
   for (MgSolver* m : solvers) {
+    const int tf = m->level(0).tiles.tz;
     if (m->levels() < 2 || m->coarse_sweeps() != solvers[0]->coarse_sweeps() ||
-        m->level(0).tiles.tz != kTZFine)
+        (tf != 4 && tf != 8 && tf != 16))
       return hipErrorInvalidValue;
-    for (int l = 1; l < m->levels(); l++)
-      if (m->level(l).tiles.tz != kTZCoarse) return hipErrorInvalidValue;
+    for (int l = 1; l < m->levels(); l++) {
+      const int tc = m->level(l).tiles.tz;
+      if ((tc != 2 && tc != 4) || m->level(l).bf16 != bf16_) return hipErrorInvalidValue;
+    }
     const MgLevel& C = m->level(m->levels() - 1);
     if (C.n > kMaxLdsCoarse || padded_cells(C.d) > kMaxLdsPadded) return hipErrorInvalidValue;
   }
@@ -414,7 +584,7 @@ hipError_t BatchSolver::build_tables() {
       D.d = M.d;
       // the finest level is read by the plane-streaming bodies only: the
       // anchor-coded kappa, no anchor mass
-      D.kap = l == 0 ? M.ks : (const float*)M.kap;
+      D.kap = l == 0 ? (const void*)M.ks : (const void*)M.kap;
       D.dg = M.dg;
       D.am = l == 0 ? nullptr : M.am;
       D.msk = M.msk;
@@ -476,12 +646,14 @@ hipError_t BatchSolver::build_tables() {
   ntiles_.assign(lmax_, 0);
   for (int l = 0; l < lmax_; l++)
     if (!all_tiles_[l].empty())
-      tiles_[l] = balloc<BatchTile>(all_tiles_[l].size() + 8, owned_, err);
+      tiles_[l] = balloc<BatchTile>(all_tiles_[l].size() + 8 * 4, owned_, err);  // group padding
   chunks_ = balloc<BatchChunk>(nchunks_, owned_, err);
   pcell_ = balloc<double>(nchunks_, owned_, err);
   pres_ = balloc<double>(nslots_, owned_, err);
+  groups_.assign(lmax_, {});
   {
-    const std::vector<BatchTile> o = xcd_order(all_tiles_[0]);
+    std::vector<int> every(n, 1);
+    const std::vector<BatchTile> o = grouped(0, every.data(), groups0_);
     nall0_ = (int)o.size();
     all0_ = balloc<BatchTile>(o.size(), owned_, err);
     if (err == hipSuccess)
@@ -518,6 +690,27 @@ hipError_t BatchSolver::build_tables() {
   return select_islands(flags_host_.data());
 }
 
+// Level l's tiles of the islands with live[i] != 0, one run per tile depth
+// (ascending), each run in XCD order; `g` receives the runs.
+std::vector<BatchTile> BatchSolver::grouped(int l, const int* live,
+                                            std::vector<TileGroup>& g) const {
+  std::vector<BatchTile> out;
+  g.clear();
+  const int n = (int)sol_.size();
+  for (int tz : {2, 4, 8, 16}) {
+    std::vector<BatchTile> v;
+    for (int i = 0; i < n; i++)
+      if (live[i] && l < sol_[i]->levels() && sol_[i]->level(l).tiles.tz == tz)
+        v.insert(v.end(), all_tiles_[l].begin() + tile_begin_[l][i],
+                 all_tiles_[l].begin() + tile_begin_[l][i + 1]);
+    if (v.empty()) continue;
+    const std::vector<BatchTile> o = xcd_order(v);
+    g.push_back(TileGroup{tz, (int)out.size(), (int)o.size()});
+    out.insert(out.end(), o.begin(), o.end());
+  }
+  return out;
+}
+
 // Launch lists holding only the tiles of islands with live[i] != 0, in
 // XCD-grouped order; the graph is re-captured at its next use.
 hipError_t BatchSolver::select_islands(const int* live) {
@@ -527,13 +720,10 @@ hipError_t BatchSolver::select_islands(const int* live) {
   listed_tiles_ = 0;
   std::vector<std::vector<BatchTile>> order(lmax_);  // alive until the copies are done
   for (int l = 0; l < lmax_; l++) {
-    std::vector<BatchTile> v;
-    for (int i = 0; i < n; i++)
-      if (live[i])
-        v.insert(v.end(), all_tiles_[l].begin() + tile_begin_[l][i],
-                 all_tiles_[l].begin() + tile_begin_[l][i + 1]);
-    if (l == 0) listed_tiles_ = (int)v.size();
-    order[l] = xcd_order(v);
+    order[l] = grouped(l, live, groups_[l]);
+    if (l == 0)
+      for (int i = 0; i < n; i++)
+        if (live[i]) listed_tiles_ += tile_begin_[0][i + 1] - tile_begin_[0][i];
     ntiles_[l] = (int)order[l].size();
     if (!order[l].empty())
       chk(hipMemcpyAsync(tiles_[l], order[l].data(), order[l].size() * sizeof(BatchTile),
@@ -566,20 +756,53 @@ hipError_t BatchSolver::capture() {
   return err;
 }
 
+// Run f(integral_constant<int, TZ>) for a finest-level (4, 8, 16) or coarse
+// (2, 4) tile depth, and g(KT) for the coarse kappa type.
+template <typename F>
+static void with_fine_tz(int tz, F&& f) {
+  if (tz == 4)
+    f(std::integral_constant<int, 4>{});
+  else if (tz == 16)
+    f(std::integral_constant<int, 16>{});
+  else
+    f(std::integral_constant<int, 8>{});
+}
+template <typename F>
+static void with_coarse_tz(int tz, F&& f) {
+  if (tz == 4)
+    f(std::integral_constant<int, 4>{});
+  else
+    f(std::integral_constant<int, 2>{});
+}
+template <typename F>
+static void with_kt(bool bf16, F&& f) {
+  if (bf16)
+    f(uint16_t{});
+  else
+    f(float{});
+}
+
 // Coarse part of every island's V-cycle below its finest level.
 hipError_t BatchSolver::vcycle_coarse(bool check) {
   hipStream_t s = stream_;
   const int n = (int)sol_.size();
-  for (int l = 1; l + 1 < lmax_; l++)
-    if (ntiles_[l] > 0)
-      kb_down<kTZCoarse, false, true><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_,
-                                                                nullptr, nullptr, 0);
-  kb_cheb<<<n, kMaxLdsCoarse, 0, s>>>(levels_, isl_d_, coef_, coarse_sweeps_, sc_, prr_,
-                                      check ? 1 : 0);
-  for (int l = lmax_ - 2; l >= 1; l--)
-    if (ntiles_[l] > 0)
-      kb_up<kTZCoarse, false><<<ntiles_[l], BS, 0, s>>>(tiles_[l], levels_, isl_d_, sc_, nullptr,
-                                                        0);
+  with_kt(bf16_, [&](auto kt) {
+    using KT = decltype(kt);
+    for (int l = 1; l + 1 < lmax_; l++)
+      for (const TileGroup& g : groups_[l])
+        with_coarse_tz(g.tz, [&](auto tz) {
+          kb_down<decltype(tz)::value, KT, false, true><<<g.count, BS, 0, s>>>(
+              tiles_[l] + g.off, levels_, isl_d_, sc_, nullptr, nullptr, 0);
+        });
+    kb_cheb<KT><<<n, kMaxLdsCoarse, 0, s>>>(levels_, isl_d_, coef_, coarse_sweeps_, sc_, prr_,
+                                            check ? 1 : 0);
+    for (int l = lmax_ - 2; l >= 1; l--)
+      for (const TileGroup& g : groups_[l])
+        with_coarse_tz(g.tz, [&](auto tz) {
+          kb_up<decltype(tz)::value, KT, false><<<g.count, BS, 0, s>>>(
+              tiles_[l] + g.off, levels_, isl_d_, sc_, nullptr, 0);
+        });
+  });
   return hipGetLastError();
 }
 
@@ -587,10 +810,17 @@ hipError_t BatchSolver::vcycle_coarse(bool check) {
 hipError_t BatchSolver::precondition() {
   hipStream_t s = stream_;
   if (ntiles_[0] == 0) return hipSuccess;
-  kbs_down<kTZFine, false><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, nullptr,
-                                                     nullptr, 0);
+  for (const TileGroup& g : groups_[0])
+    with_fine_tz(g.tz, [&](auto tz) {
+      kbs_down<decltype(tz)::value, false><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_,
+                                                                   isl_d_, sc_, nullptr, nullptr, 0);
+    });
   (void)vcycle_coarse(false);
-  kbs_up<kTZFine><<<ntiles_[0], BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 0);
+  for (const TileGroup& g : groups_[0])
+    with_fine_tz(g.tz, [&](auto tz) {
+      kbs_up<decltype(tz)::value><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_, isl_d_, sc_,
+                                                         prz_, 0);
+    });
   return hipGetLastError();
 }
 
@@ -598,11 +828,22 @@ hipError_t BatchSolver::precondition() {
 hipError_t BatchSolver::iteration(int c) {
   hipStream_t s = stream_;
   if (ntiles_[0] == 0) return hipSuccess;
-  const unsigned G = ntiles_[0];
-  kbs_spmv<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, ppq_, c);
-  kbs_down<kTZFine, true><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, ppq_, prr_, c);
+  for (const TileGroup& g : groups_[0])
+    with_fine_tz(g.tz, [&](auto tz) {
+      kbs_spmv<decltype(tz)::value><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_, isl_d_,
+                                                           sc_, prz_, ppq_, c);
+    });
+  for (const TileGroup& g : groups_[0])
+    with_fine_tz(g.tz, [&](auto tz) {
+      kbs_down<decltype(tz)::value, true><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_,
+                                                                  isl_d_, sc_, ppq_, prr_, c);
+    });
   (void)vcycle_coarse(true);
-  kbs_up<kTZFine><<<G, BS, 0, s>>>(tiles_[0], levels_, isl_d_, sc_, prz_, 1 - c);
+  for (const TileGroup& g : groups_[0])
+    with_fine_tz(g.tz, [&](auto tz) {
+      kbs_up<decltype(tz)::value><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_, isl_d_, sc_,
+                                                         prz_, 1 - c);
+    });
   return hipGetLastError();
 }
 
@@ -630,7 +871,11 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
   const int pairs_per_check = 4;
   for (int pass = 0; pass < 18 && err == hipSuccess; pass++) {
     chk(hipMemsetAsync(live_, 0, sizeof(int), s), err);
-    kb_residual_tiles<<<nall0_, BS, 0, s>>>(all0_, levels_, isl_d_, st_, pres_);
+    for (const TileGroup& g : groups0_)
+      with_fine_tz(g.tz, [&](auto tz) {
+        kb_residual_tiles<decltype(tz)::value><<<g.count, BS, 0, s>>>(all0_ + g.off, levels_,
+                                                                      isl_d_, st_, pres_);
+      });
     kb_residual_fin<<<n, BS, 0, s>>>(isl_d_, pres_, st_, sc_, tol, max_it, live_);
     kb_live<<<1, 1024, 0, s>>>(sc_, n, flags_, live_ + 1);
     chk(hipMemcpyAsync(flags_host_.data(), flags_, n * sizeof(int), hipMemcpyDeviceToHost, s),

@@ -1252,6 +1252,10 @@ struct pfsf_batch_t {
 
 pfsf_result pfsf_batch_create(const pfsf_batch_desc* desc, pfsf_batch* out) {
   if (!desc || !out || desc->count <= 0 || !desc->islands) return PFSF_ERROR_INVALID_ARG;
+  if (desc->precision != PFSF_PREC_FP32 && desc->precision != PFSF_PREC_BF16_COARSE)
+    return PFSF_ERROR_INVALID_ARG;
+  if (desc->shape != PFSF_BATCH_AUTOTUNE && (desc->shape < 0 || desc->shape >= PFSF_BATCH_SHAPES))
+    return PFSF_ERROR_INVALID_ARG;
   for (int i = 0; i < desc->count; i++) {
     const pfsf_batch_island& bi = desc->islands[i];
     if (!bi.type || !bi.kappa || !dims_ok(bi.dims)) return PFSF_ERROR_INVALID_ARG;
@@ -1276,6 +1280,18 @@ pfsf_result pfsf_batch_solve(pfsf_batch s, const float* const* b, double* const*
   for (int i = 0; i < s->b.islands(); i++)
     if (!b[i]) return PFSF_ERROR_INVALID_ARG;
   RET(s->b.solve(b, x64, phi32, info, per_island));
+}
+
+int32_t pfsf_batch_classes(pfsf_batch s, pfsf_batch_class* out, int32_t cap) {
+  if (!s) return -1;
+  for (int k = 0; k < PFSF_BATCH_CLASSES && k < cap && out; k++) out[k] = s->b.shape_class(k);
+  return PFSF_BATCH_CLASSES;
+}
+
+pfsf_result pfsf_batch_tuning_info(pfsf_batch s, pfsf_batch_tuning* out) {
+  if (!s || !out) return PFSF_ERROR_INVALID_ARG;
+  *out = s->b.tuning();
+  return PFSF_OK;
 }
 
 void pfsf_batch_destroy(pfsf_batch s) { delete s; }

@@ -107,7 +107,25 @@ class BatchIsland(C.Structure):
 class BatchDesc(C.Structure):
     _fields_ = [("count", C.c_int32), ("islands", C.POINTER(BatchIsland)),
                 ("coarse_sweeps", C.c_int32), ("tol", C.c_double), ("max_iters", C.c_int32),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("precision", C.c_int32), ("shape", C.c_int32)]
+
+
+BATCH_SHAPES = 6
+BATCH_CLASSES = 4
+BATCH_AUTOTUNE = -1
+# (finest column depth, coarse tile depth) of each candidate, in BatchClass.ms order
+BATCH_SHAPE_CANDIDATES = [(8, 2), (4, 2), (16, 2), (8, 4), (4, 4), (16, 4)]
+
+
+class BatchClass(C.Structure):
+    _fields_ = [("min_cells", C.c_int64), ("max_cells", C.c_int64), ("islands", C.c_int32),
+                ("sample", C.c_int32), ("tz_fine", C.c_int32), ("tz_coarse", C.c_int32),
+                ("ms", C.c_float * BATCH_SHAPES)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("sample", C.c_int32), ("mixed", C.c_int32), ("mixed_ms", C.c_float),
+                ("uniform_ms", C.c_float * BATCH_SHAPES)]
 
 
 class BatchInfo(C.Structure):
@@ -184,6 +202,8 @@ SIGNATURES = {
     "pfsf_batch_solve": (R, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(BatchInfo),
                              C.POINTER(SolveInfo)]),
     "pfsf_batch_destroy": (None, [P]),
+    "pfsf_batch_classes": (C.c_int32, [P, C.POINTER(BatchClass), C.c_int32]),
+    "pfsf_batch_tuning_info": (R, [P, C.POINTER(BatchTuning)]),
     "pfsf_load_path_loss": (R, [C.POINTER(LoadPathDesc), C.c_int32, C.POINTER(LoadPathInfo)]),
     "pfsf_load_path_grad": (R, [C.POINTER(LoadPathDesc), P, C.POINTER(LoadPathInfo)]),
     "pfsf_solver_profile": (R, [P, I32, I32, C.POINTER(F64)]),

@@ -193,16 +193,21 @@ class Solver:
 class BatchSolver:
     """Many independent islands solved together (pfsf_batch_*).
 
-    islands: list of (vtype, kappa) device tensors, shapes (lz, ly, lx)."""
+    islands: list of (vtype, kappa) device tensors, shapes (lz, ly, lx).
+    precision: 0 FP32, 1 BF16 coarse kappa; shape: tile-shape candidate for
+    every island (capi.BATCH_SHAPE_CANDIDATES index), or capi.BATCH_AUTOTUNE
+    to time the candidates per island-size class (classes() reports them)."""
 
-    def __init__(self, islands, tol=1e-6, max_iters=1000, coarse_sweeps=0):
+    def __init__(self, islands, tol=1e-6, max_iters=1000, coarse_sweeps=0, precision=0,
+                 shape=0):
         self._keep = list(islands)  # keep the device arrays alive
         n = len(self._keep)
         arr = (capi.BatchIsland * n)()
         for k, (vt, kp) in enumerate(self._keep):
             arr[k] = capi.BatchIsland(_dims(vt), _p(vt).value, _p(kp).value)
         self._arr = arr
-        desc = capi.BatchDesc(n, arr, coarse_sweeps, tol, max_iters, _stream().value)
+        desc = capi.BatchDesc(n, arr, coarse_sweeps, tol, max_iters, _stream().value,
+                              precision, shape)
         h = C.c_void_p()
         capi.check("pfsf_batch_create", _lib().pfsf_batch_create(C.byref(desc), C.byref(h)))
         self.h = h
@@ -222,6 +227,29 @@ class BatchSolver:
         capi.check("pfsf_batch_solve", _lib().pfsf_batch_solve(
             self.h, cast(b), cast(x), cast(ph), C.byref(info), per))
         return (info, list(per)) if per_island else info
+
+    def classes(self):
+        """Size classes with their tile shapes: list of dicts (cells range,
+        islands, sample, chosen depths, per-candidate tuning ms)."""
+        out = (capi.BatchClass * capi.BATCH_CLASSES)()
+        k = _lib().pfsf_batch_classes(self.h, out, capi.BATCH_CLASSES)
+        if k < 0:
+            raise RuntimeError("pfsf_batch_classes: no batch")
+        return [{"cells": [c.min_cells, c.max_cells], "islands": c.islands, "sample": c.sample,
+                 "tz_fine": c.tz_fine, "tz_coarse": c.tz_coarse,
+                 "ms": {f"{a}x{b}": round(float(m), 3)
+                        for (a, b), m in zip(capi.BATCH_SHAPE_CANDIDATES, c.ms) if m > 0}}
+                for c in out[:k]]
+
+    def tuning(self):
+        """The tuner's whole-batch decision on its stratified sample."""
+        t = capi.BatchTuning()
+        capi.check("pfsf_batch_tuning_info", _lib().pfsf_batch_tuning_info(self.h, C.byref(t)))
+        return {"sample": t.sample, "mixed": bool(t.mixed),
+                "mixed_ms": round(float(t.mixed_ms), 3) if t.mixed_ms > 0 else None,
+                "uniform_ms": {f"{a}x{b}": round(float(m), 3)
+                               for (a, b), m in zip(capi.BATCH_SHAPE_CANDIDATES, t.uniform_ms)
+                               if m > 0}}
 
     def close(self):
         if getattr(self, "h", None):

@@ -8,6 +8,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
+from pfsf_amd import capi
 from pfsf_amd import device as D
 from pfsf_amd import workloads as W
 
@@ -32,9 +33,9 @@ def _upload(isls, cuda):
     return vt, kp, b
 
 
-def _solve(isls, cuda, tol=1e-6, x0=None):
+def _solve(isls, cuda, tol=1e-6, x0=None, **kw):
     vt, kp, b = _upload(isls, cuda)
-    s = D.BatchSolver(list(zip(vt, kp)), tol=tol, max_iters=2000)
+    s = D.BatchSolver(list(zip(vt, kp)), tol=tol, max_iters=2000, **kw)
     xs = [torch.zeros(i.vtype.shape, dtype=torch.float64, device=cuda) if x0 is None
           else torch.from_numpy(x0[k]).to(cuda) for k, i in enumerate(isls)]
     ph = [torch.empty(i.vtype.shape, dtype=torch.float32, device=cuda) for i in isls]
@@ -56,6 +57,76 @@ def test_batch_matches_direct_solves(cuda):
         err = np.linalg.norm(phis[k] - ref) / np.linalg.norm(ref)
         assert err <= TOL_PHI, (k, err)
         assert np.array_equal(phis[k], xs[k].astype(np.float32))
+
+
+@pytest.mark.parametrize("shape,precision", [(c, 0) for c in range(6)] + [(0, 1), (3, 1)])
+def test_batch_shapes_and_bf16_coarse_match_direct_solves(cuda, shape, precision):
+    """Every tile-shape candidate (finest column depth 4/8/16, coarse tile
+    depth 2/4), and BF16 coarse kappa: every island reaches 1e-8 by the
+    oracle's FP64 residual and phi is within 1e-5 of the direct solve."""
+    isls = _islands(count=10, seed=23)
+    s, info, per, xs, phis = _solve(isls, cuda, tol=1e-8, shape=shape, precision=precision)
+    cls = s.classes()
+    s.close()
+    assert info.converged == len(isls), [(p.iterations, p.rel_residual) for p in per]
+    a, b = capi.BATCH_SHAPE_CANDIDATES[shape]
+    assert all(c["tz_fine"] == a and c["tz_coarse"] == b and c["sample"] == 0 for c in cls)
+    for k, isl in enumerate(isls):
+        assert O.residual(isl, xs[k]) <= 1e-8 * 1.01, (k, per[k].rel_residual)
+        ref = O.direct_solve(isl)
+        err = np.linalg.norm(phis[k] - ref) / np.linalg.norm(ref)
+        assert err <= TOL_PHI, (k, err)
+
+
+def test_batch_autotune_times_every_class_present(cuda):
+    """PFSF_BATCH_AUTOTUNE: each size class holding islands times all six
+    shapes on a sample; the batch then either gives every class its fastest
+    shape or one shape to all classes (the per-class mix must beat the best
+    single shape by a margin, since mixed depths split the launches); empty
+    classes stay untimed; the tuned batch still solves every island to 1e-6."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    isls = [W.grown_island(rng, n, name=f"a{k}")
+            for k, n in enumerate([80, 150, 300, 900, 2500, 6000, 12000, 20000])]
+    s, info, per, xs, phis = _solve(isls, cuda, tol=1e-6, shape=capi.BATCH_AUTOTUNE)
+    cls = s.classes()
+    tun = s.tuning()
+    s.close()
+    assert info.converged == len(isls)
+    cells = [i.n for i in isls]
+    timed = []
+    for c in cls:
+        inside = sum(c["cells"][0] <= n < c["cells"][1] for n in cells)
+        assert c["islands"] == inside
+        if inside:
+            assert c["sample"] == min(inside, 32) and len(c["ms"]) == 6
+            timed.append(c)
+        else:
+            assert c["sample"] == 0 and c["ms"] == {}
+    assert len(timed) == 3
+    chosen = [f"{c['tz_fine']}x{c['tz_coarse']}" for c in cls]
+    fastest = {min(c["ms"], key=c["ms"].get) for c in timed}
+    assert tun["sample"] == len(isls)
+    # the whole-batch options timed: the default and every class's fastest
+    assert set(tun["uniform_ms"]) == fastest | {"8x2"}
+    # a shape replaces the default (and the mix the best single shape) only
+    # when it is at least 3 % faster
+    um = tun["uniform_ms"]
+    uni = "8x2"
+    for k in ("4x2", "16x2", "8x4", "4x4", "16x4"):
+        if k in um and um[k] < 0.97 * um[uni]:
+            uni = k
+    if len(fastest) > 1:
+        assert tun["mixed_ms"] is not None
+        assert tun["mixed"] == (tun["mixed_ms"] < 0.97 * um[uni])
+    else:
+        assert tun["mixed_ms"] is None and not tun["mixed"]
+    if tun["mixed"]:
+        assert all(f"{c['tz_fine']}x{c['tz_coarse']}" == min(c["ms"], key=c["ms"].get)
+                   for c in timed)
+    else:
+        assert set(chosen) == {uni}, (cls, tun)
+    for k, isl in enumerate(isls):
+        assert O.residual(isl, xs[k]) <= 1e-6 * 1.01, k
 
 
 def test_batch_agrees_with_single_solver(cuda):
