@@ -493,8 +493,8 @@ def multi_island_tick_bench(W, count=256, steps=3):
     included), islands drawn as the batch workload's (log-uniform 64..262,144
     voxels).  kappa: each island registers its per-voxel kappa, and the
     islands are solved in one batched solve; soa6: faces only (kappa on all
-    six faces), each island's SoA-6 solve in turn.  Cold: every phi zeroed
-    before the tick."""
+    six faces), the islands' SoA-6 solves run side by side on the engine's
+    host worker threads.  Cold: every phi zeroed before the tick."""
     import numpy as np
 
     from pfsf_amd import capi

@@ -11,10 +11,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "batch_solver.h"
@@ -449,6 +452,51 @@ pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
   return post_solve(E, is);
 }
 
+// The SoA-6 islands of one tick.  Each has its own solver, vectors and
+// stream and its solve touches nothing else, so up to kSoa6Workers host
+// threads run the solves side by side and their small kernels overlap on the
+// GPU's queues.  Hierarchies are prepared before, and the post-solve steps
+// (history, damage, scan) run after, on the calling thread in island order.
+constexpr int kSoa6Workers = 8;
+
+pfsf_result solve_soa6_concurrent(pfsf_engine_t* E, const std::vector<Island*>& list) {
+  for (Island* is : list) {
+    pfsf_result r = ensure_solver(E, *is);
+    if (r != PFSF_OK) return r;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return PFSF_ERROR_VULKAN;
+  std::atomic<int> next{0};
+  std::vector<hipError_t> errs(list.size(), hipSuccess);
+  auto work = [&](bool worker) {
+    if (worker && hipSetDevice(dev) != hipSuccess) return;  // the others take its share
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= (int)list.size()) return;
+      Island& is = *list[k];
+      pfsf_solve_info info{};
+      errs[k] = is.s6->solve(is.source, is.x64, is.phi, &info);
+      is.last = info;
+    }
+  };
+  std::vector<std::thread> th;
+  const int nt = std::min<int>(kSoa6Workers, (int)list.size());
+  try {
+    for (int t = 1; t < nt; t++) th.emplace_back(work, true);
+  } catch (const std::system_error&) {
+    // fewer threads: the calling thread solves what is left
+  }
+  work(false);
+  for (std::thread& t : th) t.join();
+  for (hipError_t e : errs)
+    if (e != hipSuccess) return PFSF_ERROR_VULKAN;
+  for (Island* is : list) {
+    pfsf_result r = post_solve(E, *is);
+    if (r != PFSF_OK) return r;
+  }
+  return PFSF_OK;
+}
+
 // Several kappa-mode dirty islands in one batched solve (same numerics per
 // island).
 pfsf_result solve_batch(pfsf_engine_t* E, const std::vector<Island*>& todo) {
@@ -565,6 +613,17 @@ pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count, int32
     if (r != PFSF_OK) return r;
   } else {
     single.insert(single.begin(), batch.begin(), batch.end());
+  }
+  if (E->pcg && single.size() >= 2) {
+    std::vector<Island*> s6;
+    for (Island* is : single)
+      if (is->soa6) s6.push_back(is);
+    single.erase(std::remove_if(single.begin(), single.end(), [](Island* is) { return is->soa6; }),
+                 single.end());
+    if (!s6.empty()) {
+      pfsf_result r = solve_soa6_concurrent(E, s6);
+      if (r != PFSF_OK) return r;
+    }
   }
   for (Island* is : single) {
     pfsf_result r = solve_island(E, *is);
