@@ -268,6 +268,70 @@ __device__ __forceinline__ float stencil_apply(const float* __restrict__ v,
   return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
 }
 
+// stencil_apply's couplings at one cell, formed once for a cell whose
+// kappa coding and mask do not change (the coarsest solve's steps): the
+// masked face weights and the edge / corner neighbour codings, in
+// stencil_apply's order.
+struct CellCouplings {
+  float f[6], e[12], c[8];
+  float si;
+};
+
+template <bool MASK>
+__device__ __forceinline__ void cell_couplings(const float* __restrict__ s, int c, int sy, int sz,
+                                               uint32_t mk, CellCouplings& W) {
+  const float si = s[c], ki = si * si;
+  W.si = si;
+  auto keep = [&](float x, int b) {
+    if (!MASK) return x;
+    return __uint_as_float(__float_as_uint(x) & (0u - ((mk >> b) & 1u)));
+  };
+  auto face = [&](int k, int o, int b) {
+    const float sj = s[c + o];
+    W.f[k] = keep(is_free_s(sj) ? fmaf(sj, sj, ki) : 0.f, b);
+  };
+  face(0, -1, nbit(-1, 0, 0));
+  face(1, 1, nbit(1, 0, 0));
+  face(2, -sy, nbit(0, -1, 0));
+  face(3, sy, nbit(0, 1, 0));
+  face(4, -sz, nbit(0, 0, -1));
+  face(5, sz, nbit(0, 0, 1));
+  const int eo[12] = {-1 - sy, 1 - sy, -1 + sy, 1 + sy, -1 - sz, 1 - sz,
+                      -1 + sz, 1 + sz, -sy - sz, sy - sz, -sy + sz, sy + sz};
+  const int eb[12] = {nbit(-1, -1, 0), nbit(1, -1, 0), nbit(-1, 1, 0), nbit(1, 1, 0),
+                      nbit(-1, 0, -1), nbit(1, 0, -1), nbit(-1, 0, 1), nbit(1, 0, 1),
+                      nbit(0, -1, -1), nbit(0, 1, -1), nbit(0, -1, 1), nbit(0, 1, 1)};
+#pragma unroll
+  for (int k = 0; k < 12; k++) W.e[k] = keep(s[c + eo[k]], eb[k]);
+  const int co[8] = {-1 - sy - sz, 1 - sy - sz, -1 + sy - sz, 1 + sy - sz,
+                     -1 - sy + sz, 1 - sy + sz, -1 + sy + sz, 1 + sy + sz};
+  const int cb[8] = {nbit(-1, -1, -1), nbit(1, -1, -1), nbit(-1, 1, -1), nbit(1, 1, -1),
+                     nbit(-1, -1, 1), nbit(1, -1, 1), nbit(-1, 1, 1), nbit(1, 1, 1)};
+#pragma unroll
+  for (int k = 0; k < 8; k++) W.c[k] = keep(s[c + co[k]], cb[k]);
+}
+
+// stencil_apply with the couplings of cell_couplings: the same products and
+// sums in the same order, so the same bits.
+__device__ __forceinline__ float stencil_apply_w(const float* __restrict__ v, int c, int sy,
+                                                 int sz, const CellCouplings& W, float mi,
+                                                 float scale) {
+  const float vc = v[c];
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  const int fo[6] = {-1, 1, -sy, sy, -sz, sz};
+#pragma unroll
+  for (int k = 0; k < 6; k++) af += W.f[k] * (vc - v[c + fo[k]]);
+  const int eo[12] = {-1 - sy, 1 - sy, -1 + sy, 1 + sy, -1 - sz, 1 - sz,
+                      -1 + sz, 1 + sz, -sy - sz, sy - sz, -sy + sz, sy + sz};
+#pragma unroll
+  for (int k = 0; k < 12; k++) ae += W.e[k] * (vc - v[c + eo[k]]);
+  const int co[8] = {-1 - sy - sz, 1 - sy - sz, -1 + sy - sz, 1 + sy - sz,
+                     -1 - sy + sz, 1 - sy + sz, -1 + sy + sz, 1 + sy + sz};
+#pragma unroll
+  for (int k = 0; k < 8; k++) ac += W.c[k] * (vc - v[c + co[k]]);
+  return scale * (0.5f * af + W.si * (kEdgeP * ae + kCornerP * ac)) + mi * vc;
+}
+
 template <int TZ, bool MASK = false>
 __device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float mi, float scale,
                                             uint32_t mk = kAllCouplings) {
@@ -687,11 +751,15 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   }
   const bool live = act && ki >= 0.f;
   __syncthreads();
+  // the cell's couplings do not change across the steps (an idle thread's
+  // e is not a cell of the box: it forms none)
+  CellCouplings W;
+  if (live) cell_couplings<true>(ss, e, sy, sz, mk, W);
   int cur = 0;
   for (int st = 1; st < steps; st++) {
     const float c1 = cf[2 * st], c2 = cf[2 * st + 1];
     if (live) {
-      const float ax = stencil_apply<true>(sx[cur], ss, e, sy, sz, m, L.scale, mk);
+      const float ax = stencil_apply_w(sx[cur], e, sy, sz, W, m, L.scale);
       di = c1 * di + c2 * (ri - ax) * rcp(D);
       xi = xi + di;
       sx[cur ^ 1][e] = xi;
