@@ -95,7 +95,17 @@ struct TL {
   int ntx, nty;
   const int* ids;
   int n;  // entries in ids (the launch list, a multiple of 8)
+  const uint32_t* rows;  // per-slot row bits of the finest streaming level, or null
 };
+
+// Row bits of one column of the finest level (list slot g, words
+// rows[g * kRowWords ...]): word sg * 10 + yy describes y = y0 - 1 + yy for
+// sg = 0 the cell x0 - 1, sg = 1 the 32-cell row x0 .. x0 + 31, sg = 2 the
+// cell x0 + 32; bit pl (plane z0 - 1 + pl) is set when that row or cell
+// holds a free voxel, bit 16 + pl when it holds a free voxel or an anchor
+// (a cell whose kappa coding couples).  Columns of up to 14 planes.
+constexpr int kRowWords = 30;
+constexpr int kRowMaxPlanes = 16;
 
 // ── LDS tile machinery ─────────────────────────────────────────────────
 // Grids hold fewer than 2^31 cells (checked at creation), so the hot kernels

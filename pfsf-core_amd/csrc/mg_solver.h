@@ -64,6 +64,10 @@ struct TileList {
   int nactive = 0;                // tiles holding at least one free voxel
   int* ids = nullptr;             // device: tile id per work-group, XCD-grouped
   std::vector<int> host_active;   // host: active tile ids, ascending
+  // finest level of a standalone solver (columns of at most 14 planes):
+  // kRowWords words per list slot saying which 32-cell rows the column's
+  // streaming kernels must read (see k_row_bits)
+  uint32_t* rows = nullptr;
 };
 
 // Grids of the solver stay below 2^29 cells, so every array (FP64 included)

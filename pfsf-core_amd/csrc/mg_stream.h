@@ -56,10 +56,14 @@ struct PlaneRing {
 
 // Thread geometry of a plane: the interior cell (tx, ty) and, for threads
 // t < 84, one cell of the halo ring (top row, bottom row, left, right).
+// mo / mh: the row bits (TL::rows) of the own row and of the halo cell, all
+// set where a column carries none.
 struct PlaneGeom {
   int gx, gy, hx, hy;
   bool inxy, hown, hxy;
   int li, lh;  // LDS offsets inside a plane
+  int ow, hw;  // row-bit words of the own row and the halo cell
+  uint32_t mo, mh;
 };
 
 __device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
@@ -88,7 +92,32 @@ __device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
   g.hy = y0 + ey - 1;
   g.hxy = g.hown && g.hx >= 0 && g.hx < d.lx && g.hy >= 0 && g.hy < d.ly;
   g.lh = ey * HX + ex;
+  g.ow = 10 + ty + 1;
+  g.hw = g.hown ? (ex == 0 ? 0 : ex == HX - 1 ? 20 : 10) + ey : g.ow;
+  g.mo = g.mh = ~0u;
   return g;
+}
+
+// The row bits of list slot `slot` into g (none: every row is read).
+__device__ __forceinline__ void row_bits(const TL& M, int slot, PlaneGeom& g) {
+  if (!M.rows) return;
+  const uint32_t* w = M.rows + (int64_t)slot * kRowWords;
+  g.mo = w[g.ow];
+  g.mh = w[g.hw];
+}
+
+// Plane pl of a column: whether a row's values (free voxels) and its kappa
+// coding (free voxels or anchors) must be read.  Columns deeper than the
+// row bits describe read every row.
+template <int TZ>
+__device__ __forceinline__ bool row_free(uint32_t m, int pl) {
+  if constexpr (TZ + 2 > kRowMaxPlanes) return true;
+  return (m >> pl) & 1u;
+}
+template <int TZ>
+__device__ __forceinline__ bool row_coupled(uint32_t m, int pl) {
+  if constexpr (TZ + 2 > kRowMaxPlanes) return true;
+  return (m >> (16 + pl)) & 1u;
 }
 
 // A thread's 3 x 3 neighbourhood of one plane, index (dy + 1) * 3 + (dx + 1).
@@ -199,6 +228,9 @@ __device__ __forceinline__ void restrict_pair(float a, int tx, int ty, int gx, i
 // from the chunk of the XCD it runs on.
 struct StrideColumns {
   TL M;
+  __device__ __forceinline__ void rows(int k, PlaneGeom& g) const {
+    row_bits(M, (int)blockIdx.x + k * (int)gridDim.x, g);
+  }
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
     const int i = (int)blockIdx.x + k * (int)gridDim.x;
@@ -216,6 +248,9 @@ struct StrideColumns {
 // SlotColumn: the column of list slot blockIdx.x, one per work-group.
 struct SlotColumn {
   TL M;
+  __device__ __forceinline__ void rows(int, PlaneGeom& g) const {
+    row_bits(M, (int)blockIdx.x, g);
+  }
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
     if (k > 0) return -1;
@@ -226,6 +261,7 @@ struct SlotColumn {
 // OneColumn: the column of a batch tile record.
 struct OneColumn {
   int x0, y0, z0;
+  __device__ __forceinline__ void rows(int, PlaneGeom&) const {}
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& a, int& b, int& c) const {
     if (k > 0) return -1;
@@ -359,23 +395,42 @@ struct DnRaw {
   PlaneIdx ix;
   float k, r, q, dg, p, hk, hr, hq, hdg;
   double x;
+  bool fo;  // the own row holds a free voxel (its updates are stored)
 };
 
+// A row that holds no free voxel is not read: its cells take the values
+// staged for such cells (0, kappa -inf where nothing couples, diagonal 1),
+// and its updates are not stored (x and r stay as they are there).  The
+// loads of a row sit behind one branch, so a wave whose rows are all skipped
+// issues none of them.  Measured at 128^3 (60 %): 18.0 -> 17.4 us; the same
+// skipping in sp_load / up_load made those kernels slower (13.1 -> 13.7,
+// 12.1 -> 13.0 us): they read too few bytes per row to repay the bits.
 template <int TZ, bool UPDATE>
-__device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
-                                        const float* rin, const float* q, const float* p,
-                                        const double* x) {
+__device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, int pl,
+                                        const Lv<float>& L, const float* rin, const float* q,
+                                        const float* p, const double* x) {
   R.ix = plane_idx(g, zpl, L.d);
-  R.k = ld(L.kap, R.ix.i);
-  R.r = ld(rin, R.ix.i);
-  R.q = UPDATE ? ld(q, R.ix.i) : 0.f;
-  R.dg = ld(L.dg, R.ix.i);
-  R.p = UPDATE ? ld(p, R.ix.i) : 0.f;
-  R.x = UPDATE ? ld(x, R.ix.i) : 0.0;
-  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
-  R.hr = ld(rin, R.ix.h);
-  R.hq = UPDATE ? ld(q, R.ix.h) : 0.f;
-  R.hdg = ld(L.dg, R.ix.h);
+  const bool fo = row_free<TZ>(g.mo, pl), co = row_coupled<TZ>(g.mo, pl);
+  const bool fh = row_free<TZ>(g.mh, pl), ch = row_coupled<TZ>(g.mh, pl);
+  R.fo = fo;
+  R.k = R.hk = -INFINITY;
+  R.r = R.q = R.p = R.hr = R.hq = 0.f;
+  R.dg = R.hdg = 1.f;
+  R.x = 0.0;
+  if (co) R.k = ld(L.kap, R.ix.i);
+  if (fo) {
+    R.r = ld(rin, R.ix.i);
+    if (UPDATE) R.q = ld(q, R.ix.i);
+    R.dg = ld(L.dg, R.ix.i);
+    if (UPDATE) R.p = ld(p, R.ix.i);
+    if (UPDATE) R.x = ld(x, R.ix.i);
+  }
+  if (ch) R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  if (fh) {
+    R.hr = ld(rin, R.ix.h);
+    if (UPDATE) R.hq = ld(q, R.ix.h);
+    R.hdg = ld(L.dg, R.ix.h);
+  }
 }
 
 template <int TZ, bool UPDATE, typename Cols>
@@ -393,11 +448,12 @@ __device__ __forceinline__ void down_stream_body(
   int x0 = 0, y0 = 0, z0 = 0;
   int col = cols.template get<TZ>(0, x0, y0, z0);
   PlaneGeom g = plane_geom(x0, y0, L.d);
+  if (col > 0) cols.rows(0, g);
   DnRaw buf[kPD];
   if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
-      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
+      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, x);
     });
   if (sc->done) return;
   double alpha64 = 0.0;
@@ -427,7 +483,8 @@ __device__ __forceinline__ void down_stream_body(
         constexpr int s = decltype(S_)::value;
         const DnRaw cur = buf[s % kPD];
         if constexpr (s + kPD < TZ + 2)
-          dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, L, rin, q, p, x);
+          dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, s + kPD, L, rin, q, p,
+                              x);
         const int sl = s % kRing;
         const float ki = cur.ix.iv ? cur.k : -INFINITY;
         const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
@@ -442,7 +499,7 @@ __device__ __forceinline__ void down_stream_body(
           if (UPDATE) {
             // p and q are zero outside the free set, so the update leaves x
             // there unchanged and carries r there over
-            if (cur.ix.iv) {
+            if (cur.ix.iv && cur.fo) {
               st(x, cur.ix.i, cur.x + alpha64 * (double)cur.p);
               st(rout, cur.ix.i, ri);
             }
@@ -469,9 +526,10 @@ __device__ __forceinline__ void down_stream_body(
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
       g = plane_geom(x0, y0, L.d);
+      cols.rows(k, g);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
-        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, L, rin, q, p, x);
+        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, x);
       });
     }
   }
