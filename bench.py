@@ -276,7 +276,9 @@ def mixed_precision_bench(torch, D, W, steps=3, coarse_sweeps=0):
 
 def adjoint_bench(torch, W, steps=2):
     """The 96^3 island: 50-step load path, loss (forward) and dL/dkappa
-    (forward + checkpointed adjoint), solves to 1e-6."""
+    (forward + checkpointed adjoint), solves to 1e-6; then the same with
+    damage coupling the steps (each state's operator rebuilt from its
+    degraded kappa)."""
     import numpy as np
 
     from pfsf_amd import adjoint as A
@@ -298,6 +300,14 @@ def adjoint_bench(torch, W, steps=2):
            "gradient_pcg_iterations": g.pcg_iterations,
            "check": "finite differences on 64 voxels and an exact FP64 gradient: "
                     "tests/test_gpu_adjoint.py"}
+    fw = [A.load_path_loss(vt, kp, b, rc, damage=True) for _ in range(steps)]
+    gr = [A.load_path_grad(vt, kp, b, rc, damage=True)[0] for _ in range(steps)]
+    f, g = fw[-1], gr[-1]
+    out["damage"] = {"loss": f.loss,
+                     "forward_ms": round(float(np.median([i.ms for i in fw])), 2),
+                     "gradient_ms": round(float(np.median([i.ms for i in gr])), 2),
+                     "gradient_pcg_iterations": g.pcg_iterations,
+                     "check": "exact FP64 damage-path gradient: tests/test_gpu_adjoint.py"}
     log(f"96^3 adjoint: {out}")
     return out
 

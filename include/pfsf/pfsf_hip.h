@@ -397,6 +397,14 @@ typedef struct {
   double tol;               /* per-solve relative residual (0 = 1e-6) */
   int32_t max_iters;        /* per solve (0 = 2000) */
   void* stream;
+  /* 1: damage couples the steps.  After each state the phase-field history
+   * H = max(H, psi_e) and one damage step (the engine's hybrid AT2 step:
+   * l0 2, Gc 100, relaxation 0.3, spectral split, faces of conductivity
+   * kappa_t) update the state, and the next step solves with
+   * kappa_t = kappa * ((1 - 1e-3)(1 - d)^2 + 1e-3).  The gradient then runs
+   * through the damage, history and psi of every step as well (FP64 state),
+   * with (phi, d, H) checkpointed and each segment recomputed. */
+  int32_t damage;
 } pfsf_load_path_desc;
 
 typedef struct {

@@ -631,3 +631,207 @@ def label_summarise(vtype, ids, max_components=64):
     num = lib().or_label_summarise(*_grid_dims(t), _p(t), _p(ids), max_components, _p(rec),
                                    C.byref(ov))
     return rec, int(num), bool(ov.value)
+
+
+# ── Load path with damage: the phase-field state couples the steps ──────
+# Restated in FP64 with sparse direct solves, as the checker of the GPU
+# damage-coupled load path (pfsf_load_path_*, desc.damage = 1).  Per step t:
+#   kappa_t = kappa * g(d_{t-1}),  g(d) = (1 - K_RES) (1 - d)^2 + K_RES
+#   A(kappa_t) phi_t = (t / T) b;  L += sum softplus(f_i(phi_t; kappa_t) - rcomp_i)
+#   psi_t = 0.5 kappa_t |grad phi_t|^2 (the per-axis differences of or_psi)
+#   H_t = max(H_{t-1}, psi_t)
+#   d_t = the phase-field step of or_phase_field (hybrid AT2, split on,
+#         hydration 1, faces of conductivity kappa_t) on the free voxels
+# and its reverse-mode gradient dL/dkappa.
+K_RES = 1e-3
+# face neighbours in (axis, step) order: -x, +x, -y, +y, -z, +z on (lz, ly, lx)
+_FACES = [(2, -1), (2, 1), (1, -1), (1, 1), (0, -1), (0, 1)]
+
+
+def _shift(a, axis, step, fill=0.0):
+    """out[i] = a[i + step e_axis] (fill outside the grid)."""
+    out = np.full_like(a, fill)
+    src = [slice(None)] * 3
+    dst = [slice(None)] * 3
+    n = a.shape[axis]
+    if step > 0:
+        src[axis], dst[axis] = slice(step, n), slice(0, n - step)
+    else:
+        src[axis], dst[axis] = slice(0, n + step), slice(-step, n)
+    out[tuple(dst)] = a[tuple(src)]
+    return out
+
+
+def damage_g(d):
+    return (1.0 - K_RES) * (1.0 - d) ** 2 + K_RES
+
+
+def damage_dg(d):
+    return -2.0 * (1.0 - K_RES) * (1.0 - d)
+
+
+def psi64(occ, fr, phi, kt):
+    """psi = 0.5 kt (gx^2 + gy^2 + gz^2) on free voxels; also the per-axis
+    differences and their cases for the reverse pass."""
+    g2 = np.zeros(phi.shape)
+    parts = []
+    for axis in (2, 1, 0):
+        pm, pp = _shift(phi, axis, -1), _shift(phi, axis, 1)
+        om, op = _shift(occ, axis, -1, False), _shift(occ, axis, 1, False)
+        g = np.where(om & op, (pp - pm) * 0.5, np.where(op, pp - phi, np.where(om, phi - pm, 0.0)))
+        parts.append((axis, g, om, op))
+        g2 += g * g
+    return np.where(fr, 0.5 * kt * g2, 0.0), parts
+
+
+def psi64_vjp(occ, fr, phi, kt, psib):
+    """(dphi, dkt) of psi64 against the adjoint psib (free voxels only)."""
+    _, parts = psi64(occ, fr, phi, kt)
+    psib = np.where(fr, psib, 0.0)
+    dkt = np.zeros(phi.shape)
+    dphi = np.zeros(phi.shape)
+    for axis, g, om, op in parts:
+        dkt += 0.5 * g * g * psib
+        dg = kt * g * psib
+        both, onlyp, onlym = om & op, op & ~om, om & ~op
+        # d g / d phi_{i+e} and d g / d phi_{i-e}, scattered to those voxels
+        cp = np.where(both, 0.5 * dg, np.where(onlyp, dg, 0.0))
+        cm = np.where(both, -0.5 * dg, np.where(onlym, -dg, 0.0))
+        dphi += np.where(onlyp, -dg, 0.0) + np.where(onlym, dg, 0.0)  # phi_i in one-sided cases
+        dphi += _shift(cp, axis, -1) + _shift(cm, axis, 1)
+    return np.where(fr, dphi, 0.0), dkt
+
+
+def pf64(occ, fr, d, H, phi, kt, l0=PF_L0, gc=PF_GC, relax=PF_RELAX):
+    """One phase-field step (or_phase_field's formula in FP64, split on,
+    hydration 1, faces of conductivity kt) on free voxels with H > 0, and
+    the intermediates of its reverse pass."""
+    nf = np.zeros(phi.shape)
+    lp = np.zeros(phi.shape)
+    ld = np.zeros(phi.shape)
+    for axis, step in _FACES:
+        o = _shift(occ, axis, step, False)
+        nf += o
+        lp += np.where(o, _shift(phi, axis, step) - phi, 0.0)
+        ld += np.where(o, _shift(d, axis, step) - d, 0.0)
+    ssum = nf * kt
+    ss = np.maximum(ssum, 1e-12)
+    pp = np.maximum(lp, 0.0) / ss
+    he = np.maximum(pp, H)
+    c1, c2 = gc * l0, gc / l0
+    num = 2.0 * he + c1 * ld
+    den0 = 2.0 * he + c2
+    den = np.maximum(den0, 1e-8)
+    q0 = num / den
+    q1 = np.maximum(q0, d)
+    q = np.minimum(q1, 1.0)
+    v0 = d * (1.0 - relax) + q * relax
+    v1 = np.clip(v0, 0.0, 1.0)
+    v2 = np.maximum(v1, d)
+    active = fr.astype(bool) & (H > 0)
+    out = np.where(active, v2, d)
+    st = dict(nf=nf, lp=lp, ssum=ssum, ss=ss, pp=pp, H=H, he=he, num=num, den0=den0, den=den,
+              q0=q0, q1=q1, v0=v0, v1=v1, d=d, active=active, c1=c1, relax=relax)
+    return out, st
+
+
+def pf64_vjp(occ, st, Db):
+    """(dd_prev, dH, dphi, dkt) of pf64 against the adjoint Db of its output."""
+    a = st["active"]
+    r = st["relax"]
+    m2 = st["v1"] > st["d"]
+    dv1 = Db * m2
+    dd = Db * ~m2
+    dv0 = dv1 * ((st["v0"] > 0) & (st["v0"] < 1))
+    dd = dd + dv0 * (1.0 - r)
+    dq = dv0 * r
+    dq1 = dq * (st["q1"] < 1.0)
+    m0 = st["q0"] > st["d"]
+    dq0 = dq1 * m0
+    dd = dd + dq1 * ~m0
+    dnum = dq0 / st["den"]
+    dden = -dq0 * st["q0"] / st["den"] * (st["den0"] > 1e-8)
+    dhe = 2.0 * dnum + 2.0 * dden
+    dld = st["c1"] * dnum
+    mp = st["pp"] > st["H"]
+    dpp = dhe * mp
+    dH = dhe * ~mp
+    ml = st["lp"] > 0
+    dlp = dpp / st["ss"] * ml
+    dssum = -dpp * st["lp"] / st["ss"] ** 2 * ml * (st["ssum"] > 1e-12)
+    dkt = st["nf"] * dssum
+    # inactive voxels pass d through and depend on nothing else
+    dd = np.where(a, dd, Db)
+    dH, dld, dlp, dkt = (np.where(a, v, 0.0) for v in (dH, dld, dlp, dkt))
+    # ld and lp: +1 for each occupied face neighbour j, -1 per neighbour for i
+    dd = dd - st["nf"] * dld
+    dphi = -st["nf"] * dlp
+    for axis, step in _FACES:
+        o = _shift(occ, axis, step, False)
+        # voxel i uses neighbour j = i + step e; j receives i's coefficient
+        dd += _shift(np.where(o, dld, 0.0), axis, -step)
+        dphi += _shift(np.where(o, dlp, 0.0), axis, -step)
+    return dd, dH, dphi, dkt
+
+
+def _with_kappa(isl, kt32):
+    import dataclasses
+    return dataclasses.replace(isl, kappa=np.ascontiguousarray(kt32, np.float32))
+
+
+def _branches(fr, ps, H, st):
+    """The branch decisions of one step (max / min / clamp choices), for
+    finite-difference checks that must not straddle a kink."""
+    a = st["active"]
+    return np.concatenate([m.ravel() for m in (
+        fr & (ps > H), (st["v1"] > st["d"]) & a, ((st["v0"] > 0) & (st["v0"] < 1)) & a,
+        (st["q1"] < 1) & a, (st["q0"] > st["d"]) & a, (st["pp"] > st["H"]) & a,
+        (st["lp"] > 0) & a, a)])
+
+
+def load_path_damage(isl, steps=10, b=None, signature=False):
+    """(L, dL/dkappa) of the damage-coupled load path, FP64, direct solves;
+    with signature, also every step's branch decisions (_branches)."""
+    fr = free_mask(isl).astype(bool)
+    occ = isl.vtype != 0
+    b = isl.source if b is None else b
+    k0 = np.maximum(isl.kappa.astype(np.float64), 0.0)
+    d = np.zeros(isl.vtype.shape)
+    H = np.zeros(isl.vtype.shape)
+    L = 0.0
+    tape = []
+    for t in range(1, steps + 1):
+        kt = np.where(occ, k0 * damage_g(d), 0.0).astype(np.float32)
+        it = _with_kappa(isl, kt)
+        phi = direct_solve(it, (t / steps) * np.asarray(b, np.float64))
+        phi = np.where(fr, phi, 0.0)
+        Lt, w = flux_loss(it, fr.astype(np.uint8), phi)
+        L += Lt
+        ps, _ = psi64(occ, fr, phi, kt.astype(np.float64))
+        Hn = np.where(fr, np.maximum(H, ps), H)
+        dn, st = pf64(occ, fr, d, Hn, phi, kt.astype(np.float64))
+        tape.append((it, kt, phi, w, d, H, ps, st))
+        d, H = dn, Hn
+    sig = np.concatenate([_branches(fr, ps_, H_, st_) for _, _, _, _, _, H_, ps_, st_ in tape])
+    grad = np.zeros(isl.vtype.shape)
+    Db = np.zeros(isl.vtype.shape)
+    Hb = np.zeros(isl.vtype.shape)
+    fr8 = fr.astype(np.uint8)
+    for it, kt, phi, w, dprev, Hprev, ps, st in reversed(tape):
+        k64 = kt.astype(np.float64)
+        dd, dH, dphi_pf, dkt_pf = pf64_vjp(occ, st, Db)
+        Hbt = Hb + dH
+        mpsi = fr & (ps > Hprev)
+        psib = np.where(mpsi, Hbt, 0.0)
+        Hb = np.where(mpsi, 0.0, Hbt)
+        dphi_psi, dkt_psi = psi64_vjp(occ, fr, phi, k64, psib)
+        gphi = flux_grad_phi(it, fr8, phi, w) + np.where(fr, dphi_pf + dphi_psi, 0.0)
+        lam = direct_solve(it, gphi)
+        lam = np.where(fr, lam, 0.0)
+        kbar = (flux_grad_kappa(it, fr8, phi, w) - energy_grad_kappa(it, lam, phi)
+                + dkt_pf + dkt_psi)
+        kbar = np.where(occ, kbar, 0.0)
+        grad += kbar * damage_g(dprev)
+        Db = dd + kbar * k0 * damage_dg(dprev)
+    grad = np.where(isl.kappa > 0, grad, 0.0)
+    return (L, grad, sig) if signature else (L, grad)

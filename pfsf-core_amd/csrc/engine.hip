@@ -37,7 +37,6 @@ constexpr int kRbgsSweeps = 16;     // STEPS_MAJOR (NP/src/core/constants.h:37)
 // phase-field parameters of the reference engine (NP/src/solver/dispatcher.cpp:
 // 240-248, NP/src/core/constants.h:81-86): l0 = max(PHASE_FIELD_L0, 2),
 // G_C_CONCRETE, PHASE_FIELD_RELAX, spectral split on
-constexpr float kPfL0 = 2.0f, kPfGc = 100.0f, kPfRelax = 0.3f;
 
 struct HostBufs {
   bool registered = false;

@@ -119,6 +119,12 @@ class MgSolver {
   void allocate_pooled(DevPool& pool);
   hipError_t build_pooled(hipStream_t s);
   hipError_t update(const uint8_t* type, const float* kappa);
+  // New kappa, same type: every level's operator is rebuilt on the device
+  // with the tile lists and captured iteration graphs kept (the free set
+  // follows type, and the couplings' sign pattern with kappa's).  Where the
+  // free set does move (a coupling that vanishes or appears), a full
+  // update() follows.  Ordered after the caller's stream; standalone only.
+  hipError_t refresh_kappa(const float* kappa);
   hipError_t solve(const float* b, double* x64, float* phi32, pfsf_solve_info* info);
   hipError_t profile(int which, int reps, float* avg_ms);
   int levels() const { return nlev_; }
@@ -152,6 +158,8 @@ class MgSolver {
   T* alloc(int64_t n, hipError_t& err);
   void zero_vectors();
   hipError_t build();
+  void setup_operator(hipStream_t s);  // kappa-dependent part of build()
+  hipError_t wait_caller();            // stream_ after the caller's stream
   hipError_t build_tiles(MgLevel& L);
   hipError_t precondition(int c);       // z = M r[c], r.z
   hipError_t true_residual(const float* b, double* rel, int c, int seq = 0);  // b - A x into r[c]
@@ -198,6 +206,7 @@ class MgSolver {
   double* prz_ = nullptr;  // r.z (preconditioner)
   uint8_t* reach_ = nullptr;
   int* flag_ = nullptr;
+  uint8_t* fr_snap_ = nullptr;  // refresh_kappa: every level's free set before
   double* x_ = nullptr;  // FP64 iterate
   float* r_[2] = {nullptr, nullptr};
   float* p_[2] = {nullptr, nullptr};

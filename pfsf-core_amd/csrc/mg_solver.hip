@@ -389,6 +389,14 @@ __global__ void __launch_bounds__(BS)
   if (!f) stk(kap, i, -1.0f);
 }
 
+// flag = 1 where a level's free set differs from its copy before.
+__global__ void __launch_bounds__(BS)
+    k_fr_moved(int64_t n, const uint8_t* __restrict__ before, const uint8_t* __restrict__ fr,
+               int* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n && before[i] != fr[i]) *flag = 1;
+}
+
 // Anchor-coded kappa of the finest level (MgLevel::ks) from the final free
 // set: free kappa as is, anchors -max(kappa, FLT_MIN) (an anchor of kappa 0
 // still couples through its face weight), everything else -inf.
@@ -869,64 +877,32 @@ hipError_t MgSolver::build_tiles(MgLevel& L) {
   return err;
 }
 
+// type and kappa may still be in flight on the caller's stream (an upload
+// queued just before): set-up work waits for it.
+hipError_t MgSolver::wait_caller() {
+  hipError_t err = hipSuccess;
+  hipStream_t caller = (hipStream_t)desc_.stream;
+  if (caller == stream_) return err;
+  hipEvent_t ev = nullptr;
+  chk(hipEventCreateWithFlags(&ev, hipEventDisableTiming), err);
+  if (ev) {
+    chk(hipEventRecord(ev, caller), err);
+    chk(hipStreamWaitEvent(stream_, ev, 0), err);
+    (void)hipEventDestroy(ev);
+  }
+  return err;
+}
+
 hipError_t MgSolver::build() {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
   drop_graphs();
-  // type and kappa may still be in flight on the caller's stream (an upload
-  // queued just before): the build waits for it
-  hipStream_t caller = (hipStream_t)desc_.stream;
-  if (caller != s) {
-    hipEvent_t ev = nullptr;
-    chk(hipEventCreateWithFlags(&ev, hipEventDisableTiming), err);
-    if (ev) {
-      chk(hipEventRecord(ev, caller), err);
-      chk(hipStreamWaitEvent(s, ev, 0), err);
-      (void)hipEventDestroy(ev);
-    }
-    if (err != hipSuccess) return err;
-  }
+  if (chk(wait_caller(), err)) return err;
   // q_ is free until the first solve: it carries the component parents and
   // is cleared again afterwards (every work vector starts at zero)
   chk(anchored_mask(d0_, type_, reach_, (uint32_t*)q_, flag_, s), err);
   fill0(q_, n0_, s);
-  MgLevel& F = levels_[0];
-  k_fine_setup<<<blocks_for(n0_, BS), BS, 0, s>>>(d0_, type_, kappa_, reach_, (float*)F.kap,
-                                                  F.am, F.fr);
-  for (int l = 1; l < nlev_; l++) {
-    MgLevel& A = levels_[l - 1];
-    MgLevel& B = levels_[l];
-    unsigned nb = blocks_for(B.n, BS);
-    if (!A.bf16 && !B.bf16)
-      k_coarsen<float, float><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
-                                                (float*)B.kap, B.am, B.fr);
-    else if (!A.bf16 && B.bf16)
-      k_coarsen<float, uint16_t><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
-                                                   (uint16_t*)B.kap, B.am, B.fr);
-    else
-      k_coarsen<uint16_t, uint16_t><<<nb, BS, 0, s>>>(A.d, (const uint16_t*)A.kap, A.am, A.fr,
-                                                      B.d, (uint16_t*)B.kap, B.am, B.fr);
-  }
-  for (int l = 1; l < nlev_; l++) {
-    const MgLevel& A = levels_[l - 1];
-    MgLevel& B = levels_[l];
-    k_coarse_mask<<<blocks_for(B.n, BS), BS, 0, s>>>(A.d, A.fr, A.msk, B.d, B.fr, B.msk);
-  }
-  // diagonals need every level's free flags and masks: computed after both
-  for (int l = 0; l < nlev_; l++) {
-    MgLevel& L = levels_[l];
-    unsigned nb = blocks_for(L.n, BS);
-    if (L.bf16) {
-      k_level_diag<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(L), L.fr, L.dg);
-      k_sync_free<uint16_t><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (uint16_t*)L.kap);
-    } else {
-      k_level_diag<float><<<nb, BS, 0, s>>>(view<float>(L), L.fr, L.dg);
-      k_sync_free<float><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (float*)L.kap);
-    }
-  }
-  if (F.ks)
-    k_stream_kappa<<<blocks_for(n0_, BS), BS, 0, s>>>(n0_, type_, kappa_, F.fr,
-                                                      (const float*)F.kap, F.ks);
+  setup_operator(s);
   chk(hipGetLastError(), err);
   for (int l = 0; l < nlev_ && err == hipSuccess; l++) chk(build_tiles(levels_[l]), err);
   // Smoother weights and the Chebyshev interval use the Gershgorin bound
@@ -965,10 +941,85 @@ hipError_t MgSolver::build() {
   return err;
 }
 
+// The kappa-dependent set-up of every level: fine kappa and anchor weights,
+// coarsened operators, diagonals (the free set follows them) and the finest
+// level's anchor-coded kappa.
+void MgSolver::setup_operator(hipStream_t s) {
+  MgLevel& F = levels_[0];
+  k_fine_setup<<<blocks_for(n0_, BS), BS, 0, s>>>(d0_, type_, kappa_, reach_, (float*)F.kap,
+                                                  F.am, F.fr);
+  for (int l = 1; l < nlev_; l++) {
+    MgLevel& A = levels_[l - 1];
+    MgLevel& B = levels_[l];
+    unsigned nb = blocks_for(B.n, BS);
+    if (!A.bf16 && !B.bf16)
+      k_coarsen<float, float><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
+                                                (float*)B.kap, B.am, B.fr);
+    else if (!A.bf16 && B.bf16)
+      k_coarsen<float, uint16_t><<<nb, BS, 0, s>>>(A.d, (const float*)A.kap, A.am, A.fr, B.d,
+                                                   (uint16_t*)B.kap, B.am, B.fr);
+    else
+      k_coarsen<uint16_t, uint16_t><<<nb, BS, 0, s>>>(A.d, (const uint16_t*)A.kap, A.am, A.fr,
+                                                      B.d, (uint16_t*)B.kap, B.am, B.fr);
+  }
+  for (int l = 1; l < nlev_; l++) {
+    const MgLevel& A = levels_[l - 1];
+    MgLevel& B = levels_[l];
+    k_coarse_mask<<<blocks_for(B.n, BS), BS, 0, s>>>(A.d, A.fr, A.msk, B.d, B.fr, B.msk);
+  }
+  // diagonals need every level's free flags and masks: computed after both
+  for (int l = 0; l < nlev_; l++) {
+    MgLevel& L = levels_[l];
+    unsigned nb = blocks_for(L.n, BS);
+    if (L.bf16) {
+      k_level_diag<uint16_t><<<nb, BS, 0, s>>>(view<uint16_t>(L), L.fr, L.dg);
+      k_sync_free<uint16_t><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (uint16_t*)L.kap);
+    } else {
+      k_level_diag<float><<<nb, BS, 0, s>>>(view<float>(L), L.fr, L.dg);
+      k_sync_free<float><<<nb, BS, 0, s>>>(L.n, L.dg, L.fr, (float*)L.kap);
+    }
+  }
+  if (F.ks)
+    k_stream_kappa<<<blocks_for(n0_, BS), BS, 0, s>>>(n0_, type_, kappa_, F.fr,
+                                                      (const float*)F.kap, F.ks);
+}
+
 hipError_t MgSolver::update(const uint8_t* type, const float* kappa) {
   type_ = type;
   kappa_ = kappa;
   return build();
+}
+
+hipError_t MgSolver::refresh_kappa(const float* kappa) {
+  hipError_t err = hipSuccess;
+  if (pool_ || !sc_) return hipErrorNotSupported;
+  hipStream_t s = stream_;
+  int64_t total = 0;
+  for (const MgLevel& L : levels_) total += L.n;
+  if (!fr_snap_) {
+    fr_snap_ = alloc<uint8_t>(total, err);
+    if (err != hipSuccess) return err;
+  }
+  kappa_ = kappa;
+  if (chk(wait_caller(), err)) return err;
+  int64_t off = 0;
+  for (const MgLevel& L : levels_) {
+    chk(hipMemcpyAsync(fr_snap_ + off, L.fr, L.n, hipMemcpyDeviceToDevice, s), err);
+    off += L.n;
+  }
+  chk(hipMemsetAsync(flag_, 0, sizeof(int), s), err);
+  setup_operator(s);
+  off = 0;
+  for (const MgLevel& L : levels_) {
+    k_fr_moved<<<blocks_for(L.n, BS), BS, 0, s>>>(L.n, fr_snap_ + off, L.fr, flag_);
+    off += L.n;
+  }
+  chk(hipGetLastError(), err);
+  int moved = 0;
+  chk(hipMemcpyAsync(&moved, flag_, sizeof(int), hipMemcpyDeviceToHost, s), err);
+  chk(hipStreamSynchronize(s), err);
+  if (err != hipSuccess) return err;
+  return moved ? build() : hipSuccess;
 }
 
 // ── launch helpers: dispatch on tile depth and kappa storage ───────────

@@ -11,6 +11,11 @@ namespace pfsf {
 constexpr float kEdgeP = 0.5f;          // PFSFStencil.java:42
 constexpr float kCornerP = 0.1666667f;  // PFSFStencil.java:43
 
+// Phase-field damage step of a tick (the dispatcher's values,
+// NP/src/solver/dispatcher.cpp:243-246): length scale l0, fracture energy Gc,
+// relaxation.
+constexpr float kPfL0 = 2.0f, kPfGc = 100.0f, kPfRelax = 0.3f;
+
 // Stencil offsets in PFSFStencil.NEIGHBOR_OFFSETS order (PFSFStencil.java:70-84):
 // 6 faces, 12 edges (XY, XZ, YZ), 8 corners.  Encoded as dx,dy,dz in {-1,0,1}.
 #define PFSF_OFFSETS                                                              \

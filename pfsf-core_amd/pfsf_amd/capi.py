@@ -139,7 +139,7 @@ class LoadPathDesc(C.Structure):
     _fields_ = [("dims", Dims), ("type", C.c_void_p), ("kappa", C.c_void_p),
                 ("source", C.c_void_p), ("rcomp", C.c_void_p), ("steps", C.c_int32),
                 ("checkpoint_every", C.c_int32), ("tol", C.c_double), ("max_iters", C.c_int32),
-                ("stream", C.c_void_p)]
+                ("stream", C.c_void_p), ("damage", C.c_int32)]
 
 
 class LoadPathInfo(C.Structure):

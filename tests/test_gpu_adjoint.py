@@ -6,7 +6,9 @@ are pinned by finite differences in test_oracle.py) -- loss within 1e-6
 relative (FP64 variant) / 1e-5 (FP32 forward), gradient within 1e-4 relative
 L2; on the 96^3 island central differences of the FP64 loss (h = 1e-3 kappa_i)
 agree with the gradient on 64 random voxels within 2e-3 of the largest
-gradient entry; and the torch.autograd front end returns the same gradient."""
+gradient entry; and the torch.autograd front end returns the same gradient.
+The damage-coupled path (damage degrades the next state's kappa) meets the
+same loss and gradient bars against the oracle's FP64 damage path."""
 import numpy as np
 import pytest
 import torch
@@ -79,3 +81,33 @@ def test_autograd_function(cuda):
     _, g = A.load_path_grad(vt, kp, b, rc, 8, 4, tol=1e-9)
     assert torch.allclose(k.grad.double(), g.to(torch.float32).double(), rtol=1e-6, atol=0)
     assert abs(L.item() - A.load_path_loss(vt, kp, b, rc, 8, 4, tol=1e-9).loss) == 0.0
+
+
+@pytest.mark.parametrize("seed,L,steps,every", [(21, 7, 4, 3), (24, 10, 8, 3)])
+def test_damage_gradient_matches_exact_fp64(cuda, seed, L, steps, every):
+    isl = _loaded(seed, L, load=400.0)
+    ref_L, ref_g = O.load_path_damage(isl, steps=steps)
+    und_L, und_g = O.load_path(isl, steps=steps)
+    # the damage changes the path: the gradient moves far beyond the bar below
+    assert np.linalg.norm(ref_g - und_g) >= 0.1 * np.linalg.norm(und_g)
+    vt, kp, b, rc = _dev(isl, cuda)
+    i64 = A.load_path_loss(vt, kp, b, rc, steps, every, tol=1e-10, fp64=True, damage=True)
+    assert abs(i64.loss - ref_L) <= 1e-6 * ref_L, (i64.loss, ref_L)
+    info, g = A.load_path_grad(vt, kp, b, rc, steps, every, tol=1e-10, damage=True)
+    assert abs(info.loss - ref_L) <= 1e-5 * ref_L, (info.loss, ref_L)
+    assert info.forward_solves == steps and info.recompute_solves == steps
+    assert info.adjoint_solves == steps and info.checkpoints == (steps + every - 1) // every
+    g = g.cpu().numpy()
+    err = np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g)
+    print(f"damage seed {seed}: loss {info.loss:.6e} (undamaged {und_L:.6e}) grad rel-L2 {err:.2e}")
+    assert err <= 1e-4, err
+
+
+def test_damage_autograd_function(cuda):
+    isl = _loaded(23, 10, load=400.0)
+    vt, kp, b, rc = _dev(isl, cuda)
+    k = kp.clone().requires_grad_(True)
+    L = A.loss(k, vt, b, rc, steps=6, checkpoint_every=4, tol=1e-9, damage=True)
+    L.backward()
+    _, g = A.load_path_grad(vt, kp, b, rc, 6, 4, tol=1e-9, damage=True)
+    assert torch.allclose(k.grad.double(), g.to(torch.float32).double(), rtol=1e-6, atol=0)

@@ -378,3 +378,71 @@ def test_island_data_prep_known_answers():
     assert o["rcomp"][0, 1, 1] == 250 and o["rtens"][0, 2, 0] == 1
     assert np.isinf(o["max_phi"][0, 0, 0]) and o["max_phi"][0, 2, 1] == 0
     assert np.all(cond[:, 0, 2, 1] == 0) and o["source"][0, 2, 1] == 0
+
+
+def test_damage_steps_vjp_match_directional_derivatives():
+    """The reverse passes of the damage-coupled load path's per-step maps
+    (psi64_vjp, pf64_vjp) against central directional derivatives of psi64
+    and pf64 in FP64, with every branch exercised (split active, q clamped
+    at 1, damage changing)."""
+    isl = W.small_box_union(21, L=7)
+    occ = isl.vtype != 0
+    fr = O.free_mask(isl).astype(bool)
+    rng = np.random.default_rng(1)
+    shape = isl.vtype.shape
+    phi = np.where(fr, rng.random(shape) * 30, 0.0)
+    kt = np.where(occ, rng.uniform(0.5, 2, shape), 0.0)
+    d = np.where(fr, rng.uniform(0, 0.6, shape), 0.0)
+    H = np.where(fr, rng.uniform(0, 3, shape), 0.0)
+    Db = np.where(fr, rng.standard_normal(shape), 0.0)
+    out, st = O.pf64(occ, fr, d, H, phi, kt)
+    a = st["active"]
+    assert ((st["pp"] > H) & a).any() and ((st["q1"] >= 1) & a).any() and (out != d).any()
+    grads = dict(zip(("d", "H", "phi", "kt"), O.pf64_vjp(occ, st, Db)))
+    for name in ("d", "H", "phi", "kt"):
+        v = np.where(occ if name == "kt" else fr, rng.standard_normal(shape), 0.0)
+        args = dict(d=d, H=H, phi=phi, kt=kt)
+        hi, lo = dict(args), dict(args)
+        hi[name], lo[name] = args[name] + 1e-7 * v, args[name] - 1e-7 * v
+        fd = ((O.pf64(occ, fr, **hi)[0] - O.pf64(occ, fr, **lo)[0]) * Db).sum() / 2e-7
+        an = (grads[name] * v).sum()
+        assert abs(fd - an) <= 1e-6 * abs(an), (name, fd, an)
+    psib = rng.standard_normal(shape)
+    dphi, dkt = O.psi64_vjp(occ, fr, phi, kt, psib)
+    for name, an_g in (("phi", dphi), ("kt", dkt)):
+        v = np.where(fr if name == "phi" else occ, rng.standard_normal(shape), 0.0)
+        f = (lambda e: O.psi64(occ, fr, phi + e * v, kt)[0]) if name == "phi" else \
+            (lambda e: O.psi64(occ, fr, phi, kt + e * v)[0])
+        fd = ((f(1e-6) - f(-1e-6)) * psib).sum() / 2e-6
+        an = (an_g * v).sum()
+        assert abs(fd - an) <= 1e-6 * abs(an), (name, fd, an)
+
+
+def test_damage_load_path_gradient_matches_finite_differences():
+    """dL/dkappa of the damage-coupled load path (damage degrades kappa
+    between steps) against central differences of L, FP64, exact solves.
+    The map is piecewise smooth (max / min / clamps), so a voxel whose two
+    perturbed runs take different branches is skipped; the couplings round
+    kappa products to float32, hence the 2e-3 of the largest entry."""
+    isl = _loaded(21, 7, load=400.0)
+    L, g, sig = O.load_path_damage(isl, steps=4, signature=True)
+    rng = np.random.Generator(np.random.PCG64(3))
+    cells = np.flatnonzero(isl.vtype.ravel() != 0)
+    gmax = np.abs(g).max()
+    checked = 0
+    for m in rng.choice(cells, size=16, replace=False):
+        k0 = isl.kappa.ravel()[m]
+        vals, ks, sigs = [], [], []
+        for sgn in (1, -1):
+            isl.kappa.ravel()[m] = k0 * (1 + sgn * 3e-4)
+            ks.append(float(isl.kappa.ravel()[m]))
+            Lh, _, sh = O.load_path_damage(isl, steps=4, signature=True)
+            vals.append(Lh)
+            sigs.append(sh)
+        isl.kappa.ravel()[m] = k0
+        if not (np.array_equal(sigs[0], sig) and np.array_equal(sigs[1], sig)):
+            continue
+        checked += 1
+        fd = (vals[0] - vals[1]) / (ks[0] - ks[1])
+        assert abs(fd - g.ravel()[m]) <= 2e-3 * gmax, (m, fd, g.ravel()[m], gmax)
+    assert checked >= 12, checked
