@@ -73,6 +73,27 @@ __device__ __forceinline__ float kapf(float k) { return k > 0.0f ? k : 0.0f; }
 // transposes of each other.
 __device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 
+// ── kernel-argument batching ───────────────────────────────────────────
+// Kernel arguments arrive by scalar loads from the kernarg segment, issued
+// where a value is first used.  A use behind a branch, or after a wait for
+// another argument, puts a separate memory round trip into the prologue; the
+// iteration kernels showed five or six dependent round trips before their
+// first vector load.  pin_args(...) at kernel entry makes every argument live
+// in SGPRs at that point, so all of them load as one batch.
+__device__ __forceinline__ void pin1(const void* v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void pin1(int v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void pin1(unsigned v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void pin1(float v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void pin1(pfsf_dims d) {
+  pin1(d.lx);
+  pin1(d.ly);
+  pin1(d.lz);
+}
+template <typename... T>
+__device__ __forceinline__ void pin_args(const T&... a) {
+  (pin1(a), ...);
+}
+
 template <typename KT>
 struct Lv {
   pfsf_dims d;
@@ -93,10 +114,28 @@ constexpr uint32_t kAllCouplings = 0x07FFFFFFu;
 
 struct TL {
   int ntx, nty;
-  const int* ids;
-  int n;  // entries in ids (the launch list, a multiple of 8)
+  const int* ids;  // null: every tile of the level, in XCD-chunked order (tile_origin)
+  int n;  // entries in ids (the launch list, a multiple of 8); without ids, the tile count
   const uint32_t* rows;  // per-slot row bits of the finest streaming level, or null
 };
+
+template <typename KT>
+__device__ __forceinline__ void pin1(const Lv<KT>& L) {
+  pin1(L.d);
+  pin1(L.kap);
+  pin1(L.dg);
+  pin1(L.am);
+  pin1(L.msk);
+  pin1(L.scale);
+  pin1(L.omega);
+}
+__device__ __forceinline__ void pin1(const TL& M) {
+  pin1(M.ntx);
+  pin1(M.nty);
+  pin1(M.ids);
+  pin1(M.n);
+  pin1(M.rows);
+}
 
 // Row bits of one column of the finest level (list slot g, words
 // rows[g * kRowWords ...]): word sg * 10 + yy describes y = y0 - 1 + yy for
@@ -131,10 +170,20 @@ __device__ __forceinline__ float free_s(float k) {
 }
 __device__ __forceinline__ bool is_free_s(float s) { return __float_as_int(s) >= 0; }
 
+// Work-group g takes list entry g, or, without a list, tile (g % 8) * chunk
+// + g / 8 of the whole level (chunk = ceil(tiles / 8), the launch being
+// 8 chunk work-groups): the same XCD-chunked order without the dependent load
+// of a list entry before the operand loads.
 template <int TZ>
 __device__ __forceinline__ bool tile_origin(const TL& M, int& x0, int& y0, int& z0) {
-  int t = M.ids[blockIdx.x];
-  if (t < 0) return false;
+  int t;
+  if (M.ids) {
+    t = M.ids[blockIdx.x];
+    if (t < 0) return false;
+  } else {
+    t = (int)(blockIdx.x & 7u) * ((M.n + 7) >> 3) + (int)(blockIdx.x >> 3);
+    if (t >= M.n) return false;
+  }
   int tx = t % M.ntx;
   int r = t / M.ntx;
   x0 = tx * TX;
@@ -713,26 +762,17 @@ __device__ __forceinline__ bool convergence_test(PcgScalars* sc, const double* _
   return stop;
 }
 
-template <typename KT>
-__device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __restrict__ r,
-                                              float* __restrict__ out,
-                                              const float* __restrict__ coef, int steps,
-                                              PcgScalars* sc, const double* __restrict__ rr_part,
-                                              unsigned rr_n,
-                                              unsigned long long* mirror = nullptr) {
+// The Chebyshev steps of the coarsest level held in LDS: thread i owns cell
+// i (act: i < n) and brings its own operands (kappa, diagonal, anchor mass,
+// coupling mask, rhs) and step coefficient c_ld = coef[i] (i < 2 * steps);
+// returns the cell's iterate, 0 outside the free set.  Every thread of the
+// work-group takes part (barriers).
+__device__ __forceinline__ float cheb_lds_run(pfsf_dims d, float scale, bool act, float k_ld,
+                                              float D_ld, float m_ld, uint32_t mk_ld, float r_ld,
+                                              float c_ld, int steps) {
   __shared__ float ss[kMaxLdsPadded], sx[2][kMaxLdsPadded];
   __shared__ float cf[128];
-  const pfsf_dims d = L.d;
-  const int n = d.lx * d.ly * d.lz;
   const int i = threadIdx.x;
-  const bool act = i < n;
-  const int ic = act ? i : 0;  // cell 0 stands in for an idle thread
-  const float k_ld = ldk(L.kap, ic);
-  const float D_ld = L.dg[ic], m_ld = L.am[ic], r_ld = r[ic];
-  const uint32_t mk_ld = L.msk ? L.msk[ic] : kAllCouplings;
-  const float c_ld = i < 2 * steps ? coef[i] : 0.f;
-  if (sc->done) return;
-  if (rr_n > 0 && convergence_test(sc, rr_part, rr_n, mirror)) return;
   const int sy = d.lx + 2, sz = sy * (d.ly + 2), np = sz * (d.lz + 2);
   for (int e = threadIdx.x; e < np; e += blockDim.x) {
     ss[e] = -0.0f;
@@ -769,7 +809,7 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   for (int st = 1; st < steps; st++) {
     const float c1 = cf[2 * st], c2 = cf[2 * st + 1];
     if (live) {
-      const float ax = stencil_apply_w(sx[cur], e, sy, sz, W, m, L.scale);
+      const float ax = stencil_apply_w(sx[cur], e, sy, sz, W, m, scale);
       di = c1 * di + c2 * (ri - ax) * rcp(D);
       xi = xi + di;
       sx[cur ^ 1][e] = xi;
@@ -777,7 +817,29 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
     __syncthreads();
     cur ^= 1;
   }
-  if (act) out[i] = live ? xi : 0.f;
+  return live ? xi : 0.f;
+}
+
+template <typename KT>
+__device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __restrict__ r,
+                                              float* __restrict__ out,
+                                              const float* __restrict__ coef, int steps,
+                                              PcgScalars* sc, const double* __restrict__ rr_part,
+                                              unsigned rr_n,
+                                              unsigned long long* mirror = nullptr) {
+  const pfsf_dims d = L.d;
+  const int n = d.lx * d.ly * d.lz;
+  const int i = threadIdx.x;
+  const bool act = i < n;
+  const int ic = act ? i : 0;  // cell 0 stands in for an idle thread
+  const float k_ld = ldk(L.kap, ic);
+  const float D_ld = L.dg[ic], m_ld = L.am[ic], r_ld = r[ic];
+  const uint32_t mk_ld = L.msk ? L.msk[ic] : kAllCouplings;
+  const float c_ld = i < 2 * steps ? coef[i] : 0.f;
+  if (sc->done) return;
+  if (rr_n > 0 && convergence_test(sc, rr_part, rr_n, mirror)) return;
+  const float xi = cheb_lds_run(d, L.scale, act, k_ld, D_ld, m_ld, mk_ld, r_ld, c_ld, steps);
+  if (act) out[i] = xi;
 }
 
 // ── FP64 residual of the iterate ───────────────────────────────────────

@@ -29,6 +29,10 @@ constexpr int kTX = 32, kTY = 8;
 // Measured on the 128^3 island: 6 steps need 40 PCG iterations, 8 need 41 and
 // 12 need 39 at ~0.9 us more per step and iteration.
 constexpr int kCoarseSweeps = 6;
+// Degree of the coarsest-level polynomial when it is applied as a dense
+// matrix (standalone solvers whose coarsest level fits one work-group): the
+// degree costs nothing per iteration there, only at set-up.
+constexpr int kCoarseSweepsDense = 24;
 
 // Smallest relative drop of the recursive residual a solve pass aims for
 // before the true FP64 residual is checked (a pass stops at
@@ -62,6 +66,10 @@ struct TileList {
   int ntx = 0, nty = 0, ntz = 0;  // tiles per axis
   int nlaunch = 0;                // launch size (multiple of 8, padded with -1)
   int nactive = 0;                // tiles holding at least one free voxel
+  // launches cover every tile of the level in XCD-chunked order, computed
+  // from the work-group index (no list read); coarse levels of a standalone
+  // solver whose tiles are nearly all active
+  bool dense = false;
   int* ids = nullptr;             // device: tile id per work-group, XCD-grouped
   std::vector<int> host_active;   // host: active tile ids, ascending
   // finest level of a standalone solver (columns of at most 14 planes):
@@ -100,6 +108,10 @@ struct MgLevel {
   float* e = nullptr;     // correction returned to the finer level (levels > 0)
   float* xb = nullptr;    // Chebyshev ping-pong (coarsest)
   float* dv = nullptr;    // Chebyshev direction (coarsest)
+  // coarsest level of a standalone solver: its Chebyshev polynomial as a
+  // dense symmetric matrix, n rows of pm_ld floats (k_cheb_dense)
+  float* pm = nullptr;
+  int pm_ld = 0;
   float scale = 1.f;      // coupling scale 2^level
   float omega = 1.f;      // damped-Jacobi weight
   float lmax = 2.f;       // estimate of lambda_max(D^-1 A)
@@ -190,6 +202,9 @@ class MgSolver {
   bool bf16_ = false;
   int nlev_ = 1;
   int coarse_sweeps_ = 12;
+  // standalone, not batchable: coarse levels launch over all their tiles and
+  // the coarsest solve is a dense matrix when it fits one work-group
+  bool dense_ok_ = false;
   std::vector<MgLevel> levels_;
   std::vector<float> coef_host_;
   std::vector<void*> owned_;

@@ -146,6 +146,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
            const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
            float* __restrict__ rc, PcgScalars* sc, const double* __restrict__ pq_part,
            unsigned pq_n, double* partials) {
+  pin_args(L, M, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n, partials);
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
   down_body<TZ, KT, UPDATE, MASK>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part,
@@ -156,6 +157,7 @@ template <int TZ, typename KT, bool FINEST, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_up(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ r, const float* __restrict__ ec,
          float* __restrict__ out, const PcgScalars* sc, double* partials) {
+  pin_args(L, M, c, r, ec, out, sc, partials);
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
   up_body<TZ, KT, FINEST, MASK>(L, have, x0, y0, z0, c, r, ec, out, sc, partials, blockIdx.x);
@@ -166,7 +168,89 @@ __global__ void __launch_bounds__(kMaxLdsCoarse)
     k_cheb_lds(Lv<KT> L, const float* __restrict__ r, float* __restrict__ out,
                const float* __restrict__ coef, int steps, PcgScalars* sc,
                const double* __restrict__ rr_part, unsigned rr_n, unsigned long long* mirror) {
+  pin_args(L, r, out, coef, steps, sc, rr_part, rr_n, mirror);
   cheb_lds_body<KT>(L, r, out, coef, steps, sc, rr_part, rr_n, mirror);
+}
+
+// ── coarsest level as a dense matrix ──────────────────────────────────
+// The coarsest solve is a fixed polynomial in the coarsest operator: `steps`
+// Chebyshev steps from a zero guess give x = P r with P = p(D^-1 A) D^-1,
+// symmetric in exact arithmetic.  P is formed once per operator (column j =
+// the steps applied to the unit vector of cell j), symmetrised, and applied
+// each V-cycle as a matrix-vector product spread over many work-groups, in
+// place of a one-work-group chain of `steps` dependent stencil passes.  Rows
+// are padded to ld (a multiple of 256) with zeros; cells outside the free
+// set have zero rows and columns.
+
+// Column j of P, stored as row j (blockIdx.x = j; kMaxLdsCoarse threads).
+template <typename KT>
+__global__ void __launch_bounds__(kMaxLdsCoarse)
+    k_cheb_columns(Lv<KT> L, const float* __restrict__ coef, int steps, float* __restrict__ pm,
+                   int ld) {
+  const pfsf_dims d = L.d;
+  const int n = d.lx * d.ly * d.lz;
+  const int j = blockIdx.x, i = threadIdx.x;
+  if (ldk(L.kap, j) < 0.f) return;  // outside the free set: a zero column (pre-filled)
+  const bool act = i < n;
+  const int ic = act ? i : 0;
+  const float k_ld = ldk(L.kap, ic);
+  const float D_ld = L.dg[ic], m_ld = L.am[ic];
+  const uint32_t mk_ld = L.msk ? L.msk[ic] : kAllCouplings;
+  const float c_ld = i < 2 * steps ? coef[i] : 0.f;
+  const float xi = cheb_lds_run(d, L.scale, act, k_ld, D_ld, m_ld, mk_ld, i == j ? 1.f : 0.f,
+                                c_ld, steps);
+  if (act) pm[(int64_t)j * ld + i] = xi;
+}
+
+// P <- (P + P^T) / 2: thread (i, j) with i < j owns both entries.
+__global__ void __launch_bounds__(BS) k_sym_dense(float* __restrict__ pm, int n, int ld) {
+  const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  const int i = (int)(t / n), j = (int)(t % n);
+  if (i >= j) return;
+  const float a = pm[(int64_t)i * ld + j], b = pm[(int64_t)j * ld + i];
+  const float v = 0.5f * (a + b);
+  pm[(int64_t)i * ld + j] = v;
+  pm[(int64_t)j * ld + i] = v;
+}
+
+// out = P r: one row per wave, NP passes of 64 lanes x 4 columns (ld = 256 NP).
+// With rr_n > 0, work-group 0 first runs the PCG iteration's convergence test
+// (as the one-work-group coarsest solve does); the other work-groups form
+// their rows regardless, and the up-sweeps behind this launch read the stop.
+template <int NP>
+__global__ void __launch_bounds__(BS)
+    k_cheb_dense(const float* __restrict__ pm, int n, const float* __restrict__ r,
+                 float* __restrict__ out, PcgScalars* sc, const double* __restrict__ rr_part,
+                 unsigned rr_n, unsigned long long* mirror) {
+  pin_args(pm, n, r, out, sc, rr_part, rr_n, mirror);
+  constexpr int ld = 256 * NP;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  const int rw = row < n ? row : 0;  // row 0 stands in for a row past the end
+  float4 a[NP];
+  float rv[NP][4];
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const int k = p * 256 + lane * 4;
+    a[p] = *reinterpret_cast<const float4*>(pm + (int64_t)rw * ld + k);
+#pragma unroll
+    for (int c = 0; c < 4; c++) rv[p][c] = r[min(k + c, n - 1)];  // zero-padded columns
+  }
+  if (sc->done) return;
+  if (rr_n > 0 && blockIdx.x == 0 && convergence_test(sc, rr_part, rr_n, mirror)) return;
+  float acc = 0.f;
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const int k = p * 256 + lane * 4;
+    acc += a[p].x * (k < n ? rv[p][0] : 0.f);
+    acc += a[p].y * (k + 1 < n ? rv[p][1] : 0.f);
+    acc += a[p].z * (k + 2 < n ? rv[p][2] : 0.f);
+    acc += a[p].w * (k + 3 < n ? rv[p][3] : 0.f);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0 && row < n) out[row] = acc;
 }
 
 // Jacobi-preconditioned variant of the update (single-level solve):
@@ -637,6 +721,9 @@ static Lv<float> sview(const MgLevel& l) {
 }
 
 static TL tl(const MgLevel& l) {
+  if (l.tiles.dense)
+    return TL{l.tiles.ntx, l.tiles.nty, nullptr, l.tiles.ntx * l.tiles.nty * l.tiles.ntz,
+              nullptr};
   return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids, l.tiles.nlaunch, l.tiles.rows};
 }
 
@@ -745,6 +832,10 @@ hipError_t MgSolver::allocate(bool standalone) {
     if (l == nlev_ - 1 && nlev_ > 1) {
       L.xb = alloc<float>(L.n, err);
       L.dv = alloc<float>(L.n, err);
+      if (dense_ok_ && L.n <= kMaxLdsCoarse && padded_cells(L.d) <= kMaxLdsPadded) {
+        L.pm_ld = (int)((L.n + 255) / 256 * 256);
+        L.pm = alloc<float>(L.n * L.pm_ld, err);
+      }
     }
     int64_t nt = (int64_t)L.tiles.ntx * L.tiles.nty * L.tiles.ntz;
     L.tiles.ids = alloc<int>(nt + 8, err);
@@ -789,10 +880,16 @@ void MgSolver::zero_vectors() {
 hipError_t MgSolver::create(const pfsf_solver_desc& desc, bool batchable) {
   hipError_t err = hipSuccess;
   if (nvox(desc.dims) > kMaxSolverCells) return hipErrorInvalidValue;
-  if (batchable)
+  if (batchable) {
     configure(desc, 2, kBatchTZFine, kBatchTZCoarse);
-  else
+  } else {
     configure(desc, 1, 0, 0);
+    dense_ok_ = true;
+    const MgLevel& C = levels_.back();
+    if (nlev_ > 1 && desc.coarse_sweeps <= 0 && C.n <= kMaxLdsCoarse &&
+        padded_cells(C.d) <= kMaxLdsPadded)
+      coarse_sweeps_ = kCoarseSweepsDense;
+  }
   // The solver runs on a stream of its own (default priority) so that its
   // iterations can be captured into a HIP graph; solve() orders it after the
   // caller's stream and waits for it before returning.
@@ -866,6 +963,10 @@ hipError_t MgSolver::build_tiles(MgLevel& L) {
   L.tiles.nactive = T;
   L.tiles.nlaunch = 8 * chunk;
   L.tiles.host_active = act;
+  // a coarse level whose tiles are nearly all active launches over all of
+  // them (empty tiles find no free voxel and write zeros)
+  L.tiles.dense = dense_ok_ && &L != &levels_[0] && (int64_t)T * 8 >= nt * 7;
+  if (L.tiles.dense) L.tiles.nlaunch = (int)((nt + 7) / 8 * 8);
   chk(hipMemcpyAsync(L.tiles.ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice,
                      stream_),
       err);
@@ -902,9 +1003,6 @@ hipError_t MgSolver::build() {
   // is cleared again afterwards (every work vector starts at zero)
   chk(anchored_mask(d0_, type_, reach_, (uint32_t*)q_, flag_, s), err);
   fill0(q_, n0_, s);
-  setup_operator(s);
-  chk(hipGetLastError(), err);
-  for (int l = 0; l < nlev_ && err == hipSuccess; l++) chk(build_tiles(levels_[l]), err);
   // Smoother weights and the Chebyshev interval use the Gershgorin bound
   // lambda_max(D^-1 A) <= 2 (each diagonal is at least the sum of its row's
   // off-diagonal magnitudes).  A bound, unlike an eigenvalue estimate, keeps
@@ -934,9 +1032,13 @@ hipError_t MgSolver::build() {
     }
     coef_host_ = cf;
     if (coef_)
-      chk(hipMemcpyAsync(coef_, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, s),
+      chk(hipMemcpyAsync(coef_, coef_host_.data(), sizeof(float) * coef_host_.size(),
+                         hipMemcpyHostToDevice, s),
           err);
   }
+  setup_operator(s);
+  chk(hipGetLastError(), err);
+  for (int l = 0; l < nlev_ && err == hipSuccess; l++) chk(build_tiles(levels_[l]), err);
   chk(hipStreamSynchronize(s), err);
   return err;
 }
@@ -982,6 +1084,19 @@ void MgSolver::setup_operator(hipStream_t s) {
   if (F.ks)
     k_stream_kappa<<<blocks_for(n0_, BS), BS, 0, s>>>(n0_, type_, kappa_, F.fr,
                                                       (const float*)F.kap, F.ks);
+  MgLevel& C = levels_[nlev_ - 1];
+  if (C.pm) {  // the coarsest polynomial (coefficients already on the stream)
+    const int st = std::min(coarse_sweeps_, 63);
+    const int64_t nn = C.n * (int64_t)C.pm_ld;
+    fill0(C.pm, nn, s);
+    if (C.bf16)
+      k_cheb_columns<uint16_t><<<(unsigned)C.n, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), coef_,
+                                                                        st, C.pm, C.pm_ld);
+    else
+      k_cheb_columns<float><<<(unsigned)C.n, kMaxLdsCoarse, 0, s>>>(view<float>(C), coef_, st,
+                                                                     C.pm, C.pm_ld);
+    k_sym_dense<<<blocks_for(C.n * C.n, BS), BS, 0, s>>>(C.pm, (int)C.n, C.pm_ld);
+  }
 }
 
 hipError_t MgSolver::update(const uint8_t* type, const float* kappa) {
@@ -1117,7 +1232,23 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
   for (int l = from; l < nlev_ - 1; l++)
     launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
-  if (lds) {
+  if (C.pm) {
+    const unsigned g = (unsigned)((C.n + BS / 64 - 1) / (BS / 64));
+    switch (C.pm_ld / 256) {
+      case 1:
+        k_cheb_dense<1><<<g, BS, 0, s>>>(C.pm, (int)C.n, C.r, C.e, sc_, prr_, rr_n, mirror_);
+        break;
+      case 2:
+        k_cheb_dense<2><<<g, BS, 0, s>>>(C.pm, (int)C.n, C.r, C.e, sc_, prr_, rr_n, mirror_);
+        break;
+      case 3:
+        k_cheb_dense<3><<<g, BS, 0, s>>>(C.pm, (int)C.n, C.r, C.e, sc_, prr_, rr_n, mirror_);
+        break;
+      default:
+        k_cheb_dense<4><<<g, BS, 0, s>>>(C.pm, (int)C.n, C.r, C.e, sc_, prr_, rr_n, mirror_);
+        break;
+    }
+  } else if (lds) {
     if (C.bf16)
       k_cheb_lds<uint16_t><<<1, kMaxLdsCoarse, 0, s>>>(view<uint16_t>(C), C.r, C.e, coef_, st,
                                                         sc_, prr_, rr_n, mirror_);

@@ -379,6 +379,7 @@ __global__ void __launch_bounds__(BS)
     ks_spmv(Lv<float> L, TL M, const float* __restrict__ z, const float* __restrict__ pin,
             float* __restrict__ pout, float* __restrict__ q, PcgScalars* sc,
             const double* __restrict__ rz_part, unsigned rz_n, int slot, double* partials) {
+  pin_args(L, M, z, pin, pout, q, sc, rz_part, rz_n, slot, partials);
   if constexpr (LOOP)
     spmv_stream_body<TZ>(L, StrideColumns{M}, z, pin, pout, q, sc, rz_part, rz_n, slot,
                          blockIdx.x == 0, partials, blockIdx.x);
@@ -542,6 +543,7 @@ __global__ void __launch_bounds__(BS)
             float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
             double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
             const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
+  pin_args(L, M, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n, partials);
   if constexpr (LOOP)
     down_stream_body<TZ, UPDATE>(L, StrideColumns{M}, c, rin, rout, q, p, x, rc, sc, pq_part,
                                  pq_n, blockIdx.x == 0, partials, blockIdx.x);
@@ -647,6 +649,7 @@ __global__ void __launch_bounds__(BS)
     ks_up(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ r,
           const float* __restrict__ ec, float* __restrict__ out, const PcgScalars* sc,
           double* partials) {
+  pin_args(L, M, c, r, ec, out, sc, partials);
   if constexpr (LOOP)
     up_stream_body<TZ, FINEST>(L, StrideColumns{M}, c, r, ec, out, sc, partials, blockIdx.x);
   else
