@@ -239,7 +239,9 @@ typedef struct {
   const float* kappa;       /* device, N */
   int32_t mg_levels;        /* 1 = Jacobi-PCG; >= 2 geometric V-cycle levels; 0 = auto */
   int32_t precision;        /* PFSF_PREC_* */
-  int32_t coarse_sweeps;    /* Chebyshev sweeps on the coarsest level (0 = 6) */
+  int32_t coarse_sweeps;    /* Chebyshev degree of the coarsest-level solve (0: 12 when that
+                               level fits one work-group and is applied as a dense matrix,
+                               else 6) */
   double tol;               /* ||b - A x||_2 / ||b||_2 target (FP64 residual) */
   int32_t max_iters;        /* cap on total PCG iterations */
   void* stream;

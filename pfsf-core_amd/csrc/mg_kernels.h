@@ -50,6 +50,26 @@ template <typename T>
 __device__ __forceinline__ void st(T* p, int i, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (unsigned)i * (unsigned)sizeof(T)) = v;
 }
+// Raw buffer access (MUBUF) to an FP32 array: the base sits in a 128-bit
+// resource in SGPRs and each access adds one 32-bit vector byte offset, so a
+// load or store costs no 64-bit address arithmetic (the compiler does not
+// always fold ld()/st() into the SGPR-base form).  Accesses at or past
+// `bytes` read 0 and write nothing.  0x00020000: the resource word of raw
+// 32-bit buffer access on gfx9-family targets.
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Buf buf_of(const void* p, int64_t bytes) {
+  return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                               (int)(uint32_t)bytes, 0x00020000)};
+}
+__device__ __forceinline__ float bld(const Buf& b, int i) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(b.r, (int)((unsigned)i * 4u), 0, 0));
+}
+__device__ __forceinline__ void bst(const Buf& b, int i, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), b.r, (int)((unsigned)i * 4u), 0, 0);
+}
+
 __device__ __forceinline__ float ldk32(const float* p, int i) { return ld(p, i); }
 __device__ __forceinline__ float ldk32(const uint16_t* p, int i) {
   return __uint_as_float(((unsigned)ld(p, i)) << 16);

@@ -25,14 +25,17 @@ struct IrScalars {
 // Stencil tiles: 32 x 8 x tz voxels per 256-thread work-group.
 constexpr int kTX = 32, kTY = 8;
 
-// Chebyshev steps of the coarsest-level solve (interval [lmax / 100, lmax]).
-// Measured on the 128^3 island: 6 steps need 40 PCG iterations, 8 need 41 and
-// 12 need 39 at ~0.9 us more per step and iteration.
+// Chebyshev steps of the coarsest-level solve (interval [lmax / 100, lmax])
+// when it runs inside one work-group's LDS (batched and batchable solvers):
+// each step costs ~1 us per iteration there.  Measured on the 128^3 island:
+// 6 steps need 40 PCG iterations, 8 need 41 and 12 need 39.
 constexpr int kCoarseSweeps = 6;
 // Degree of the coarsest-level polynomial when it is applied as a dense
 // matrix (standalone solvers whose coarsest level fits one work-group): the
-// degree costs nothing per iteration there, only at set-up.
-constexpr int kCoarseSweepsDense = 24;
+// degree costs nothing per iteration there, only at set-up.  Measured PCG
+// iterations for degrees 6 / 8 / 12 / 16 / 32: 128^3 38 / 36 / 35 / 35 / 35,
+// 256^3 80 / 78 / 71 / 76 / 76.
+constexpr int kCoarseSweepsDense = 12;
 
 // Smallest relative drop of the recursive residual a solve pass aims for
 // before the true FP64 residual is checked (a pass stops at

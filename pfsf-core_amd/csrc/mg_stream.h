@@ -61,6 +61,8 @@ struct PlaneRing {
 struct PlaneGeom {
   int gx, gy, hx, hy;
   bool inxy, hown, hxy;
+  int oxy, hxy_off;  // flat offsets of the two cells inside a plane (0 when outside)
+  int cxy, chxy;     // their coarse parents' offsets inside a coarse plane (plane_geom_coarse)
   int li, lh;  // LDS offsets inside a plane
   int ow, hw;  // row-bit words of the own row and the halo cell
   uint32_t mo, mh;
@@ -91,11 +93,22 @@ __device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
   g.hx = x0 + ex - 1;
   g.hy = y0 + ey - 1;
   g.hxy = g.hown && g.hx >= 0 && g.hx < d.lx && g.hy >= 0 && g.hy < d.ly;
+  // a plane's flat offsets once per column, so a plane adds only z * lx * ly
+  g.oxy = g.inxy ? g.gx + d.lx * g.gy : 0;
+  g.hxy_off = g.hxy ? g.hx + d.lx * g.hy : 0;
+  g.cxy = g.chxy = 0;
   g.lh = ey * HX + ex;
   g.ow = 10 + ty + 1;
   g.hw = g.hown ? (ex == 0 ? 0 : ex == HX - 1 ? 20 : 10) + ey : g.ow;
   g.mo = g.mh = ~0u;
   return g;
+}
+
+// The coarse parents' offsets (up-sweep: x >> 1, y >> 1 inside a plane of
+// the coarse grid c).
+__device__ __forceinline__ void plane_geom_coarse(PlaneGeom& g, pfsf_dims c) {
+  g.cxy = g.inxy ? (g.gx >> 1) + c.lx * (g.gy >> 1) : 0;
+  g.chxy = g.hxy ? (g.hx >> 1) + c.lx * (g.hy >> 1) : 0;
 }
 
 // The row bits of list slot `slot` into g (none: every row is read).
@@ -190,8 +203,9 @@ __device__ __forceinline__ PlaneIdx plane_idx(const PlaneGeom& g, int z, pfsf_di
   const bool zin = z >= 0 && z < d.lz;
   p.iv = g.inxy && zin;
   p.hv = g.hxy && zin;
-  p.i = p.iv ? flat32(g.gx, g.gy, z, d) : 0;
-  p.h = p.hv ? flat32(g.hx, g.hy, z, d) : 0;
+  const int zo = z * (d.lx * d.ly);
+  p.i = p.iv ? g.oxy + zo : 0;
+  p.h = p.hv ? g.hxy_off + zo : 0;
   return p;
 }
 
@@ -278,16 +292,22 @@ struct SpRaw {
   float k, z, p, hk, hz, hp;
 };
 
+// The arrays of one finest-level system as raw buffers (loads through
+// 32-bit offsets; see Buf).
+struct SpBufs {
+  Buf k, z, p, po, q;
+};
+
 template <int TZ>
-__device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
-                                        const float* z, const float* pin) {
-  R.ix = plane_idx(g, zpl, L.d);
-  R.k = ld(L.kap, R.ix.i);
-  R.z = ld(z, R.ix.i);
-  R.p = ld(pin, R.ix.i);
-  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
-  R.hz = ld(z, R.ix.h);
-  R.hp = ld(pin, R.ix.h);
+__device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, pfsf_dims d,
+                                        const SpBufs& B) {
+  R.ix = plane_idx(g, zpl, d);
+  R.k = bld(B.k, R.ix.i);
+  R.z = bld(B.z, R.ix.i);
+  R.p = bld(B.p, R.ix.i);
+  R.hk = bld(B.k, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hz = bld(B.z, R.ix.h);
+  R.hp = bld(B.p, R.ix.h);
 }
 
 // The columns of `cols` of one system; `publish` (one work-group per system)
@@ -304,6 +324,9 @@ __device__ __forceinline__ void spmv_stream_body(
   static_assert((TZ + 2) % kRing == 0, "ring slots continue across columns");
   // the first planes' loads do not depend on beta: they are in flight while
   // the done flag and the r.z partials are read
+  const int64_t nb = (int64_t)L.d.lx * L.d.ly * L.d.lz * 4;
+  const SpBufs B{buf_of(L.kap, nb), buf_of(z, nb), buf_of(pin, nb), buf_of(pout, nb),
+                 buf_of(q, nb)};
   int x0 = 0, y0 = 0, z0 = 0;
   int col = cols.template get<TZ>(0, x0, y0, z0);
   PlaneGeom g = plane_geom(x0, y0, L.d);
@@ -311,7 +334,7 @@ __device__ __forceinline__ void spmv_stream_body(
   if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
-      sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
+      sp_load<TZ>(buf[s], g, z0 - 1 + s, L.d, B);
     });
   if (sc->done) return;
   const bool first = !sc->have_dir;
@@ -334,7 +357,7 @@ __device__ __forceinline__ void spmv_stream_body(
       static_for<TZ + 2>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
         const SpRaw cur = buf[s % kPD];
-        if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, z, pin);
+        if constexpr (s + kPD < TZ + 2) sp_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L.d, B);
         const int sl = s % kRing;
         const float ki = cur.ix.iv ? cur.k : -INFINITY;
         const float pv = ki >= 0.f ? cur.z + beta * cur.p : 0.f;
@@ -344,7 +367,7 @@ __device__ __forceinline__ void spmv_stream_body(
           ring_put(R, sl, g.lh, kh, cur.hz + beta * cur.hp);
         }
         if constexpr (s >= 1 && s <= TZ) {
-          if (cur.ix.iv) st(pout, cur.ix.i, pv);
+          if (cur.ix.iv) bst(B.po, cur.ix.i, pv);
           pv_own[s & 1] = pv;
           fr_own[s & 1] = ki >= 0.f;
           ix_own[s & 1] = cur.ix;
@@ -356,7 +379,7 @@ __device__ __forceinline__ void spmv_stream_body(
           float qi = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
           qi = fr_own[o] ? qi : 0.f;
           acc += (double)pv_own[o] * qi;
-          if (ix_own[o].iv) st(q, ix_own[o].i, qi);
+          if (ix_own[o].iv) bst(B.q, ix_own[o].i, qi);
         }
       });
     }
@@ -365,7 +388,7 @@ __device__ __forceinline__ void spmv_stream_body(
       g = plane_geom(x0, y0, L.d);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
-        sp_load<TZ>(buf[s], g, z0 - 1 + s, L, z, pin);
+        sp_load<TZ>(buf[s], g, z0 - 1 + s, L.d, B);
       });
     }
   }
@@ -560,19 +583,23 @@ struct UpRaw {
   float k, r, dg, ec, hk, hr, hdg, hec;
 };
 
+struct UpBufs {
+  Buf k, r, dg, ec, out;
+};
+
 template <int TZ>
-__device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, const Lv<float>& L,
-                                        pfsf_dims c, const float* r, const float* ec) {
-  R.ix = plane_idx(g, zpl, L.d);
-  const int zc = (R.ix.iv || R.ix.hv) ? zpl >> 1 : 0;
-  R.k = ld(L.kap, R.ix.i);
-  R.r = ld(r, R.ix.i);
-  R.dg = ld(L.dg, R.ix.i);
-  R.ec = ld(ec, R.ix.iv ? flat32(g.gx >> 1, g.gy >> 1, zc, c) : 0);
-  R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
-  R.hr = ld(r, R.ix.h);
-  R.hdg = ld(L.dg, R.ix.h);
-  R.hec = ld(ec, R.ix.hv ? flat32(g.hx >> 1, g.hy >> 1, zc, c) : 0);
+__device__ __forceinline__ void up_load(UpRaw& R, const PlaneGeom& g, int zpl, pfsf_dims d,
+                                        pfsf_dims c, const UpBufs& B) {
+  R.ix = plane_idx(g, zpl, d);
+  const int zco = (zpl >> 1) * (c.lx * c.ly);  // used only for planes inside the grid
+  R.k = bld(B.k, R.ix.i);
+  R.r = bld(B.r, R.ix.i);
+  R.dg = bld(B.dg, R.ix.i);
+  R.ec = bld(B.ec, R.ix.iv ? g.cxy + zco : 0);
+  R.hk = bld(B.k, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hr = bld(B.r, R.ix.h);
+  R.hdg = bld(B.dg, R.ix.h);
+  R.hec = bld(B.ec, R.ix.hv ? g.chxy + zco : 0);
 }
 
 template <int TZ, bool FINEST, typename Cols>
@@ -584,14 +611,18 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
   __shared__ PlaneRing R;
   __shared__ double sh[1][BS / 64];
   static_assert((TZ + 2) % kRing == 0, "ring slots continue across columns");
+  const int64_t nb = (int64_t)L.d.lx * L.d.ly * L.d.lz * 4;
+  const UpBufs B{buf_of(L.kap, nb), buf_of(r, nb), buf_of(L.dg, nb),
+                 buf_of(ec, (int64_t)c.lx * c.ly * c.lz * 4), buf_of(out, nb)};
   int x0 = 0, y0 = 0, z0 = 0;
   int col = cols.template get<TZ>(0, x0, y0, z0);
   PlaneGeom g = plane_geom(x0, y0, L.d);
+  plane_geom_coarse(g, c);
   UpRaw buf[kPD];
   if (col > 0)  // in flight while the done flag is read
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
-      up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
+      up_load<TZ>(buf[s], g, z0 - 1 + s, L.d, c, B);
     });
   if (sc->done) return;
   double acc = 0.0;
@@ -605,7 +636,7 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
       static_for<TZ + 2>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
         const UpRaw cur = buf[s % kPD];
-        if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L, c, r, ec);
+        if constexpr (s + kPD < TZ + 2) up_load<TZ>(buf[s % kPD], g, z0 - 1 + s + kPD, L.d, c, B);
         const int sl = s % kRing;
         const float ki = cur.ix.iv ? cur.k : -INFINITY;
         ring_put(R, sl, g.li, ki, om * cur.r * rcp(cur.dg) + cur.ec);
@@ -628,16 +659,17 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
           float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
           zi = fr_own[o] ? zi : 0.f;
           if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
-          if (ix_own[o].iv) st(out, ix_own[o].i, zi);
+          if (ix_own[o].iv) bst(B.out, ix_own[o].i, zi);
         }
       });
     }
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
       g = plane_geom(x0, y0, L.d);
+      plane_geom_coarse(g, c);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
-        up_load<TZ>(buf[s], g, z0 - 1 + s, L, c, r, ec);
+        up_load<TZ>(buf[s], g, z0 - 1 + s, L.d, c, B);
       });
     }
   }
