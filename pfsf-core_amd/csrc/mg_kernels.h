@@ -708,10 +708,17 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       dgr[tz] = ld(L.dg, i);
       mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
     }
+    // the halo loads are in flight before the done flag is tested
+    float kk[Stage<TZ>::NP];
+    float3 raw[Stage<TZ>::NP];  // r, diagonal, coarse correction
+    stage_load<TZ>(L, x0, y0, z0,
+                   [&](int j, int gx_, int gy_, int gz_) {
+                     return make_float3(ld(r, j), ld(L.dg, j),
+                                        ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c)));
+                   },
+                   kk, raw);
     if (done) return;
-    stage<TZ>(T, L, x0, y0, z0, [&](int j, int gx_, int gy_, int gz_) {
-      return om * ld(r, j) * rcp(ld(L.dg, j)) + ld(ec, flat32(gx_ >> 1, gy_ >> 1, gz_ >> 1, c));
-    });
+    stage_put<TZ>(T, kk, raw, [&](float3 v) { return om * v.x * rcp(v.y) + v.z; });
     __syncthreads();
     const int c0 = tcell(tx, ty, 0);
 #pragma unroll

@@ -140,13 +140,23 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
 }
 
 // MASK: the level carries a coupling mask (every level below the finest).
+// With rr_n > 0 the launch has one work-group more than tiles, and that last
+// work-group runs the PCG iteration's convergence test from the r.r
+// partials of the finest update (mirrored to the host words) while the
+// others sweep their tiles: the test leaves the critical path of the
+// coarsest solve, and every later launch of the iteration reads the stop.
 template <int TZ, typename KT, bool UPDATE, bool MASK = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     k_down(Lv<KT> L, TL M, pfsf_dims c, const float* __restrict__ rin, float* __restrict__ rout,
            const float* __restrict__ q, const float* __restrict__ p, double* __restrict__ x,
            float* __restrict__ rc, PcgScalars* sc, const double* __restrict__ pq_part,
-           unsigned pq_n, double* partials) {
-  pin_args(L, M, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n, partials);
+           unsigned pq_n, double* partials, const double* __restrict__ rr_part = nullptr,
+           unsigned rr_n = 0u, unsigned long long* mirror = nullptr) {
+  pin_args(L, M, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n, partials, rr_part, rr_n, mirror);
+  if (rr_n > 0 && blockIdx.x == gridDim.x - 1) {
+    if (!sc->done) (void)convergence_test(sc, rr_part, rr_n, mirror);
+    return;
+  }
   int x0 = 0, y0 = 0, z0 = 0;
   const bool have = tile_origin<TZ>(M, x0, y0, z0);
   down_body<TZ, KT, UPDATE, MASK>(L, have, x0, y0, z0, c, rin, rout, q, p, x, rc, sc, pq_part,
@@ -1174,26 +1184,30 @@ hipError_t MgSolver::refresh_kappa(const float* kappa) {
     PFSF_TZF_DISPATCH(L, CALL);             \
   }
 
+// With rr_n > 0 the launch carries the convergence test (see k_down).
 template <typename KT>
 static void launch_down_t(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
-                          hipStream_t s) {
+                          hipStream_t s, const double* rr_part, unsigned rr_n,
+                          unsigned long long* mirror) {
+  const unsigned g = A.tiles.nlaunch + (rr_n > 0 ? 1u : 0u);
   if (A.msk) {
-    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false, true><<<A.tiles.nlaunch, BS, 0, s>>>(
+    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false, true><<<g, BS, 0, s>>>(
                              view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
-                             sc, nullptr, 0u, nullptr)));
+                             sc, nullptr, 0u, nullptr, rr_part, rr_n, mirror)));
   } else {
-    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<A.tiles.nlaunch, BS, 0, s>>>(
+    PFSF_TZ_DISPATCH(A, (k_down<TZV, KT, false><<<g, BS, 0, s>>>(
                              view<KT>(A), tl(A), B.d, r, nullptr, nullptr, nullptr, nullptr, B.r,
-                             sc, nullptr, 0u, nullptr)));
+                             sc, nullptr, 0u, nullptr, rr_part, rr_n, mirror)));
   }
 }
 
 static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgScalars* sc,
-                        hipStream_t s) {
+                        hipStream_t s, const double* rr_part = nullptr, unsigned rr_n = 0u,
+                        unsigned long long* mirror = nullptr) {
   if (A.bf16)
-    launch_down_t<uint16_t>(A, B, r, sc, s);
+    launch_down_t<uint16_t>(A, B, r, sc, s, rr_part, rr_n, mirror);
   else
-    launch_down_t<float>(A, B, r, sc, s);
+    launch_down_t<float>(A, B, r, sc, s, rr_part, rr_n, mirror);
 }
 
 template <typename KT>
@@ -1228,10 +1242,18 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   MgLevel& C = levels_[nlev_ - 1];
   const bool lds = C.n <= kMaxLdsCoarse && padded_cells(C.d) <= kMaxLdsPadded;
   int st = std::min(coarse_sweeps_, 63);
-  const unsigned rr_n = (check && lds) ? G : 0u;
-  if (check && !lds) k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
-  for (int l = from; l < nlev_ - 1; l++)
-    launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
+  // the convergence test runs beside the first coarse down-sweep when there
+  // is one, else inside the one-work-group coarsest solve
+  const bool test_in_down = check && from < nlev_ - 1;
+  const unsigned rr_n = (check && lds && !test_in_down) ? G : 0u;
+  if (check && !lds && !test_in_down)
+    k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
+  for (int l = from; l < nlev_ - 1; l++) {
+    if (l == from && test_in_down)
+      launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s, prr_, G, mirror_);
+    else
+      launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
+  }
   if (C.pm) {
     const unsigned g = (unsigned)((C.n + BS / 64 - 1) / (BS / 64));
     switch (C.pm_ld / 256) {
@@ -1601,13 +1623,15 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
                                   view<uint16_t>(A), tl(A), B.d, (const float*)A.r,
                                   (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
                                   (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
-                                  (double*)nullptr));
+                                  (double*)nullptr, (const double*)nullptr, 0u,
+                                  (unsigned long long*)nullptr));
         } else {
           PFSF_TZ_DISPATCH(A, ext(k_down<TZV, float, false, true>, A.tiles.nlaunch, a, b,
                                   view<float>(A), tl(A), B.d, (const float*)A.r,
                                   (float*)nullptr, (const float*)nullptr, (const float*)nullptr,
                                   (double*)nullptr, B.r, sc_, (const double*)nullptr, 0u,
-                                  (double*)nullptr));
+                                  (double*)nullptr, (const double*)nullptr, 0u,
+                                  (unsigned long long*)nullptr));
         }
         break;
       }
