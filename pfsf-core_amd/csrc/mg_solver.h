@@ -36,6 +36,12 @@ constexpr int kCoarseSweeps = 6;
 // iterations for degrees 6 / 8 / 12 / 16 / 32: 128^3 38 / 36 / 35 / 35 / 35,
 // 256^3 80 / 78 / 71 / 76 / 76.
 constexpr int kCoarseSweepsDense = 12;
+// Largest level folded into the dense coarsest solve (its cells are the
+// columns of the folded matrix; 16^3 = 4,096 with an 8^3 coarsest level:
+// 512 x 4,096 FP32, 8 MB), and that matrix's row length: the next of 1,024,
+// 2,048, 4,096 or 8,192 columns.
+constexpr int64_t kMaxFoldCells = 8192;
+inline int fold_ld(int64_t n) { return n <= 1024 ? 1024 : n <= 2048 ? 2048 : n <= 4096 ? 4096 : 8192; }
 
 // Smallest relative drop of the recursive residual a solve pass aims for
 // before the true FP64 residual is checked (a pass stops at
@@ -115,6 +121,14 @@ struct MgLevel {
   // dense symmetric matrix, n rows of pm_ld floats (k_cheb_dense)
   float* pm = nullptr;
   int pm_ld = 0;
+  // with the dense matrix, hierarchies of at least four levels fold the
+  // down-sweep of the level above into it: fg = P R (n rows of fg_ld
+  // floats), R = P^T (I - A S) of that level, built from the column table
+  // fpar / fval (8 parent slots per cell of that level); k_fold_dense
+  float* fg = nullptr;
+  int fg_ld = 0;
+  int* fpar = nullptr;
+  float* fval = nullptr;
   float scale = 1.f;      // coupling scale 2^level
   float omega = 1.f;      // damped-Jacobi weight
   float lmax = 2.f;       // estimate of lambda_max(D^-1 A)

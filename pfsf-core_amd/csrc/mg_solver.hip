@@ -263,6 +263,120 @@ __global__ void __launch_bounds__(BS)
   if (lane == 0 && row < n) out[row] = acc;
 }
 
+// ── down-sweep folded into the dense coarsest solve ────────────────────
+// The coarsest rhs is r_c = R r_b with R = P^T (I - A S) the down-sweep of
+// the level b above (S = omega D^-1: pre-smoothing from a zero guess, then
+// the restricted residual), so the coarsest correction is e = P R r_b =
+// G r_b.  G is formed once per operator; each V-cycle then reads r_b and
+// skips the level-b down-sweep launch.  Column j of R, for a free cell j,
+// holds 1 - omega at parent(j) and omega w_kj / D_j at parent(k) for every
+// coupled free neighbour k (w as in stencil_apply), summed into the at most
+// 8 parents of j's neighbourhood (slot = per-axis offset from
+// ((x - 1) >> 1, (y - 1) >> 1, (z - 1) >> 1)).
+template <typename KT>
+__global__ void __launch_bounds__(BS)
+    k_fold_columns(Lv<KT> L, const uint8_t* __restrict__ fr, pfsf_dims c, int* __restrict__ par,
+                   float* __restrict__ val) {
+  const pfsf_dims d = L.d;
+  const int64_t j = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (j >= (int64_t)d.lx * d.ly * d.lz) return;
+  int x, y, z;
+  unflat(j, d, x, y, z);
+  const int bx = (x - 1) >> 1, by = (y - 1) >> 1, bz = (z - 1) >> 1;
+  auto slot = [&](int px, int py, int pz) { return (px - bx) + 2 * (py - by) + 4 * (pz - bz); };
+  float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (fr[j]) {
+    const float cj = L.omega / L.dg[j];
+    v[slot(x >> 1, y >> 1, z >> 1)] += 1.f - L.omega;
+    const float kj = ldk(L.kap, j), sj = sqrtf(kj);
+    const uint32_t mk = L.msk ? L.msk[j] : kAllCouplings;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int cls = (dx != 0) + (dy != 0) + (dz != 0);
+          const int nx = x + dx, ny = y + dy, nz = z + dz;
+          if (cls == 0 || nx < 0 || ny < 0 || nz < 0 || nx >= d.lx || ny >= d.ly || nz >= d.lz)
+            continue;
+          if (!((mk >> nbit(dx, dy, dz)) & 1u)) continue;
+          const int64_t k = flat(nx, ny, nz, d);
+          if (!fr[k]) continue;
+          const float kk = ldk(L.kap, k);
+          const float w = cls == 1 ? L.scale * 0.5f * (kj + kk)
+                                   : L.scale * sj * (cls == 2 ? kEdgeP : kCornerP) * sqrtf(kk);
+          v[slot(nx >> 1, ny >> 1, nz >> 1)] += w * cj;
+        }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int px = bx + (q & 1), py = by + ((q >> 1) & 1), pz = bz + (q >> 2);
+    const bool in = px >= 0 && py >= 0 && pz >= 0 && px < c.lx && py < c.ly && pz < c.lz;
+    par[j * 8 + q] = in && v[q] != 0.f ? (int)flat(px, py, pz, c) : -1;
+    val[j * 8 + q] = v[q];
+  }
+}
+
+// G[i][j] = sum over j's parent slots of P[p][i] val (P symmetric).
+__global__ void __launch_bounds__(BS)
+    k_fold_matrix(const float* __restrict__ pm, int ldp, int nc, const int* __restrict__ par,
+                  const float* __restrict__ val, int nb, float* __restrict__ g, int ldg) {
+  const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (t >= (int64_t)nc * nb) return;
+  const int i = (int)(t / nb), j = (int)(t % nb);
+  float a = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int p = par[j * 8 + q];
+    if (p >= 0) a += pm[(int64_t)p * ldp + i] * val[j * 8 + q];
+  }
+  g[(int64_t)i * ldg + j] = a;
+}
+
+// out = G r_b: r_b staged in LDS, one row of G per wave, NP passes of 64
+// lanes x 4 columns (ld = 256 NP, padded with zeros).  With rr_n > 0,
+// work-group 0 first runs the PCG iteration's convergence test.
+template <int NP>
+__global__ void __launch_bounds__(BS)
+    k_fold_dense(const float* __restrict__ g, int nc, int nb, const float* __restrict__ rb,
+                 float* __restrict__ out, PcgScalars* sc, const double* __restrict__ rr_part,
+                 unsigned rr_n, unsigned long long* mirror) {
+  pin_args(g, nc, nb, rb, out, sc, rr_part, rr_n, mirror);
+  constexpr int ld = 256 * NP;
+  __shared__ float rs[ld];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+  const int rw = row < nc ? row : 0;  // row 0 stands in for a row past the end
+  float rv[ld / BS];
+#pragma unroll
+  for (int k = 0; k < ld / BS; k++) {
+    const int e = k * BS + (int)threadIdx.x;
+    rv[k] = rb[min(e, nb - 1)];
+  }
+  float4 a[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+    a[p] = *reinterpret_cast<const float4*>(g + (int64_t)rw * ld + p * 256 + lane * 4);
+  if (sc->done) return;
+  if (rr_n > 0 && blockIdx.x == 0 && convergence_test(sc, rr_part, rr_n, mirror)) return;
+#pragma unroll
+  for (int k = 0; k < ld / BS; k++) {
+    const int e = k * BS + (int)threadIdx.x;
+    rs[e] = e < nb ? rv[k] : 0.f;
+  }
+  __syncthreads();
+  float acc = 0.f;
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const float4 r4 = *reinterpret_cast<const float4*>(rs + p * 256 + lane * 4);
+    acc += a[p].x * r4.x;
+    acc += a[p].y * r4.y;
+    acc += a[p].z * r4.z;
+    acc += a[p].w * r4.w;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0 && row < nc) out[row] = acc;
+}
+
 // Jacobi-preconditioned variant of the update (single-level solve):
 // x64 += alpha p; r_out = r_in - alpha q; z = D^-1 r_out; r.r and r.z.
 __global__ void __launch_bounds__(BS)
@@ -845,6 +959,13 @@ hipError_t MgSolver::allocate(bool standalone) {
       if (dense_ok_ && L.n <= kMaxLdsCoarse && padded_cells(L.d) <= kMaxLdsPadded) {
         L.pm_ld = (int)((L.n + 255) / 256 * 256);
         L.pm = alloc<float>(L.n * L.pm_ld, err);
+        const MgLevel& B = levels_[l - 1];
+        if (nlev_ >= 4 && B.n <= kMaxFoldCells) {
+          L.fg_ld = fold_ld(B.n);
+          L.fg = alloc<float>(L.n * L.fg_ld, err);
+          L.fpar = alloc<int>(8 * B.n, err);
+          L.fval = alloc<float>(8 * B.n, err);
+        }
       }
     }
     int64_t nt = (int64_t)L.tiles.ntx * L.tiles.nty * L.tiles.ntz;
@@ -1106,6 +1227,18 @@ void MgSolver::setup_operator(hipStream_t s) {
       k_cheb_columns<float><<<(unsigned)C.n, kMaxLdsCoarse, 0, s>>>(view<float>(C), coef_, st,
                                                                      C.pm, C.pm_ld);
     k_sym_dense<<<blocks_for(C.n * C.n, BS), BS, 0, s>>>(C.pm, (int)C.n, C.pm_ld);
+    if (C.fg) {  // the level above's down-sweep folded in: G = P R
+      const MgLevel& B = levels_[nlev_ - 2];
+      if (B.bf16)
+        k_fold_columns<uint16_t><<<blocks_for(B.n, BS), BS, 0, s>>>(view<uint16_t>(B), B.fr, C.d,
+                                                                     C.fpar, C.fval);
+      else
+        k_fold_columns<float><<<blocks_for(B.n, BS), BS, 0, s>>>(view<float>(B), B.fr, C.d,
+                                                                  C.fpar, C.fval);
+      fill0(C.fg, C.n * (int64_t)C.fg_ld, s);
+      k_fold_matrix<<<blocks_for(C.n * B.n, BS), BS, 0, s>>>(C.pm, C.pm_ld, (int)C.n, C.fpar,
+                                                              C.fval, (int)B.n, C.fg, C.fg_ld);
+    }
   }
 }
 
@@ -1248,13 +1381,37 @@ hipError_t MgSolver::vcycle_below(int from, bool check) {
   const unsigned rr_n = (check && lds && !test_in_down) ? G : 0u;
   if (check && !lds && !test_in_down)
     k_finalize<<<1, BS, 0, s>>>(FIN_UPDATE, prr_, G, sc_, ir_, mirror_);
-  for (int l = from; l < nlev_ - 1; l++) {
+  // with the fold, the level above the coarsest is restricted inside the
+  // coarsest solve (k_fold_dense reads its residual)
+  const bool fold = C.fg && from < nlev_ - 2;
+  for (int l = from; l < nlev_ - (fold ? 2 : 1); l++) {
     if (l == from && test_in_down)
       launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s, prr_, G, mirror_);
     else
       launch_down(levels_[l], levels_[l + 1], levels_[l].r, sc_, s);
   }
-  if (C.pm) {
+  if (fold) {
+    const MgLevel& B = levels_[nlev_ - 2];
+    const unsigned g = (unsigned)((C.n + BS / 64 - 1) / (BS / 64));
+    switch (C.fg_ld / 256) {
+      case 4:
+        k_fold_dense<4><<<g, BS, 0, s>>>(C.fg, (int)C.n, (int)B.n, B.r, C.e, sc_, prr_, rr_n,
+                                         mirror_);
+        break;
+      case 8:
+        k_fold_dense<8><<<g, BS, 0, s>>>(C.fg, (int)C.n, (int)B.n, B.r, C.e, sc_, prr_, rr_n,
+                                         mirror_);
+        break;
+      case 16:
+        k_fold_dense<16><<<g, BS, 0, s>>>(C.fg, (int)C.n, (int)B.n, B.r, C.e, sc_, prr_, rr_n,
+                                          mirror_);
+        break;
+      default:
+        k_fold_dense<32><<<g, BS, 0, s>>>(C.fg, (int)C.n, (int)B.n, B.r, C.e, sc_, prr_, rr_n,
+                                          mirror_);
+        break;
+    }
+  } else if (C.pm) {
     const unsigned g = (unsigned)((C.n + BS / 64 - 1) / (BS / 64));
     switch (C.pm_ld / 256) {
       case 1:
