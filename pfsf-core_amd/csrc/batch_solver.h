@@ -94,6 +94,14 @@ class BatchSolver {
   hipError_t solve(const float* const* b, double* const* x64, float* const* phi32,
                    pfsf_batch_info* info, pfsf_solve_info* per_island);
   int islands() const { return (int)sol_.size(); }
+  // Preconditioner-only use by an outer Krylov solve over the attached
+  // islands (another operator per island): z = M r into each solver's
+  // precond_out() from its precond_in(), every island at once, on stream().
+  // An island whose scalars (island_scalars()[i].done != 0) say so is
+  // skipped; kernels only, no synchronisation, capturable.
+  hipError_t apply_preconditioner() { return precondition(); }
+  PcgScalars* island_scalars() const { return sc_; }
+  hipStream_t stream() const { return stream_; }
   pfsf_batch_class shape_class(int k) const { return classes_[k]; }
   pfsf_batch_tuning tuning() const { return tuning_; }
 

@@ -677,6 +677,12 @@ hipError_t BatchSolver::build_tables() {
   chk(hipMemcpyAsync(chunks_, chunks.data(), chunks.size() * sizeof(BatchChunk),
                      hipMemcpyHostToDevice, s),
       err);
+  // the island records hold every pointer the V-cycle kernels follow (the
+  // caller's b / x / phi come with each solve), so apply_preconditioner()
+  // may run straight after attach
+  chk(hipMemcpyAsync(isl_d_, host_isl_.data(), n * sizeof(BatchIsland), hipMemcpyHostToDevice,
+                     s),
+      err);
   // every island uses the same coarse-solve polynomial (same bound, steps)
   chk(hipMemcpyAsync(coef_, sol_[0]->chebyshev_coefficients(), 2 * 64 * sizeof(float),
                      hipMemcpyHostToDevice, s),

@@ -359,7 +359,8 @@ pfsf_result ensure_solver(pfsf_engine_t* E, Island& is) {
     if (!is.s6) {
       auto sv = std::make_unique<Soa6Solver>();
       hipError_t e = sv->create(is.dims, is.type, is.cond, is.cw, is.kappa,
-                                E->cfg.enable_multigrid != 0, E->tol, E->max_iters, E->stream);
+                                E->cfg.enable_multigrid != 0, E->tol, E->max_iters, E->stream,
+                                /*batchable=*/true);
       if (e != hipSuccess) return from_hip(e);
       const int64_t b = sv->device_bytes();
       is.extra_bytes += b;
@@ -451,11 +452,12 @@ pfsf_result solve_island(pfsf_engine_t* E, Island& is) {
   return post_solve(E, is);
 }
 
-// The SoA-6 islands of one tick.  Each has its own solver, vectors and
-// stream and its solve touches nothing else, so up to kSoa6Workers host
-// threads run the solves side by side and their small kernels overlap on the
-// GPU's queues.  Hierarchies are prepared before, and the post-solve steps
-// (history, damage, scan) run after, on the calling thread in island order.
+// The SoA-6 islands of one tick without multigrid (with it, several are
+// solved as one Soa6Batch).  Each has its own solver, vectors and stream and
+// its solve touches nothing else, so up to kSoa6Workers host threads run the
+// solves side by side and their small kernels overlap on the GPU's queues.
+// Hierarchies are prepared before, and the post-solve steps (history,
+// damage, scan) run after, on the calling thread in island order.
 constexpr int kSoa6Workers = 8;
 
 pfsf_result solve_soa6_concurrent(pfsf_engine_t* E, const std::vector<Island*>& list) {
@@ -491,6 +493,36 @@ pfsf_result solve_soa6_concurrent(pfsf_engine_t* E, const std::vector<Island*>& 
     if (e != hipSuccess) return PFSF_ERROR_VULKAN;
   for (Island* is : list) {
     pfsf_result r = post_solve(E, *is);
+    if (r != PFSF_OK) return r;
+  }
+  return PFSF_OK;
+}
+
+// Several SoA-6 dirty islands in one batched solve (Soa6Batch: the FP64 PCG
+// of each island, every vector pass and the proxies' V-cycles batched).
+pfsf_result solve_soa6_batch(pfsf_engine_t* E, const std::vector<Island*>& list) {
+  std::vector<Soa6Solver*> sv;
+  std::vector<const float*> b;
+  std::vector<double*> x;
+  std::vector<float*> phi;
+  for (Island* is : list) {
+    pfsf_result r = ensure_solver(E, *is);
+    if (r != PFSF_OK) return r;
+    sv.push_back(is->s6.get());
+    b.push_back(is->source);
+    x.push_back(is->x64);
+    phi.push_back(is->phi);
+  }
+  Soa6Batch batch;
+  hipError_t e = batch.attach(sv, E->tol, E->max_iters, E->stream);
+  if (e == hipErrorInvalidValue) return solve_soa6_concurrent(E, list);  // not batchable
+  if (e != hipSuccess) return from_hip(e);
+  std::vector<pfsf_solve_info> per(list.size());
+  e = batch.solve(b.data(), x.data(), phi.data(), per.data());
+  if (e != hipSuccess) return from_hip(e);
+  for (size_t k = 0; k < list.size(); k++) {
+    list[k]->last = per[k];
+    pfsf_result r = post_solve(E, *list[k]);
     if (r != PFSF_OK) return r;
   }
   return PFSF_OK;
@@ -580,8 +612,8 @@ pfsf_result report(pfsf_engine_t* E, const std::vector<Island*>& todo, int32_t* 
 // The reference walks the dirty islands one at a time, uploading and solving
 // each within the tick budget (NP/src/pfsf_engine.cpp:424-600).  Here the
 // islands that fit the budget are uploaded first; with PCG and multigrid on,
-// two or more kappa-mode islands are then solved in one batched solve, the
-// rest one by one; each then gets its history update and failure scan, and
+// two or more kappa-mode islands are then solved in one batched solve, two or
+// more SoA-6 islands in another, the rest one by one; each then gets its history update and failure scan, and
 // the reports of all islands come back together.
 pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count, int32_t* out,
                       int64_t cap) {
@@ -620,7 +652,8 @@ pfsf_result tick_impl(pfsf_engine_t* E, const int32_t* ids, int32_t count, int32
     single.erase(std::remove_if(single.begin(), single.end(), [](Island* is) { return is->soa6; }),
                  single.end());
     if (!s6.empty()) {
-      pfsf_result r = solve_soa6_concurrent(E, s6);
+      pfsf_result r = E->cfg.enable_multigrid && s6.size() >= 2 ? solve_soa6_batch(E, s6)
+                                                                : solve_soa6_concurrent(E, s6);
       if (r != PFSF_OK) return r;
     }
   }

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 
+#include "batch_solver.h"
 #include "soa6.h"
 
 namespace pfsf {
@@ -495,6 +496,229 @@ __global__ void k_s6_pass(S6Scalars* sc) {
   if (sc->iters < sc->max_iters) sc->done = 0;
 }
 
+// ── batched SoA-6 PCG (Soa6Batch) ──────────────────────────────────────
+// One 256-cell chunk of one island per work-group; per-island scalars; an
+// island whose `done` says so takes no part (its work-groups return at
+// once).  The per-cell work is that of the single-island kernels above.
+constexpr int kS6Finished = 3;
+
+struct ChunkCell {
+  int k;      // island
+  int64_t i;  // cell (valid when < I.n)
+};
+
+__device__ __forceinline__ ChunkCell chunk_cell(const S6Chunk* __restrict__ ch) {
+  const S6Chunk c = ch[blockIdx.x];
+  return ChunkCell{c.island, (int64_t)c.begin + threadIdx.x};
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_begin(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+              double* __restrict__ part) {
+  const ChunkCell c = chunk_cell(ch);
+  const S6Isl& I = isl[c.k];
+  double bb = 0.0;
+  if (c.i < I.n) {
+    const bool f = I.fr[c.i] != 0;
+    I.x[c.i] = (f && I.guess) ? I.guess[c.i] : 0.0;
+    const double bi = f ? (double)I.b[c.i] : 0.0;
+    bb = bi * bi;
+  }
+  put_partial(bb, part);
+}
+
+__global__ void ks6_start(S6Scalars* sc, int n, int max_iters) {
+  const int k = blockIdx.x * BS + threadIdx.x;
+  if (k >= n) return;
+  sc[k].done = 0;
+  sc[k].iters = 0;
+  sc[k].max_iters = max_iters;
+  sc[k].first = 1;
+  sc[k].pass_end = 0;
+}
+
+// resid: r = b - A x and r.r partials on every unfinished island; else
+// q = A p and p.q partials on every iterating island (as k_s6_apply).
+__global__ void __launch_bounds__(BS)
+    ks6_apply(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+              const S6Scalars* __restrict__ sc, int resid, double* __restrict__ part) {
+  constexpr int O[26][3] = PFSF_OFFSETS;
+  const ChunkCell c = chunk_cell(ch);
+  const int dn = sc[c.k].done;
+  if (resid ? dn == kS6Finished : dn != 0) return;
+  const S6Isl& I = isl[c.k];
+  const pfsf_dims d = I.d;
+  const int64_t n = I.n, i = c.i;
+  const double* __restrict__ v = resid ? I.x : I.p;
+  double* __restrict__ y = resid ? I.r : I.q;
+  double pv = 0.0;
+  if (i < n) {
+    double out = 0.0;
+    if (I.fr[i]) {
+      const P3 q3 = unflat3(i, d);
+      const double vi = v[i];
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 26; k++) {
+        const int nx = q3.x + O[k][0], ny = q3.y + O[k][1], nz = q3.z + O[k][2];
+        if (!inside(nx, ny, nz, d)) continue;
+        const int64_t j = flat3(nx, ny, nz, d);
+        const double w = wgt(I.cw, n, k, i, j);
+        const double vj = I.fr[j] ? v[j] : 0.0;
+        acc += w * (vi - vj);
+      }
+      out = resid ? (double)I.b[i] - acc : acc;
+      pv = resid ? out * out : vi * out;
+    }
+    y[i] = out;
+  }
+  put_partial(pv, part);
+}
+
+// Per island (one work-group each): the scalar step of k_s6_fin from the
+// island's chunk partials.  With vsc, the V-cycle of an island that stops
+// is switched off (vsc[k].done).
+__global__ void __launch_bounds__(BS)
+    ks6_fin(int mode, const S6Isl* __restrict__ isl, const double* __restrict__ part,
+            S6Scalars* sc, double tol, PcgScalars* vsc) {
+  const int k = blockIdx.x;
+  S6Scalars* s = sc + k;
+  const int dn = s->done;
+  if (mode == S6_RR0 ? dn == kS6Finished : (mode >= S6_PQ && dn != 0)) return;
+  const S6Isl& I = isl[k];
+  const double v = sum_partials(part + I.cb, (unsigned)I.cn);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case S6_BB:
+      s->bb = v;
+      break;
+    case S6_RR0:
+      s->pass_end = dn;
+      s->rr = v;
+      s->stop2 = tol * tol * (s->bb > 0.0 ? s->bb : 1.0);
+      s->done = (v <= s->stop2) ? 1 : 0;
+      s->first = 1;
+      break;
+    case S6_PQ:
+      s->pq = v;
+      if (v > 0.0) {
+        s->alpha = s->rz / v;
+      } else {
+        s->alpha = 0.0;
+        s->done = 2;
+      }
+      break;
+    case S6_RR:
+      s->rr = v;
+      s->iters += 1;
+      if (v <= s->stop2 || s->iters >= s->max_iters) s->done = 1;
+      break;
+    case S6_RZ:
+      s->beta = s->first ? 0.0 : v / s->rz;
+      s->first = 0;
+      s->rz = v;
+      break;
+  }
+  if (vsc) vsc[k].done = s->done != 0 ? 1 : 0;
+}
+
+// A pass starts for the islands with flags[k] (those left with room
+// continue, as k_s6_pass); the others are finished for this solve.
+__global__ void ks6_pass(S6Scalars* sc, int n, const int* __restrict__ flags, PcgScalars* vsc) {
+  const int k = blockIdx.x * BS + threadIdx.x;
+  if (k >= n) return;
+  if (!flags[k]) {
+    sc[k].done = kS6Finished;
+  } else if (sc[k].iters < sc[k].max_iters) {
+    sc[k].done = 0;
+  }
+  vsc[k].done = sc[k].done != 0 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_to_mg(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+              const S6Scalars* __restrict__ sc) {
+  const ChunkCell c = chunk_cell(ch);
+  if (sc[c.k].done) return;
+  const S6Isl& I = isl[c.k];
+  if (c.i < I.n) I.rmg[c.i] = (I.fr[c.i] && I.mgfr[c.i]) ? (float)I.r[c.i] : 0.0f;
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_update(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+               const S6Scalars* __restrict__ sc, double* __restrict__ part) {
+  const ChunkCell c = chunk_cell(ch);
+  const S6Scalars& S = sc[c.k];
+  if (S.done) return;
+  const S6Isl& I = isl[c.k];
+  double rr = 0.0;
+  if (c.i < I.n) {
+    const int64_t i = c.i;
+    float rm = 0.0f;
+    if (I.fr[i]) {
+      const double a = S.alpha;
+      I.x[i] += a * I.p[i];
+      const double ri = I.r[i] - a * I.q[i];
+      I.r[i] = ri;
+      rr = ri * ri;
+      rm = I.mgfr[i] ? (float)ri : 0.0f;
+    }
+    I.rmg[i] = rm;
+  }
+  put_partial(rr, part);
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_zfix(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+             const S6Scalars* __restrict__ sc, double* __restrict__ part) {
+  const ChunkCell c = chunk_cell(ch);
+  if (sc[c.k].done) return;
+  const S6Isl& I = isl[c.k];
+  double rz = 0.0;
+  if (c.i < I.n) {
+    const int64_t i = c.i;
+    double zi = 0.0;
+    if (I.fr[i]) {
+      zi = I.mgfr[i] ? (double)I.zmg[i] : I.r[i] / I.dg[i];
+      rz = I.r[i] * zi;
+    }
+    I.z[i] = zi;
+  }
+  put_partial(rz, part);
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_dir(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl,
+            const S6Scalars* __restrict__ sc) {
+  const ChunkCell c = chunk_cell(ch);
+  const S6Scalars& S = sc[c.k];
+  if (S.done) return;
+  const S6Isl& I = isl[c.k];
+  if (c.i < I.n) I.p[c.i] = S.beta == 0.0 ? I.z[c.i] : I.z[c.i] + S.beta * I.p[c.i];
+}
+
+__global__ void __launch_bounds__(BS)
+    ks6_end(const S6Chunk* __restrict__ ch, const S6Isl* __restrict__ isl) {
+  const ChunkCell c = chunk_cell(ch);
+  const S6Isl& I = isl[c.k];
+  if (c.i >= I.n) return;
+  const double v = I.x[c.i];
+  if (I.xio && I.xio != I.x) I.xio[c.i] = v;
+  if (I.phio) I.phio[c.i] = (float)v;
+}
+
+// Islands still iterating (done == 0), into live[0].
+__global__ void __launch_bounds__(1024) ks6_live(const S6Scalars* __restrict__ sc, int n, int* live) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int c = 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) c += sc[k].done == 0;
+  if (c) atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) live[0] = cnt;
+}
+
 }  // namespace
 
 // ── launch wrappers ────────────────────────────────────────────────────
@@ -568,7 +792,7 @@ int64_t Soa6Solver::device_bytes() const {
 
 hipError_t Soa6Solver::create(pfsf_dims d, const uint8_t* type, const float* cond,
                               const double* cw, const float* proxy_kappa, bool multigrid,
-                              double tol, int max_iters, hipStream_t caller) {
+                              double tol, int max_iters, hipStream_t caller, bool batchable) {
   hipError_t err = hipSuccess;
   d_ = d;
   n_ = nvox(d);
@@ -590,7 +814,7 @@ hipError_t Soa6Solver::create(pfsf_dims d, const uint8_t* type, const float* con
   sd.max_iters = max_iters_;
   sd.stream = caller;
   mg_ = std::make_unique<MgSolver>();
-  if (ck(mg_->create(sd), err)) return err;
+  if (ck(mg_->create(sd, batchable && multigrid), err)) return err;
   nblk_ = blocks_for(n_, BS);
   auto take = [&](size_t bytes) -> void* {
     void* p = nullptr;
@@ -734,6 +958,216 @@ hipError_t Soa6Solver::solve(const float* b, double* x64, float* phi32, pfsf_sol
     info->rel_residual = rel;
     info->solve_ms = ms;
   }
+  return err;
+}
+
+// ── Soa6Batch ──────────────────────────────────────────────────────────
+Soa6Batch::~Soa6Batch() { release(); }
+
+void Soa6Batch::release() {
+  if (vc_ && vc_->stream()) (void)hipStreamSynchronize(vc_->stream());
+  for (void* q : owned_) (void)hipFree(q);
+  owned_.clear();
+  if (sc_host_) (void)hipHostFree(sc_host_);
+  if (flags_host_) (void)hipHostFree(flags_host_);
+  if (live_host_) (void)hipHostFree(live_host_);
+  sc_host_ = nullptr;
+  flags_host_ = live_host_ = nullptr;
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  ev0_ = ev1_ = nullptr;
+  vc_.reset();
+}
+
+hipError_t Soa6Batch::attach(const std::vector<Soa6Solver*>& solvers, double tol, int max_iters,
+                             hipStream_t caller) {
+  hipError_t err = hipSuccess;
+  if (solvers.empty()) return hipErrorInvalidValue;
+  sol_ = solvers;
+  caller_ = caller;
+  tol_ = tol > 0 ? tol : 1e-6;
+  max_iters_ = max_iters > 0 ? max_iters : 500;
+  std::vector<MgSolver*> proxies;
+  for (Soa6Solver* v : sol_) proxies.push_back(v->mg_.get());
+  vc_ = std::make_unique<BatchSolver>();
+  if (ck(vc_->attach(proxies, tol_, max_iters_, caller_), err)) return err;
+  const int n = (int)sol_.size();
+  host_.assign(n, S6Isl{});
+  std::vector<S6Chunk> ch;
+  for (int k = 0; k < n; k++) {
+    Soa6Solver& v = *sol_[k];
+    S6Isl& I = host_[k];
+    I.d = v.d_;
+    I.n = v.n_;
+    I.fr = v.free_;
+    I.mgfr = v.mg_->free_mask();
+    I.cw = v.cw_;
+    I.dg = v.dg_;
+    I.r = v.r_;
+    I.p = v.p_;
+    I.q = v.q_;
+    I.z = v.z_;
+    I.rmg = v.mg_->precond_in();
+    I.zmg = v.mg_->precond_out();
+    I.cb = (int)ch.size();
+    for (int64_t c0 = 0; c0 < v.n_; c0 += BS) ch.push_back(S6Chunk{k, (int)c0});
+    I.cn = (int)ch.size() - I.cb;
+  }
+  nch_ = (int)ch.size();
+  auto take = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (ck(hipMalloc(&q, std::max<size_t>(bytes, 8)), err)) return nullptr;
+    owned_.push_back(q);
+    return q;
+  };
+  isl_ = (S6Isl*)take(sizeof(S6Isl) * n);
+  ch_ = (S6Chunk*)take(sizeof(S6Chunk) * std::max(nch_, 1));
+  part_ = (double*)take(sizeof(double) * std::max(nch_, 1));
+  sc_ = (S6Scalars*)take(sizeof(S6Scalars) * n);
+  flags_ = (int*)take(sizeof(int) * n);
+  live_ = (int*)take(sizeof(int));
+  if (err != hipSuccess) return err;
+  ck(hipHostMalloc((void**)&sc_host_, sizeof(S6Scalars) * n), err);
+  ck(hipHostMalloc((void**)&flags_host_, sizeof(int) * n), err);
+  ck(hipHostMalloc((void**)&live_host_, sizeof(int)), err);
+  ck(hipEventCreate(&ev0_), err);
+  ck(hipEventCreate(&ev1_), err);
+  if (err != hipSuccess) return err;
+  if (nch_ > 0)
+    ck(hipMemcpyAsync(ch_, ch.data(), sizeof(S6Chunk) * nch_, hipMemcpyHostToDevice,
+                      vc_->stream()),
+       err);
+  ck(hipStreamSynchronize(vc_->stream()), err);
+  return err;
+}
+
+// One PCG iteration of every iterating island (Soa6Solver::queue_iteration).
+hipError_t Soa6Batch::iteration() {
+  hipStream_t s = vc_->stream();
+  const int n = (int)sol_.size();
+  PcgScalars* vsc = vc_->island_scalars();
+  ks6_apply<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, 0, part_);
+  ks6_fin<<<n, BS, 0, s>>>(S6_PQ, isl_, part_, sc_, tol_, vsc);
+  ks6_update<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, part_);
+  ks6_fin<<<n, BS, 0, s>>>(S6_RR, isl_, part_, sc_, tol_, vsc);
+  hipError_t e = vc_->apply_preconditioner();
+  if (e != hipSuccess) return e;
+  ks6_zfix<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, part_);
+  ks6_fin<<<n, BS, 0, s>>>(S6_RZ, isl_, part_, sc_, tol_, nullptr);
+  ks6_dir<<<nch_, BS, 0, s>>>(ch_, isl_, sc_);
+  return hipGetLastError();
+}
+
+// Soa6Solver::solve for every island at once: per island the same passes
+// (from the FP64 true residual, stop at the tolerance, a stalled pass ends
+// the island's solve), the host following the count of islands still
+// iterating after every few iterations.
+hipError_t Soa6Batch::solve(const float* const* b, double* const* x64, float* const* phi32,
+                            pfsf_solve_info* per_island) {
+  hipError_t err = hipSuccess;
+  const int n = (int)sol_.size();
+  if (n == 0 || nch_ == 0) return hipSuccess;
+  hipStream_t s = vc_->stream();
+  PcgScalars* vsc = vc_->island_scalars();
+  for (int k = 0; k < n; k++) {
+    S6Isl& I = host_[k];
+    I.b = b[k];
+    I.guess = x64 ? x64[k] : nullptr;
+    I.x = (x64 && x64[k]) ? x64[k] : sol_[k]->xo_;
+    I.xio = x64 ? x64[k] : nullptr;
+    I.phio = phi32 ? phi32[k] : nullptr;
+  }
+  ck(hipEventRecord(ev0_, caller_), err);  // the islands' arrays were uploaded there
+  ck(hipStreamWaitEvent(s, ev0_, 0), err);
+  // each solver's set-up (diagonal, free set: Soa6Solver::refresh) runs on
+  // its proxy's stream
+  for (Soa6Solver* v : sol_) {
+    ck(hipEventRecord(ev0_, v->mg_->stream()), err);
+    ck(hipStreamWaitEvent(s, ev0_, 0), err);
+  }
+  ck(hipEventRecord(ev0_, s), err);
+  ck(hipMemcpyAsync(isl_, host_.data(), sizeof(S6Isl) * n, hipMemcpyHostToDevice, s), err);
+  const unsigned nbi = (unsigned)((n + BS - 1) / BS);
+  ks6_begin<<<nch_, BS, 0, s>>>(ch_, isl_, part_);
+  ks6_fin<<<n, BS, 0, s>>>(S6_BB, isl_, part_, sc_, tol_, nullptr);
+  ks6_start<<<nbi, BS, 0, s>>>(sc_, n, max_iters_);
+  ks6_apply<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, 1, part_);
+  ks6_fin<<<n, BS, 0, s>>>(S6_RR0, isl_, part_, sc_, tol_, nullptr);
+  ck(hipGetLastError(), err);
+  ck(hipMemcpyAsync(sc_host_, sc_, sizeof(S6Scalars) * n, hipMemcpyDeviceToHost, s), err);
+  ck(hipStreamSynchronize(s), err);
+  std::vector<double> rel(n, 0.0);
+  std::vector<int> passes(n, 0), fin(n, 0);
+  auto rel_of = [&](int k) {
+    const double bb = sc_host_[k].bb > 0.0 ? sc_host_[k].bb : 1.0;
+    return sqrt(sc_host_[k].rr / bb);
+  };
+  for (int k = 0; k < n; k++) {
+    rel[k] = rel_of(k);
+    fin[k] = !(rel[k] > tol_) || sc_host_[k].iters >= max_iters_;
+  }
+  std::vector<int> before(n, 0);
+  for (int pass = 0; pass < 8 && err == hipSuccess; pass++) {
+    int active = 0;
+    for (int k = 0; k < n; k++) {
+      flags_host_[k] = fin[k] ? 0 : 1;
+      active += flags_host_[k];
+      before[k] = sc_host_[k].iters;
+    }
+    if (active == 0) break;
+    ck(hipMemcpyAsync(flags_, flags_host_, sizeof(int) * n, hipMemcpyHostToDevice, s), err);
+    ks6_pass<<<nbi, BS, 0, s>>>(sc_, n, flags_, vsc);
+    // pass start: z = M r, p = z
+    ks6_to_mg<<<nch_, BS, 0, s>>>(ch_, isl_, sc_);
+    ck(vc_->apply_preconditioner(), err);
+    ks6_zfix<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, part_);
+    ks6_fin<<<n, BS, 0, s>>>(S6_RZ, isl_, part_, sc_, tol_, nullptr);
+    ks6_dir<<<nch_, BS, 0, s>>>(ch_, isl_, sc_);
+    ck(hipGetLastError(), err);
+    int chunk = 4;
+    for (int queued = 0; err == hipSuccess && queued < max_iters_;) {
+      for (int k = 0; k < chunk && err == hipSuccess; k++) ck(iteration(), err);
+      queued += chunk;
+      ks6_live<<<1, 1024, 0, s>>>(sc_, n, live_);
+      ck(hipMemcpyAsync(live_host_, live_, sizeof(int), hipMemcpyDeviceToHost, s), err);
+      ck(hipStreamSynchronize(s), err);
+      if (err != hipSuccess || *live_host_ == 0) break;
+      chunk = std::min(chunk * 2, 32);
+    }
+    // the true residual of every island that took part
+    ks6_apply<<<nch_, BS, 0, s>>>(ch_, isl_, sc_, 1, part_);
+    ks6_fin<<<n, BS, 0, s>>>(S6_RR0, isl_, part_, sc_, tol_, nullptr);
+    ck(hipGetLastError(), err);
+    ck(hipMemcpyAsync(sc_host_, sc_, sizeof(S6Scalars) * n, hipMemcpyDeviceToHost, s), err);
+    ck(hipStreamSynchronize(s), err);
+    for (int k = 0; k < n && err == hipSuccess; k++) {
+      if (fin[k]) continue;
+      const int it = sc_host_[k].iters - before[k];
+      passes[k]++;
+      const bool breakdown = sc_host_[k].pass_end == 2;
+      const double nrel = rel_of(k);
+      const bool stalled = it == 0 || (breakdown && nrel >= rel[k]);
+      rel[k] = nrel;
+      if (stalled || !(rel[k] > tol_) || sc_host_[k].iters >= max_iters_) fin[k] = 1;
+    }
+  }
+  ks6_end<<<nch_, BS, 0, s>>>(ch_, isl_);
+  ck(hipGetLastError(), err);
+  ck(hipEventRecord(ev1_, s), err);
+  ck(hipEventSynchronize(ev1_), err);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0_, ev1_);
+  ck(hipStreamWaitEvent(caller_, ev1_, 0), err);
+  if (per_island)
+    for (int k = 0; k < n; k++) {
+      pfsf_solve_info& f = per_island[k];
+      f.iterations = sc_host_[k].iters;
+      f.refinements = passes[k];
+      f.converged = rel[k] <= tol_ ? 1 : 0;
+      f.levels = sol_[k]->mg_->levels();
+      f.rel_residual = rel[k];
+      f.solve_ms = ms;
+    }
   return err;
 }
 
