@@ -1343,9 +1343,22 @@ static void launch_down(const MgLevel& A, const MgLevel& B, const float* r, PcgS
     launch_down_t<float>(A, B, r, sc, s, rr_part, rr_n, mirror);
 }
 
+// A coarse level launched over all its tiles runs its up-sweep on one-plane
+// tiles: twice the work-groups of the 2-plane tiles, each thread one cell.
+// The up-sweep restricts nothing, so it needs no z-pair in a thread, and
+// these small launches are bound by one wave's chain of loads, LDS and
+// stencil work, which halves (128^3: 3 up-sweeps, iteration 68.4 -> 65.6 us;
+// the same per-cell arithmetic, so the same bits).
 template <typename KT>
 static void launch_up_t(const MgLevel& A, const MgLevel& B, const float* r, float* out,
                         PcgScalars* sc, hipStream_t s) {
+  if (A.msk && A.tiles.dense) {
+    const int nt = A.tiles.ntx * A.tiles.nty * A.d.lz;
+    k_up<1, KT, false, true><<<(unsigned)((nt + 7) / 8 * 8), BS, 0, s>>>(
+        view<KT>(A), TL{A.tiles.ntx, A.tiles.nty, nullptr, nt, nullptr}, B.d, r, B.e, out, sc,
+        nullptr);
+    return;
+  }
   if (A.msk) {
     PFSF_TZ_DISPATCH(A, (k_up<TZV, KT, false, true><<<A.tiles.nlaunch, BS, 0, s>>>(
                              view<KT>(A), tl(A), B.d, r, B.e, out, sc, nullptr)));
