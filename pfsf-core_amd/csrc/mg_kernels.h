@@ -136,7 +136,7 @@ struct TL {
   int ntx, nty;
   const int* ids;  // null: every tile of the level, in XCD-chunked order (tile_origin)
   int n;  // entries in ids (the launch list, a multiple of 8); without ids, the tile count
-  const uint32_t* rows;  // per-slot row bits of the finest streaming level, or null
+  const uint32_t* cells;  // per-slot cell bits of the finest streaming level, or null
 };
 
 template <typename KT>
@@ -154,17 +154,17 @@ __device__ __forceinline__ void pin1(const TL& M) {
   pin1(M.nty);
   pin1(M.ids);
   pin1(M.n);
-  pin1(M.rows);
+  pin1(M.cells);
 }
 
-// Row bits of one column of the finest level (list slot g, words
-// rows[g * kRowWords ...]): word sg * 10 + yy describes y = y0 - 1 + yy for
-// sg = 0 the cell x0 - 1, sg = 1 the 32-cell row x0 .. x0 + 31, sg = 2 the
-// cell x0 + 32; bit pl (plane z0 - 1 + pl) is set when that row or cell
-// holds a free voxel, bit 16 + pl when it holds a free voxel or an anchor
-// (a cell whose kappa coding couples).  Columns of up to 14 planes.
-constexpr int kRowWords = 30;
-constexpr int kRowMaxPlanes = 16;
+// Cell bits of one column of the finest level (list slot g, words
+// cells[g * kCellWords ...]): word ey * HX + ex describes the cell
+// (x0 - 1 + ex, y0 - 1 + ey) of the column's plane footprint with its halo,
+// in the layout of an LDS plane; bit pl (plane z0 - 1 + pl) is set when the
+// cell is free, bit 16 + pl when it is free or an anchor (a cell whose kappa
+// coding couples).  Columns of up to 14 planes.
+constexpr int kCellWords = HXY;
+constexpr int kCellMaxPlanes = 16;
 
 // ── LDS tile machinery ─────────────────────────────────────────────────
 // Grids hold fewer than 2^31 cells (checked at creation), so the hot kernels

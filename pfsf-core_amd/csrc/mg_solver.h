@@ -82,9 +82,9 @@ struct TileList {
   int* ids = nullptr;             // device: tile id per work-group, XCD-grouped
   std::vector<int> host_active;   // host: active tile ids, ascending
   // finest level of a standalone solver (columns of at most 14 planes):
-  // kRowWords words per list slot saying which 32-cell rows the column's
-  // streaming kernels must read (see k_row_bits)
-  uint32_t* rows = nullptr;
+  // kCellWords words per list slot saying which cells of the column the
+  // streaming kernels must read (see k_cell_bits)
+  uint32_t* cells = nullptr;
 };
 
 // Grids of the solver stay below 2^29 cells, so every array (FP64 included)

@@ -622,33 +622,33 @@ __global__ void __launch_bounds__(BS)
   ks[i] = v;
 }
 
-// Row bits of every listed column (see kRowWords): one work-group per list
+// Cell bits of every listed column (see kCellWords): one work-group per list
 // slot, one thread per word.  A cell is free where fr is set and couples
-// where it is free or an anchor (the finest level's anchor-coded kappa).
-__global__ void __launch_bounds__(64)
-    k_row_bits(pfsf_dims d, const uint8_t* __restrict__ type, const uint8_t* __restrict__ fr,
-               const int* __restrict__ ids, int ntx, int nty, int tz, uint32_t* __restrict__ rows) {
-  const int g = blockIdx.x, w = threadIdx.x;
-  if (w >= kRowWords) return;
+// where it is free or an anchor (the finest level's anchor-coded kappa);
+// cells outside the grid have no bits.
+__global__ void __launch_bounds__(BS)
+    k_cell_bits(pfsf_dims d, const uint8_t* __restrict__ type, const uint8_t* __restrict__ fr,
+                const int* __restrict__ ids, int ntx, int nty, int tz,
+                uint32_t* __restrict__ cells) {
+  const int g = blockIdx.x;
   const int t = ids[g];
-  uint32_t bits = 0;
-  if (t >= 0) {
-    const int x0 = (t % ntx) * TX, r = t / ntx;
-    const int y0 = (r % nty) * TY, z0 = (r / nty) * tz;
-    const int sg = w / 10, y = y0 - 1 + w % 10;
-    const int xa = sg == 0 ? x0 - 1 : sg == 1 ? x0 : x0 + TX;
-    const int xb = sg == 1 ? x0 + TX - 1 : xa;
-    for (int pl = 0; pl < tz + 2; pl++) {
-      const int z = z0 - 1 + pl;
-      if (y < 0 || y >= d.ly || z < 0 || z >= d.lz) continue;
-      for (int x = max(xa, 0); x <= min(xb, d.lx - 1); x++) {
-        const int i = flat32(x, y, z, d);
-        const bool f = fr[i] != 0, a = f || type[i] == 2;
-        bits |= (f ? 1u << pl : 0u) | (a ? 1u << (16 + pl) : 0u);
-      }
+  for (int w = threadIdx.x; w < kCellWords; w += BS) {
+    uint32_t bits = 0;
+    if (t >= 0) {
+      const int r = t / ntx;
+      const int x = (t % ntx) * TX - 1 + w % HX, y = (r % nty) * TY - 1 + w / HX;
+      const int z0 = (r / nty) * tz;
+      if (x >= 0 && x < d.lx && y >= 0 && y < d.ly)
+        for (int pl = 0; pl < tz + 2; pl++) {
+          const int z = z0 - 1 + pl;
+          if (z < 0 || z >= d.lz) continue;
+          const int i = flat32(x, y, z, d);
+          const bool f = fr[i] != 0, a = f || type[i] == 2;
+          bits |= (f ? 1u << pl : 0u) | (a ? 1u << (16 + pl) : 0u);
+        }
     }
+    cells[(int64_t)g * kCellWords + w] = bits;
   }
-  rows[(int64_t)g * kRowWords + w] = bits;
 }
 
 // Tiles that hold at least one free voxel.
@@ -848,7 +848,7 @@ static TL tl(const MgLevel& l) {
   if (l.tiles.dense)
     return TL{l.tiles.ntx, l.tiles.nty, nullptr, l.tiles.ntx * l.tiles.nty * l.tiles.ntz,
               nullptr};
-  return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids, l.tiles.nlaunch, l.tiles.rows};
+  return TL{l.tiles.ntx, l.tiles.nty, l.tiles.ids, l.tiles.nlaunch, l.tiles.cells};
 }
 
 static void fill0(float* p, int64_t n, hipStream_t s) {
@@ -970,10 +970,10 @@ hipError_t MgSolver::allocate(bool standalone) {
     }
     int64_t nt = (int64_t)L.tiles.ntx * L.tiles.nty * L.tiles.ntz;
     L.tiles.ids = alloc<int>(nt + 8, err);
-    // row bits of the streaming columns (standalone solvers only: a batch
+    // cell bits of the streaming columns (standalone solvers only: a batch
     // streams its columns through its own tile records)
-    if (l == 0 && nlev_ > 1 && standalone && L.tiles.tz + 2 <= kRowMaxPlanes)
-      L.tiles.rows = alloc<uint32_t>((nt + 8) * kRowWords, err);
+    if (l == 0 && nlev_ > 1 && standalone && L.tiles.tz + 2 <= kCellMaxPlanes)
+      L.tiles.cells = alloc<uint32_t>((nt + 8) * kCellWords, err);
     maxblk = std::max(maxblk, nt + 8);
   }
   if (standalone) {
@@ -1101,9 +1101,9 @@ hipError_t MgSolver::build_tiles(MgLevel& L) {
   chk(hipMemcpyAsync(L.tiles.ids, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice,
                      stream_),
       err);
-  if (L.tiles.rows)
-    k_row_bits<<<L.tiles.nlaunch, 64, 0, stream_>>>(L.d, type_, L.fr, L.tiles.ids, L.tiles.ntx,
-                                                     L.tiles.nty, L.tiles.tz, L.tiles.rows);
+  if (L.tiles.cells)
+    k_cell_bits<<<L.tiles.nlaunch, BS, 0, stream_>>>(L.d, type_, L.fr, L.tiles.ids, L.tiles.ntx,
+                                                     L.tiles.nty, L.tiles.tz, L.tiles.cells);
   chk(hipGetLastError(), err);
   chk(hipStreamSynchronize(stream_), err);
   return err;

@@ -56,15 +56,14 @@ struct PlaneRing {
 
 // Thread geometry of a plane: the interior cell (tx, ty) and, for threads
 // t < 84, one cell of the halo ring (top row, bottom row, left, right).
-// mo / mh: the row bits (TL::rows) of the own row and of the halo cell, all
-// set where a column carries none.
+// mo / mh: the cell bits (TL::cells) of the own cell and of the halo cell
+// (none for a thread without one), all set where a column carries none.
 struct PlaneGeom {
   int gx, gy, hx, hy;
   bool inxy, hown, hxy;
   int oxy, hxy_off;  // flat offsets of the two cells inside a plane (0 when outside)
   int cxy, chxy;     // their coarse parents' offsets inside a coarse plane (plane_geom_coarse)
-  int li, lh;  // LDS offsets inside a plane
-  int ow, hw;  // row-bit words of the own row and the halo cell
+  int li, lh;  // LDS offsets inside a plane (and cell-bit words of a column)
   uint32_t mo, mh;
 };
 
@@ -98,8 +97,6 @@ __device__ __forceinline__ PlaneGeom plane_geom(int x0, int y0, pfsf_dims d) {
   g.hxy_off = g.hxy ? g.hx + d.lx * g.hy : 0;
   g.cxy = g.chxy = 0;
   g.lh = ey * HX + ex;
-  g.ow = 10 + ty + 1;
-  g.hw = g.hown ? (ex == 0 ? 0 : ex == HX - 1 ? 20 : 10) + ey : g.ow;
   g.mo = g.mh = ~0u;
   return g;
 }
@@ -111,25 +108,25 @@ __device__ __forceinline__ void plane_geom_coarse(PlaneGeom& g, pfsf_dims c) {
   g.chxy = g.hxy ? (g.hx >> 1) + c.lx * (g.hy >> 1) : 0;
 }
 
-// The row bits of list slot `slot` into g (none: every row is read).
-__device__ __forceinline__ void row_bits(const TL& M, int slot, PlaneGeom& g) {
-  if (!M.rows) return;
-  const uint32_t* w = M.rows + (int64_t)slot * kRowWords;
-  g.mo = w[g.ow];
-  g.mh = w[g.hw];
+// The cell bits of list slot `slot` into g (none: every cell is read).
+__device__ __forceinline__ void cell_bits(const TL& M, int slot, PlaneGeom& g) {
+  if (!M.cells) return;
+  const uint32_t* w = M.cells + (int64_t)slot * kCellWords;
+  g.mo = w[g.li];
+  g.mh = g.hown ? w[g.lh] : 0u;
 }
 
-// Plane pl of a column: whether a row's values (free voxels) and its kappa
-// coding (free voxels or anchors) must be read.  Columns deeper than the
-// row bits describe read every row.
+// Plane pl of a column: whether a cell's value (a free voxel) and its kappa
+// coding (a free voxel or an anchor) must be read.  Columns deeper than the
+// cell bits describe read every cell.
 template <int TZ>
-__device__ __forceinline__ bool row_free(uint32_t m, int pl) {
-  if constexpr (TZ + 2 > kRowMaxPlanes) return true;
+__device__ __forceinline__ bool cell_free(uint32_t m, int pl) {
+  if constexpr (TZ + 2 > kCellMaxPlanes) return true;
   return (m >> pl) & 1u;
 }
 template <int TZ>
-__device__ __forceinline__ bool row_coupled(uint32_t m, int pl) {
-  if constexpr (TZ + 2 > kRowMaxPlanes) return true;
+__device__ __forceinline__ bool cell_coupled(uint32_t m, int pl) {
+  if constexpr (TZ + 2 > kCellMaxPlanes) return true;
   return (m >> (16 + pl)) & 1u;
 }
 
@@ -169,6 +166,50 @@ __device__ __forceinline__ float stencil_win(const Win& Wm, const Win& W0, const
   face(W0, 7);  // +y
   face(Wm, 4);  // -z
   face(Wp, 4);  // +z
+  edge(W0, 0);
+  edge(W0, 2);
+  edge(W0, 6);
+  edge(W0, 8);
+  edge(Wm, 3);
+  edge(Wm, 5);
+  edge(Wp, 3);
+  edge(Wp, 5);
+  edge(Wm, 1);
+  edge(Wm, 7);
+  edge(Wp, 1);
+  edge(Wp, 7);
+  corner(Wm, 0);
+  corner(Wm, 2);
+  corner(Wm, 6);
+  corner(Wm, 8);
+  corner(Wp, 0);
+  corner(Wp, 2);
+  corner(Wp, 6);
+  corner(Wp, 8);
+  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac));
+}
+
+// The off-diagonal sum of stencil_win, sum_j w_ij v_j, so that (A v)_i =
+// D_i v_i - offsum_win with D_i the level's diagonal (k_level_diag sums the
+// same weights over free neighbours and anchors).  Every cell off the free
+// set is staged at 0, so its term vanishes without the coupling test of
+// stencil_win: one FMA per neighbour (two per face).  The V-cycle kernels use
+// it where the centre term cancels in closed form; q = A p keeps stencil_win,
+// whose differences are exact for neighbouring values within a factor 2.
+__device__ __forceinline__ float offsum_win(const Win& Wm, const Win& W0, const Win& Wp,
+                                            float scale) {
+  const float si = W0.s[4];
+  const float ki = si * si;
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  auto face = [&](const Win& W, int k) { af = fmaf(fmaf(W.s[k], W.s[k], ki), W.v[k], af); };
+  auto edge = [&](const Win& W, int k) { ae = fmaf(W.s[k], W.v[k], ae); };
+  auto corner = [&](const Win& W, int k) { ac = fmaf(W.s[k], W.v[k], ac); };
+  face(W0, 3);
+  face(W0, 5);
+  face(W0, 1);
+  face(W0, 7);
+  face(Wm, 4);
+  face(Wp, 4);
   edge(W0, 0);
   edge(W0, 2);
   edge(W0, 6);
@@ -242,8 +283,8 @@ __device__ __forceinline__ void restrict_pair(float a, int tx, int ty, int gx, i
 // from the chunk of the XCD it runs on.
 struct StrideColumns {
   TL M;
-  __device__ __forceinline__ void rows(int k, PlaneGeom& g) const {
-    row_bits(M, (int)blockIdx.x + k * (int)gridDim.x, g);
+  __device__ __forceinline__ void bits(int k, PlaneGeom& g) const {
+    cell_bits(M, (int)blockIdx.x + k * (int)gridDim.x, g);
   }
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
@@ -262,8 +303,8 @@ struct StrideColumns {
 // SlotColumn: the column of list slot blockIdx.x, one per work-group.
 struct SlotColumn {
   TL M;
-  __device__ __forceinline__ void rows(int, PlaneGeom& g) const {
-    row_bits(M, (int)blockIdx.x, g);
+  __device__ __forceinline__ void bits(int, PlaneGeom& g) const {
+    cell_bits(M, (int)blockIdx.x, g);
   }
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& x0, int& y0, int& z0) const {
@@ -275,7 +316,7 @@ struct SlotColumn {
 // OneColumn: the column of a batch tile record.
 struct OneColumn {
   int x0, y0, z0;
-  __device__ __forceinline__ void rows(int, PlaneGeom&) const {}
+  __device__ __forceinline__ void bits(int, PlaneGeom&) const {}
   template <int TZ>
   __device__ __forceinline__ int get(int k, int& a, int& b, int& c) const {
     if (k > 0) return -1;
@@ -298,6 +339,9 @@ struct SpBufs {
   Buf k, z, p, po, q;
 };
 
+// Every cell of a plane is read: gating these loads by the column's cell
+// bits (as dn_load does) made this kernel and ks_up slower at 128^3 (12.5 ->
+// 13.1 us, 11.2 -> 12.4 us), though they read fewer bytes.
 template <int TZ>
 __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, pfsf_dims d,
                                         const SpBufs& B) {
@@ -419,23 +463,22 @@ struct DnRaw {
   PlaneIdx ix;
   float k, r, q, dg, p, hk, hr, hq, hdg;
   double x;
-  bool fo;  // the own row holds a free voxel (its updates are stored)
+  bool fo;  // the own cell is a free voxel (its updates are stored)
 };
 
-// A row that holds no free voxel is not read: its cells take the values
-// staged for such cells (0, kappa -inf where nothing couples, diagonal 1),
-// and its updates are not stored (x and r stay as they are there).  The
-// loads of a row sit behind one branch, so a wave whose rows are all skipped
-// issues none of them.  Measured at 128^3 (60 %): 18.0 -> 17.4 us; the same
-// skipping in sp_load / up_load made those kernels slower (13.1 -> 13.7,
-// 12.1 -> 13.0 us): they read too few bytes per row to repay the bits.
+// Only the cells a value or a coupling comes from are read (the column's
+// cell bits): a cell that is not free takes the values staged for such cells
+// (0, kappa -inf where nothing couples, diagonal 1), and its updates are not
+// stored (x and r stay as they are there).  A lane's loads sit behind its
+// own bits, so a wave issues only the loads of its free cells and the line
+// segments that hold none are not fetched.
 template <int TZ, bool UPDATE>
 __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, int pl,
                                         const Lv<float>& L, const float* rin, const float* q,
                                         const float* p, const double* x) {
   R.ix = plane_idx(g, zpl, L.d);
-  const bool fo = row_free<TZ>(g.mo, pl), co = row_coupled<TZ>(g.mo, pl);
-  const bool fh = row_free<TZ>(g.mh, pl), ch = row_coupled<TZ>(g.mh, pl);
+  const bool fo = cell_free<TZ>(g.mo, pl), co = cell_coupled<TZ>(g.mo, pl);
+  const bool fh = cell_free<TZ>(g.mh, pl), ch = cell_coupled<TZ>(g.mh, pl);
   R.fo = fo;
   R.k = R.hk = -INFINITY;
   R.r = R.q = R.p = R.hr = R.hq = 0.f;
@@ -472,7 +515,7 @@ __device__ __forceinline__ void down_stream_body(
   int x0 = 0, y0 = 0, z0 = 0;
   int col = cols.template get<TZ>(0, x0, y0, z0);
   PlaneGeom g = plane_geom(x0, y0, L.d);
-  if (col > 0) cols.rows(0, g);
+  if (col > 0) cols.bits(0, g);
   DnRaw buf[kPD];
   if (col > 0)
     static_for<kPD>([&](auto S_) {
@@ -495,7 +538,7 @@ __device__ __forceinline__ void down_stream_body(
     if (!ok) return;
   }
   double acc = 0.0;
-  const float alpha = (float)alpha64, om = L.omega;
+  const float alpha = (float)alpha64, om = L.omega, om1 = 1.f - om;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   for (int k = 1; col >= 0; k++) {
     if (col > 0) {
@@ -536,8 +579,9 @@ __device__ __forceinline__ void down_stream_body(
         read_win(R, sl, g.li, W[s % 3]);
         if constexpr (s >= 2) {
           const int o = (s - 1) & 1;
-          const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
-          const float res = fr_own[o] ? r_own[o] - ax : 0.f;
+          // r - A x with x = omega D^-1 r: (1 - omega) r + sum_j w_ij x_j
+          const float os = offsum_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
+          const float res = fr_own[o] ? fmaf(om1, r_own[o], os) : 0.f;
           const int zo = z0 + s - 2;  // z0 is even: zo has the parity of s
           if constexpr ((s & 1) == 0) {
             pair = res;
@@ -550,7 +594,7 @@ __device__ __forceinline__ void down_stream_body(
     col = cols.template get<TZ>(k, x0, y0, z0);
     if (col > 0) {
       g = plane_geom(x0, y0, L.d);
-      cols.rows(k, g);
+      cols.bits(k, g);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
         dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, x);
@@ -626,7 +670,7 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
     });
   if (sc->done) return;
   double acc = 0.0;
-  const float om = L.omega;
+  const float om = L.omega, om1 = 1.f - om;
   for (int k = 1; col >= 0; k++) {
     if (col > 0) {
       float r_own[2] = {0.f, 0.f}, dg_own[2] = {1.f, 1.f};
@@ -654,9 +698,10 @@ __device__ __forceinline__ void up_stream_body(const Lv<float>& L, const Cols& c
         read_win(R, sl, g.li, W[s % 3]);
         if constexpr (s >= 2) {
           const int o = (s - 1) & 1;
-          const float ax = stencil_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
+          // x + omega D^-1 (r - A x) = (1 - omega) x + omega D^-1 (r + sum_j w_ij x_j)
+          const float os = offsum_win(W[(s - 2) % 3], W[(s - 1) % 3], W[s % 3], L.scale);
           const float xi = W[(s - 1) % 3].v[4];
-          float zi = xi + om * (r_own[o] - ax) * rcp(dg_own[o]);
+          float zi = fmaf(om1, xi, om * (r_own[o] + os) * rcp(dg_own[o]));
           zi = fr_own[o] ? zi : 0.f;
           if (FINEST) acc += fr_own[o] ? (double)r_own[o] * zi : 0.0;
           if (ix_own[o].iv) bst(B.out, ix_own[o].i, zi);
