@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BS)
 
 // Pass boundary, per island: an island at the tolerance, out of iterations
 // or stalled finishes; the others restart PCG from the new residual (the pass
-// stops when the recursive residual has dropped by max(0.5 tol / rel, 2e-6)).
+// stops when the recursive residual has dropped by max(0.85 tol / rel, 2e-6)).
 // *live counts the restarts.
 __global__ void __launch_bounds__(BS)
     kb_residual_fin(const BatchIsland* __restrict__ isl, const double* __restrict__ part,
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(BS)
     P.done = 3;
     return;
   }
-  const double stop_rel = fmax(0.5 * tol / rel, PFSF_PASS_FLOOR);
+  const double stop_rel = fmax(0.85 * tol / rel, PFSF_PASS_FLOOR);
   P.done = 0;
   P.iters = 0;
   P.have_dir = 0;

@@ -1625,8 +1625,13 @@ hipError_t MgSolver::true_residual(const float* b, double* rel, int c, int seq) 
 
 // Solve passes: PCG on FP32 vectors with the iterate accumulated in FP64,
 // started from the true FP64 residual; a pass ends when its recursive
-// residual falls below half the target (or the FP32 resolution limit), then
-// the true residual decides whether a further pass (a restart) is needed.
+// residual falls below 0.85 of the target (or the FP32 resolution limit),
+// then the true residual decides whether a further pass (a restart) is
+// needed.  A pass that starts near the target tracks the true residual
+// closely (its FP32 recursion error scales with its own small start), so the
+// margin only has to cover that: 0.85 against 0.5 saved one iteration of
+// the 128^3 solve's second pass (35 -> 33 iterations in all, final residual
+// 6.3e-7) and the 256^3 one (71 -> 70).
 hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve_info* info) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
@@ -1648,7 +1653,7 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
   int passes = 0, total = 0;
   const IrScalars* irh = (const IrScalars*)(sc_host_ + 1);
   for (; rel > tol && total < max_it && err == hipSuccess && passes < 16;) {
-    double stop_rel = std::max(0.5 * tol / rel, PFSF_PASS_FLOOR);
+    double stop_rel = std::max(0.85 * tol / rel, PFSF_PASS_FLOOR);
     mirror_[0] = mirror_[1] = 0;  // the stream is idle here (true_residual synchronised)
     k_pcg_begin<<<1, 1, 0, s>>>(sc_, ir_, stop_rel, max_it - total, 1);
     chk(precondition(0), err);
