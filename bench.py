@@ -28,11 +28,17 @@ METRIC = "time-to-1e-6 residual at 128^3 + batched islands/s vs plain, % of HBM 
 # array element counted once per launch, neighbour reuse free; kappa is the
 # anchor-coded array, so no anchor-mass array is read):
 #   0 ks_spmv   read z, p_in, kappa 12; write p_out, q 8                     = 20
-#   1 ks_down   read r_in, q, kappa, diag, p 20, x (f64) 8;
-#               write x (f64) 8, r_out 4, coarse rhs 0.5                     = 40.5
+#   1 ks_down of an odd iteration (moves the FP64 iterate by the directions
+#               of both iterations of the pair): read r_in, q, kappa, diag 16,
+#               p, p_prev 8, x (f64) 8; write x (f64) 8, r_out 4,
+#               coarse rhs 0.5                                               = 44.5
+#   5 ks_down of an even iteration (iterate update left pending): read r_in,
+#               q, kappa, diag 16; write r_out 4, coarse rhs 0.5             = 20.5
 #   3 ks_up     read r, kappa, diag 12, coarse correction 0.5; write z 4     = 16.5
-ALG_BYTES = {0: 20.0, 1: 40.5, 3: 16.5}
-KERNEL_NAMES = {0: "ks_spmv", 1: "ks_down(update)", 3: "ks_up(finest)"}
+ALG_BYTES = {0: 20.0, 1: 44.5, 5: 20.5, 3: 16.5}
+KERNEL_NAMES = {0: "ks_spmv", 1: "ks_down(odd)", 5: "ks_down(even)", 3: "ks_up(finest)"}
+# finest-level bytes of an average iteration (the two ks_down kinds alternate)
+ITER_ALG_BYTES = ALG_BYTES[0] + 0.5 * (ALG_BYTES[1] + ALG_BYTES[5]) + ALG_BYTES[3]
 # HBM bytes per launch of these kernels on this workload, from the committed
 # FETCH_SIZE / WRITE_SIZE counter summary (copy-kernel calibrated).
 TRAFFIC_FILE = ROOT / "profiles" / "solve_128_hbm_traffic.md"
@@ -621,7 +627,7 @@ def main():
     # live per-kernel timing (HIP events on the solver stream)
     prof = {}
     for k in KERNEL_NAMES:
-        if k == 3 and last.levels < 2:
+        if k in (3, 5) and last.levels < 2:
             continue
         prof[k] = solver.profile(k, reps=100)
     iter_ms = solver.profile(4, reps=30)
@@ -629,7 +635,7 @@ def main():
     ach = ALG_BYTES[dom] * n_act / (prof[dom] * 1e-3)
     traffic = measured_traffic().get(KERNEL_NAMES[dom])
     # whole-iteration algorithmic rate
-    it_bytes = sum(ALG_BYTES.values()) * n_act
+    it_bytes = ITER_ALG_BYTES * n_act
     # whole solve: per active voxel, a fused PCG iteration (SpMV + dot 13,
     # update + z + dot 29, direction 17 = 59 B) plus one 2-level V-cycle
     # (67 B) per iteration, and 17 B for the initial residual

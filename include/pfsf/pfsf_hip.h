@@ -276,12 +276,15 @@ PFSF_API void pfsf_solver_destroy(pfsf_solver s);
 /* Measurement hook: average device time (ms) of one launch of a solver
  * kernel over `reps` launches on the solver's stream with the convergence
  * stop disabled.  which: 0 finest-level direction update + SpMV + p.Ap,
- * 1 finest vector update + r.r + down-sweep (smooth, residual, restrict),
- * 2 second-level down-sweep, 3 finest up-sweep (prolong, smooth, r.z) --
- * each timed by the kernel's own start/stop event pair -- and 4 one PCG
- * iteration including the whole V-cycle (events around back-to-back
- * iterations).  Use after pfsf_solver_solve; it overwrites the solver's
- * scratch vectors and its internal FP64 iterate, not the caller's arrays. */
+ * 1 finest vector update + r.r + down-sweep (smooth, residual, restrict) of
+ * an odd PCG iteration, which also moves the FP64 iterate by the last two
+ * directions, 5 the same of an even iteration (the iterate update left
+ * pending), 2 second-level down-sweep, 3 finest up-sweep (prolong, smooth,
+ * r.z) -- each timed by the kernel's own start/stop event pair -- and 4 one
+ * PCG iteration including the whole V-cycle (the average over back-to-back
+ * iterations of alternating parity, between events).  Use after
+ * pfsf_solver_solve; it overwrites the solver's scratch vectors and its
+ * internal FP64 iterate, not the caller's arrays. */
 PFSF_API pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps,
                                          double* avg_ms);
 

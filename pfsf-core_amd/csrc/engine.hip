@@ -1327,7 +1327,7 @@ pfsf_result pfsf_solver_update(pfsf_solver s, const uint8_t* type, const float* 
 void pfsf_solver_destroy(pfsf_solver s) { delete s; }
 
 pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps, double* avg_ms) {
-  if (!s || !avg_ms || reps <= 0 || which < 0 || which > 4) return PFSF_ERROR_INVALID_ARG;
+  if (!s || !avg_ms || reps <= 0 || which < 0 || which > 5) return PFSF_ERROR_INVALID_ARG;
   float ms = 0.f;
   hipError_t e = s->mg.profile(which, reps, &ms);
   if (e == hipErrorInvalidValue) return PFSF_ERROR_INVALID_ARG;

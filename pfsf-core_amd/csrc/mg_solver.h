@@ -12,6 +12,9 @@ namespace pfsf {
 struct PcgScalars {
   double rz, pq, alpha, beta, rr, stop2;
   double rzs[2];  // r.z of the last two iterations (slot = iteration parity)
+  // alpha of the iterate update an even iteration of a pass leaves pending
+  // (the odd iteration after it applies both; k_x_pending one left at a stop)
+  double xa;
   int done;  // 1 converged / capped, 2 breakdown
   int iters;
   int max_iters;

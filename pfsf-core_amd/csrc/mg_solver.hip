@@ -762,9 +762,23 @@ __global__ void k_pcg_begin(PcgScalars* sc, const IrScalars* ir, double stop_rel
   sc->rzs[0] = sc->rzs[1] = 0.0;
   sc->beta = 0.0;
   sc->alpha = 0.0;
+  sc->xa = 0.0;
   sc->rr = ir->rr;
   sc->stop2 = ir->rr * stop_rel * stop_rel;
   sc->max_iters = max_iters;
+}
+
+// The iterate update a pass leaves pending: when its last iteration (or the
+// last one before a breakdown) was an even one, x += xa p (p = p_[1], that
+// iteration's direction, zero off the free set), the FMA of a single update.
+// A pass has run an odd number of convergence tests exactly then.
+__global__ void __launch_bounds__(BS)
+    k_x_pending(int64_t n, const PcgScalars* __restrict__ sc, const float* __restrict__ p,
+                double* __restrict__ x) {
+  if ((sc->iters & 1) == 0) return;
+  const double a = sc->xa;
+  int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] = fma(a, (double)p[i], x[i]);
 }
 
 // Profiling: keep every kernel live (no convergence stop) for timed replays.
@@ -1485,7 +1499,8 @@ hipError_t MgSolver::precondition(int c) {
   const bool loop = fine_loop();
   PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, false, LOOPV><<<G, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr,
-                                    nullptr, B.r, sc_, nullptr, 0u, nullptr)));
+                                    nullptr, (int)kXNow, nullptr, B.r, sc_, nullptr, 0u,
+                                    nullptr)));
   (void)vcycle_below(1, false);
   PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<G, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[c], B.e, z_, sc_, prz_)));
@@ -1497,8 +1512,10 @@ hipError_t MgSolver::precond_begin() {
   return hipGetLastError();
 }
 
-// One PCG iteration reading r_[c], p_[c] and writing r_[1-c], p_[1-c]; the
-// FP64 iterate is updated in place.
+// One PCG iteration reading r_[c], p_[c] and writing r_[1-c], p_[1-c]
+// (iterations of a pass alternate c = 0, 1, ...).  The FP64 iterate is
+// updated in place by the odd iterations, with the even one's direction
+// p_[1] and their own p_[0] (see ks_down).
 hipError_t MgSolver::iteration(int c) {
   hipStream_t s = stream_;
   MgLevel& F = levels_[0];
@@ -1521,8 +1538,9 @@ hipError_t MgSolver::iteration(int c) {
                                     sview(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prz_, GS, c,
                                     ppq_)));
   PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
-                                    sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c], x_,
-                                    B.r, sc_, ppq_, GS, prr_)));
+                                    sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c],
+                                    p_[c], (int)(c ? kXPair : kXDefer), x_, B.r, sc_, ppq_, GS,
+                                    prr_)));
   (void)vcycle_below(1, true);
   PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, prz_)));
@@ -1708,6 +1726,7 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
     // a restart from the true residual: continuing the recursion with the
     // replaced residual and the old direction (residual replacement) was
     // measured slower here, 48 iterations against 43 at 128^3
+    if (nlev_ > 1) k_x_pending<<<nb0, BS, 0, s>>>(n0_, sc_, p_[1], x_);
     double nrel = rel;
     chk(true_residual(b, &nrel, 0), err);
     bool stalled = (it == 0) || (sc_host_->done == 2 && nrel >= rel);
@@ -1733,12 +1752,15 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
 
 // Average device time of one launch of a solver kernel over `reps` launches
 // on the solver stream.  which: 0 finest SpMV + direction + p.q, 1 finest
-// update + down-sweep (or the Jacobi update), 2 second-level down-sweep,
-// 3 finest up-sweep + r.z: each launch is bracketed by the kernel's own
+// update + down-sweep of an odd iteration (r and the FP64 iterate with two
+// directions; or the Jacobi update), 5 the same of an even iteration (r
+// only, the iterate update left pending), 2 second-level down-sweep, 3
+// finest up-sweep + r.z: each launch is bracketed by the kernel's own
 // start/stop timestamps (hipExtLaunchKernel events), so the figure is the
-// kernel duration alone.  4: one whole PCG iteration, timed between events
-// around `reps` back-to-back iterations (launch gaps included).  Call after a
-// solve so the vectors hold finite data.
+// kernel duration alone.  4: one whole PCG iteration, the average over
+// `reps` back-to-back iterations of alternating parity timed between events
+// (launch gaps included).  Call after a solve so the vectors hold finite
+// data.
 hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
@@ -1746,14 +1768,14 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   const unsigned G = F.tiles.nlaunch;  // tiled launches
   const unsigned GS = nlev_ > 1 ? fine_groups() : G;  // streaming launches
   const bool loop = nlev_ > 1 && fine_loop();
-  if (which < 0 || which > 4 || reps <= 0) return hipErrorInvalidValue;
-  if ((which == 2 || which == 3) && nlev_ < 2) return hipErrorInvalidValue;
+  if (which < 0 || which > 5 || reps <= 0) return hipErrorInvalidValue;
+  if ((which == 2 || which == 3 || which == 5) && nlev_ < 2) return hipErrorInvalidValue;
   if (which == 2 && nlev_ < 3) return hipErrorInvalidValue;
   k_profile_begin<<<1, 1, 0, s>>>(sc_);
   if (which == 4) {
-    (void)iteration(0);  // warm
+    (void)iteration(1);  // warm
     chk(hipEventRecord(ev0_, s), err);
-    for (int k = 0; k < reps; k++) (void)iteration(0);
+    for (int k = 0; k < reps; k++) (void)iteration(k & 1);  // as the passes alternate
     chk(hipEventRecord(ev1_, s), err);
     chk(hipEventSynchronize(ev1_), err);
     float ms = 0.f;
@@ -1786,9 +1808,17 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
         } else {
           PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV>, GS, a, b, sview(F),
                                             tl(F), levels_[1].d, (const float*)r_[0], r_[1],
-                                            (const float*)q_, (const float*)p_[1], x_,
+                                            (const float*)q_, (const float*)p_[1],
+                                            (const float*)p_[0], (int)kXPair, x_,
                                             levels_[1].r, sc_, (const double*)ppq_, GS, prr_));
         }
+        break;
+      case 5:
+        PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV>, GS, a, b, sview(F), tl(F),
+                                          levels_[1].d, (const float*)r_[0], r_[1],
+                                          (const float*)q_, (const float*)p_[1],
+                                          (const float*)p_[0], (int)kXDefer, x_, levels_[1].r,
+                                          sc_, (const double*)ppq_, GS, prr_));
         break;
       case 2: {
         const MgLevel& A = levels_[1];

@@ -341,7 +341,10 @@ struct SpBufs {
 
 // Every cell of a plane is read: gating these loads by the column's cell
 // bits (as dn_load does) made this kernel and ks_up slower at 128^3 (12.5 ->
-// 13.1 us, 11.2 -> 12.4 us), though they read fewer bytes.
+// 13.1 us, 11.2 -> 12.4 us), though they read fewer bytes.  Their stores
+// cover every cell of a column too: storing only the free cells (whose
+// values are the only ones read) left 128^3 unchanged and made the 256^3
+// solve 1.4 % slower.
 template <int TZ>
 __device__ __forceinline__ void sp_load(SpRaw& R, const PlaneGeom& g, int zpl, pfsf_dims d,
                                         const SpBufs& B) {
@@ -456,12 +459,20 @@ __global__ void __launch_bounds__(BS)
 }
 
 // ── down-sweep of the finest level (see down_body) ─────────────────────
-// With UPDATE: x64 += alpha p, r_out = r_in - alpha q (stored as each owned
-// plane is staged), r.r partials; then x = omega D^-1 r_out and the coarse
-// rhs P^T (r_out - A x).
+// With UPDATE: r_out = r_in - alpha q (stored as each owned plane is
+// staged), r.r partials; then x = omega D^-1 r_out and the coarse rhs
+// P^T (r_out - A x).  The FP64 iterate x64 is updated by mode:
+//   kXNow   x64 += alpha p;
+//   kXDefer (even iterations of a pass) alpha p stays pending (PcgScalars::xa);
+//   kXPair  (odd iterations) x64 + xa pprev + alpha p, pprev being the even
+//           iteration's direction, still in the other direction buffer,
+// with the FMAs of single updates, so pairs give the same iterate while
+// x64 and the directions are streamed every other iteration only.
+enum XMode { kXNow = 0, kXDefer = 1, kXPair = 2 };
+
 struct DnRaw {
   PlaneIdx ix;
-  float k, r, q, dg, p, hk, hr, hq, hdg;
+  float k, r, q, dg, p, pp, hk, hr, hq, hdg;
   double x;
   bool fo;  // the own cell is a free voxel (its updates are stored)
 };
@@ -475,13 +486,14 @@ struct DnRaw {
 template <int TZ, bool UPDATE>
 __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, int pl,
                                         const Lv<float>& L, const float* rin, const float* q,
-                                        const float* p, const double* x) {
+                                        const float* p, const float* pprev, int xmode,
+                                        const double* x) {
   R.ix = plane_idx(g, zpl, L.d);
   const bool fo = cell_free<TZ>(g.mo, pl), co = cell_coupled<TZ>(g.mo, pl);
   const bool fh = cell_free<TZ>(g.mh, pl), ch = cell_coupled<TZ>(g.mh, pl);
   R.fo = fo;
   R.k = R.hk = -INFINITY;
-  R.r = R.q = R.p = R.hr = R.hq = 0.f;
+  R.r = R.q = R.p = R.pp = R.hr = R.hq = 0.f;
   R.dg = R.hdg = 1.f;
   R.x = 0.0;
   if (co) R.k = ld(L.kap, R.ix.i);
@@ -489,8 +501,11 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, i
     R.r = ld(rin, R.ix.i);
     if (UPDATE) R.q = ld(q, R.ix.i);
     R.dg = ld(L.dg, R.ix.i);
-    if (UPDATE) R.p = ld(p, R.ix.i);
-    if (UPDATE) R.x = ld(x, R.ix.i);
+    if (UPDATE && xmode != kXDefer) {
+      R.p = ld(p, R.ix.i);
+      if (xmode == kXPair) R.pp = ld(pprev, R.ix.i);
+      R.x = ld(x, R.ix.i);
+    }
   }
   if (ch) R.hk = ld(L.kap, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
   if (fh) {
@@ -504,7 +519,8 @@ template <int TZ, bool UPDATE, typename Cols>
 __device__ __forceinline__ void down_stream_body(
     const Lv<float>& L, const Cols& cols, pfsf_dims c,
     const float* __restrict__ rin, float* __restrict__ rout, const float* __restrict__ q,
-    const float* __restrict__ p, double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
+    const float* __restrict__ p, const float* __restrict__ pprev, int xmode,
+    double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
     const double* __restrict__ pq_part, unsigned pq_n, bool publish, double* partials,
     unsigned pslot) {
   __shared__ PlaneRing R;
@@ -520,22 +536,25 @@ __device__ __forceinline__ void down_stream_body(
   if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
-      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, x);
+      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, xmode, x);
     });
   if (sc->done) return;
-  double alpha64 = 0.0;
+  double alpha64 = 0.0, xa = 0.0;
   if (UPDATE) {
     const double rz = sc->rz;
     const double pq = block_total(pq_part, pq_n);
     const bool ok = pq > 0.0;
     alpha64 = ok ? rz / pq : 0.0;
+    // written by the even iteration before (kXPair) or by this one (kXDefer)
+    if (xmode == kXPair) xa = sc->xa;
     if (publish && threadIdx.x == 0) {
       sc->pq = pq;
       sc->alpha = alpha64;
+      if (xmode == kXDefer) sc->xa = alpha64;
       sc->have_dir = 1;
       if (!ok) sc->done = 2;  // breakdown: the direction carries no energy
     }
-    if (!ok) return;
+    if (!ok) return;  // an update left pending stays so (k_x_pending)
   }
   double acc = 0.0;
   const float alpha = (float)alpha64, om = L.omega, om1 = 1.f - om;
@@ -551,7 +570,7 @@ __device__ __forceinline__ void down_stream_body(
         const DnRaw cur = buf[s % kPD];
         if constexpr (s + kPD < TZ + 2)
           dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, s + kPD, L, rin, q, p,
-                              x);
+                              pprev, xmode, x);
         const int sl = s % kRing;
         const float ki = cur.ix.iv ? cur.k : -INFINITY;
         const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
@@ -564,10 +583,12 @@ __device__ __forceinline__ void down_stream_body(
         if constexpr (s >= 1 && s <= TZ) {
           const bool fr = ki >= 0.f;
           if (UPDATE) {
-            // p and q are zero outside the free set, so the update leaves x
-            // there unchanged and carries r there over
+            // r and x change on the free set only
             if (cur.ix.iv && cur.fo) {
-              st(x, cur.ix.i, cur.x + alpha64 * (double)cur.p);
+              if (xmode == kXPair)
+                st(x, cur.ix.i, fma(alpha64, (double)cur.p, fma(xa, (double)cur.pp, cur.x)));
+              else if (xmode == kXNow)
+                st(x, cur.ix.i, fma(alpha64, (double)cur.p, cur.x));
               st(rout, cur.ix.i, ri);
             }
             acc += fr ? (double)ri * ri : 0.0;
@@ -597,7 +618,7 @@ __device__ __forceinline__ void down_stream_body(
       cols.bits(k, g);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
-        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, x);
+        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, xmode, x);
       });
     }
   }
@@ -608,15 +629,16 @@ template <int TZ, bool UPDATE, bool LOOP>
 __global__ void __launch_bounds__(BS)
     ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
             float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
-            double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
-            const double* __restrict__ pq_part, unsigned pq_n, double* partials) {
-  pin_args(L, M, c, rin, rout, q, p, x, rc, sc, pq_part, pq_n, partials);
+            const float* __restrict__ pprev, int xmode, double* __restrict__ x,
+            float* __restrict__ rc, PcgScalars* sc, const double* __restrict__ pq_part,
+            unsigned pq_n, double* partials) {
+  pin_args(L, M, c, rin, rout, q, p, pprev, xmode, x, rc, sc, pq_part, pq_n, partials);
   if constexpr (LOOP)
-    down_stream_body<TZ, UPDATE>(L, StrideColumns{M}, c, rin, rout, q, p, x, rc, sc, pq_part,
-                                 pq_n, blockIdx.x == 0, partials, blockIdx.x);
+    down_stream_body<TZ, UPDATE>(L, StrideColumns{M}, c, rin, rout, q, p, pprev, xmode, x, rc,
+                                 sc, pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
   else
-    down_stream_body<TZ, UPDATE>(L, SlotColumn{M}, c, rin, rout, q, p, x, rc, sc, pq_part,
-                                 pq_n, blockIdx.x == 0, partials, blockIdx.x);
+    down_stream_body<TZ, UPDATE>(L, SlotColumn{M}, c, rin, rout, q, p, pprev, xmode, x, rc, sc,
+                                 pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── up-sweep of the finest level (see up_body) ─────────────────────────
