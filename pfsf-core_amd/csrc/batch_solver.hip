@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(BS)
 // tile per work-group (every island's tiles, in XCD order), with the ||r||^2
 // partial in the tile's slot.
 template <int TZ>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
     kb_residual_tiles(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
                       const BatchIsland* __restrict__ isl, const BatchState* __restrict__ st,
                       double* part) {
@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   // before the first pass an island without an initial guess has x = 0
   const bool xzero = S.passes == 0 && !I.xio;
-  true_residual_tile<TZ>(I.d, true, t.x0, t.y0, t.z0, I.type, I.kappa, I.fr, I.b, I.x,
-                              I.r0, part, (unsigned)t.slot, xzero);
+  residual_stream_body<TZ>(I.d, OneColumn{t.x0, t.y0, t.z0}, I.type, I.kappa, I.fr, I.b, I.x,
+                           I.r0, part, (unsigned)t.slot, xzero);
 }
 
 // Pass boundary, per island: an island at the tolerance, out of iterations

@@ -869,7 +869,9 @@ __device__ __forceinline__ void cheb_lds_body(const Lv<KT>& L, const float* __re
   if (act) out[i] = xi;
 }
 
-// ── FP64 residual of the iterate ───────────────────────────────────────
+// ── FP64 coupling of the exact operator ────────────────────────────────
+// Face 1/2 fl(ki + kj), edge / corner kE / kC sqrt(fl(ki kj)) in FP64 (the
+// reference's float product, root in double).
 __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   if (cls == 1) {
     float s = ki + kj;
@@ -878,137 +880,6 @@ __device__ __forceinline__ double couple_f64(int cls, float ki, float kj) {
   float p = ki * kj;
   double r = sqrt((double)p);
   return r * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
-}
-
-// The same coupling with the square root of the FP32 product p formed from
-// the hardware FP32 square root s (relative error below 2^-23) plus one
-// correction e * (0.5 / s), e = p - s^2 (one FP32 FMA, within an FP32
-// rounding of the exact difference, itself about 2^-23 p) and 0.5 / s from
-// the FP32 reciprocal square root: the correction is small, so its FP32
-// error stays near 2^-45 of the root, and the sum is formed in FP64.  The
-// result is within about 3e-14 of the correctly rounded square root at one
-// FP64 add instead of an FP64 square root: the convergence check needs the
-// residual to far better than its 1e-6 target, not to the last bit.
-__device__ __forceinline__ double couple_f64_fast(int cls, float ki, float kj) {
-  if (cls == 1) {
-    float s = ki + kj;
-    return 0.5 * (double)s;
-  }
-  const float p = ki * kj;
-  const float s0 = __builtin_amdgcn_sqrtf(p);
-  const float corr = fmaf(-s0, s0, p) * (0.5f * __builtin_amdgcn_rsqf(p));
-  const double sq = p > 0.f ? (double)s0 + (double)corr : 0.0;
-  return sq * (cls == 2 ? (double)kEdgeP : (double)kCornerP);
-}
-
-// True residual over the tiles of the finest level in FP64 (couplings of
-// couple_f64_fast): the tile
-// stages the iterate (0 off the free set) and the coupling kappa (-1 on air)
-// with a one-voxel halo, then each thread forms b - A x for its column from
-// LDS.  Cells of tiles without free voxels are never written and stay 0.
-template <int TZ>
-struct TileR {
-  static constexpr int HN = Tile<TZ>::HN;
-  double x[HN];
-  float k[HN];
-};
-
-// With xzero (the iterate is known to be 0 everywhere) the residual is b on
-// the free set: the same values, without the stencil.
-template <int TZ>
-__device__ __forceinline__ void true_residual_tile(
-    pfsf_dims d, bool have, int x0, int y0, int z0, const uint8_t* __restrict__ type,
-    const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
-    const float* __restrict__ b, const double* __restrict__ x, float* __restrict__ r32,
-    double* partials, unsigned pslot, bool xzero = false) {
-  __shared__ TileR<TZ> T;
-  __shared__ double sh[1][BS / 64];
-  double acc = 0.0;
-  if (have && xzero) {
-    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    const int gx = x0 + tx, gy = y0 + ty, sz = d.lx * d.ly;
-    const int i0 = flat32(gx, gy, z0, d);
-    if (gx < d.lx && gy < d.ly) {
-      for (int tz = 0; tz < TZ; tz++) {
-        if (z0 + tz >= d.lz) break;
-        const int i = i0 + tz * sz;
-        if (!fr[i]) continue;
-        const double ri = (double)b[i];
-        r32[i] = (float)ri;
-        acc += ri * ri;
-      }
-    }
-  } else if (have) {
-    constexpr int NROWS = Tile<TZ>::NROWS, RPP = BS / TX;
-    constexpr int NP = (NROWS + RPP - 1) / RPP + 1;
-    const int t = threadIdx.x;
-    auto cell = [&](int e, int gx, int gy, int gz, bool ok) {
-      const int j = ok ? flat32(gx, gy, gz, d) : 0;
-      const uint8_t tj = type[j];
-      const float kj = kappa[j];
-      const uint8_t fj = fr[j];
-      const double xj = x[j];
-      T.k[e] = (ok && tj != 0) ? kapf(kj) : -1.f;
-      T.x[e] = (ok && fj) ? xj : 0.0;
-    };
-    const int tx = t % TX, tr = t / TX;
-    const int gx = x0 + tx;
-#pragma unroll
-    for (int p = 0; p < NP - 1; p++) {
-      const int row = tr + p * RPP;
-      const int ey = row % HY, ez = row / HY;
-      const int gy = y0 + ey - 1, gz = z0 + ez - 1;
-      if (p < NP - 2 || row < NROWS)
-        cell(row * HX + tx + 1, gx, gy, gz,
-             gx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
-    }
-    const int ex = (t & 1) ? HX - 1 : 0;
-    for (int hrow = t >> 1; hrow < NROWS; hrow += BS / 2) {  // the two halo columns
-      const int ey = hrow % HY, ez = hrow / HY;
-      const int hx = x0 + ex - 1, gy = y0 + ey - 1, gz = z0 + ez - 1;
-      cell(hrow * HX + ex, hx, gy, gz,
-           hx >= 0 && hx < d.lx && gy >= 0 && gy < d.ly && gz >= 0 && gz < d.lz);
-    }
-    const int ty = t / TX, gy = y0 + ty, sz = d.lx * d.ly;
-    const bool col = gx < d.lx && gy < d.ly;
-    const int i0 = flat32(gx, gy, z0, d);
-    // own cells' free flags and loads, requested with the halo
-    bool fown[TZ];
-    float bown[TZ];
-#pragma unroll
-    for (int tz = 0; tz < TZ; tz++) {
-      const bool in = col && z0 + tz < d.lz;
-      const int i = in ? i0 + tz * sz : 0;
-      fown[tz] = in && fr[i];
-      bown[tz] = b[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int tz = 0; tz < TZ; tz++) {
-      if (!fown[tz]) continue;
-      const int i = i0 + tz * sz;
-      const int c = tcell(tx, ty, tz);
-      const float ki = T.k[c];
-      const double xi = T.x[c];
-      double ax = 0.0;
-#pragma unroll
-      for (int dz = -1; dz <= 1; dz++)
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-          for (int dx = -1; dx <= 1; dx++) {
-            const int cls = (dx != 0) + (dy != 0) + (dz != 0);
-            if (cls == 0) continue;
-            const int o = dx + dy * HX + dz * HXY;
-            const float kj = T.k[c + o];
-            if (kj >= 0.f) ax += couple_f64_fast(cls, ki, kj) * (xi - T.x[c + o]);
-          }
-      const double ri = (double)bown[tz] - ax;
-      r32[i] = (float)ri;
-      acc += ri * ri;
-    }
-  }
-  write_partial_at(acc, partials, sh, pslot);
 }
 
 }  // namespace

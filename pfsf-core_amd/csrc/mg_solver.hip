@@ -670,30 +670,19 @@ __global__ void __launch_bounds__(BS) k_fill(int64_t n, float* __restrict__ v, f
 }
 
 // True residual of the FP64 iterate: r = b - A x with the exact coupling rule
-// (FP64), written as r32 = (float) r, plus partial ||r||^2 per work-group.
-// Work-group g takes the lower or upper TZ / 2 planes of tile ids[g / 2], so
-// it stages half the FP64 halo box of a tile in LDS and twice as many
-// work-groups are resident.  A half below the grid's last plane writes
-// nothing and a zero partial.
+// (FP64), written as r32 = (float) r, plus partial ||r||^2 per work-group
+// (residual_stream_body): work-group g streams the column of list slot g.
 template <int TZ>
-__global__ void __launch_bounds__(BS)
-    k_true_residual_halves(pfsf_dims d, TL M, const uint8_t* __restrict__ type,
-                           const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
-                           const float* __restrict__ b, const double* __restrict__ x,
-                           float* __restrict__ r32, double* partials, const int* nz, int seq) {
-  const int t = M.ids[blockIdx.x >> 1];
-  const bool have = t >= 0;
-  int x0 = 0, y0 = 0, z0 = 0;
-  if (have) {
-    const int r = t / M.ntx;
-    x0 = (t % M.ntx) * TX;
-    y0 = (r % M.nty) * TY;
-    z0 = (r / M.nty) * TZ + (int)(blockIdx.x & 1) * (TZ / 2);
-  }
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WPE)))
+    k_true_residual(pfsf_dims d, TL M, const uint8_t* __restrict__ type,
+                    const float* __restrict__ kappa, const uint8_t* __restrict__ fr,
+                    const float* __restrict__ b, const double* __restrict__ x,
+                    float* __restrict__ r32, double* partials, const int* nz, int seq) {
+  pin_args(d, M, type, kappa, fr, b, x, r32, partials, nz, seq);
   // nz holds the number of the last solve whose initial guess had a nonzero
   const bool xzero = nz && *nz != seq;
-  true_residual_tile<TZ / 2>(d, have, x0, y0, z0, type, kappa, fr, b, x, r32, partials,
-                             blockIdx.x, xzero);
+  residual_stream_body<TZ>(d, SlotColumn{M}, type, kappa, fr, b, x, r32, partials, blockIdx.x,
+                           xzero);
 }
 
 // Solve set-up over all cells, one pass (grid-stride, at most kBeginBlocks
@@ -1627,12 +1616,12 @@ hipError_t MgSolver::true_residual(const float* b, double* rel, int c, int seq) 
   hipStream_t s = stream_;
   const MgLevel& F = levels_[0];
   const int* nz = seq > 0 ? nz_flag_ : nullptr;
-  // one work-group per half tile: 2 x nlaunch partials, within the partial
-  // buffer (4 x the largest launch), not live between passes
-  const unsigned G = 2u * (unsigned)F.tiles.nlaunch;
+  // one work-group per column: nlaunch partials, within the partial buffer
+  // (4 x the largest launch), not live between passes
+  const unsigned G = (unsigned)F.tiles.nlaunch;
   if ((int64_t)G > npartials_) return hipErrorInvalidValue;
-  PFSF_TZF_DISPATCH(F, (k_true_residual_halves<TZV><<<G, BS, 0, s>>>(
-                          d0_, tl(F), type_, kappa_, F.fr, b, x_, r_[c], partials_, nz, seq)));
+  PFSF_TZF_DISPATCH(F, (k_true_residual<TZV><<<G, BS, 0, s>>>(d0_, tl(F), type_, kappa_, F.fr, b,
+                                                              x_, r_[c], partials_, nz, seq)));
   k_finalize<<<1, BS, 0, s>>>(FIN_IR, partials_, G, sc_, ir_, nullptr);
   IrScalars* irh = (IrScalars*)(sc_host_ + 1);
   chk(hipMemcpyAsync(irh, ir_, sizeof(IrScalars), hipMemcpyDeviceToHost, s), err);

@@ -755,5 +755,154 @@ __global__ void __launch_bounds__(BS)
     up_stream_body<TZ, FINEST>(L, SlotColumn{M}, c, r, ec, out, sc, partials, blockIdx.x);
 }
 
+// ── FP64 true residual of the iterate, plane streaming ─────────────────
+// r = b - A x over the columns of `cols` with the exact coupling rule
+// (couple_f64: face 1/2 fl(ki + kj), edge / corner kE / kC sqrt(fl(ki kj))
+// in FP64, every non-air neighbour coupled, x 0 off the free set), written
+// as r32 = (float) r on free cells, plus the work-group's ||r||^2 partial;
+// with xzero (the iterate is 0 everywhere) r = b without the stencil.
+//
+// A plane's cells are staged once with their coupling kappa (-1 on air),
+// 1 / kappa, FP64 sqrt(kappa) and x, so a coupling takes no square root:
+// with p = fl(ki kj) and e = ki kj - p, the exact error of that FP32 product
+// (one FMA), sqrt(p) = sqrt(ki) sqrt(kj) (1 - t / 2 - t^2 / 8 - ...) for
+// t = e / (ki kj), |t| < 2^-23.  The first-order form is within 1e-14 of
+// the correctly rounded root (t^2 / 8 and the 1-ulp FP32 reciprocals);
+// the convergence test needs the residual to far better than its 1e-6
+// target, not to the last bit.  Air stages 1 / kappa = sqrt = x = 0, so an
+// edge or corner term vanishes without a test; a face keeps one (its weight
+// 1/2 (ki + kj) does not vanish by itself).  Four LDS plane slots take one
+// barrier per plane: the slot written at plane s was last read two planes
+// earlier.
+struct ResRing {
+  float2 kr[4][HXY];   // kappa (-1 on air), 1 / kappa (0 on air or kappa 0)
+  double2 xs[4][HXY];  // x (0 off the free set), sqrt(kappa) (0 on air)
+};
+
+struct ResRaw {
+  PlaneIdx ix;
+  float k, b, hk;
+  double x, hx;
+  bool fo, ho, ft, ht;  // free (own, halo); non-air (own, halo)
+};
+
+__device__ __forceinline__ void res_load(ResRaw& R, const PlaneGeom& g, int zpl, pfsf_dims d,
+                                         const uint8_t* __restrict__ type,
+                                         const float* __restrict__ kappa,
+                                         const uint8_t* __restrict__ fr,
+                                         const float* __restrict__ b,
+                                         const double* __restrict__ x) {
+  R.ix = plane_idx(g, zpl, d);
+  R.k = ld(kappa, R.ix.i);
+  R.b = ld(b, R.ix.i);
+  R.x = ld(x, R.ix.i);
+  R.ft = ld(type, R.ix.i) != 0;
+  R.fo = ld(fr, R.ix.i) != 0;
+  R.hk = ld(kappa, R.ix.h);  // halo cell (cell 0 for threads without one: unused)
+  R.hx = ld(x, R.ix.h);
+  R.ht = ld(type, R.ix.h) != 0;
+  R.ho = ld(fr, R.ix.h) != 0;
+}
+
+__device__ __forceinline__ void res_put(ResRing& T, int sl, int off, bool in, bool nonair,
+                                        bool fre, float kv, double xv) {
+  const float k = (in && nonair) ? kapf(kv) : -1.f;
+  T.kr[sl][off] = make_float2(k, k > 0.f ? 1.f / k : 0.f);
+  T.xs[sl][off] = make_double2((in && fre) ? xv : 0.0, k >= 0.f ? sqrt((double)k) : 0.0);
+}
+
+template <int TZ, typename Cols>
+__device__ __forceinline__ void residual_stream_body(
+    pfsf_dims d, const Cols& cols, const uint8_t* __restrict__ type,
+    const float* __restrict__ kappa, const uint8_t* __restrict__ fr, const float* __restrict__ b,
+    const double* __restrict__ x, float* __restrict__ r32, double* partials, unsigned pslot,
+    bool xzero) {
+  __shared__ ResRing T;
+  __shared__ double sh[1][BS / 64];
+  double acc = 0.0;
+  int x0 = 0, y0 = 0, z0 = 0;
+  int col = cols.template get<TZ>(0, x0, y0, z0);
+  for (int k = 1; col >= 0; k++) {
+    if (col > 0) {
+      const PlaneGeom g = plane_geom(x0, y0, d);
+      if (xzero) {
+        for (int tz = 0; tz < TZ; tz++) {
+          const PlaneIdx ix = plane_idx(g, z0 + tz, d);
+          if (ix.iv && ld(fr, ix.i)) {
+            const double ri = (double)ld(b, ix.i);
+            st(r32, ix.i, (float)ri);
+            acc += ri * ri;
+          }
+        }
+      } else {
+        // a run-time plane loop: unrolled, the compiler interleaves planes
+        // and needs ~148 VGPRs at TZ = 8 (spills below 128)
+        ResRaw cur, nx1, nx2;
+        res_load(cur, g, z0 - 1, d, type, kappa, fr, b, x);
+        res_load(nx1, g, z0, d, type, kappa, fr, b, x);
+        bool f_prev = false;
+        float b_prev = 0.f;
+        int i_prev = 0;
+#pragma unroll 1
+        for (int s = 0; s < TZ + 2; s++) {
+          if (s + 2 < TZ + 2) res_load(nx2, g, z0 + 1 + s, d, type, kappa, fr, b, x);
+          const int sl = s & 3;
+          res_put(T, sl, g.li, cur.ix.iv, cur.ft, cur.fo, cur.k, cur.x);
+          if (g.hown) res_put(T, sl, g.lh, cur.ix.hv, cur.ht, cur.ho, cur.hk, cur.hx);
+          __syncthreads();
+          if (s >= 2 && f_prev) {
+            // plane s - 1: slots (s - 2) & 3, (s - 1) & 3, s & 3 for dz = -1, 0, 1
+            const int c = g.li;
+            const int sc = (s - 1) & 3;
+            const float2 kri = T.kr[sc][c];
+            const double2 xsi = T.xs[sc][c];
+            const float ki = kri.x, hi = -0.5f * kri.y;
+            const double xi = xsi.x, si = xsi.y;
+            double af = 0.0, ae = 0.0, ac = 0.0;
+            static_for<27>([&](auto N_) {
+              constexpr int n = decltype(N_)::value;
+              constexpr int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
+              constexpr int cls = (dx != 0) + (dy != 0) + (dz != 0);
+              if constexpr (cls != 0) {
+                const int slot = (s - 1 + dz) & 3;
+                const int cj = c + dy * HX + dx;
+                const float2 krj = T.kr[slot][cj];
+                const double2 xsj = T.xs[slot][cj];
+                const double dxj = xi - xsj.x;
+                if constexpr (cls == 1) {
+                  const float w = krj.x >= 0.f ? ki + krj.x : 0.f;
+                  af = fma((double)w, dxj, af);
+                } else {
+                  const float p = ki * krj.x;
+                  const float e = fmaf(ki, krj.x, -p);
+                  const float t = (e * hi) * krj.y;
+                  const double sq = si * xsj.y;
+                  const double w = fma(sq, (double)t, sq);
+                  if constexpr (cls == 2)
+                    ae = fma(w, dxj, ae);
+                  else
+                    ac = fma(w, dxj, ac);
+                }
+              }
+            });
+            const double ax = 0.5 * af + (double)kEdgeP * ae + (double)kCornerP * ac;
+            const double ri = (double)b_prev - ax;
+            st(r32, i_prev, (float)ri);
+            acc += ri * ri;
+          }
+          f_prev = s >= 1 && s <= TZ && cur.ix.iv && cur.fo;
+          b_prev = cur.b;
+          i_prev = cur.ix.i;
+          cur = nx1;
+          nx1 = nx2;
+        }
+        __syncthreads();  // the next column's first planes reuse the slots read last
+      }
+    }
+    col = cols.template get<TZ>(k, x0, y0, z0);
+  }
+  write_partial_at(acc, partials, sh, pslot);
+}
+
 }  // namespace
 }  // namespace pfsf
