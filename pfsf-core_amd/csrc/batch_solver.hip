@@ -111,10 +111,10 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_stream_body<TZ, UPDATE>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin, rout, I.q,
-                               c ? I.p0 : I.p1, nullptr, kXNow, I.x, D.rc, sc + D.island,
-                               pq_part + I.tb, (unsigned)I.tn, t.slot == I.tb, rr_part,
-                               (unsigned)t.slot);
+  down_stream_body<TZ, UPDATE, kXNow>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin,
+                                      rout, I.q, c ? I.p0 : I.p1, nullptr, I.x, D.rc,
+                                      sc + D.island, pq_part + I.tb, (unsigned)I.tn,
+                                      t.slot == I.tb, rr_part, (unsigned)t.slot);
 }
 
 template <int TZ>

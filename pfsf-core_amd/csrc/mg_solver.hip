@@ -1488,8 +1488,7 @@ hipError_t MgSolver::precondition(int c) {
   const bool loop = fine_loop();
   PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, false, LOOPV><<<G, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[c], nullptr, nullptr, nullptr,
-                                    nullptr, (int)kXNow, nullptr, B.r, sc_, nullptr, 0u,
-                                    nullptr)));
+                                    nullptr, nullptr, B.r, sc_, nullptr, 0u, nullptr)));
   (void)vcycle_below(1, false);
   PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<G, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[c], B.e, z_, sc_, prz_)));
@@ -1526,10 +1525,15 @@ hipError_t MgSolver::iteration(int c) {
   PFSF_STREAM_DISPATCH(F, loop, (ks_spmv<TZV, LOOPV><<<GS, BS, 0, s>>>(
                                     sview(F), tl(F), z_, p_[c], p_[1 - c], q_, sc_, prz_, GS, c,
                                     ppq_)));
-  PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
-                                    sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c],
-                                    p_[c], (int)(c ? kXPair : kXDefer), x_, B.r, sc_, ppq_, GS,
-                                    prr_)));
+  if (c) {
+    PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, true, LOOPV, kXPair><<<GS, BS, 0, s>>>(
+                                      sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c],
+                                      p_[c], x_, B.r, sc_, ppq_, GS, prr_)));
+  } else {
+    PFSF_STREAM_DISPATCH(F, loop, (ks_down<TZV, true, LOOPV, kXDefer><<<GS, BS, 0, s>>>(
+                                      sview(F), tl(F), B.d, r_[c], r_[1 - c], q_, p_[1 - c],
+                                      nullptr, x_, B.r, sc_, ppq_, GS, prr_)));
+  }
   (void)vcycle_below(1, true);
   PFSF_STREAM_DISPATCH(F, loop, (ks_up<TZV, true, LOOPV><<<GS, BS, 0, s>>>(
                                     sview(F), tl(F), B.d, r_[1 - c], B.e, z_, sc_, prz_)));
@@ -1795,19 +1799,19 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
           ext(k_update_jacobi, nb, a, b, n0_, (const float*)F.dg, x_, (const float*)r_[0], r_[1],
               (const float*)p_[1], (const float*)q_, z_, (const PcgScalars*)sc_, prr_);
         } else {
-          PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV>, GS, a, b, sview(F),
-                                            tl(F), levels_[1].d, (const float*)r_[0], r_[1],
-                                            (const float*)q_, (const float*)p_[1],
-                                            (const float*)p_[0], (int)kXPair, x_,
-                                            levels_[1].r, sc_, (const double*)ppq_, GS, prr_));
+          PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV, kXPair>, GS, a, b,
+                                            sview(F), tl(F), levels_[1].d, (const float*)r_[0],
+                                            r_[1], (const float*)q_, (const float*)p_[1],
+                                            (const float*)p_[0], x_, levels_[1].r, sc_,
+                                            (const double*)ppq_, GS, prr_));
         }
         break;
       case 5:
-        PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV>, GS, a, b, sview(F), tl(F),
-                                          levels_[1].d, (const float*)r_[0], r_[1],
-                                          (const float*)q_, (const float*)p_[1],
-                                          (const float*)p_[0], (int)kXDefer, x_, levels_[1].r,
-                                          sc_, (const double*)ppq_, GS, prr_));
+        PFSF_STREAM_DISPATCH(F, loop, ext(ks_down<TZV, true, LOOPV, kXDefer>, GS, a, b,
+                                          sview(F), tl(F), levels_[1].d, (const float*)r_[0],
+                                          r_[1], (const float*)q_, (const float*)p_[1],
+                                          (const float*)nullptr, x_, levels_[1].r, sc_,
+                                          (const double*)ppq_, GS, prr_));
         break;
       case 2: {
         const MgLevel& A = levels_[1];

@@ -483,11 +483,10 @@ struct DnRaw {
 // stored (x and r stay as they are there).  A lane's loads sit behind its
 // own bits, so a wave issues only the loads of its free cells and the line
 // segments that hold none are not fetched.
-template <int TZ, bool UPDATE>
+template <int TZ, bool UPDATE, int XM>
 __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, int pl,
                                         const Lv<float>& L, const float* rin, const float* q,
-                                        const float* p, const float* pprev, int xmode,
-                                        const double* x) {
+                                        const float* p, const float* pprev, const double* x) {
   R.ix = plane_idx(g, zpl, L.d);
   const bool fo = cell_free<TZ>(g.mo, pl), co = cell_coupled<TZ>(g.mo, pl);
   const bool fh = cell_free<TZ>(g.mh, pl), ch = cell_coupled<TZ>(g.mh, pl);
@@ -501,9 +500,9 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, i
     R.r = ld(rin, R.ix.i);
     if (UPDATE) R.q = ld(q, R.ix.i);
     R.dg = ld(L.dg, R.ix.i);
-    if (UPDATE && xmode != kXDefer) {
+    if (UPDATE && XM != kXDefer) {
       R.p = ld(p, R.ix.i);
-      if (xmode == kXPair) R.pp = ld(pprev, R.ix.i);
+      if (XM == kXPair) R.pp = ld(pprev, R.ix.i);
       R.x = ld(x, R.ix.i);
     }
   }
@@ -515,12 +514,12 @@ __device__ __forceinline__ void dn_load(DnRaw& R, const PlaneGeom& g, int zpl, i
   }
 }
 
-template <int TZ, bool UPDATE, typename Cols>
+template <int TZ, bool UPDATE, int XM, typename Cols>
 __device__ __forceinline__ void down_stream_body(
     const Lv<float>& L, const Cols& cols, pfsf_dims c,
     const float* __restrict__ rin, float* __restrict__ rout, const float* __restrict__ q,
-    const float* __restrict__ p, const float* __restrict__ pprev, int xmode,
-    double* __restrict__ x, float* __restrict__ rc, PcgScalars* sc,
+    const float* __restrict__ p, const float* __restrict__ pprev, double* __restrict__ x,
+    float* __restrict__ rc, PcgScalars* sc,
     const double* __restrict__ pq_part, unsigned pq_n, bool publish, double* partials,
     unsigned pslot) {
   __shared__ PlaneRing R;
@@ -536,7 +535,7 @@ __device__ __forceinline__ void down_stream_body(
   if (col > 0)
     static_for<kPD>([&](auto S_) {
       constexpr int s = decltype(S_)::value;
-      dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, xmode, x);
+      dn_load<TZ, UPDATE, XM>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, x);
     });
   if (sc->done) return;
   double alpha64 = 0.0, xa = 0.0;
@@ -546,11 +545,11 @@ __device__ __forceinline__ void down_stream_body(
     const bool ok = pq > 0.0;
     alpha64 = ok ? rz / pq : 0.0;
     // written by the even iteration before (kXPair) or by this one (kXDefer)
-    if (xmode == kXPair) xa = sc->xa;
+    if constexpr (XM == kXPair) xa = sc->xa;
     if (publish && threadIdx.x == 0) {
       sc->pq = pq;
       sc->alpha = alpha64;
-      if (xmode == kXDefer) sc->xa = alpha64;
+      if constexpr (XM == kXDefer) sc->xa = alpha64;
       sc->have_dir = 1;
       if (!ok) sc->done = 2;  // breakdown: the direction carries no energy
     }
@@ -569,8 +568,8 @@ __device__ __forceinline__ void down_stream_body(
         constexpr int s = decltype(S_)::value;
         const DnRaw cur = buf[s % kPD];
         if constexpr (s + kPD < TZ + 2)
-          dn_load<TZ, UPDATE>(buf[s % kPD], g, z0 - 1 + s + kPD, s + kPD, L, rin, q, p,
-                              pprev, xmode, x);
+          dn_load<TZ, UPDATE, XM>(buf[s % kPD], g, z0 - 1 + s + kPD, s + kPD, L, rin, q, p,
+                                  pprev, x);
         const int sl = s % kRing;
         const float ki = cur.ix.iv ? cur.k : -INFINITY;
         const float ri = UPDATE ? cur.r - alpha * cur.q : cur.r;
@@ -585,9 +584,9 @@ __device__ __forceinline__ void down_stream_body(
           if (UPDATE) {
             // r and x change on the free set only
             if (cur.ix.iv && cur.fo) {
-              if (xmode == kXPair)
+              if constexpr (XM == kXPair)
                 st(x, cur.ix.i, fma(alpha64, (double)cur.p, fma(xa, (double)cur.pp, cur.x)));
-              else if (xmode == kXNow)
+              else if constexpr (XM == kXNow)
                 st(x, cur.ix.i, fma(alpha64, (double)cur.p, cur.x));
               st(rout, cur.ix.i, ri);
             }
@@ -618,27 +617,31 @@ __device__ __forceinline__ void down_stream_body(
       cols.bits(k, g);
       static_for<kPD>([&](auto S_) {
         constexpr int s = decltype(S_)::value;
-        dn_load<TZ, UPDATE>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, xmode, x);
+        dn_load<TZ, UPDATE, XM>(buf[s], g, z0 - 1 + s, s, L, rin, q, p, pprev, x);
       });
     }
   }
   if (UPDATE) write_partial_at(acc, partials, sh, pslot);
 }
 
-template <int TZ, bool UPDATE, bool LOOP>
-__global__ void __launch_bounds__(BS)
+// One column per work-group: every column of a grid up to 1,024 columns is
+// resident at four waves per SIMD, so registers stay within 128 (the pair
+// update's two directions would take 129).  The column-loop form runs three
+// work-groups per CU by design.
+template <int TZ, bool UPDATE, bool LOOP, int XM = kXNow>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(LOOP ? 1 : PFSF_WPE)))
     ks_down(Lv<float> L, TL M, pfsf_dims c, const float* __restrict__ rin,
             float* __restrict__ rout, const float* __restrict__ q, const float* __restrict__ p,
-            const float* __restrict__ pprev, int xmode, double* __restrict__ x,
-            float* __restrict__ rc, PcgScalars* sc, const double* __restrict__ pq_part,
-            unsigned pq_n, double* partials) {
-  pin_args(L, M, c, rin, rout, q, p, pprev, xmode, x, rc, sc, pq_part, pq_n, partials);
+            const float* __restrict__ pprev, double* __restrict__ x, float* __restrict__ rc,
+            PcgScalars* sc, const double* __restrict__ pq_part, unsigned pq_n,
+            double* partials) {
+  pin_args(L, M, c, rin, rout, q, p, pprev, x, rc, sc, pq_part, pq_n, partials);
   if constexpr (LOOP)
-    down_stream_body<TZ, UPDATE>(L, StrideColumns{M}, c, rin, rout, q, p, pprev, xmode, x, rc,
-                                 sc, pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
+    down_stream_body<TZ, UPDATE, XM>(L, StrideColumns{M}, c, rin, rout, q, p, pprev, x, rc, sc,
+                                     pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
   else
-    down_stream_body<TZ, UPDATE>(L, SlotColumn{M}, c, rin, rout, q, p, pprev, xmode, x, rc, sc,
-                                 pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
+    down_stream_body<TZ, UPDATE, XM>(L, SlotColumn{M}, c, rin, rout, q, p, pprev, x, rc, sc,
+                                     pq_part, pq_n, blockIdx.x == 0, partials, blockIdx.x);
 }
 
 // ── up-sweep of the finest level (see up_body) ─────────────────────────
