@@ -110,6 +110,8 @@ class BatchSolver {
   hipError_t tune(const pfsf_batch_desc& desc, const std::vector<int>& cls);
   hipError_t build_tables();
   hipError_t iteration(int c);
+  // the FP64 iterate updates that unfinished islands' passes left pending
+  hipError_t apply_pending();
   hipError_t capture();
   hipError_t select_islands(const int* live);  // tile lists of the live islands only
   std::vector<BatchTile> grouped(int l, const int* live, std::vector<TileGroup>& g) const;

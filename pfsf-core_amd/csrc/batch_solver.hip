@@ -100,7 +100,10 @@ __global__ void __launch_bounds__(BS)
                        t.slot == I.tb, pq_part, (unsigned)t.slot);
 }
 
-template <int TZ, bool UPDATE>
+// Iterations of a pair (c = 0, 1; passes start at c = 0) update the FP64
+// iterate as the single solver's do: the even one leaves alpha p pending,
+// the odd one applies both directions (kXDefer / kXPair, see ks_down).
+template <int TZ, bool UPDATE, int XM = kXNow>
 __global__ void __launch_bounds__(BS)
     kbs_down(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
              const BatchIsland* __restrict__ isl, PcgScalars* sc,
@@ -111,10 +114,10 @@ __global__ void __launch_bounds__(BS)
   const BatchIsland& I = isl[D.island];
   const float* rin = UPDATE ? (c ? I.r1 : I.r0) : D.r;
   float* rout = UPDATE ? (c ? I.r0 : I.r1) : nullptr;
-  down_stream_body<TZ, UPDATE, kXNow>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin,
-                                      rout, I.q, c ? I.p0 : I.p1, nullptr, I.x, D.rc,
-                                      sc + D.island, pq_part + I.tb, (unsigned)I.tn,
-                                      t.slot == I.tb, rr_part, (unsigned)t.slot);
+  down_stream_body<TZ, UPDATE, XM>(lv_of<float>(D), OneColumn{t.x0, t.y0, t.z0}, D.c, rin, rout,
+                                   I.q, c ? I.p0 : I.p1, I.p1, I.x, D.rc, sc + D.island,
+                                   pq_part + I.tb, (unsigned)I.tn, t.slot == I.tb, rr_part,
+                                   (unsigned)t.slot);
 }
 
 template <int TZ>
@@ -197,6 +200,31 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(PFSF_WP
                            I.r0, part, (unsigned)t.slot, xzero);
 }
 
+// The iterate update an unfinished island's pass left pending (its last
+// iteration, or the last one before a breakdown, was an even one: an odd
+// number of convergence tests): x += xa p1 over the island's finest tiles,
+// the FMA of a single update (p1 is zero off the free set).  Runs before
+// each residual phase and once after the last pass; an island is finished
+// only after a residual phase, so no update is applied twice.
+template <int TZ>
+__global__ void __launch_bounds__(BS)
+    kb_x_pending(const BatchTile* __restrict__ tiles, const BatchLevel* __restrict__ lv,
+                 const BatchIsland* __restrict__ isl, const BatchState* __restrict__ st,
+                 const PcgScalars* __restrict__ sc) {
+  const BatchTile t = tiles[blockIdx.x];
+  if (t.level < 0) return;
+  const int k = lv[t.level].island;
+  if (st[k].finished || (sc[k].iters & 1) == 0) return;
+  const BatchIsland& I = isl[k];
+  const double a = sc[k].xa;
+  const int x = t.x0 + (int)threadIdx.x % TX, y = t.y0 + (int)threadIdx.x / TX;
+  if (x >= I.d.lx || y >= I.d.ly) return;
+  for (int z = t.z0; z < t.z0 + TZ && z < I.d.lz; z++) {
+    const int i = flat32(x, y, z, I.d);
+    I.x[i] = fma(a, (double)I.p1[i], I.x[i]);
+  }
+}
+
 // Pass boundary, per island: an island at the tolerance, out of iterations
 // or stalled finishes; the others restart PCG from the new residual (the pass
 // stops when the recursive residual has dropped by max(0.85 tol / rel, 2e-6)).
@@ -229,7 +257,7 @@ __global__ void __launch_bounds__(BS)
   P.done = 0;
   P.iters = 0;
   P.have_dir = 0;
-  P.rz = P.beta = P.alpha = P.pq = 0.0;
+  P.rz = P.beta = P.alpha = P.pq = P.xa = 0.0;
   P.rzs[0] = P.rzs[1] = 0.0;
   P.rr = rr;
   P.stop2 = rr * stop_rel * stop_rel;
@@ -832,6 +860,16 @@ hipError_t BatchSolver::precondition() {
 }
 
 // One PCG iteration of every live island, reading r[c], p[c].
+hipError_t BatchSolver::apply_pending() {
+  hipStream_t s = stream_;
+  for (const TileGroup& g : groups0_)
+    with_fine_tz(g.tz, [&](auto tz) {
+      kb_x_pending<decltype(tz)::value><<<g.count, BS, 0, s>>>(all0_ + g.off, levels_, isl_d_,
+                                                               st_, sc_);
+    });
+  return hipGetLastError();
+}
+
 hipError_t BatchSolver::iteration(int c) {
   hipStream_t s = stream_;
   if (ntiles_[0] == 0) return hipSuccess;
@@ -842,8 +880,12 @@ hipError_t BatchSolver::iteration(int c) {
     });
   for (const TileGroup& g : groups_[0])
     with_fine_tz(g.tz, [&](auto tz) {
-      kbs_down<decltype(tz)::value, true><<<g.count, BS, 0, s>>>(tiles_[0] + g.off, levels_,
-                                                                  isl_d_, sc_, ppq_, prr_, c);
+      if (c)
+        kbs_down<decltype(tz)::value, true, kXPair><<<g.count, BS, 0, s>>>(
+            tiles_[0] + g.off, levels_, isl_d_, sc_, ppq_, prr_, c);
+      else
+        kbs_down<decltype(tz)::value, true, kXDefer><<<g.count, BS, 0, s>>>(
+            tiles_[0] + g.off, levels_, isl_d_, sc_, ppq_, prr_, c);
     });
   (void)vcycle_coarse(true);
   for (const TileGroup& g : groups_[0])
@@ -878,6 +920,7 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
   const int pairs_per_check = 4;
   for (int pass = 0; pass < 18 && err == hipSuccess; pass++) {
     chk(hipMemsetAsync(live_, 0, sizeof(int), s), err);
+    chk(apply_pending(), err);
     for (const TileGroup& g : groups0_)
       with_fine_tz(g.tz, [&](auto tz) {
         kb_residual_tiles<decltype(tz)::value><<<g.count, BS, 0, s>>>(all0_ + g.off, levels_,
@@ -911,6 +954,7 @@ hipError_t BatchSolver::solve(const float* const* b, double* const* x64, float* 
       if (4 * live_tiles < 3 * listed_tiles_) chk(select_islands(flags_host_.data()), err);
     }
   }
+  chk(apply_pending(), err);  // a pass cut off by the pass limit
   kb_finish_part<<<nchunks_, BS, 0, s>>>(chunks_, isl_d_);
   chk(hipGetLastError(), err);
   chk(hipEventRecord(ev1_, s), err);
