@@ -417,6 +417,64 @@ __device__ __forceinline__ float tile_apply(const Tile<TZ>& T, int c, float mi, 
   return stencil_apply<MASK>(T.v, T.s, c, HX, HXY, mi, scale, mk);
 }
 
+// The off-diagonal sum of stencil_apply, scale sum_j w_ij v_j over the
+// (masked) couplings, so that (A v)_i = D_i v_i - offsum_apply with D_i the
+// level's diagonal (k_level_diag: the same weights plus the anchor mass).
+// Every cell off the free set holds v = 0, so its term vanishes without
+// stencil_apply's coupling test; a masked-off free neighbour is cut by the
+// mask.  The smoothers use it where the centre term cancels in closed form.
+template <bool MASK = false>
+__device__ __forceinline__ float offsum_apply(const float* __restrict__ v,
+                                              const float* __restrict__ s, int c, int sy, int sz,
+                                              float scale, uint32_t mk = kAllCouplings) {
+  const float si = s[c];
+  const float ki = si * si;
+  float af = 0.f, ae = 0.f, ac = 0.f;
+  auto keep = [&](float x, int b) {
+    if (!MASK) return x;
+    return __uint_as_float(__float_as_uint(x) & (0u - ((mk >> b) & 1u)));
+  };
+  auto face = [&](int o, int b) {
+    const float sj = s[c + o];
+    af = fmaf(keep(fmaf(sj, sj, ki), b), v[c + o], af);
+  };
+  auto edge = [&](int o, int b) { ae = fmaf(keep(s[c + o], b), v[c + o], ae); };
+  auto corner = [&](int o, int b) { ac = fmaf(keep(s[c + o], b), v[c + o], ac); };
+  face(-1, nbit(-1, 0, 0));
+  face(1, nbit(1, 0, 0));
+  face(-sy, nbit(0, -1, 0));
+  face(sy, nbit(0, 1, 0));
+  face(-sz, nbit(0, 0, -1));
+  face(sz, nbit(0, 0, 1));
+  edge(-1 - sy, nbit(-1, -1, 0));
+  edge(1 - sy, nbit(1, -1, 0));
+  edge(-1 + sy, nbit(-1, 1, 0));
+  edge(1 + sy, nbit(1, 1, 0));
+  edge(-1 - sz, nbit(-1, 0, -1));
+  edge(1 - sz, nbit(1, 0, -1));
+  edge(-1 + sz, nbit(-1, 0, 1));
+  edge(1 + sz, nbit(1, 0, 1));
+  edge(-sy - sz, nbit(0, -1, -1));
+  edge(sy - sz, nbit(0, 1, -1));
+  edge(-sy + sz, nbit(0, -1, 1));
+  edge(sy + sz, nbit(0, 1, 1));
+  corner(-1 - sy - sz, nbit(-1, -1, -1));
+  corner(1 - sy - sz, nbit(1, -1, -1));
+  corner(-1 + sy - sz, nbit(-1, 1, -1));
+  corner(1 + sy - sz, nbit(1, 1, -1));
+  corner(-1 - sy + sz, nbit(-1, -1, 1));
+  corner(1 - sy + sz, nbit(1, -1, 1));
+  corner(-1 + sy + sz, nbit(-1, 1, 1));
+  corner(1 + sy + sz, nbit(1, 1, 1));
+  return scale * (0.5f * af + si * (kEdgeP * ae + kCornerP * ac));
+}
+
+template <int TZ, bool MASK = false>
+__device__ __forceinline__ float tile_offsum(const Tile<TZ>& T, int c, float scale,
+                                             uint32_t mk = kAllCouplings) {
+  return offsum_apply<MASK>(T.v, T.s, c, HX, HXY, scale, mk);
+}
+
 __device__ __forceinline__ int tcell(int tx, int ty, int tz) {
   return (tz + 1) * HXY + (ty + 1) * HX + (tx + 1);
 }
@@ -600,7 +658,7 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
   // own-voxel operands (cell 0 stands in for a cell outside the grid; its
   // values are never used)
   bool inb[TZ];
-  float ro[TZ], qo[TZ], amr[TZ], pr[TZ];
+  float ro[TZ], qo[TZ], pr[TZ];
   double xr[TZ];
   uint32_t mkr[TZ];
   float kk[Stage<TZ>::NP];
@@ -612,7 +670,6 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
       const int i = inb[tz] ? i0 + tz * sz : 0;
       ro[tz] = ld(rin, i);
       qo[tz] = UPDATE ? ld(q, i) : 0.f;
-      amr[tz] = ld(L.am, i);
       pr[tz] = UPDATE ? ld(p, i) : 0.f;
       xr[tz] = UPDATE ? ld(x, i) : 0.0;
       mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
@@ -665,8 +722,9 @@ __device__ __forceinline__ void down_body(const Lv<KT>& L, bool have, int x0, in
       const int cc = c0 + tz * HXY;
       const float ri = rr_[tz];
       const bool fr = is_free_s(T.s[cc]);
-      const float ax = tile_apply<TZ, MASK>(T, cc, amr[tz], L.scale, mkr[tz]);
-      res[tz] = fr ? ri - ax : 0.f;
+      // r - A x with x = omega D^-1 r: (1 - omega) r + sum_j w_ij x_j
+      const float os = tile_offsum<TZ, MASK>(T, cc, L.scale, mkr[tz]);
+      res[tz] = fr ? fmaf(1.f - om, ri, os) : 0.f;
       if (UPDATE) acc[0] += fr ? (double)ri * ri : 0.0;
     }
     restrict_tile<TZ>(res, x0, y0, z0, tx, ty, c, rc);
@@ -697,14 +755,13 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
     // own-voxel operands, loaded before the barrier (cell 0 stands in for a
     // cell outside the grid; its values are never used)
     bool inb[TZ];
-    float rr_[TZ], amr[TZ], dgr[TZ];
+    float rr_[TZ], dgr[TZ];
     uint32_t mkr[TZ];
 #pragma unroll
     for (int tz = 0; tz < TZ; tz++) {
       inb[tz] = col && z0 + tz < d.lz;
       const int i = inb[tz] ? i0 + tz * sz : 0;
       rr_[tz] = ld(r, i);
-      amr[tz] = ld(L.am, i);
       dgr[tz] = ld(L.dg, i);
       mkr[tz] = MASK ? L.msk[i] : kAllCouplings;
     }
@@ -726,8 +783,9 @@ __device__ __forceinline__ void up_body(const Lv<KT>& L, bool have, int x0, int 
       const int cc = c0 + tz * HXY;
       const float xi = T.v[cc], ri = rr_[tz];
       const bool fr = is_free_s(T.s[cc]);
-      const float ax = tile_apply<TZ, MASK>(T, cc, amr[tz], L.scale, mkr[tz]);
-      float zi = xi + om * (ri - ax) * rcp(dgr[tz]);
+      // x + omega D^-1 (r - A x) = (1 - omega) x + omega D^-1 (r + sum_j w_ij x_j)
+      const float os = tile_offsum<TZ, MASK>(T, cc, L.scale, mkr[tz]);
+      float zi = fmaf(1.f - om, xi, om * (ri + os) * rcp(dgr[tz]));
       zi = fr ? zi : 0.f;
       if (FINEST) acc[0] += fr ? (double)ri * zi : 0.0;
       if (inb[tz]) st(out, i0 + tz * sz, zi);
