@@ -282,7 +282,7 @@ PFSF_API void pfsf_solver_destroy(pfsf_solver s);
  * pending), 2 second-level down-sweep, 3 finest up-sweep (prolong, smooth,
  * r.z) -- each timed by the kernel's own start/stop event pair -- and 4 one
  * PCG iteration including the whole V-cycle (the average over back-to-back
- * iterations of alternating parity, between events).  Use after
+ * iterations launched as the solve launches them, between events).  Use after
  * pfsf_solver_solve; it overwrites the solver's scratch vectors and its
  * internal FP64 iterate, not the caller's arrays. */
 PFSF_API pfsf_result pfsf_solver_profile(pfsf_solver s, int32_t which, int32_t reps,

@@ -1751,9 +1751,10 @@ hipError_t MgSolver::solve(const float* b, double* x64, float* phi32, pfsf_solve
 // finest up-sweep + r.z: each launch is bracketed by the kernel's own
 // start/stop timestamps (hipExtLaunchKernel events), so the figure is the
 // kernel duration alone.  4: one whole PCG iteration, the average over
-// `reps` back-to-back iterations of alternating parity timed between events
-// (launch gaps included).  Call after a solve so the vectors hold finite
-// data.
+// `reps` (rounded up to a multiple of 4) back-to-back iterations launched as
+// the solve launches them (graphs of four iterations of alternating parity),
+// timed between events, launch gaps included.  Call after a solve so the
+// vectors hold finite data.
 hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   hipError_t err = hipSuccess;
   hipStream_t s = stream_;
@@ -1766,14 +1767,17 @@ hipError_t MgSolver::profile(int which, int reps, float* avg_ms) {
   if (which == 2 && nlev_ < 3) return hipErrorInvalidValue;
   k_profile_begin<<<1, 1, 0, s>>>(sc_);
   if (which == 4) {
-    (void)iteration(1);  // warm
+    // the solve's own launches: captured graphs of four iterations (parities
+    // 0, 1, 0, 1) where graphs are in use, else direct launches
+    const int n4 = (reps + 3) / 4;
+    chk(launch_iters(4, 0), err);  // warm (and capture)
     chk(hipEventRecord(ev0_, s), err);
-    for (int k = 0; k < reps; k++) (void)iteration(k & 1);  // as the passes alternate
+    for (int k = 0; k < n4 && err == hipSuccess; k++) chk(launch_iters(4, 0), err);
     chk(hipEventRecord(ev1_, s), err);
     chk(hipEventSynchronize(ev1_), err);
     float ms = 0.f;
     chk(hipEventElapsedTime(&ms, ev0_, ev1_), err);
-    *avg_ms = ms / (float)reps;
+    *avg_ms = ms / (float)(4 * n4);
     return err;
   }
   auto ext = [&](auto kern, unsigned grid, hipEvent_t a, hipEvent_t b, auto... args) {
