@@ -77,6 +77,11 @@ struct BatchState {
   int passes;
   int finished;     // 1: converged or given up
   int converged;
+  double rr0;       // ||r||^2 of the true residual a pass started from
+  // the first pass's floor (0: PFSF_PASS_FLOOR), kept across solves of the
+  // batch: raised while first passes end well above their recursive
+  // residual (as MgSolver::solve does)
+  double floor0;
 };
 
 class BatchSolver {

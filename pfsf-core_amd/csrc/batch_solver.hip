@@ -245,6 +245,18 @@ __global__ void __launch_bounds__(BS)
   const bool first = S.passes == 0;
   S.total += pass_iters;
   const bool stalled = !first && (pass_iters == 0 || (pass_done == 2 && rel >= S.rel));
+  // the first pass's drift from the true residual sets the next solve's
+  // first-pass floor (see MgSolver::solve)
+  if (S.passes == 1 && pass_done == 1 && pass_iters > 0 && P.rr > 0.0 && S.rr0 > 0.0 &&
+      S.rel > 0.0) {
+    const double rec = S.rel * sqrt(P.rr / S.rr0);
+    const double drift = rel / rec;
+    const double f0 = S.floor0 > 0.0 ? S.floor0 : PFSF_PASS_FLOOR;
+    if (drift > 3.0)
+      S.floor0 = fmin(fmax(0.5 * rel / S.rel, PFSF_PASS_FLOOR), 1e-2);
+    else if (drift < 1.5)
+      S.floor0 = fmax(0.5 * f0, PFSF_PASS_FLOOR);
+  }
   S.prev_rel = S.rel;
   S.rel = rel;
   if (rel <= tol || stalled || S.total >= max_iters || S.passes >= 16) {
@@ -253,7 +265,9 @@ __global__ void __launch_bounds__(BS)
     P.done = 3;
     return;
   }
-  const double stop_rel = fmax(0.85 * tol / rel, PFSF_PASS_FLOOR);
+  const double floor = (first && S.floor0 > 0.0) ? S.floor0 : PFSF_PASS_FLOOR;
+  const double stop_rel = fmax(0.85 * tol / rel, floor);
+  S.rr0 = rr;
   P.done = 0;
   P.iters = 0;
   P.have_dir = 0;
@@ -691,6 +705,7 @@ hipError_t BatchSolver::build_tables() {
   flags_ = balloc<int>(n, owned_, err);
   sc_ = balloc<PcgScalars>(n, owned_, err);
   st_ = balloc<BatchState>(n, owned_, err);
+  if (st_) chk(hipMemset(st_, 0, n * sizeof(BatchState)), err);  // floor0 = 0 (the default)
   ppq_ = balloc<double>(nslots_, owned_, err);
   prr_ = balloc<double>(nslots_, owned_, err);
   prz_ = balloc<double>(nslots_, owned_, err);

@@ -255,6 +255,9 @@ class MgSolver {
   hipGraph_t graph_[6] = {};
   hipGraphExec_t graph_exec_[6] = {};
   double rho_pass_[4] = {};  // per-iteration r.r contraction of each pass of the last solve
+  // floor of the first pass (see solve()): PFSF_PASS_FLOOR, raised while the
+  // first passes of earlier solves ended well above their recursive residual
+  double floor0_ = PFSF_PASS_FLOOR;
   bool use_graph_ = true;
 };
 
