@@ -48,7 +48,8 @@ inline int fold_ld(int64_t n) { return n <= 1024 ? 1024 : n <= 2048 ? 2048 : n <
 
 // Smallest relative drop of the recursive residual a solve pass aims for
 // before the true FP64 residual is checked (a pass stops at
-// max(0.85 tol / rel, this)).
+// max(0.85 tol / rel, this); a first pass at max(0.85 tol / rel, floor0_),
+// floor0_ starting here and adapting to the FP32 recursion's drift).
 #ifndef PFSF_PASS_FLOOR
 #define PFSF_PASS_FLOOR 2e-6
 #endif
