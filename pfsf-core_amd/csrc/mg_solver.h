@@ -174,6 +174,9 @@ class MgSolver {
   const uint8_t* type() const { return type_; }
   const float* kappa() const { return kappa_; }
   const uint8_t* free_mask() const { return levels_.empty() ? nullptr : levels_[0].fr; }
+  // solid voxels 26-connected to an anchor (the orphan mask of the last
+  // build, on stream())
+  const uint8_t* reach() const { return reach_; }
   hipStream_t stream() const { return stream_; }
   // Preconditioner-only use by an outer Krylov solve with another operator:
   // precond_begin() clears the stop flag, then each precond_apply() writes

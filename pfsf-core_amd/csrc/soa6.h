@@ -83,7 +83,6 @@ class Soa6Solver {
   double *dg_ = nullptr, *r_ = nullptr, *p_ = nullptr, *q_ = nullptr, *z_ = nullptr;
   double* xo_ = nullptr;  // own iterate when the caller passes no x64
   uint8_t* free_ = nullptr;
-  uint8_t* reach_ = nullptr;
   double* part_ = nullptr;
   unsigned nblk_ = 0;
   S6Scalars* sc_ = nullptr;

@@ -829,7 +829,6 @@ hipError_t Soa6Solver::create(pfsf_dims d, const uint8_t* type, const float* con
   z_ = (double*)take(8 * n_);
   xo_ = (double*)take(8 * n_);
   free_ = (uint8_t*)take(n_);
-  reach_ = (uint8_t*)take(n_);
   part_ = (double*)take(8 * (size_t)nblk_);
   sc_ = (S6Scalars*)take(sizeof(S6Scalars));
   if (err != hipSuccess) return err;
@@ -841,8 +840,7 @@ hipError_t Soa6Solver::create(pfsf_dims d, const uint8_t* type, const float* con
   ck(hipMemsetAsync(p_, 0, 8 * n_, s), err);
   ck(hipMemsetAsync(z_, 0, 8 * n_, s), err);
   if (err != hipSuccess) return err;
-  if (ck(rebuild(), err)) return err;
-  return refresh();
+  return refresh();  // the proxy was just built from the same type and kappa
 }
 
 hipError_t Soa6Solver::rebuild() {
@@ -851,10 +849,8 @@ hipError_t Soa6Solver::rebuild() {
   // the caller's stream holds the uploads of type and kappa
   ck(hipEventRecord(ev0_, caller_), err);
   ck(hipStreamWaitEvent(s, ev0_, 0), err);
-  if (ck(mg_->update(type_, kappa_), err)) return err;
-  // orphan mask on the same union-find as the proxy's (q_ as its scratch:
-  // 8 n bytes >= 4 n + 4)
-  ck(anchored_mask(d_, type_, reach_, (uint32_t*)q_, (int*)(r_), s), err);
+  // the proxy's build also forms the orphan mask of type (MgSolver::reach)
+  ck(mg_->update(type_, kappa_), err);
   return err;
 }
 
@@ -863,7 +859,7 @@ hipError_t Soa6Solver::refresh() {
   hipStream_t s = mg_->stream();
   ck(hipEventRecord(ev0_, caller_), err);
   ck(hipStreamWaitEvent(s, ev0_, 0), err);
-  k_s6_setup<<<nblk_, BS, 0, s>>>(d_, type_, cw_, reach_, dg_, free_);
+  k_s6_setup<<<nblk_, BS, 0, s>>>(d_, type_, cw_, mg_->reach(), dg_, free_);
   ck(hipGetLastError(), err);
   return err;
 }
